@@ -1,0 +1,127 @@
+/*
+ * d3hip.h -- C ABI of libd3hip.so, the MI355X implementation of the DFT-D3
+ * energy / gradient / virial path.
+ *
+ * Part 1 mirrors the subset of the simple-dftd3 v1.6.0 C API that its Python
+ * layer binds for this path (include/s-dftd3.h of the reference): the same
+ * symbol names, argument meaning, NULL conventions and error strings, with
+ * host pointers.  A program or binding written against s-dftd3.h for these
+ * entry points links against libd3hip.so unchanged.
+ *
+ * Part 2 adds entry points that take device pointers (e.g. PyTorch-ROCm
+ * tensors) and a HIP stream, for single systems and batches of systems.
+ *
+ * Units: bohr and Hartree.  positions [n][3] row-major, lattice [3][3] with
+ * rows = lattice vectors, gradient [n][3], sigma [3][3] in the order the
+ * reference returns it (sigma(a,b) at [b][a]; the array is symmetric).
+ */
+#ifndef D3HIP_H
+#define D3HIP_H
+
+#ifdef __cplusplus
+#define D3HIP_API extern "C"
+#else
+#include <stdbool.h>
+#define D3HIP_API extern
+#endif
+
+typedef struct _dftd3_error* dftd3_error;
+typedef struct _dftd3_structure* dftd3_structure;
+typedef struct _dftd3_model* dftd3_model;
+typedef struct _dftd3_param* dftd3_param;
+
+/* ------------------------------------------------------------------ Part 1 */
+
+/* s-dftd3.h:71-72 dftd3_get_version: major*10000 + minor*100 + patch of the
+ * API this library mirrors (1.6.0). */
+D3HIP_API int dftd3_get_version(void);
+
+/* s-dftd3.h:85-100 error handle (api.f90:176-238). */
+D3HIP_API dftd3_error dftd3_new_error(void);
+D3HIP_API int dftd3_check_error(dftd3_error error);
+D3HIP_API void dftd3_get_error(dftd3_error error, char* buffer, const int* buffersize);
+D3HIP_API void dftd3_delete_error(dftd3_error* error);
+
+/* s-dftd3.h:107-113 dftd3_new_structure (api.f90:243-280): positions are
+ * wrapped into the central cell; an error "Too close interatomic distances
+ * found" is set for any pair closer than 1e-9 bohr.  lattice and periodic may
+ * be NULL; a lattice without periodicity is periodic in three dimensions. */
+D3HIP_API dftd3_structure dftd3_new_structure(dftd3_error error, const int natoms, const int* numbers,
+                                              const double* positions, const double* lattice,
+                                              const bool* periodic);
+
+/* s-dftd3.h:116-117 (api.f90:284-296) */
+D3HIP_API void dftd3_delete_structure(dftd3_structure* mol);
+
+/* s-dftd3.h:120-124 dftd3_update_structure (api.f90:300-335); lattice may be NULL. */
+D3HIP_API void dftd3_update_structure(dftd3_error error, dftd3_structure mol, const double* positions,
+                                      const double* lattice);
+
+/* s-dftd3.h:131-133 dftd3_new_d3_model (api.f90:339-364). */
+D3HIP_API dftd3_model dftd3_new_d3_model(dftd3_error error, dftd3_structure mol);
+
+/* s-dftd3.h:138-143 (api.f90:369-390), widths set to zero. */
+D3HIP_API void dftd3_set_model_realspace_cutoff(dftd3_error error, dftd3_model model, double disp2, double disp3,
+                                                double cn);
+
+/* s-dftd3.h:146-150 dftd3_set_model_ghost_index: ghost atoms (1-based
+ * indices) contribute no dispersion (model.f90:166-174). */
+D3HIP_API void dftd3_set_model_ghost_index(dftd3_error error, dftd3_model model, const int* ghost, int nidx);
+
+/* s-dftd3.h:153-160 (api.f90:393-416) */
+D3HIP_API void dftd3_set_model_realspace_cutoff_smooth(dftd3_error error, dftd3_model model, double disp2,
+                                                       double disp3, double cn, double width2, double width3);
+
+/* s-dftd3.h:208-209 (api.f90) */
+D3HIP_API void dftd3_delete_model(dftd3_model* disp);
+
+/* s-dftd3.h:216-229 zero damping (api.f90:625-694, param.f90:805-1090). */
+D3HIP_API dftd3_param dftd3_new_zero_damping(dftd3_error error, double s6, double s8, double s9, double rs6,
+                                             double rs8, double alp);
+D3HIP_API dftd3_param dftd3_load_zero_damping(dftd3_error error, char* method, bool atm);
+
+/* s-dftd3.h:232-245 rational damping (api.f90:555-622, param.f90:302-800). */
+D3HIP_API dftd3_param dftd3_new_rational_damping(dftd3_error error, double s6, double s8, double s9, double a1,
+                                                 double a2, double alp);
+D3HIP_API dftd3_param dftd3_load_rational_damping(dftd3_error error, char* method, bool atm);
+
+/* s-dftd3.h:330-331 */
+D3HIP_API void dftd3_delete_param(dftd3_param* param);
+
+/* s-dftd3.h:380-388 dftd3_get_dispersion (api.f90:1038-1111): gradient and
+ * sigma may be NULL; requesting either evaluates both derivatives. */
+D3HIP_API void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model disp, dftd3_param param,
+                                    double* energy, double* gradient, double* sigma);
+
+/* ------------------------------------------------------------------ Part 2
+ * No counterpart in the reference: device-pointer entry points. */
+
+/* Arithmetic of the pair and triple terms: FP64 (default) or FP32 with FP64
+ * accumulation of energies, gradients and virials. */
+#define D3HIP_PRECISION_FP64 0
+#define D3HIP_PRECISION_MIXED 1
+
+D3HIP_API void d3hip_set_model_precision(dftd3_error error, dftd3_model model, int precision);
+
+/* Evaluate a batch of independent systems in one pass.  Host arrays:
+ *   natoms[nsys], numbers[sum natoms], lattices[nsys][3][3] (may be NULL
+ *   when no system is periodic), periodic[nsys][3] (may be NULL),
+ *   params[nsys][7] = {kind (0 rational, 1 zero), s6, s8, s9, a1|rs6,
+ *   a2|rs8, alp}, cutoffs[nsys][5] = {disp2, disp3, cn, width2, width3}
+ *   (NULL selects the defaults 60, 40, 40, 0, 0).
+ * Device arrays: positions[sum natoms][3] (not wrapped by this call),
+ *   energy[nsys], gradient[sum natoms][3] (may be NULL), sigma[nsys][3][3]
+ *   (may be NULL).  stream is a hipStream_t (NULL = default stream).
+ * The call is asynchronous with respect to the host except for one
+ * device-to-host read of the three-body list size. */
+D3HIP_API void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
+                                          const int* numbers, const double* lattices, const bool* periodic,
+                                          const double* params, const double* cutoffs, int precision,
+                                          const double* positions, double* energy, double* gradient,
+                                          double* sigma, void* stream);
+
+/* A model without a structure, usable only as the workspace (device buffers
+ * and stream) of d3hip_get_dispersion_batch. */
+D3HIP_API dftd3_model d3hip_new_workspace(dftd3_error error);
+
+#endif /* D3HIP_H */

@@ -1,0 +1,810 @@
+/*
+ * d3_oracle.c -- CPU oracle for the DFT-D3 energy / gradient / virial path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP library is
+ * compared against and the CPU baseline timed beside it; the library never
+ * links, loads or calls it.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it.
+ *
+ * It is a plain FP64 C restatement of the loops of simple-dftd3 v1.6.0
+ * (the reference), each function citing the reference file:line it follows.
+ * Loop nests, the triangular index sets, the distance formulas and their
+ * operand order follow the reference so that the set of pair / triple terms
+ * is the same; OpenMP parallel loops with thread-private accumulators merged
+ * in a critical section mirror the reference's shared-memory strategy.
+ *
+ * Pinning (see tests/test_oracle_golden.py): reference golden vectors
+ * (test/unit/test_regression.f90 Ta90 3D, test/unit/test_periodic_2d.f90
+ * gh185 and BP86 2D, test/validation/0{1,2,6,7}-*.json,
+ * python/dftd3/test_interface.py energies) plus finite differences of the
+ * energy for gradient and virial, closed forms (dimer CN, isolated atom C6,
+ * weight normalisation) and brute-force enumeration of the lattice images.
+ *
+ * Conventions: positions xyz[n][3] in bohr (Fortran xyz(3,n)), lattice
+ * lat[3][3] with rows = lattice vectors (Fortran lattice(3,3) columns),
+ * gradient[n][3], sigma[3][3] stored in Fortran order sigma(a,b) at
+ * sigma[(b)*3 + (a)] -- i.e. the array the C API of the reference returns.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAX_ELEM 103
+#define MAX_REF 7
+#define NPAIR (MAX_ELEM * (MAX_ELEM + 1) / 2)
+#define EPS DBL_EPSILON
+
+/* ------------------------------------------------------------------ tables */
+static int g_nref[MAX_ELEM];
+static double g_refcn[MAX_ELEM][MAX_REF];
+static double *g_c6ref; /* [NPAIR][MAX_REF][MAX_REF] as [ic][jref][iref] */
+static double g_r4r2raw[118];
+static double g_vdwrad_aa[NPAIR];
+static double g_covrad_aa[118];
+static double g_aatoau;
+
+/* Copy the element tables (data/d3_tables.npz) into the oracle. */
+void d3o_set_tables(const int *nref, const double *refcn, const double *c6ref,
+                    const double *r4_over_r2, const double *vdwrad_aa,
+                    const double *covrad_aa, double aatoau)
+{
+    memcpy(g_nref, nref, sizeof g_nref);
+    memcpy(g_refcn, refcn, sizeof g_refcn);
+    free(g_c6ref);
+    g_c6ref = (double *)malloc(sizeof(double) * NPAIR * MAX_REF * MAX_REF);
+    memcpy(g_c6ref, c6ref, sizeof(double) * NPAIR * MAX_REF * MAX_REF);
+    memcpy(g_r4r2raw, r4_over_r2, sizeof g_r4r2raw);
+    memcpy(g_vdwrad_aa, vdwrad_aa, sizeof g_vdwrad_aa);
+    memcpy(g_covrad_aa, covrad_aa, sizeof g_covrad_aa);
+    g_aatoau = aatoau;
+}
+
+/* mctc-lib covalent_rad_d3 = 4/3 * covalent_rad_2009 (model.f90:125). */
+double d3o_rcov(int z) { return 4.0 / 3.0 * (g_aatoau * g_covrad_aa[z - 1]); }
+
+/* data/r4r2.f90:64-66,91-105: sqrt(0.5 * r4r2(Z) * sqrt(Z)). */
+double d3o_r4r2(int z)
+{
+    if (z <= 0 || z > 118) return 0.0;
+    return sqrt(0.5 * (g_r4r2raw[z - 1] * sqrt((double)z)));
+}
+
+/* data/vdwrad.f90:38,1163-1197 (only the in-table branch: Z <= 103). */
+double d3o_rvdw(int z1, int z2)
+{
+    int a = z1 > z2 ? z1 : z2, b = z1 > z2 ? z2 : z1;
+    return g_aatoau * g_vdwrad_aa[b + a * (a - 1) / 2 - 1];
+}
+
+/* reference.f90:160-171 get_c6(iref, jref, ati, atj), 1-based refs. */
+double d3o_get_c6(int iref, int jref, int ati, int atj)
+{
+    int ic;
+    if (ati > atj) {
+        ic = atj + ati * (ati - 1) / 2;
+        return g_c6ref[((size_t)(ic - 1) * MAX_REF + (jref - 1)) * MAX_REF + (iref - 1)];
+    }
+    ic = ati + atj * (atj - 1) / 2;
+    return g_c6ref[((size_t)(ic - 1) * MAX_REF + (iref - 1)) * MAX_REF + (jref - 1)];
+}
+
+/* ------------------------------------------------------------ geometry */
+static void cross(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - b[1] * a[2];
+    c[1] = a[2] * b[0] - b[2] * a[0];
+    c[2] = a[0] * b[1] - b[0] * a[1];
+}
+
+static double norm2v(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* cutoff.f90:204-227 get_translations: rep from plane normals. */
+static void get_translations(const double lat[3][3], double rthr, int rep[3])
+{
+    double n[3][3];
+    cross(lat[1], lat[2], n[0]);
+    cross(lat[2], lat[0], n[1]);
+    cross(lat[0], lat[1], n[2]);
+    for (int k = 0; k < 3; ++k) {
+        double l = norm2v(n[k]);
+        for (int c = 0; c < 3; ++c) n[k][c] = n[k][c] / l;
+    }
+    for (int k = 0; k < 3; ++k) {
+        double cs = n[k][0] * lat[k][0] + n[k][1] * lat[k][1] + n[k][2] * lat[k][2];
+        rep[k] = (int)ceil(fabs(rthr / cs));
+    }
+}
+
+/*
+ * cutoff.f90:140-186 get_lattice_points.  lat[k] is lattice vector k.
+ * Writes at most maxtr translations [ntr][3], returns ntr (or the needed
+ * count if it exceeds maxtr).  Loop order ix outer, iy, iz inner.
+ */
+int d3o_lattice_points(const int *periodic, const double *latflat, double rthr,
+                       double *trans, int maxtr)
+{
+    double lat[3][3];
+    int rep[3] = {0, 0, 0};
+    int np = periodic[0] + periodic[1] + periodic[2];
+    memcpy(lat, latflat, sizeof lat);
+    if (np == 0) {
+        if (maxtr >= 1) trans[0] = trans[1] = trans[2] = 0.0;
+        return 1;
+    }
+    if (np == 3) {
+        get_translations(lat, rthr, rep);
+    } else if (np == 1) {
+        for (int k = 0; k < 3; ++k)
+            rep[k] = periodic[k] ? (int)ceil(rthr / norm2v(lat[k])) : 0;
+    } else {
+        /* get_normals, then where(spread(periodic, 2, 3)) norms = lat:
+         * the mask selects Cartesian component c of every vector. */
+        double nrm[3][3];
+        for (int k = 0; k < 3; ++k) cross(lat[(k + 1) % 3], lat[(k + 2) % 3], nrm[k]);
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c)
+                if (periodic[c]) nrm[k][c] = lat[k][c];
+        get_translations(nrm, rthr, rep);
+        for (int k = 0; k < 3; ++k)
+            if (!periodic[k]) rep[k] = 0;
+    }
+    int ntr = (2 * rep[0] + 1) * (2 * rep[1] + 1) * (2 * rep[2] + 1);
+    if (ntr > maxtr) return ntr;
+    int itr = 0;
+    for (int ix = -rep[0]; ix <= rep[0]; ++ix)
+        for (int iy = -rep[1]; iy <= rep[1]; ++iy)
+            for (int iz = -rep[2]; iz <= rep[2]; ++iz) {
+                for (int c = 0; c < 3; ++c)
+                    trans[3 * itr + c] = lat[0][c] * ix + lat[1][c] * iy + lat[2][c] * iz;
+                ++itr;
+            }
+    return ntr;
+}
+
+/* mctc-lib matinv_3x3 (adjugate over determinant), used by utils.f90:38. */
+static void matinv3(const double a[3][3], double b[3][3])
+{
+    /* a is Fortran lattice(3,3): a[col][row] in our row-vector storage */
+    double m[3][3]; /* m[r][c] = Fortran a(r+1,c+1) */
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r][c] = a[c][r];
+    double det = m[0][0] * m[1][1] * m[2][2] - m[0][0] * m[1][2] * m[2][1]
+               - m[0][1] * m[1][0] * m[2][2] + m[0][1] * m[1][2] * m[2][0]
+               + m[0][2] * m[1][0] * m[2][1] - m[0][2] * m[1][1] * m[2][0];
+    double di = 1.0 / det;
+    double inv[3][3];
+    inv[0][0] = +di * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+    inv[1][0] = -di * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+    inv[2][0] = +di * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    inv[0][1] = -di * (m[0][1] * m[2][2] - m[0][2] * m[2][1]);
+    inv[1][1] = +di * (m[0][0] * m[2][2] - m[0][2] * m[2][0]);
+    inv[2][1] = -di * (m[0][0] * m[2][1] - m[0][1] * m[2][0]);
+    inv[0][2] = +di * (m[0][1] * m[1][2] - m[0][2] * m[1][1]);
+    inv[1][2] = -di * (m[0][0] * m[1][2] - m[0][2] * m[1][0]);
+    inv[2][2] = +di * (m[0][0] * m[1][1] - m[0][1] * m[1][0]);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) b[r][c] = inv[r][c]; /* b[r][c] = inv(r,c) */
+}
+
+/* utils.f90:48-62 shift_back_abc. */
+static double shift_back(double in)
+{
+    const double eps = 1.0e-14;
+    double out = in;
+    if (in < (0.0 - eps)) out = in + ceil(-in);
+    if (in > (1.0 + eps)) out = in - floor(in);
+    if (fabs(in - 1.0) < eps) out = in - 1.0;
+    return out;
+}
+
+/* utils.f90:29-45 wrap_to_central_cell. */
+void d3o_wrap(int nat, double *xyz, const double *latflat, const int *periodic)
+{
+    if (!(periodic[0] || periodic[1] || periodic[2])) return;
+    double lat[3][3], inv[3][3];
+    memcpy(lat, latflat, sizeof lat);
+    matinv3(lat, inv);
+    for (int i = 0; i < nat; ++i) {
+        double *x = xyz + 3 * i, f[3];
+        for (int r = 0; r < 3; ++r) f[r] = inv[r][0] * x[0] + inv[r][1] * x[1] + inv[r][2] * x[2];
+        for (int r = 0; r < 3; ++r) f[r] = shift_back(f[r]);
+        /* matmul(lattice, vec): x(r) = sum_c lattice(r,c) f(c) = sum_c lat[c][r] f(c) */
+        for (int r = 0; r < 3; ++r) x[r] = lat[0][r] * f[0] + lat[1][r] * f[1] + lat[2][r] * f[2];
+    }
+}
+
+/* api.f90:1517-1530 verify_structure: 1 if any pair is closer than 1e-9. */
+int d3o_verify(int nat, const double *xyz)
+{
+    int stat = 0;
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j < i; ++j) {
+            double d[3] = {xyz[3 * j] - xyz[3 * i], xyz[3 * j + 1] - xyz[3 * i + 1],
+                           xyz[3 * j + 2] - xyz[3 * i + 2]};
+            if (norm2v(d) < 1.0e-9) ++stat;
+        }
+    return stat > 0;
+}
+
+/* cutoff.f90:73-99 smooth_cutoff. */
+static void smooth_cutoff(double r, double cutoff, double width, double *sw, double *dswdr)
+{
+    if (width <= 0.0 || width >= cutoff) {
+        *sw = 1.0; *dswdr = 0.0;
+    } else {
+        double inner = cutoff - width;
+        if (r <= inner) { *sw = 1.0; *dswdr = 0.0; }
+        else if (r >= cutoff) { *sw = 0.0; *dswdr = 0.0; }
+        else {
+            double x = (cutoff - r) / width;
+            *sw = x * x * x * (10.0 + x * (-15.0 + 6.0 * x));
+            *dswdr = -30.0 * x * x * (1.0 - x) * (1.0 - x) / width;
+        }
+    }
+}
+
+/* ------------------------------------------------------ coordination */
+#define KCN 16.0
+
+/* ncoord.f90:86-144 get_partitioned_coordination_number (no partition). */
+void d3o_cn(int nat, const int *num, const double *xyz, int ntr, const double *trans,
+            double cutoff, double *cn)
+{
+    double cutoff2 = cutoff * cutoff;
+    memset(cn, 0, sizeof(double) * nat);
+#pragma omp parallel
+    {
+        double *loc = (double *)calloc(nat, sizeof(double));
+#pragma omp for schedule(static)
+        for (int iat = 0; iat < nat; ++iat) {
+            for (int jat = 0; jat <= iat; ++jat) {
+                double rc = d3o_rcov(num[iat]) + d3o_rcov(num[jat]);
+                for (int itr = 0; itr < ntr; ++itr) {
+                    const double *t = trans + 3 * itr;
+                    double v0 = xyz[3 * iat] - (xyz[3 * jat] + t[0]);
+                    double v1 = xyz[3 * iat + 1] - (xyz[3 * jat + 1] + t[1]);
+                    double v2 = xyz[3 * iat + 2] - (xyz[3 * jat + 2] + t[2]);
+                    double r2 = v0 * v0 + v1 * v1 + v2 * v2;
+                    if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+                    double r1 = sqrt(r2);
+                    double countf = 1.0 / (1.0 + exp(-KCN * (rc / r1 - 1.0)));
+                    loc[iat] += countf;
+                    if (iat != jat) loc[jat] += countf;
+                }
+            }
+        }
+#pragma omp critical(d3o_cn_)
+        for (int i = 0; i < nat; ++i) cn[i] += loc[i];
+        free(loc);
+    }
+}
+
+/* ncoord.f90:152-228 add_coordination_number_derivs. */
+void d3o_cn_derivs(int nat, const int *num, const double *xyz, int ntr, const double *trans,
+                   double cutoff, const double *dEdcn, double *gradient, double *sigma)
+{
+    double cutoff2 = cutoff * cutoff;
+#pragma omp parallel
+    {
+        double *gl = (double *)calloc(3 * (size_t)nat, sizeof(double));
+        double sl[9] = {0};
+#pragma omp for schedule(static)
+        for (int iat = 0; iat < nat; ++iat) {
+            for (int jat = 0; jat <= iat; ++jat) {
+                double rc = d3o_rcov(num[iat]) + d3o_rcov(num[jat]);
+                for (int itr = 0; itr < ntr; ++itr) {
+                    const double *t = trans + 3 * itr;
+                    double v[3];
+                    for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + t[c]);
+                    double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+                    double r1 = sqrt(r2);
+                    double expterm = exp(-KCN * (rc / r1 - 1.0));
+                    double cf = 1.0 / (1.0 + expterm);
+                    double dcf = -cf * (1.0 - cf) * KCN * rc / r2;
+                    double cd[3];
+                    for (int c = 0; c < 3; ++c) cd[c] = dcf * v[c] / r1;
+                    double de = dEdcn[iat] + dEdcn[jat];
+                    for (int c = 0; c < 3; ++c) {
+                        gl[3 * iat + c] += cd[c] * de;
+                        gl[3 * jat + c] -= cd[c] * de;
+                    }
+                    double ds = dEdcn[iat] + (jat != iat ? dEdcn[jat] : 0.0);
+                    /* ds(a,b) = countd(b)*vec(a); sigma(a,b) at [b*3+a] */
+                    for (int b = 0; b < 3; ++b)
+                        for (int a = 0; a < 3; ++a) sl[b * 3 + a] += cd[b] * v[a] * ds;
+                }
+            }
+        }
+#pragma omp critical(d3o_cnd_)
+        {
+            for (int i = 0; i < 3 * nat; ++i) gradient[i] += gl[i];
+            for (int k = 0; k < 9; ++k) sigma[k] += sl[k];
+        }
+        free(gl);
+    }
+}
+
+/* ---------------------------------------------------------- weights */
+static int is_exceptional(double v) { return isnan(v) || fabs(v) > DBL_MAX; }
+
+/*
+ * model.f90:201-324 weight_references, wf = 4 (model.f90:80).
+ * gwvec/gwdcn are [nat][mref]; gwdcn may be NULL (energy-only branch).
+ */
+void d3o_weights(int nat, const int *num, const double *cn, int mref, double *gwvec, double *gwdcn)
+{
+    const double wf = 4.0, wf2 = 2.0 * wf;
+    memset(gwvec, 0, sizeof(double) * nat * mref);
+    if (gwdcn) memset(gwdcn, 0, sizeof(double) * nat * mref);
+#pragma omp parallel for schedule(static)
+    for (int iat = 0; iat < nat; ++iat) {
+        int z = num[iat], nr = g_nref[z - 1];
+        double gwt[MAX_REF], norm = 0.0, dnorm = 0.0, maxcn = -1.0e300;
+        for (int r = 0; r < nr; ++r) {
+            double d = cn[iat] - g_refcn[z - 1][r];
+            double gw = exp(-wf * (d * d));
+            gwt[r] = gw;
+            double dcn = g_refcn[z - 1][r] - cn[iat];
+            norm += gw;
+            dnorm += wf2 * dcn * gw;
+            if (g_refcn[z - 1][r] > maxcn) maxcn = g_refcn[z - 1][r];
+        }
+        norm = 1.0 / norm;
+        for (int r = 0; r < nr; ++r) {
+            double expw = gwt[r];
+            double dcn = g_refcn[z - 1][r] - cn[iat];
+            double expd = wf2 * dcn * expw;
+            double gwk = expw * norm;
+            if (is_exceptional(gwk)) gwk = (maxcn == g_refcn[z - 1][r]) ? 1.0 : 0.0;
+            gwvec[iat * mref + r] = gwk;
+            if (gwdcn) {
+                double dgwk = expd * norm - expw * dnorm * (norm * norm);
+                if (is_exceptional(dgwk)) dgwk = 0.0;
+                gwdcn[iat * mref + r] = dgwk;
+            }
+        }
+    }
+}
+
+/*
+ * model.f90:340-491 get_atomic_c6, first-derivative and plain branches.
+ * c6 and dc6dcn are [nat][nat] with c6[i*nat+j] = Fortran c6(i,j) and
+ * dc6dcn[i*nat+j] = Fortran dc6dcn(i,j) = dC6_ij / dCN_i.
+ */
+void d3o_c6(int nat, const int *num, int mref, const double *gwvec, const double *gwdcn,
+            const int *ghost, double *c6, double *dc6dcn)
+{
+    memset(c6, 0, sizeof(double) * nat * nat);
+    if (dc6dcn) memset(dc6dcn, 0, sizeof(double) * nat * nat);
+#pragma omp parallel for schedule(static)
+    for (int iat = 0; iat < nat; ++iat) {
+        if (ghost && ghost[iat]) continue;
+        int zi = num[iat];
+        for (int jat = 0; jat <= iat; ++jat) {
+            if (ghost && ghost[jat]) continue;
+            int zj = num[jat];
+            double dc6 = 0.0, dci = 0.0, dcj = 0.0;
+            for (int ir = 0; ir < g_nref[zi - 1]; ++ir)
+                for (int jr = 0; jr < g_nref[zj - 1]; ++jr) {
+                    double refc6 = d3o_get_c6(ir + 1, jr + 1, zi, zj);
+                    dc6 += gwvec[iat * mref + ir] * gwvec[jat * mref + jr] * refc6;
+                    if (dc6dcn) {
+                        dci += gwdcn[iat * mref + ir] * gwvec[jat * mref + jr] * refc6;
+                        dcj += gwvec[iat * mref + ir] * gwdcn[jat * mref + jr] * refc6;
+                    }
+                }
+            c6[(size_t)iat * nat + jat] = dc6;
+            c6[(size_t)jat * nat + iat] = dc6;
+            if (dc6dcn) {
+                dc6dcn[(size_t)iat * nat + jat] = dci;
+                dc6dcn[(size_t)jat * nat + iat] = dcj;
+            }
+        }
+    }
+}
+
+/* --------------------------------------------------------- two-body */
+typedef struct {
+    int kind;          /* 0 = rational (BJ), 1 = zero (CHG) */
+    double s6, s8, s9; /* scaling */
+    double a1, a2;     /* rational: a1, a2 ; zero: rs6, rs8 */
+    double alp;
+} d3o_param;
+
+/*
+ * rational.f90:220-514 get_dispersion2 (BJ) and zero.f90:243-553 (zero).
+ * grad != 0 selects the E+grad+sigma branch (rational.f90:378-514,
+ * zero.f90:409-553), otherwise the energy branch (rational.f90:282-374,
+ * zero.f90:306-405).
+ */
+static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz, int ntr,
+                  const double *trans, double cutoff, double width, const double *c6,
+                  const double *dc6dcn, double *energy, double *dEdcn, double *gradient,
+                  double *sigma, int grad)
+{
+    if (fabs(p->s6) < EPS && fabs(p->s8) < EPS) return;
+    double cutoff2 = cutoff * cutoff;
+#pragma omp parallel
+    {
+        double *el = (double *)calloc(nat, sizeof(double));
+        double *dl = grad ? (double *)calloc(nat, sizeof(double)) : NULL;
+        double *gl = grad ? (double *)calloc(3 * (size_t)nat, sizeof(double)) : NULL;
+        double sl[9] = {0};
+#pragma omp for schedule(static)
+        for (int iat = 0; iat < nat; ++iat) {
+            int zi = num[iat];
+            for (int jat = 0; jat <= iat; ++jat) {
+                int zj = num[jat];
+                double rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
+                double c6ij = c6[(size_t)jat * nat + iat];
+                double r0ij = 0, r0ij6 = 0, r0ij8 = 0, rs6r0 = 0, rs8r0 = 0;
+                if (p->kind == 0) {
+                    r0ij = p->a1 * sqrt(rrij) + p->a2;
+                    double r0ij2 = r0ij * r0ij;
+                    r0ij6 = r0ij2 * r0ij2 * r0ij2;
+                    r0ij8 = r0ij6 * r0ij2;
+                } else {
+                    r0ij = d3o_rvdw(zj, zi);
+                    rs6r0 = p->a1 * r0ij;
+                    rs8r0 = p->a2 * r0ij;
+                }
+                for (int jtr = 0; jtr < ntr; ++jtr) {
+                    const double *t = trans + 3 * jtr;
+                    double v[3];
+                    for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + t[c]);
+                    double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    if (r2 > cutoff2 || r2 < EPS) continue;
+                    double r1 = sqrt(r2), sw, dswdr;
+                    smooth_cutoff(r1, cutoff, width, &sw, &dswdr);
+                    double edisp0, gdisp0 = 0.0;
+                    if (p->kind == 0) {
+                        double r4 = r2 * r2;
+                        double t6 = 1.0 / (r4 * r2 + r0ij6);
+                        double t8 = 1.0 / (r4 * r4 + r0ij8);
+                        edisp0 = p->s6 * t6 + p->s8 * rrij * t8;
+                        if (grad) {
+                            double d6 = -6 * r4 * t6 * t6;
+                            double d8 = -8 * r4 * r2 * t8 * t8;
+                            gdisp0 = p->s6 * d6 + p->s8 * rrij * d8;
+                        }
+                    } else {
+                        double alp6 = p->alp, alp8 = p->alp + 2.0;
+                        double r6 = r2 * r2 * r2, r8 = r6 * r2;
+                        double t6 = pow(rs6r0 / r1, alp6);
+                        double t8 = pow(rs8r0 / r1, alp8);
+                        double f6 = 1.0 / (1.0 + 6.0 * t6);
+                        double f8 = 1.0 / (1.0 + 6.0 * t8);
+                        edisp0 = p->s6 * f6 / r6 + p->s8 * rrij * f8 / r8;
+                        if (grad) {
+                            double d6 = -6.0 * f6 / r2 + 6.0 * alp6 * t6 * f6 * f6 / r2;
+                            double d8 = -8.0 * f8 / r2 + 6.0 * alp8 * t8 * f8 * f8 / r2;
+                            gdisp0 = p->s6 * d6 / r6 + p->s8 * rrij * d8 / r8;
+                        }
+                    }
+                    double edisp = sw * edisp0;
+                    double dE = -c6ij * edisp * 0.5;
+                    el[iat] += dE;
+                    if (iat != jat) el[jat] += dE;
+                    if (!grad) continue;
+                    double gdisp = sw * gdisp0 + dswdr * edisp0 / r1;
+                    double dG[3], dS[9];
+                    for (int c = 0; c < 3; ++c) dG[c] = -c6ij * gdisp * v[c];
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) dS[b * 3 + a] = dG[a] * v[b] * 0.5;
+                    dl[iat] -= dc6dcn[(size_t)iat * nat + jat] * edisp;
+                    for (int k = 0; k < 9; ++k) sl[k] += dS[k];
+                    if (iat != jat) {
+                        dl[jat] -= dc6dcn[(size_t)jat * nat + iat] * edisp;
+                        for (int c = 0; c < 3; ++c) {
+                            gl[3 * iat + c] += dG[c];
+                            gl[3 * jat + c] -= dG[c];
+                        }
+                        for (int k = 0; k < 9; ++k) sl[k] += dS[k];
+                    }
+                }
+            }
+        }
+#pragma omp critical(d3o_disp2_)
+        {
+            for (int i = 0; i < nat; ++i) energy[i] += el[i];
+            if (grad) {
+                for (int i = 0; i < nat; ++i) dEdcn[i] += dl[i];
+                for (int i = 0; i < 3 * nat; ++i) gradient[i] += gl[i];
+                for (int k = 0; k < 9; ++k) sigma[k] += sl[k];
+            }
+        }
+        free(el); free(dl); free(gl);
+    }
+}
+
+/* atm.f90:954-980 triple_scale (0-based indices). */
+static double triple_scale(int ii, int jj, int kk)
+{
+    if (ii == jj) return (ii == kk) ? 1.0 / 6.0 : 0.5;
+    if (ii != kk && jj != kk) return 1.0;
+    return 0.5;
+}
+
+/*
+ * atm.f90:433-825 get_atm_dispersion with rs9 = 4/3 and alp + 2
+ * (rational.f90:518-566, zero.f90:557-604).  grad selects
+ * get_atm_dispersion_derivs (atm.f90:625-825) or the energy branch
+ * (atm.f90:500-621).
+ */
+static void atm(int nat, const int *num, const double *xyz, int ntr, const double *trans,
+                double cutoff, double width, double s9, double rs9, double alp,
+                const double *c6, const double *dc6dcn, double *energy, double *dEdcn,
+                double *gradient, double *sigma, int grad)
+{
+    if (fabs(s9) < EPS) return;
+    double cutoff2 = cutoff * cutoff, alp3 = alp / 3.0;
+#pragma omp parallel
+    {
+        double *el = (double *)calloc(nat, sizeof(double));
+        double *dl = grad ? (double *)calloc(nat, sizeof(double)) : NULL;
+        double *gl = grad ? (double *)calloc(3 * (size_t)nat, sizeof(double)) : NULL;
+        double sl[9] = {0};
+#pragma omp for schedule(static)
+        for (int iat = 0; iat < nat; ++iat) {
+            int zi = num[iat];
+            for (int jat = 0; jat <= iat; ++jat) {
+                int zj = num[jat];
+                double c6ij = c6[(size_t)jat * nat + iat];
+                double r0ij = rs9 * d3o_rvdw(zj, zi);
+                for (int jtr = 0; jtr < ntr; ++jtr) {
+                    const double *tj = trans + 3 * jtr;
+                    double vij[3];
+                    for (int c = 0; c < 3; ++c) vij[c] = xyz[3 * jat + c] + tj[c] - xyz[3 * iat + c];
+                    double r2ij = vij[0] * vij[0] + vij[1] * vij[1] + vij[2] * vij[2];
+                    if (r2ij > cutoff2 || r2ij < EPS) continue;
+                    double rij = sqrt(r2ij), swij, dswijdr;
+                    smooth_cutoff(rij, cutoff, width, &swij, &dswijdr);
+                    for (int kat = 0; kat <= jat; ++kat) {
+                        int zk = num[kat];
+                        double c6ik = c6[(size_t)kat * nat + iat];
+                        double c6jk = c6[(size_t)kat * nat + jat];
+                        if (grad && (fabs(c6ij) < EPS || fabs(c6ik) < EPS || fabs(c6jk) < EPS))
+                            continue;
+                        double c9 = -s9 * sqrt(fabs(c6ij * c6ik * c6jk));
+                        double r0ik = rs9 * d3o_rvdw(zk, zi);
+                        double r0jk = rs9 * d3o_rvdw(zk, zj);
+                        double r0 = r0ij * r0ik * r0jk;
+                        double triple = triple_scale(iat, jat, kat);
+                        for (int ktr = 0; ktr < ntr; ++ktr) {
+                            const double *tk = trans + 3 * ktr;
+                            double vik[3], vjk[3];
+                            for (int c = 0; c < 3; ++c) vik[c] = xyz[3 * kat + c] + tk[c] - xyz[3 * iat + c];
+                            double r2ik = vik[0] * vik[0] + vik[1] * vik[1] + vik[2] * vik[2];
+                            if (r2ik > cutoff2 || r2ik < EPS) continue;
+                            double rik = sqrt(r2ik), swik, dswikdr;
+                            smooth_cutoff(rik, cutoff, width, &swik, &dswikdr);
+                            for (int c = 0; c < 3; ++c) vjk[c] = vik[c] - vij[c];
+                            double r2jk = vjk[0] * vjk[0] + vjk[1] * vjk[1] + vjk[2] * vjk[2];
+                            if (r2jk > cutoff2 || r2jk < EPS) continue;
+                            double rjk = sqrt(r2jk), swjk, dswjkdr;
+                            smooth_cutoff(rjk, cutoff, width, &swjk, &dswjkdr);
+                            double sw = swij * swik * swjk;
+                            double r2 = r2ij * r2ik * r2jk;
+                            double r1 = sqrt(r2);
+                            double r3 = r2 * r1;
+                            double r5 = r3 * r2;
+                            double fdmp = 1.0 / (1.0 + 6.0 * pow(r0 / r1, alp3));
+                            double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik)
+                                       * (-r2ij + r2jk + r2ik) / r5 + 1.0 / r3;
+                            double rr = ang * fdmp;
+                            if (!grad) {
+                                double dE = rr * c9 * triple * sw / 3.0;
+                                el[iat] -= dE;
+                                el[jat] -= dE;
+                                el[kat] -= dE;
+                                continue;
+                            }
+                            double r0r1alp3 = pow(r0 / r1, alp3);
+                            double dfdmp = -2.0 * alp * r0r1alp3 * fdmp * fdmp;
+                            double dang, dGij[3], dGik[3], dGjk[3];
+                            dang = -0.375 * (r2ij * r2ij * r2ij + r2ij * r2ij * (r2jk + r2ik)
+                                   + r2ij * (3.0 * r2jk * r2jk + 2.0 * r2jk * r2ik + 3.0 * r2ik * r2ik)
+                                   - 5.0 * (r2jk - r2ik) * (r2jk - r2ik) * (r2jk + r2ik)) / r5;
+                            double dE0 = rr * c9;
+                            for (int c = 0; c < 3; ++c)
+                                dGij[c] = sw * c9 * (-dang * fdmp + ang * dfdmp) / r2ij * vij[c]
+                                        - dE0 * dswijdr / rij * swik * swjk * vij[c];
+                            dang = -0.375 * (r2ik * r2ik * r2ik + r2ik * r2ik * (r2jk + r2ij)
+                                   + r2ik * (3.0 * r2jk * r2jk + 2.0 * r2jk * r2ij + 3.0 * r2ij * r2ij)
+                                   - 5.0 * (r2jk - r2ij) * (r2jk - r2ij) * (r2jk + r2ij)) / r5;
+                            for (int c = 0; c < 3; ++c)
+                                dGik[c] = sw * c9 * (-dang * fdmp + ang * dfdmp) / r2ik * vik[c]
+                                        - dE0 * dswikdr / rik * swij * swjk * vik[c];
+                            dang = -0.375 * (r2jk * r2jk * r2jk + r2jk * r2jk * (r2ik + r2ij)
+                                   + r2jk * (3.0 * r2ik * r2ik + 2.0 * r2ik * r2ij + 3.0 * r2ij * r2ij)
+                                   - 5.0 * (r2ik - r2ij) * (r2ik - r2ij) * (r2ik + r2ij)) / r5;
+                            for (int c = 0; c < 3; ++c)
+                                dGjk[c] = sw * c9 * (-dang * fdmp + ang * dfdmp) / r2jk * vjk[c]
+                                        - dE0 * dswjkdr / rjk * swij * swik * vjk[c];
+                            double dE = dE0 * triple * sw;
+                            el[iat] -= dE / 3.0;
+                            el[jat] -= dE / 3.0;
+                            el[kat] -= dE / 3.0;
+                            for (int c = 0; c < 3; ++c) {
+                                gl[3 * iat + c] -= (dGij[c] + dGik[c]) * triple;
+                                gl[3 * jat + c] += (dGij[c] - dGjk[c]) * triple;
+                                gl[3 * kat + c] += (dGik[c] + dGjk[c]) * triple;
+                            }
+                            for (int a = 0; a < 3; ++a)
+                                for (int b = 0; b < 3; ++b)
+                                    sl[b * 3 + a] += (dGij[a] * vij[b] + dGik[a] * vik[b]
+                                                      + dGjk[a] * vjk[b]) * triple;
+                            dl[iat] -= dE * 0.5 * (dc6dcn[(size_t)iat * nat + jat] / c6ij
+                                                   + dc6dcn[(size_t)iat * nat + kat] / c6ik);
+                            dl[jat] -= dE * 0.5 * (dc6dcn[(size_t)jat * nat + iat] / c6ij
+                                                   + dc6dcn[(size_t)jat * nat + kat] / c6jk);
+                            dl[kat] -= dE * 0.5 * (dc6dcn[(size_t)kat * nat + iat] / c6ik
+                                                   + dc6dcn[(size_t)kat * nat + jat] / c6jk);
+                        }
+                    }
+                }
+            }
+        }
+#pragma omp critical(d3o_atm_)
+        {
+            for (int i = 0; i < nat; ++i) energy[i] += el[i];
+            if (grad) {
+                for (int i = 0; i < nat; ++i) dEdcn[i] += dl[i];
+                for (int i = 0; i < 3 * nat; ++i) gradient[i] += gl[i];
+                for (int k = 0; k < 9; ++k) sigma[k] += sl[k];
+            }
+        }
+        free(el); free(dl); free(gl);
+    }
+}
+
+/* ------------------------------------------------------- orchestrator */
+static int lattice_alloc(const int *periodic, const double *lat, double rthr, double **trans)
+{
+    int n = d3o_lattice_points(periodic, lat, rthr, NULL, 0);
+    *trans = (double *)malloc(sizeof(double) * 3 * n);
+    d3o_lattice_points(periodic, lat, rthr, *trans, n);
+    return n;
+}
+
+/*
+ * disp.f90:56-185 get_dispersion_atomic_v2 followed by disp.f90:352-399
+ * (energy = sum of atomic energies).  par = {s6, s8, s9, a1|rs6, a2|rs8, alp},
+ * cut = {cn, disp2, disp3, width2, width3}.  gradient / sigma may be NULL
+ * (grad = present(gradient) .or. present(sigma), disp.f90:102).
+ * Optional intermediates (may be NULL): cn_out[nat], dEdcn_out[nat],
+ * energies_out[nat].  Returns 0 on success.
+ */
+int d3o_get_dispersion(int nat, const int *num, const double *xyz, const double *lattice,
+                       const int *periodic, int kind, const double *par, const double *cut,
+                       const int *ghost, double *energy, double *gradient, double *sigma,
+                       double *cn_out, double *dEdcn_out, double *energies_out)
+{
+    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5]};
+    int grad = (gradient != NULL) || (sigma != NULL);
+    int mref = 0;
+    for (int i = 0; i < nat; ++i)
+        if (g_nref[num[i] - 1] > mref) mref = g_nref[num[i] - 1];
+    double zero_lat[9] = {0};
+    int zero_pbc[3] = {0, 0, 0};
+    if (!lattice) lattice = zero_lat;
+    if (!periodic) periodic = zero_pbc;
+
+    double *cn = (double *)calloc(nat, sizeof(double));
+    double *trans;
+    int ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+    d3o_cn(nat, num, xyz, ntr, trans, cut[0], cn);
+    free(trans);
+
+    double *gw = (double *)calloc((size_t)nat * mref, sizeof(double));
+    double *gwd = grad ? (double *)calloc((size_t)nat * mref, sizeof(double)) : NULL;
+    d3o_weights(nat, num, cn, mref, gw, gwd);
+    double *c6 = (double *)calloc((size_t)nat * nat, sizeof(double));
+    double *dc6 = grad ? (double *)calloc((size_t)nat * nat, sizeof(double)) : NULL;
+    d3o_c6(nat, num, mref, gw, gwd, ghost, c6, dc6);
+
+    double *en = (double *)calloc(nat, sizeof(double));
+    double *dEdcn = grad ? (double *)calloc(nat, sizeof(double)) : NULL;
+    double *gl = grad ? (double *)calloc(3 * (size_t)nat, sizeof(double)) : NULL;
+    double sl[9] = {0};
+
+    ntr = lattice_alloc(periodic, lattice, cut[1], &trans);
+    disp2(&p, nat, num, xyz, ntr, trans, cut[1], cut[3], c6, dc6, en, dEdcn, gl, sl, grad);
+    free(trans);
+    ntr = lattice_alloc(periodic, lattice, cut[2], &trans);
+    atm(nat, num, xyz, ntr, trans, cut[2], cut[4], p.s9, 4.0 / 3.0, p.alp + 2.0, c6, dc6, en,
+        dEdcn, gl, sl, grad);
+    free(trans);
+    if (grad) {
+        ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+        d3o_cn_derivs(nat, num, xyz, ntr, trans, cut[0], dEdcn, gl, sl);
+        free(trans);
+        if (gradient) memcpy(gradient, gl, sizeof(double) * 3 * nat);
+        if (sigma) memcpy(sigma, sl, sizeof sl);
+    }
+    double e = 0.0;
+    for (int i = 0; i < nat; ++i) e += en[i];
+    *energy = e;
+    if (cn_out) memcpy(cn_out, cn, sizeof(double) * nat);
+    if (dEdcn_out && grad) memcpy(dEdcn_out, dEdcn, sizeof(double) * nat);
+    if (energies_out) memcpy(energies_out, en, sizeof(double) * nat);
+    free(cn); free(gw); free(gwd); free(c6); free(dc6); free(en); free(dEdcn); free(gl);
+    return 0;
+}
+
+/* Count of the terms the reference loops visit (the work units). */
+void d3o_count_terms(int nat, const double *xyz, const double *lattice, const int *periodic,
+                     const double *cut, long long *n_cn, long long *n_pair, long long *n_triple)
+{
+    double *trans;
+    double c2;
+    int ntr;
+    long long cnt = 0;
+    ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+    c2 = cut[0] * cut[0];
+#pragma omp parallel for reduction(+ : cnt) schedule(static)
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int t = 0; t < ntr; ++t) {
+                double v[3];
+                for (int c = 0; c < 3; ++c) v[c] = xyz[3 * i + c] - (xyz[3 * j + c] + trans[3 * t + c]);
+                double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (!(r2 > c2 || r2 < 1.0e-12)) ++cnt;
+            }
+    free(trans);
+    *n_cn = cnt;
+    cnt = 0;
+    ntr = lattice_alloc(periodic, lattice, cut[1], &trans);
+    c2 = cut[1] * cut[1];
+#pragma omp parallel for reduction(+ : cnt) schedule(static)
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int t = 0; t < ntr; ++t) {
+                double v[3];
+                for (int c = 0; c < 3; ++c) v[c] = xyz[3 * i + c] - (xyz[3 * j + c] + trans[3 * t + c]);
+                double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (!(r2 > c2 || r2 < EPS)) ++cnt;
+            }
+    free(trans);
+    *n_pair = cnt;
+    if (!n_triple) return;
+    cnt = 0;
+    ntr = lattice_alloc(periodic, lattice, cut[2], &trans);
+    c2 = cut[2] * cut[2];
+#pragma omp parallel for reduction(+ : cnt) schedule(dynamic)
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int tj = 0; tj < ntr; ++tj) {
+                double vij[3];
+                for (int c = 0; c < 3; ++c) vij[c] = xyz[3 * j + c] + trans[3 * tj + c] - xyz[3 * i + c];
+                double r2ij = vij[0] * vij[0] + vij[1] * vij[1] + vij[2] * vij[2];
+                if (r2ij > c2 || r2ij < EPS) continue;
+                for (int k = 0; k <= j; ++k)
+                    for (int tk = 0; tk < ntr; ++tk) {
+                        double vik[3], vjk[3];
+                        for (int c = 0; c < 3; ++c) vik[c] = xyz[3 * k + c] + trans[3 * tk + c] - xyz[3 * i + c];
+                        double r2ik = vik[0] * vik[0] + vik[1] * vik[1] + vik[2] * vik[2];
+                        if (r2ik > c2 || r2ik < EPS) continue;
+                        for (int c = 0; c < 3; ++c) vjk[c] = vik[c] - vij[c];
+                        double r2jk = vjk[0] * vjk[0] + vjk[1] * vjk[1] + vjk[2] * vjk[2];
+                        if (r2jk > c2 || r2jk < EPS) continue;
+                        ++cnt;
+                    }
+            }
+    free(trans);
+    *n_triple = cnt;
+}
+
+int d3o_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,205 @@
+"""ctypes front end of the CPU oracle (oracle/d3_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module as the checker.  The
+library under test never imports it.
+
+The oracle restates simple-dftd3 v1.6.0 in FP64; the C file cites the
+reference file:line of every function.  ``ORACLE_AATOAU`` is the
+Angstrom -> bohr factor of mctc-lib: the Bohr radius hbar/(m_e c alpha) from
+the CODATA 2018 constants (h and c exact, m_e, alpha measured).  Both tight
+2D goldens of test/unit/test_periodic_2d.f90 (gh185 BJ, BP86 zero damping)
+agree with it to 100 epsilon, while the rounded value 0.529177210903 A misses
+the BP86 one by 2.4e-13 Eh (tests/test_oracle_golden.py).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libd3oracle.so")
+TABLES = os.path.join(HERE, "..", "simple-dftd3_amd", "data", "d3_tables.npz")
+
+
+def _aatoau():
+    import math
+    hbar = 6.62607015e-34 / (2.0 * math.pi)  # J s
+    bohr = hbar / (9.1093837015e-31 * 299792458.0 * 7.2973525693e-3)  # m
+    return 1.0 / (bohr * 1.0e10)
+
+
+ORACLE_AATOAU = _aatoau()
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force=False):
+    """Compile the oracle in place (make)."""
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+
+
+def _ptr(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def lib(aatoau=ORACLE_AATOAU):
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.d3o_rcov.restype = ctypes.c_double
+        _lib.d3o_rcov.argtypes = [ctypes.c_int]
+        _lib.d3o_r4r2.restype = ctypes.c_double
+        _lib.d3o_r4r2.argtypes = [ctypes.c_int]
+        _lib.d3o_rvdw.restype = ctypes.c_double
+        _lib.d3o_rvdw.argtypes = [ctypes.c_int, ctypes.c_int]
+        _lib.d3o_get_c6.restype = ctypes.c_double
+        _lib.d3o_get_c6.argtypes = [ctypes.c_int] * 4
+        _lib.d3o_num_threads.restype = ctypes.c_int
+        set_tables(aatoau)
+    return _lib
+
+
+def set_tables(aatoau=ORACLE_AATOAU):
+    t = np.load(TABLES)
+    keep = [np.ascontiguousarray(t["nref"], dtype=np.int32),
+            np.ascontiguousarray(t["refcn"], dtype=np.float64),
+            np.ascontiguousarray(t["c6ref"], dtype=np.float64),
+            np.ascontiguousarray(t["r4_over_r2"], dtype=np.float64),
+            np.ascontiguousarray(t["vdwrad_aa"], dtype=np.float64),
+            np.ascontiguousarray(t["covrad_2009_aa"], dtype=np.float64)]
+    _lib.d3o_set_tables(_ptr(keep[0], _ip), *[_ptr(a) for a in keep[1:]], ctypes.c_double(aatoau))
+
+
+def _geom(numbers, positions, lattice, periodic):
+    num = np.ascontiguousarray(numbers, dtype=np.int32)
+    xyz = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    lat = np.zeros((3, 3)) if lattice is None else np.ascontiguousarray(lattice, dtype=np.float64).reshape(3, 3)
+    if periodic is None:
+        periodic = [lattice is not None] * 3
+    pbc = np.ascontiguousarray(np.array(periodic, dtype=bool).astype(np.int32).reshape(-1))
+    if pbc.size == 1:  # a single flag applies to all directions (mctc-lib)
+        pbc = np.repeat(pbc, 3)
+    return num, xyz, lat, pbc
+
+
+def lattice_points(periodic, lattice, cutoff):
+    L = lib()
+    lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(3, 3)
+    pbc = np.ascontiguousarray(np.array(periodic, dtype=np.int32).reshape(3))
+    n = L.d3o_lattice_points(_ptr(pbc, _ip), _ptr(lat), ctypes.c_double(cutoff), None, 0)
+    out = np.zeros((n, 3))
+    L.d3o_lattice_points(_ptr(pbc, _ip), _ptr(lat), ctypes.c_double(cutoff), _ptr(out), n)
+    return out
+
+
+def wrap(positions, lattice, periodic):
+    L = lib()
+    xyz = np.array(positions, dtype=np.float64).reshape(-1, 3).copy()
+    lat = np.ascontiguousarray(lattice, dtype=np.float64).reshape(3, 3)
+    pbc = np.ascontiguousarray(np.array(periodic, dtype=np.int32).reshape(3))
+    L.d3o_wrap(len(xyz), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip))
+    return xyz
+
+
+def too_close(positions):
+    xyz = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    return bool(lib().d3o_verify(len(xyz), _ptr(xyz)))
+
+
+def coordination_number(numbers, positions, lattice=None, periodic=None, cutoff=40.0):
+    L = lib()
+    num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
+    tr = lattice_points(pbc, lat, cutoff)
+    cn = np.zeros(len(num))
+    L.d3o_cn(len(num), _ptr(num, _ip), _ptr(xyz), len(tr), _ptr(tr), ctypes.c_double(cutoff), _ptr(cn))
+    return cn
+
+
+def weights(numbers, cn, grad=True):
+    L = lib()
+    num = np.ascontiguousarray(numbers, dtype=np.int32)
+    cn = np.ascontiguousarray(cn, dtype=np.float64)
+    t = np.load(TABLES)
+    mref = int(max(t["nref"][z - 1] for z in num))
+    gw = np.zeros((len(num), mref))
+    gwd = np.zeros((len(num), mref)) if grad else None
+    L.d3o_weights(len(num), _ptr(num, _ip), _ptr(cn), mref, _ptr(gw), _ptr(gwd))
+    return gw, gwd
+
+
+def atomic_c6(numbers, gw, gwd=None, ghost=None):
+    L = lib()
+    num = np.ascontiguousarray(numbers, dtype=np.int32)
+    n = len(num)
+    gw = np.ascontiguousarray(gw, dtype=np.float64)
+    gwd = None if gwd is None else np.ascontiguousarray(gwd, dtype=np.float64)
+    gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
+    c6 = np.zeros((n, n))
+    dc6 = np.zeros((n, n)) if gwd is not None else None
+    L.d3o_c6(n, _ptr(num, _ip), gw.shape[1], _ptr(gw), _ptr(gwd), _ptr(gh, _ip), _ptr(c6), _ptr(dc6))
+    return c6, dc6
+
+
+RATIONAL, ZERO = 0, 1
+DEFAULT_CUTOFF = (40.0, 60.0, 40.0, 0.0, 0.0)  # cn, disp2, disp3, width2, width3
+
+
+def dispersion(numbers, positions, lattice=None, periodic=None, kind=RATIONAL,
+               param=None, cutoff=DEFAULT_CUTOFF, grad=True, ghost=None, wrap_cell=False):
+    """FP64 oracle of get_dispersion.
+
+    param: dict with s6, s8, s9 and a1, a2 (rational) or rs6, rs8 (zero), alp.
+    Returns a dict with energy, gradient (n,3), sigma (3,3 Fortran order as
+    returned by the C API, i.e. sigma[b][a] = sigma(a,b)), virial (= the
+    Python interface's array), cn, dEdcn and energies (atomic).
+    """
+    L = lib()
+    num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
+    if wrap_cell:
+        xyz = wrap(xyz, lat, pbc)
+    p = dict(param)
+    s6 = p.get("s6", 1.0)
+    if kind == RATIONAL:
+        par = [s6, p["s8"], p.get("s9", 1.0), p["a1"], p["a2"], p.get("alp", 14.0)]
+    else:
+        par = [s6, p["s8"], p.get("s9", 1.0), p["rs6"], p.get("rs8", 1.0), p.get("alp", 14.0)]
+    par = np.array(par, dtype=np.float64)
+    cut = np.array(cutoff, dtype=np.float64)
+    n = len(num)
+    gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
+    energy = np.zeros(1)
+    g = np.zeros((n, 3)) if grad else None
+    s = np.zeros((3, 3)) if grad else None
+    cn = np.zeros(n)
+    dedcn = np.zeros(n)
+    en = np.zeros(n)
+    L.d3o_get_dispersion(n, _ptr(num, _ip), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip), kind, _ptr(par),
+                         _ptr(cut), _ptr(gh, _ip), _ptr(energy), _ptr(g), _ptr(s), _ptr(cn),
+                         _ptr(dedcn), _ptr(en))
+    out = dict(energy=float(energy[0]), cn=cn, energies=en)
+    if grad:
+        out.update(gradient=g, sigma=s, virial=s.copy(), dEdcn=dedcn)
+    return out
+
+
+def count_terms(positions, lattice=None, periodic=None, cutoff=DEFAULT_CUTOFF, triples=True):
+    """Exact counts of CN, two-body and ATM terms the reference loops visit."""
+    L = lib()
+    num, xyz, lat, pbc = _geom(np.zeros(len(np.reshape(positions, (-1, 3))), dtype=np.int32) + 1,
+                               positions, lattice, periodic)
+    cut = np.array(cutoff, dtype=np.float64)
+    a, b, c = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+    L.d3o_count_terms(len(xyz), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip), _ptr(cut), ctypes.byref(a),
+                      ctypes.byref(b), ctypes.byref(c) if triples else None)
+    return a.value, b.value, (c.value if triples else None)
+
+
+def num_threads():
+    return lib().d3o_num_threads()

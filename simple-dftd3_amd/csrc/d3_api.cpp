@@ -1,0 +1,463 @@
+// C ABI of libd3hip.so (declared in include/d3hip.h).
+//
+// Part 1 mirrors src/dftd3/api.f90 of simple-dftd3 v1.6.0 for the energy /
+// gradient / virial path: opaque handles, the error handle, NULL means
+// "absent", and the same error messages.  The computation itself runs on the
+// GPU through d3::Engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "d3_engine.hpp"
+
+#include "../../include/d3hip.h"
+
+namespace {
+#include "d3_params_data.inc"
+}  // namespace
+
+struct _dftd3_error {
+  bool set = false;
+  std::string message;
+};
+
+struct _dftd3_structure {
+  int nat = 0;
+  std::vector<int> num;
+  std::vector<double> xyz;  // [nat][3]
+  double lattice[9] = {0};
+  int periodic[3] = {0, 0, 0};
+};
+
+struct _dftd3_model {
+  bool has_structure = false;
+  int nat = 0;
+  std::vector<int> num;
+  double cutoff[5] = {40.0, 60.0, 40.0, 0.0, 0.0};  // cn, disp2, disp3, width2, width3 (cutoff.f90:26-32)
+  std::vector<int> ghost;
+  int precision = d3::kFP64;
+  std::unique_ptr<d3::Engine> engine;
+  hipStream_t stream = nullptr;
+  // device buffers of the host-pointer entry point
+  double* d_xyz = nullptr;
+  double* d_out = nullptr;  // energy | sigma[9] | gradient[nat][3]
+  size_t cap_atoms = 0;
+  ~_dftd3_model() {
+    if (d_xyz) (void)hipFree(d_xyz);
+    if (d_out) (void)hipFree(d_out);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct _dftd3_param {
+  int kind = d3::kRational;
+  double s6 = 1.0, s8 = 1.0, s9 = 0.0, a1 = 0.4, a2 = 5.0, alp = 14.0;  // param.f90:38-48 defaults
+};
+
+namespace {
+
+void fatal(dftd3_error err, const std::string& msg) {
+  if (!err) return;
+  err->set = true;
+  err->message = msg;
+}
+
+template <typename F>
+void guarded(dftd3_error err, F&& f) {
+  try {
+    f();
+  } catch (const std::exception& e) {
+    fatal(err, e.what());
+  }
+}
+
+// mctc-lib matinv_3x3 on the Fortran lattice(3,3) (columns = vectors).
+void wrap_to_central_cell(int nat, double* xyz, const double* lat, const int* periodic) {
+  // utils.f90:29-64
+  if (!(periodic[0] || periodic[1] || periodic[2])) return;
+  double m[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) m[r][c] = lat[3 * c + r];
+  const double det = m[0][0] * m[1][1] * m[2][2] - m[0][0] * m[1][2] * m[2][1] - m[0][1] * m[1][0] * m[2][2] +
+                     m[0][1] * m[1][2] * m[2][0] + m[0][2] * m[1][0] * m[2][1] - m[0][2] * m[1][1] * m[2][0];
+  const double di = 1.0 / det;
+  double inv[3][3];
+  inv[0][0] = +di * (m[1][1] * m[2][2] - m[1][2] * m[2][1]);
+  inv[1][0] = -di * (m[1][0] * m[2][2] - m[1][2] * m[2][0]);
+  inv[2][0] = +di * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  inv[0][1] = -di * (m[0][1] * m[2][2] - m[0][2] * m[2][1]);
+  inv[1][1] = +di * (m[0][0] * m[2][2] - m[0][2] * m[2][0]);
+  inv[2][1] = -di * (m[0][0] * m[2][1] - m[0][1] * m[2][0]);
+  inv[0][2] = +di * (m[0][1] * m[1][2] - m[0][2] * m[1][1]);
+  inv[1][2] = -di * (m[0][0] * m[1][2] - m[0][2] * m[1][0]);
+  inv[2][2] = +di * (m[0][0] * m[1][1] - m[0][1] * m[1][0]);
+  auto shift_back = [](double in) {
+    const double eps = 1.0e-14;
+    double out = in;
+    if (in < (0.0 - eps)) out = in + std::ceil(-in);
+    if (in > (1.0 + eps)) out = in - std::floor(in);
+    if (std::fabs(in - 1.0) < eps) out = in - 1.0;
+    return out;
+  };
+  for (int i = 0; i < nat; ++i) {
+    double* x = xyz + 3 * i;
+    double f[3];
+    for (int r = 0; r < 3; ++r) f[r] = shift_back(inv[r][0] * x[0] + inv[r][1] * x[1] + inv[r][2] * x[2]);
+    for (int r = 0; r < 3; ++r) x[r] = m[r][0] * f[0] + m[r][1] * f[1] + m[r][2] * f[2];
+  }
+}
+
+// api.f90:1517-1530 verify_structure, with a cell grid so that large
+// structures stay O(N): only pairs in neighbouring cells of edge >= 1e-9 can be
+// closer than 1e-9 bohr.
+bool too_close(int nat, const double* xyz) {
+  const double thr = 1.0e-9;
+  if (nat < 2) return false;
+  if (nat <= 2048) {
+    for (int i = 0; i < nat; ++i)
+      for (int j = 0; j < i; ++j) {
+        const double d0 = xyz[3 * j] - xyz[3 * i], d1 = xyz[3 * j + 1] - xyz[3 * i + 1],
+                     d2 = xyz[3 * j + 2] - xyz[3 * i + 2];
+        if (std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) < thr) return true;
+      }
+    return false;
+  }
+  // sort by x and compare neighbours within thr in x (sweep)
+  std::vector<int> idx(nat);
+  for (int i = 0; i < nat; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return xyz[3 * a] < xyz[3 * b]; });
+  for (int p = 0; p < nat; ++p) {
+    const int i = idx[p];
+    for (int q = p + 1; q < nat; ++q) {
+      const int j = idx[q];
+      if (xyz[3 * j] - xyz[3 * i] >= thr) break;
+      const double d0 = xyz[3 * j] - xyz[3 * i], d1 = xyz[3 * j + 1] - xyz[3 * i + 1],
+                   d2 = xyz[3 * j + 2] - xyz[3 * i + 2];
+      if (std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) < thr) return true;
+    }
+  }
+  return false;
+}
+
+// param.f90:96-300 get_method_id: drop '-' and compare lower case.
+const char* method_id(const char* method) {
+  std::string key;
+  for (const char* c = method; *c; ++c)
+    if (*c != '-') key.push_back(static_cast<char>(std::tolower(static_cast<unsigned char>(*c))));
+  while (!key.empty() && key.back() == ' ') key.pop_back();
+  for (const auto& a : kMethodAliases)
+    if (key == a.name) return a.id;
+  return nullptr;
+}
+
+const ParamRecord* find_record(const ParamRecord* table, int n, const char* id) {
+  if (!id) return nullptr;
+  for (int k = 0; k < n; ++k)
+    if (std::strcmp(table[k].id, id) == 0) return &table[k];
+  return nullptr;
+}
+
+void ensure_model_buffers(_dftd3_model* m, int nat) {
+  if (!m->stream) d3::hip_check(hipStreamCreate(&m->stream), "hipStreamCreate");
+  if (static_cast<size_t>(nat) > m->cap_atoms || !m->d_xyz) {
+    if (m->d_xyz) d3::hip_check(hipFree(m->d_xyz), "hipFree");
+    if (m->d_out) d3::hip_check(hipFree(m->d_out), "hipFree");
+    m->d_xyz = nullptr;
+    m->d_out = nullptr;
+    d3::hip_check(hipMalloc(&m->d_xyz, sizeof(double) * 3 * std::max(nat, 1)), "hipMalloc");
+    d3::hip_check(hipMalloc(&m->d_out, sizeof(double) * (10 + 3 * std::max(nat, 1))), "hipMalloc");
+    m->cap_atoms = nat;
+  }
+}
+
+d3::SysParam make_sys_param(const _dftd3_param* p, const double* cutoff) {
+  d3::SysParam sp{};
+  sp.kind = p->kind;
+  sp.s6 = p->s6; sp.s8 = p->s8; sp.s9 = p->s9; sp.a1 = p->a1; sp.a2 = p->a2; sp.alp = p->alp;
+  sp.cut_cn = cutoff[0]; sp.cut2 = cutoff[1]; sp.cut3 = cutoff[2];
+  sp.width2 = cutoff[3]; sp.width3 = cutoff[4];
+  return sp;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ error handle
+int dftd3_get_version(void) { return 10600; }
+
+dftd3_error dftd3_new_error(void) { return new _dftd3_error(); }
+
+int dftd3_check_error(dftd3_error error) { return error && error->set ? 1 : 0; }
+
+void dftd3_get_error(dftd3_error error, char* buffer, const int* buffersize) {
+  if (!error || !buffer) return;
+  size_t maxlen = buffersize ? static_cast<size_t>(std::max(*buffersize, 1)) : error->message.size() + 1;
+  size_t n = std::min(error->message.size(), maxlen - 1);
+  std::memcpy(buffer, error->message.data(), n);
+  buffer[n] = '\0';
+}
+
+void dftd3_delete_error(dftd3_error* error) {
+  if (error && *error) {
+    delete *error;
+    *error = nullptr;
+  }
+}
+
+// ------------------------------------------------------------- structure
+dftd3_structure dftd3_new_structure(dftd3_error error, const int natoms, const int* numbers,
+                                    const double* positions, const double* lattice, const bool* periodic) {
+  if (!error) return nullptr;
+  dftd3_structure mol = nullptr;
+  guarded(error, [&] {
+    if (natoms <= 0 || !numbers || !positions) throw d3::Error("Invalid molecular structure data provided");
+    mol = new _dftd3_structure();
+    mol->nat = natoms;
+    mol->num.assign(numbers, numbers + natoms);
+    mol->xyz.assign(positions, positions + 3 * natoms);
+    if (lattice) std::memcpy(mol->lattice, lattice, sizeof mol->lattice);
+    if (periodic) {
+      for (int k = 0; k < 3; ++k) mol->periodic[k] = periodic[k] ? 1 : 0;
+    } else if (lattice) {
+      mol->periodic[0] = mol->periodic[1] = mol->periodic[2] = 1;
+    }
+    wrap_to_central_cell(natoms, mol->xyz.data(), mol->lattice, mol->periodic);
+    if (too_close(natoms, mol->xyz.data())) fatal(error, "Too close interatomic distances found");
+  });
+  return mol;
+}
+
+void dftd3_delete_structure(dftd3_structure* mol) {
+  if (mol && *mol) {
+    delete *mol;
+    *mol = nullptr;
+  }
+}
+
+void dftd3_update_structure(dftd3_error error, dftd3_structure mol, const double* positions,
+                            const double* lattice) {
+  if (!error) return;
+  if (!mol) return fatal(error, "Molecular structure data is missing");
+  if (mol->nat <= 0 || mol->num.empty()) return fatal(error, "Invalid molecular structure data provided");
+  guarded(error, [&] {
+    std::memcpy(mol->xyz.data(), positions, sizeof(double) * 3 * mol->nat);
+    if (lattice) std::memcpy(mol->lattice, lattice, sizeof mol->lattice);
+    wrap_to_central_cell(mol->nat, mol->xyz.data(), mol->lattice, mol->periodic);
+    if (too_close(mol->nat, mol->xyz.data())) fatal(error, "Too close interatomic distances found");
+  });
+}
+
+// ----------------------------------------------------------------- model
+dftd3_model dftd3_new_d3_model(dftd3_error error, dftd3_structure mol) {
+  if (!error) return nullptr;
+  if (!mol) {
+    fatal(error, "Molecular structure data is missing");
+    return nullptr;
+  }
+  dftd3_model disp = nullptr;
+  guarded(error, [&] {
+    for (int z : mol->num)
+      if (z < 1 || z > d3::kMaxElem) throw d3::Error("Unsupported element with atomic number " + std::to_string(z));
+    auto m = std::make_unique<_dftd3_model>();
+    m->has_structure = true;
+    m->nat = mol->nat;
+    m->num = mol->num;
+    m->ghost.assign(mol->nat, 0);
+    m->engine = std::make_unique<d3::Engine>();
+    disp = m.release();
+  });
+  return disp;
+}
+
+dftd3_model d3hip_new_workspace(dftd3_error error) {
+  if (!error) return nullptr;
+  dftd3_model disp = nullptr;
+  guarded(error, [&] {
+    auto m = std::make_unique<_dftd3_model>();
+    m->engine = std::make_unique<d3::Engine>();
+    disp = m.release();
+  });
+  return disp;
+}
+
+void dftd3_set_model_realspace_cutoff(dftd3_error error, dftd3_model model, double disp2, double disp3, double cn) {
+  dftd3_set_model_realspace_cutoff_smooth(error, model, disp2, disp3, cn, 0.0, 0.0);
+}
+
+void dftd3_set_model_realspace_cutoff_smooth(dftd3_error error, dftd3_model model, double disp2, double disp3,
+                                             double cn, double width2, double width3) {
+  if (!error) return;
+  if (!model) return fatal(error, "D3 dispersion model is missing");
+  model->cutoff[0] = cn;
+  model->cutoff[1] = disp2;
+  model->cutoff[2] = disp3;
+  model->cutoff[3] = width2;
+  model->cutoff[4] = width3;
+}
+
+void dftd3_set_model_ghost_index(dftd3_error error, dftd3_model model, const int* ghost, int nidx) {
+  if (!error) return;
+  if (!model) return fatal(error, "D3 dispersion model is missing");
+  std::fill(model->ghost.begin(), model->ghost.end(), 0);
+  for (int k = 0; k < nidx; ++k)  // model.f90:166-174: out-of-range indices are ignored
+    if (ghost[k] > 0 && ghost[k] <= model->nat) model->ghost[ghost[k] - 1] = 1;
+}
+
+void d3hip_set_model_precision(dftd3_error error, dftd3_model model, int precision) {
+  if (!error) return;
+  if (!model) return fatal(error, "D3 dispersion model is missing");
+  if (precision != d3::kFP64 && precision != d3::kMixed) return fatal(error, "Unknown precision mode");
+  model->precision = precision;
+}
+
+void dftd3_delete_model(dftd3_model* disp) {
+  if (disp && *disp) {
+    delete *disp;
+    *disp = nullptr;
+  }
+}
+
+// ------------------------------------------------------------ parameters
+dftd3_param dftd3_new_rational_damping(dftd3_error error, double s6, double s8, double s9, double a1, double a2,
+                                       double alp) {
+  if (!error) return nullptr;
+  auto* p = new _dftd3_param();
+  p->kind = d3::kRational;
+  p->s6 = s6; p->s8 = s8; p->s9 = s9; p->a1 = a1; p->a2 = a2; p->alp = alp;
+  return p;
+}
+
+dftd3_param dftd3_new_zero_damping(dftd3_error error, double s6, double s8, double s9, double rs6, double rs8,
+                                   double alp) {
+  if (!error) return nullptr;
+  auto* p = new _dftd3_param();
+  p->kind = d3::kZero;
+  p->s6 = s6; p->s8 = s8; p->s9 = s9; p->a1 = rs6; p->a2 = rs8; p->alp = alp;
+  return p;
+}
+
+namespace {
+dftd3_param load_damping(dftd3_error error, const char* method, bool atm, int kind) {
+  if (!error) return nullptr;
+  if (!method) {
+    fatal(error, "No entry for '' present");
+    return nullptr;
+  }
+  const char* id = method_id(method);
+  const ParamRecord* rec = kind == d3::kRational ? find_record(kRationalParams, kNumRational, id)
+                                                 : find_record(kZeroParams, kNumZero, id);
+  if (!rec) {
+    fatal(error, std::string("No entry for '") + method + "' present");
+    return nullptr;
+  }
+  auto* p = new _dftd3_param();
+  p->kind = kind;
+  p->s6 = rec->s6;
+  p->s8 = rec->s8;
+  p->s9 = atm ? 1.0 : rec->s9;  // api.f90:606 (atm -> s9 = 1), param.f90:798-800
+  p->a1 = kind == d3::kRational ? rec->a1 : rec->rs6;
+  p->a2 = kind == d3::kRational ? rec->a2 : rec->rs8;
+  p->alp = rec->alp;
+  return p;
+}
+}  // namespace
+
+dftd3_param dftd3_load_rational_damping(dftd3_error error, char* method, bool atm) {
+  return load_damping(error, method, atm, d3::kRational);
+}
+
+dftd3_param dftd3_load_zero_damping(dftd3_error error, char* method, bool atm) {
+  return load_damping(error, method, atm, d3::kZero);
+}
+
+void dftd3_delete_param(dftd3_param* param) {
+  if (param && *param) {
+    delete *param;
+    *param = nullptr;
+  }
+}
+
+// ------------------------------------------------------------ evaluation
+void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model disp, dftd3_param param,
+                          double* energy, double* gradient, double* sigma) {
+  // api.f90:1038-1111
+  if (!error) return;
+  if (!mol) return fatal(error, "Molecular structure data is missing");
+  if (!disp) return fatal(error, "Dispersion model is missing");
+  if (!param) return fatal(error, "Damping parameters are missing");
+  if (!disp->has_structure || disp->nat != mol->nat)
+    return fatal(error, "Dispersion model does not match the molecular structure");
+  guarded(error, [&] {
+    const int nat = mol->nat;
+    ensure_model_buffers(disp, nat);
+    std::vector<d3::SystemDesc> sys(1);
+    sys[0].natoms = nat;
+    std::memcpy(sys[0].lattice, mol->lattice, sizeof mol->lattice);
+    std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
+    sys[0].param = make_sys_param(param, disp->cutoff);
+    const bool grad = gradient || sigma;
+    d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
+                                 disp->stream),
+                  "upload positions");
+    d3::BatchOutput out;
+    out.energy = disp->d_out;
+    out.sigma = grad ? disp->d_out + 1 : nullptr;
+    out.gradient = grad ? disp->d_out + 10 : nullptr;
+    bool any_ghost = false;
+    for (int g : disp->ghost) any_ghost |= g != 0;
+    disp->engine->run(sys, mol->num.data(), disp->d_xyz, disp->precision, out, disp->stream, nullptr, nullptr,
+                      any_ghost ? disp->ghost.data() : nullptr);
+    std::vector<double> host(grad ? 10 + 3 * nat : 1);
+    d3::hip_check(hipMemcpyAsync(host.data(), disp->d_out, sizeof(double) * host.size(), hipMemcpyDeviceToHost,
+                                 disp->stream),
+                  "download results");
+    d3::hip_check(hipStreamSynchronize(disp->stream), "hipStreamSynchronize");
+    if (energy) *energy = host[0];
+    if (sigma) std::memcpy(sigma, host.data() + 1, sizeof(double) * 9);
+    if (gradient) std::memcpy(gradient, host.data() + 10, sizeof(double) * 3 * nat);
+  });
+}
+
+void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
+                                const int* numbers, const double* lattices, const bool* periodic,
+                                const double* params, const double* cutoffs, int precision,
+                                const double* positions, double* energy, double* gradient, double* sigma,
+                                void* stream) {
+  if (!error) return;
+  if (!workspace) return fatal(error, "Dispersion model is missing");
+  if (nsys <= 0 || !natoms || !numbers || !params || !positions || !energy)
+    return fatal(error, "Invalid batch description");
+  if (precision != d3::kFP64 && precision != d3::kMixed) return fatal(error, "Unknown precision mode");
+  guarded(error, [&] {
+    std::vector<d3::SystemDesc> sys(nsys);
+    for (int k = 0; k < nsys; ++k) {
+      d3::SystemDesc& s = sys[k];
+      s.natoms = natoms[k];
+      if (s.natoms <= 0) throw d3::Error("Invalid molecular structure data provided");
+      if (lattices) std::memcpy(s.lattice, lattices + 9 * k, sizeof s.lattice);
+      if (periodic)
+        for (int c = 0; c < 3; ++c) s.periodic[c] = periodic[3 * k + c] ? 1 : 0;
+      const double* p = params + 7 * k;
+      _dftd3_param prm;
+      prm.kind = static_cast<int>(p[0]);
+      if (prm.kind != d3::kRational && prm.kind != d3::kZero) throw d3::Error("Unknown damping kind");
+      prm.s6 = p[1]; prm.s8 = p[2]; prm.s9 = p[3]; prm.a1 = p[4]; prm.a2 = p[5]; prm.alp = p[6];
+      double cut[5] = {40.0, 60.0, 40.0, 0.0, 0.0};
+      if (cutoffs) {
+        const double* c = cutoffs + 5 * k;
+        cut[0] = c[2]; cut[1] = c[0]; cut[2] = c[1]; cut[3] = c[3]; cut[4] = c[4];
+      }
+      s.param = make_sys_param(&prm, cut);
+    }
+    d3::BatchOutput out;
+    out.energy = energy;
+    out.gradient = gradient;
+    out.sigma = sigma;
+    workspace->engine->run(sys, numbers, positions, precision, out, static_cast<hipStream_t>(stream));
+  });
+}

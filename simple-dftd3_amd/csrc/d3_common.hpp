@@ -1,0 +1,59 @@
+// Shared definitions of the D3 library: element tables, per-system
+// descriptors and the device data layout.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace d3 {
+
+constexpr int kMaxElem = 103;   // elements with reference data (reference.f90:27)
+constexpr int kMaxRef = 7;      // reference systems per element (reference.f90:28)
+constexpr int kNumPairs = kMaxElem * (kMaxElem + 1) / 2;
+constexpr int kMaxSpecies = 24; // distinct elements in one batch (LDS tables)
+
+enum DampingKind : int { kRational = 0, kZero = 1 };
+enum Precision : int { kFP64 = 0, kMixed = 1 };
+
+// Damping parameters and cutoffs of one system (rational.f90:34-70,
+// zero.f90:34-70, cutoff.f90:42-66).  For zero damping a1/a2 hold rs6/rs8.
+struct SysParam {
+  int kind;
+  int grad;
+  double s6, s8, s9, a1, a2, alp;
+  double cut_cn, cut2, cut3, width2, width3;
+};
+
+// Atom range and lattice-image ranges of one system of a batch.  Images are
+// stored per cutoff (CN, two-body, three-body) exactly as get_lattice_points
+// enumerates them (cutoff.f90:140-186), so the pair and triple sets are the
+// reference's.
+struct SysGeom {
+  int atom_begin, natoms;
+  int img_cn_begin, img_cn_count;
+  int img2_begin, img2_count;
+  int img3_begin, img3_count;
+  long long c6_begin;  // offset of the n x n C6 block of this system
+};
+
+// Host-side element tables (generated from data/d3_tables.npz).
+struct ElementTables {
+  int nref[kMaxElem];
+  double refcn[kMaxElem][kMaxRef];
+  const double* c6ref;  // [kNumPairs][kMaxRef][kMaxRef], [ic][jref][iref]
+  double r4_over_r2[118];
+  double vdwrad_aa[kNumPairs];
+  double covrad_aa[118];
+};
+
+const ElementTables& element_tables();
+
+// Angstrom -> bohr (mctc-lib): Bohr radius hbar / (m_e c alpha), CODATA 2018.
+double aatoau();
+
+double covalent_radius(int z);           // 4/3 * Pyykkoe radius, bohr
+double r4r2_value(int z);                // sqrt(0.5 r4/r2 sqrt(Z)) (r4r2.f90:91-105)
+double vdw_radius(int z1, int z2);       // pairwise radius, bohr (vdwrad.f90:1163-1197)
+double reference_c6(int iref, int jref, int zi, int zj);  // get_c6, 0-based refs
+
+}  // namespace d3

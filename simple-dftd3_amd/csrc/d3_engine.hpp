@@ -1,0 +1,71 @@
+// Batch engine of the D3 library: plans a batch of systems on the host and
+// runs the energy / gradient / virial kernels on one HIP stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "d3_common.hpp"
+
+namespace d3 {
+
+// A failure that the C API reports through its error handle.
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+void hip_check(hipError_t err, const char* what);
+
+// One system of a batch, as the host describes it.
+struct SystemDesc {
+  int natoms = 0;
+  double lattice[9] = {0};  // rows = lattice vectors (bohr)
+  int periodic[3] = {0, 0, 0};
+  SysParam param{};
+};
+
+// Outputs of a batch run (device pointers; any may be null except energy).
+struct BatchOutput {
+  double* energy = nullptr;    // [nsys]
+  double* gradient = nullptr;  // [natoms][3]
+  double* sigma = nullptr;     // [nsys][9], sigma(a,b) at [b*3+a]
+};
+
+// Optional per-atom intermediates for testing (device pointers).
+struct BatchDebug {
+  double* cn = nullptr;     // [natoms]
+  double* dedcn = nullptr;  // [natoms]
+};
+
+struct RunStats {
+  long long atm_list_entries = 0;  // size of the three-body neighbour lists
+};
+
+// Lattice points within a cutoff, in the order of cutoff.f90:140-186.
+std::vector<double> lattice_points(const int periodic[3], const double lattice[9], double rthr);
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // Evaluate a batch.  numbers are host arrays (atomic numbers, 1..103),
+  // positions a device array [natoms][3] in bohr.  Systems are consecutive
+  // in the atom arrays.  ghost_host (optional, host, [natoms]) marks atoms
+  // whose C6 coefficients vanish (model.f90:166-174, 436-440).
+  void run(const std::vector<SystemDesc>& systems, const int* numbers_host,
+           const double* positions_dev, int precision, const BatchOutput& out,
+           hipStream_t stream, const BatchDebug* debug = nullptr, RunStats* stats = nullptr,
+           const int* ghost_host = nullptr);
+
+ private:
+  struct Impl;
+  Impl* impl_;
+};
+
+}  // namespace d3

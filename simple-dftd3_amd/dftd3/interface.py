@@ -1,0 +1,232 @@
+"""Drop-in Python interface of the D3 energy / gradient / virial path.
+
+Same module path, class names, keyword arguments, defaults and error
+messages as python/dftd3/interface.py of simple-dftd3 v1.6.0 for the path
+this library implements:
+
+* ``Structure(numbers, positions, lattice=None, periodic=None)`` and
+  ``Structure.update(positions, lattice=None)`` (interface.py:63-152),
+* ``RationalDampingParam`` / ``ZeroDampingParam`` built from explicit
+  parameters or loaded by method name (interface.py:155-287),
+* ``DispersionModel(numbers, positions, lattice, periodic)`` with
+  ``set_realspace_cutoff``, ``set_ghost_atoms`` and
+  ``get_dispersion(param, grad)`` returning ``energy`` (0-d array),
+  ``gradient`` (n, 3) and ``virial`` (3, 3) in Hartree and bohr
+  (interface.py:505-616).
+
+All evaluation happens on the GPU through libd3hip.so.  Additions beyond the
+reference: ``DispersionModel.set_precision("fp64" | "mixed")``.  The damping
+families, Hessian, pairwise energies, Ewald summation and counterpoise
+correction of the reference are not part of this library; their classes
+exist so that imports keep working, and raise ``NotImplementedError`` when
+used.
+"""
+from typing import Optional
+
+import numpy as np
+
+from . import library
+
+
+class Structure:
+    """Molecular structure data with immutable species and periodicity.
+
+    Raises ValueError on invalid input (wrong shapes), RuntimeError with the
+    library message otherwise (e.g. "Too close interatomic distances found").
+    """
+
+    _mol = library.StructureHandle.null()
+
+    def __init__(self, numbers: np.ndarray, positions: np.ndarray, lattice: Optional[np.ndarray] = None,
+                 periodic: Optional[np.ndarray] = None):
+        numbers = np.asarray(numbers)
+        positions = np.asarray(positions)
+        if positions.size % 3 != 0:
+            raise ValueError("Expected tripels of cartesian coordinates")
+        if 3 * numbers.size != positions.size:
+            raise ValueError("Dimension missmatch between numbers and positions")
+        self._natoms = len(numbers)
+        _numbers = np.ascontiguousarray(numbers, dtype="i4")
+        _positions = np.ascontiguousarray(positions, dtype=float)
+        _lattice = None
+        if lattice is not None:
+            lattice = np.asarray(lattice)
+            if lattice.size != 9:
+                raise ValueError("Invalid lattice provided")
+            _lattice = np.ascontiguousarray(lattice, dtype="float")
+        _periodic = None
+        if periodic is not None:
+            periodic = np.asarray(periodic)
+            if periodic.size != 3:
+                raise ValueError("Invalid periodicity provided")
+            _periodic = np.ascontiguousarray(periodic, dtype="bool")
+        self._mol = library.new_structure(self._natoms, _numbers, _positions, _lattice, _periodic)
+
+    def __len__(self):
+        return self._natoms
+
+    def update(self, positions: np.ndarray, lattice: Optional[np.ndarray] = None) -> None:
+        """Update coordinates and (optionally) the lattice, both in bohr."""
+        positions = np.asarray(positions)
+        if 3 * len(self) != positions.size:
+            raise ValueError("Dimension missmatch for positions")
+        _positions = np.ascontiguousarray(positions, dtype="float")
+        _lattice = None
+        if lattice is not None:
+            lattice = np.asarray(lattice)
+            if lattice.size != 9:
+                raise ValueError("Invalid lattice provided")
+            _lattice = np.ascontiguousarray(lattice, dtype="float")
+        library.update_structure(self._mol, _positions, _lattice)
+
+
+class DampingParam:
+    """Damping parameters, created from a method name or explicit values.
+
+    Mixing both ways is not allowed: with ``method`` the parameters are
+    loaded, otherwise all required values must be given as keywords.
+    """
+
+    _param = library.ParamHandle.null()
+
+    def __init__(self, **kwargs):
+        if "method" in kwargs and kwargs["method"] is None:
+            del kwargs["method"]
+        if "method" in kwargs:
+            self._param = self.load_param(**kwargs)
+        else:
+            self._param = self.new_param(**kwargs)
+
+    @staticmethod
+    def load_param(method, **kwargs) -> library.ParamHandle:
+        raise NotImplementedError("Child class has to define parameter loading")
+
+    @staticmethod
+    def new_param(**kwargs) -> library.ParamHandle:
+        raise NotImplementedError("Child class has to define parameter construction")
+
+
+class RationalDampingParam(DampingParam):
+    """Rational (Becke-Johnson) damping; the ATM term uses zero damping."""
+
+    def __init__(self, **kwargs):
+        _rename_kwargs(kwargs, "alpha6", "alp")
+        DampingParam.__init__(self, **kwargs)
+
+    @staticmethod
+    def load_param(method: str, atm: bool = False) -> library.ParamHandle:
+        return library.load_rational_damping(method, atm)
+
+    @staticmethod
+    def new_param(*, s6: float = 1.0, s8: float, s9: float = 1.0, a1: float, a2: float,
+                  alp: float = 14.0) -> library.ParamHandle:
+        return library.new_rational_damping(s6, s8, s9, a1, a2, alp)
+
+
+class ZeroDampingParam(DampingParam):
+    """Zero (Chai-Head-Gordon) damping of the original DFT-D3."""
+
+    def __init__(self, **kwargs):
+        _rename_kwargs(kwargs, "sr6", "rs6")
+        _rename_kwargs(kwargs, "sr8", "rs8")
+        _rename_kwargs(kwargs, "alpha6", "alp")
+        DampingParam.__init__(self, **kwargs)
+
+    @staticmethod
+    def load_param(method: str, atm: bool = False) -> library.ParamHandle:
+        return library.load_zero_damping(method, atm)
+
+    @staticmethod
+    def new_param(*, s6: float = 1.0, s8: float, s9: float = 1.0, rs6: float, rs8: float = 1.0,
+                  alp: float = 14.0) -> library.ParamHandle:
+        return library.new_zero_damping(s6, s8, s9, rs6, rs8, alp)
+
+
+class _Unavailable(DampingParam):
+    """Damping family of the reference that this library does not provide."""
+
+    def __init__(self, **kwargs):
+        raise NotImplementedError(f"{type(self).__name__} is not available in this library")
+
+
+class ModifiedRationalDampingParam(_Unavailable):
+    pass
+
+
+class ModifiedZeroDampingParam(_Unavailable):
+    pass
+
+
+class OptimizedPowerDampingParam(_Unavailable):
+    pass
+
+
+class CSODampingParam(_Unavailable):
+    pass
+
+
+class ZDampingParam(_Unavailable):
+    pass
+
+
+class DispersionModel(Structure):
+    """Dispersion model bound to its structure (species and periodicity)."""
+
+    _disp = library.ModelHandle.null()
+
+    def __init__(self, numbers: np.ndarray, positions: np.ndarray, lattice: Optional[np.ndarray] = None,
+                 periodic: Optional[np.ndarray] = None):
+        Structure.__init__(self, numbers, positions, lattice, periodic)
+        self._disp = library.new_d3_model(self._mol)
+
+    def set_realspace_cutoff(self, disp2: float, disp3: float, cn: float, width2: float = 0.0,
+                             width3: float = 0.0):
+        """Set realspace cutoffs for evaluation of interactions"""
+        library.set_model_realspace_cutoff(self._disp, disp2, disp3, cn, width2, width3)
+
+    def set_ghost_atoms(self, ghost_atoms) -> None:
+        """Disable dispersion contributions from selected atoms (0-based)."""
+        library.set_model_ghost_index(self._disp, np.asarray(ghost_atoms, dtype="i4"))
+
+    def set_precision(self, precision: str) -> None:
+        """'fp64' (default) or 'mixed' (FP32 arithmetic, FP64 accumulation)."""
+        modes = {"fp64": library.PRECISION_FP64, "mixed": library.PRECISION_MIXED}
+        if precision not in modes:
+            raise ValueError(f"Unknown precision '{precision}', expected one of {sorted(modes)}")
+        library.set_model_precision(self._disp, modes[precision])
+
+    def get_dispersion(self, param: DampingParam, grad: bool) -> dict:
+        """Perform actual evaluation of the dispersion correction"""
+        _energy = np.array(0.0)
+        if grad:
+            _gradient = np.zeros((len(self), 3))
+            _sigma = np.zeros((3, 3))
+        else:
+            _gradient = None
+            _sigma = None
+        library.get_dispersion(self._mol, self._disp, param._param, _energy, _gradient, _sigma)
+        results = dict(energy=_energy)
+        if _gradient is not None:
+            results.update(gradient=_gradient)
+        if _sigma is not None:
+            results.update(virial=_sigma)
+        return results
+
+    def set_ewald_summation(self, *args, **kwargs) -> None:
+        raise NotImplementedError("Ewald summation is not available in this library")
+
+    def set_work_partition(self, part: int, nparts: int) -> None:
+        raise NotImplementedError("Work partitions are not available in this library")
+
+    def get_pairwise_dispersion(self, param: DampingParam) -> dict:
+        raise NotImplementedError("Pairwise dispersion energies are not available in this library")
+
+    def get_hessian(self, param: DampingParam) -> dict:
+        raise NotImplementedError("The dispersion Hessian is not available in this library")
+
+
+def _rename_kwargs(kwargs, old_name, new_name):
+    """Accept the legacy spelling of a damping parameter."""
+    if old_name in kwargs and new_name not in kwargs:
+        kwargs[new_name] = kwargs[old_name]
+        del kwargs[old_name]

@@ -1,0 +1,220 @@
+"""ctypes binding of libd3hip.so, the C ABI declared in include/d3hip.h.
+
+Mirrors python/dftd3/library.py of simple-dftd3 v1.6.0 (which binds the same
+entry points through CFFI): handle classes whose lifetime is tied to the
+Python object, an ``error_check`` wrapper that raises ``RuntimeError`` with
+the library's message (library.py:114-128 of the reference), and one function
+per C entry point with the reference's names.
+
+The library is required: importing this module fails with ``ImportError`` if
+libd3hip.so is missing or cannot be loaded.  There is no CPU fallback.
+"""
+import ctypes
+import functools
+import os
+from typing import Optional
+
+import numpy as np
+
+_LIBNAME = "libd3hip.so"
+_PATH = os.environ.get("D3HIP_LIBRARY", os.path.join(os.path.dirname(os.path.abspath(__file__)), _LIBNAME))
+
+try:
+    lib = ctypes.CDLL(_PATH)
+except OSError as exc:  # pragma: no cover - exercised by deployments without a build
+    raise ImportError(f"DFT-D3 HIP library unimportable ({_PATH}): {exc}") from exc
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_bp = ctypes.POINTER(ctypes.c_bool)
+_d = ctypes.c_double
+
+# Signatures of the exported entry points (include/d3hip.h).
+_SIGNATURES = {
+    "dftd3_get_version": (ctypes.c_int, []),
+    "dftd3_new_error": (_vp, []),
+    "dftd3_check_error": (ctypes.c_int, [_vp]),
+    "dftd3_get_error": (None, [_vp, ctypes.c_char_p, _ip]),
+    "dftd3_delete_error": (None, [ctypes.POINTER(_vp)]),
+    "dftd3_new_structure": (_vp, [_vp, ctypes.c_int, _ip, _dp, _dp, _bp]),
+    "dftd3_delete_structure": (None, [ctypes.POINTER(_vp)]),
+    "dftd3_update_structure": (None, [_vp, _vp, _dp, _dp]),
+    "dftd3_new_d3_model": (_vp, [_vp, _vp]),
+    "dftd3_set_model_realspace_cutoff": (None, [_vp, _vp, _d, _d, _d]),
+    "dftd3_set_model_realspace_cutoff_smooth": (None, [_vp, _vp, _d, _d, _d, _d, _d]),
+    "dftd3_set_model_ghost_index": (None, [_vp, _vp, _ip, ctypes.c_int]),
+    "dftd3_delete_model": (None, [ctypes.POINTER(_vp)]),
+    "dftd3_new_zero_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d]),
+    "dftd3_load_zero_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
+    "dftd3_new_rational_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d]),
+    "dftd3_load_rational_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
+    "dftd3_delete_param": (None, [ctypes.POINTER(_vp)]),
+    "dftd3_get_dispersion": (None, [_vp, _vp, _vp, _vp, _dp, _dp, _dp]),
+    "d3hip_set_model_precision": (None, [_vp, _vp, ctypes.c_int]),
+    "d3hip_new_workspace": (_vp, [_vp]),
+    "d3hip_get_dispersion_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, ctypes.c_int,
+                                          _vp, _vp, _vp, _vp, _vp]),
+}
+
+for _name, (_res, _args) in _SIGNATURES.items():
+    _fn = getattr(lib, _name)  # AttributeError: library does not export the symbol
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+
+def get_api_version() -> str:
+    """API version of the C interface as 'major.minor.patch'."""
+    v = lib.dftd3_get_version()
+    return "{}.{}.{}".format(v // 10000, v % 10000 // 100, v % 100)
+
+
+class Handle:
+    """Owner of an opaque library handle, released when garbage collected."""
+
+    _deleter = None
+
+    def __init__(self, handle=None):
+        self.handle = handle
+
+    @classmethod
+    def with_gc(cls, handle):
+        return cls(handle)
+
+    @classmethod
+    def null(cls):
+        return cls(None)
+
+    def __del__(self):
+        if self.handle and self._deleter is not None:
+            ptr = _vp(self.handle)
+            getattr(lib, self._deleter)(ctypes.byref(ptr))
+            self.handle = None
+
+
+class StructureHandle(Handle):
+    _deleter = "dftd3_delete_structure"
+
+
+class ModelHandle(Handle):
+    _deleter = "dftd3_delete_model"
+
+
+class ParamHandle(Handle):
+    _deleter = "dftd3_delete_param"
+
+
+class _Error:
+    def __init__(self):
+        self.handle = lib.dftd3_new_error()
+
+    def __del__(self):
+        if self.handle:
+            ptr = _vp(self.handle)
+            lib.dftd3_delete_error(ctypes.byref(ptr))
+            self.handle = None
+
+
+def error_check(func):
+    """Run a library function with a fresh error handle, raise on error."""
+
+    @functools.wraps(func)
+    def handle_error(*args, **kwargs):
+        err = _Error()
+        value = func(err.handle, *args, **kwargs)
+        if lib.dftd3_check_error(err.handle):
+            buf = ctypes.create_string_buffer(512)
+            size = ctypes.c_int(512)
+            lib.dftd3_get_error(err.handle, buf, ctypes.byref(size))
+            raise RuntimeError(buf.value.decode())
+        return value
+
+    return handle_error
+
+
+def _cast(ctype, array: Optional[np.ndarray]):
+    return None if array is None else array.ctypes.data_as(ctype)
+
+
+def _char(value: Optional[str]):
+    return None if value is None else value.encode()
+
+
+def new_structure(natoms: int, numbers: np.ndarray, positions: np.ndarray, lattice: Optional[np.ndarray],
+                  periodic: Optional[np.ndarray]) -> StructureHandle:
+    """Create new molecular structure data"""
+    return StructureHandle.with_gc(
+        error_check(lib.dftd3_new_structure)(natoms, _cast(_ip, numbers), _cast(_dp, positions),
+                                             _cast(_dp, lattice), _cast(_bp, periodic)))
+
+
+def update_structure(mol: StructureHandle, positions: np.ndarray, lattice: Optional[np.ndarray]) -> None:
+    """Update the molecular structure data"""
+    return error_check(lib.dftd3_update_structure)(mol.handle, _cast(_dp, positions), _cast(_dp, lattice))
+
+
+def new_d3_model(mol: StructureHandle) -> ModelHandle:
+    """Create new D3 dispersion model"""
+    return ModelHandle.with_gc(error_check(lib.dftd3_new_d3_model)(mol.handle))
+
+
+def new_workspace() -> ModelHandle:
+    """Create a device workspace for batched evaluations"""
+    return ModelHandle.with_gc(error_check(lib.d3hip_new_workspace)())
+
+
+def set_model_realspace_cutoff(disp: ModelHandle, disp2: float, disp3: float, cn: float, width2: float = 0.0,
+                               width3: float = 0.0) -> None:
+    """Set the realspace cutoffs of the model"""
+    return error_check(lib.dftd3_set_model_realspace_cutoff_smooth)(disp.handle, disp2, disp3, cn, width2, width3)
+
+
+def set_model_ghost_index(disp: ModelHandle, ghost: np.ndarray) -> None:
+    """Disable dispersion contributions from selected atoms (0-based indices)"""
+    idx = np.ascontiguousarray(np.asarray(ghost, dtype="i4") + 1)
+    return error_check(lib.dftd3_set_model_ghost_index)(disp.handle, _cast(_ip, idx), idx.size)
+
+
+def set_model_precision(disp: ModelHandle, precision: int) -> None:
+    """Select FP64 (0) or FP32-arithmetic / FP64-accumulation (1) pair terms"""
+    return error_check(lib.d3hip_set_model_precision)(disp.handle, precision)
+
+
+def new_zero_damping(s6: float, s8: float, s9: float, rs6: float, rs8: float, alp: float) -> ParamHandle:
+    """Create new zero damping parameters"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_new_zero_damping)(s6, s8, s9, rs6, rs8, alp))
+
+
+def load_zero_damping(method: str, atm: bool) -> ParamHandle:
+    """Load zero damping parameters from internal storage"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_load_zero_damping)(_char(method), atm))
+
+
+def new_rational_damping(s6: float, s8: float, s9: float, a1: float, a2: float, alp: float) -> ParamHandle:
+    """Create new rational damping parameters"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_new_rational_damping)(s6, s8, s9, a1, a2, alp))
+
+
+def load_rational_damping(method: str, atm: bool) -> ParamHandle:
+    """Load rational damping parameters from internal storage"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_load_rational_damping)(_char(method), atm))
+
+
+def get_dispersion(mol: StructureHandle, disp: ModelHandle, param: ParamHandle, energy: np.ndarray,
+                   gradient: Optional[np.ndarray], sigma: Optional[np.ndarray]) -> None:
+    """Evaluate the dispersion energy and its derivatives"""
+    return error_check(lib.dftd3_get_dispersion)(mol.handle, disp.handle, param.handle, _cast(_dp, energy),
+                                                 _cast(_dp, gradient), _cast(_dp, sigma))
+
+
+def get_dispersion_batch(workspace: ModelHandle, natoms, numbers, lattices, periodic, params, cutoffs,
+                         precision: int, positions_ptr: int, energy_ptr: int, gradient_ptr: Optional[int],
+                         sigma_ptr: Optional[int], stream: Optional[int]) -> None:
+    """Batched evaluation on device pointers (include/d3hip.h, Part 2)"""
+    return error_check(lib.d3hip_get_dispersion_batch)(
+        workspace.handle, len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices),
+        _cast(_bp, periodic), _cast(_dp, params), _cast(_dp, cutoffs), precision, positions_ptr, energy_ptr,
+        gradient_ptr, sigma_ptr, stream)

@@ -1,0 +1,173 @@
+"""The drop-in boundary without a GPU: library loading, exported symbols and
+the host-side behaviour of the C ABI and of dftd3.interface (argument
+validation, error strings, parameter loading), mirroring
+python/dftd3/test_interface.py:103-231 of the reference.  No compute call
+is made here except one that must fail loudly when no GPU is present."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "d3hip.h")).read()
+    return re.findall(r"D3HIP_API\s+[\w\s\*]+?\b((?:dftd3|d3hip)_\w+)\s*\(", text)
+
+
+def test_library_exports_every_declared_symbol():
+    from dftd3 import library
+    names = _declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(library.lib, name), name
+        assert name in library._SIGNATURES, name
+
+
+def test_api_version():
+    from dftd3 import library
+    assert library.get_api_version() == "1.6.0"
+
+
+@pytest.fixture
+def numbers():
+    return np.array([6, 7, 6, 7, 6, 6, 6, 8, 7, 6, 8, 7, 6, 6, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1])
+
+
+@pytest.fixture
+def positions(numbers):
+    rng = np.random.default_rng(1)
+    pos = []
+    while len(pos) < len(numbers):
+        x = rng.uniform(0, 12, 3)
+        if all(np.linalg.norm(x - y) > 1.5 for y in pos):
+            pos.append(x)
+    return np.array(pos)
+
+
+def test_damping_param_missing_arguments():
+    from dftd3.interface import RationalDampingParam, ZeroDampingParam
+    with pytest.raises(TypeError):
+        RationalDampingParam()
+    with pytest.raises(TypeError, match="s8"):
+        RationalDampingParam(a1=0.4, a2=5.0)
+    with pytest.raises(TypeError, match="a1"):
+        RationalDampingParam(s8=1.0, a2=5.0)
+    with pytest.raises(TypeError, match="a2"):
+        RationalDampingParam(s8=1.0, a1=0.4)
+    with pytest.raises(TypeError, match="rs6"):
+        ZeroDampingParam(s8=1.0)
+
+
+def test_damping_param_loading():
+    from dftd3.interface import RationalDampingParam, ZeroDampingParam
+    RationalDampingParam(method="pbe0")
+    RationalDampingParam(method="PBE-0", atm=True)  # '-' removed, lower case (param.f90:104-114)
+    ZeroDampingParam(method="b3lyp", atm=True)
+    RationalDampingParam(method=None, s8=1.0, a1=0.4, a2=5.0)
+    ZeroDampingParam(sr6=1.2, s8=1.0, alpha6=14.0)  # legacy keyword names
+    with pytest.raises(RuntimeError, match="No entry for 'abc' present"):
+        RationalDampingParam(method="abc")
+    with pytest.raises(RuntimeError, match="No entry for 'not-a-method' present"):
+        ZeroDampingParam(method="not-a-method")
+
+
+def test_unavailable_damping_families():
+    from dftd3.interface import ModifiedRationalDampingParam
+    with pytest.raises(NotImplementedError):
+        ModifiedRationalDampingParam(method="pbe")
+
+
+@pytest.mark.parametrize("cls_name", ["Structure", "DispersionModel"])
+def test_structure_validation(cls_name, numbers, positions):
+    from dftd3 import interface
+    cls = getattr(interface, cls_name)
+    rng = np.random.default_rng()
+    with pytest.raises(RuntimeError, match="Too close interatomic distances found"):
+        cls(numbers, np.zeros((24, 3)))
+    with pytest.raises(ValueError, match="Dimension missmatch"):
+        cls(np.array([1, 1, 1]), positions)
+    with pytest.raises(ValueError, match="Expected tripels"):
+        cls(numbers, rng.random(7))
+    with pytest.raises(ValueError, match="Invalid lattice provided"):
+        cls(numbers, positions, lattice=np.eye(2))
+    with pytest.raises(ValueError, match="Invalid periodicity provided"):
+        cls(numbers, positions, lattice=np.eye(3) * 20, periodic=np.array([True, False]))
+    inst = cls(numbers, positions)
+    assert len(inst) == 24
+    with pytest.raises(ValueError, match="Dimension missmatch for positions"):
+        inst.update(rng.random(7))
+    with pytest.raises(ValueError, match="Invalid lattice provided"):
+        inst.update(positions, rng.random(7))
+    with pytest.raises(RuntimeError, match="Too close interatomic distances found"):
+        inst.update(np.zeros((24, 3)))
+    inst.update(positions + 0.1)
+
+
+def test_c_abi_null_handles():
+    from dftd3 import library as L
+    err = L._Error()
+    vp = ctypes.c_void_p
+    e = ctypes.c_double()
+    L.lib.dftd3_get_dispersion(err.handle, None, None, None, ctypes.byref(e), None, None)
+    assert L.lib.dftd3_check_error(err.handle)
+    buf = ctypes.create_string_buffer(128)
+    L.lib.dftd3_get_error(err.handle, buf, None)
+    assert buf.value.decode() == "Molecular structure data is missing"
+    # a truncated message respects the buffer size
+    size = ctypes.c_int(10)
+    L.lib.dftd3_get_error(err.handle, buf, ctypes.byref(size))
+    assert buf.value.decode() == "Molecular"
+    err2 = L._Error()
+    L.lib.dftd3_new_d3_model(err2.handle, None)
+    L.lib.dftd3_get_error(err2.handle, buf, None)
+    assert buf.value.decode() == "Molecular structure data is missing"
+    # delete functions null the handle and accept NULL
+    h = vp(L.lib.dftd3_new_rational_damping(err.handle, 1.0, 1.0, 0.0, 0.4, 5.0, 14.0))
+    L.lib.dftd3_delete_param(ctypes.byref(h))
+    assert h.value is None
+    L.lib.dftd3_delete_param(ctypes.byref(h))
+
+
+def test_c_abi_dispersion_missing_pieces(numbers, positions):
+    from dftd3 import library as L
+    mol = L.new_structure(len(numbers), np.ascontiguousarray(numbers, dtype="i4"),
+                          np.ascontiguousarray(positions), None, None)
+    disp = L.new_d3_model(mol)
+    energy = np.zeros(())
+    with pytest.raises(RuntimeError, match="Dispersion model is missing"):
+        L.get_dispersion(mol, L.ModelHandle.null(), L.ParamHandle.null(), energy, None, None)
+    with pytest.raises(RuntimeError, match="Damping parameters are missing"):
+        L.get_dispersion(mol, disp, L.ParamHandle.null(), energy, None, None)
+
+
+def test_unsupported_element_rejected():
+    from dftd3.interface import DispersionModel
+    with pytest.raises(RuntimeError, match="Unsupported element"):
+        DispersionModel(np.array([1, 104]), np.array([[0.0, 0, 0], [0, 0, 3.0]]))
+
+
+def test_no_cpu_fallback(numbers, positions):
+    """Without a GPU the evaluation must fail with an error, never compute."""
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from dftd3.interface import DispersionModel, RationalDampingParam
+    model = DispersionModel(numbers, positions)
+    with pytest.raises(RuntimeError, match="HIP"):
+        model.get_dispersion(RationalDampingParam(method="pbe0"), grad=True)
+
+
+def test_synthetic_workloads_are_seeded():
+    from dftd3 import synthetic
+    a = synthetic.molecule_c0()
+    b = synthetic.molecule_c0()
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert len(a[0]) == 200
+    d = np.linalg.norm(a[1][:, None] - a[1][None], axis=-1) + np.eye(200) * 1e9
+    assert d.min() >= 1.0 * synthetic.AATOAU - 1e-12
+    assert sorted(set(a[0].tolist())) == [1, 6, 7, 8]

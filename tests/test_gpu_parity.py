@@ -1,0 +1,223 @@
+"""GPU parity of the HIP library against the CPU oracle and the reference's
+golden vectors.
+
+Bars (checked against the FP64 oracle on the same inputs):
+  plain FP64 path:  |dE| <= 1e-10 |E| (+1e-14), max |dg| <= 1e-9 Eh/bohr,
+                    max |d sigma| <= 1e-9 Eh
+  mixed path:       |dE| <= 1e-8 Eh per atom, max |dg| <= 1e-7 Eh/bohr
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+E_REL, G_ABS, S_ABS = 1e-10, 1e-9, 1e-9
+MIX_E_PER_ATOM, MIX_G = 1e-8, 1e-7
+
+BJ = dict(s6=1.0, s8=1.2177, a1=0.4145, a2=4.8593, s9=1.0, alp=14.0)
+ZERO = dict(s6=1.0, s8=1.703, rs6=1.261, rs8=1.0, s9=1.0, alp=14.0)
+
+
+def _step(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _oracle(numbers, pos, lat=None, pbc=None, kind="rational", param=BJ, cutoff=O.DEFAULT_CUTOFF, grad=True,
+            ghost=None, wrap=False):
+    return O.dispersion(numbers, pos, lat, pbc, O.RATIONAL if kind == "rational" else O.ZERO, param, cutoff,
+                        grad=grad, ghost=ghost, wrap_cell=wrap)
+
+
+def _check(res, ref, nat, mixed=False):
+    e, er = float(res["energy"]), ref["energy"]
+    if mixed:
+        assert abs(e - er) <= MIX_E_PER_ATOM * nat, (e, er)
+    else:
+        assert abs(e - er) <= E_REL * abs(er) + 1e-14, (e, er, abs(e - er) / abs(er))
+    if "gradient" in res and "gradient" in ref:
+        dg = np.abs(np.asarray(res["gradient"]) - ref["gradient"]).max()
+        assert dg <= (MIX_G if mixed else G_ABS), dg
+        ds = np.abs(np.asarray(res["virial"]) - ref["virial"]).max()
+        assert ds <= (MIX_G * nat if mixed else S_ABS), ds
+
+
+def _batch(systems, precision="fp64", grad=True):
+    from dftd3.torch_api import get_dispersion_batch
+    out = get_dispersion_batch(systems, precision=precision, grad=grad)
+    res = []
+    off = 0
+    for k, s in enumerate(systems):
+        n = len(s["numbers"])
+        r = dict(energy=out["energy"][k].item())
+        if grad:
+            r["gradient"] = out["gradient"][off:off + n].cpu().numpy()
+            r["virial"] = out["sigma"][k].cpu().numpy()
+        off += n
+        res.append(r)
+    return res
+
+
+@pytest.mark.parametrize("name", ["ta90_3d_bj", "bp86_zero_c4_2d", "bp86_zero_c4_vac3d", "gh185_bj_2d",
+                                  "gh185_bj_vac3d"])
+def test_reference_tight_goldens(goldens, name):
+    f = goldens[name]
+    _step(f"tight golden {name}")
+    cut = dict(cn=f["cutoff"][0], disp2=f["cutoff"][1], disp3=f["cutoff"][2])
+    sysd = dict(numbers=f["numbers"], positions=f["positions"], lattice=f["lattice"], periodic=f["periodic"],
+                kind=f["kind"], param=f["param"], cutoff=cut)
+    r = _batch([sysd], grad=f["grad"])[0]
+    ref = f["expected"]["energy"]
+    assert abs(r["energy"] - ref) <= max(f["atol"], E_REL * abs(ref)), (r["energy"], ref)
+    if "gradient_norm" in f["expected"]:
+        assert abs(np.linalg.norm(r["gradient"]) - f["expected"]["gradient_norm"]) <= G_ABS
+    orc = _run_golden_oracle(f)
+    _check(r, orc, len(f["numbers"]))
+
+
+def _run_golden_oracle(f):
+    return _oracle(f["numbers"], f["positions"], f["lattice"], f["periodic"], f["kind"], f["param"], f["cutoff"],
+                   grad=f["grad"], wrap=False)
+
+
+@pytest.mark.parametrize("name", ["validation_01-energy-d3-bj", "validation_02-energy-d3-zero",
+                                  "validation_06-gradient-d3-bj", "validation_07-gradient-d3-zero"])
+def test_validation_goldens_through_interface(goldens, name):
+    from dftd3.interface import DispersionModel, RationalDampingParam, ZeroDampingParam
+    f = goldens[name]
+    _step(f"validation {name}")
+    model = DispersionModel(np.array(f["numbers"]), np.array(f["positions"]),
+                            None if f["lattice"] is None else np.array(f["lattice"]),
+                            None if f["periodic"] is None else np.array(f["periodic"]))
+    cls = RationalDampingParam if f["kind"] == "rational" else ZeroDampingParam
+    res = model.get_dispersion(cls(method=f["method"], atm=f["atm"]), grad=f["grad"])
+    e = f["expected"]
+    assert abs(float(res["energy"]) - e["energy"]) <= f["atol"]
+    if "gradient" in e:
+        np.testing.assert_allclose(res["gradient"], np.array(e["gradient"]), atol=f["atol"], rtol=0)
+        np.testing.assert_allclose(res["virial"], np.array(e["virial"]), atol=f["atol"], rtol=0)
+    _check(res, _run_golden_oracle(dict(f, wrap=True)), len(f["numbers"]))
+
+
+@pytest.mark.parametrize("atm", [True, False])
+@pytest.mark.parametrize("kind,method", [("rational", "pbe0"), ("zero", "b3lyp")])
+def test_python_interface_model(python_model, kind, method, atm):
+    from dftd3.interface import DispersionModel, RationalDampingParam, ZeroDampingParam
+    _step(f"python model {kind} {method} atm={atm}")
+    m = DispersionModel(np.array(python_model["numbers"]), np.array(python_model["positions"]))
+    cls = RationalDampingParam if kind == "rational" else ZeroDampingParam
+    res = m.get_dispersion(cls(method=method, atm=atm), grad=False)
+    ref = python_model["energies"][f"{kind}:{method}"]["atm" if atm else "noatm"]
+    assert float(res["energy"]) == pytest.approx(ref, rel=python_model["rel"])
+    assert set(res) == {"energy"} and res["energy"].shape == ()
+
+
+def test_config0_molecule_fp64():
+    from dftd3 import synthetic
+    from dftd3.interface import DispersionModel, RationalDampingParam
+    _step("config 0, 200-atom molecule, BJ + ATM, FP64")
+    num, pos = synthetic.molecule_c0()
+    model = DispersionModel(num, pos)
+    res = model.get_dispersion(RationalDampingParam(**BJ), grad=True)
+    assert res["gradient"].shape == (200, 3) and res["virial"].shape == (3, 3)
+    _check(res, _oracle(num, pos), 200)
+
+
+def test_config0_molecule_mixed():
+    from dftd3 import synthetic
+    from dftd3.interface import DispersionModel, RationalDampingParam
+    _step("config 0, mixed precision")
+    num, pos = synthetic.molecule_c0()
+    model = DispersionModel(num, pos)
+    model.set_precision("mixed")
+    res = model.get_dispersion(RationalDampingParam(**BJ), grad=True)
+    _check(res, _oracle(num, pos), 200, mixed=True)
+
+
+@pytest.mark.parametrize("kind", ["rational", "zero"])
+def test_energy_only_path(kind):
+    from dftd3 import synthetic
+    _step(f"energy only {kind}")
+    num, pos = synthetic.molecule_c0(seed=5, n=80)
+    p = BJ if kind == "rational" else ZERO
+    r = _batch([dict(numbers=num, positions=pos, kind=kind, param=p)], grad=False)[0]
+    _check(r, _oracle(num, pos, kind=kind, param=p, grad=False), 80)
+
+
+def test_zero_damping_atm_ghost_and_smooth_cutoff():
+    from dftd3 import synthetic
+    from dftd3.interface import DispersionModel, ZeroDampingParam
+    _step("zero damping, ghosts, smooth cutoffs")
+    num, pos = synthetic.molecule_c0(seed=9, n=60)
+    model = DispersionModel(num, pos)
+    model.set_ghost_atoms([0, 5, 17])
+    model.set_realspace_cutoff(disp2=14.0, disp3=10.0, cn=25.0, width2=4.0, width3=3.0)
+    res = model.get_dispersion(ZeroDampingParam(**ZERO), grad=True)
+    ghost = np.zeros(60, dtype=np.int32)
+    ghost[[0, 5, 17]] = 1
+    ref = _oracle(num, pos, kind="zero", param=ZERO, cutoff=(25.0, 14.0, 10.0, 4.0, 3.0), ghost=ghost)
+    _check(res, ref, 60)
+
+
+def _cell(seed, n, edge):
+    rng = np.random.default_rng(seed)
+    lat = np.eye(3) * edge + rng.uniform(-0.08, 0.08, (3, 3)) * edge
+    frac = []
+    while len(frac) < n:
+        f = rng.uniform(0, 1, 3)
+        ok = True
+        for g in frac:
+            d = f - g
+            d -= np.round(d)
+            if np.linalg.norm(d @ lat) < 1.9:
+                ok = False
+                break
+        if ok:
+            frac.append(f)
+    num = rng.choice([1, 6, 7, 8, 14, 16], size=n)
+    return num, np.array(frac) @ lat, lat
+
+
+@pytest.mark.parametrize("kind", ["rational", "zero"])
+def test_periodic_cell_default_cutoffs(kind):
+    from dftd3.interface import DispersionModel, RationalDampingParam, ZeroDampingParam
+    _step(f"periodic 16-atom cell {kind} + ATM")
+    num, pos, lat = _cell(21, 16, 11.0)
+    model = DispersionModel(num, pos, lat, np.array([True, True, True]))
+    p = BJ if kind == "rational" else ZERO
+    cls = RationalDampingParam if kind == "rational" else ZeroDampingParam
+    res = model.get_dispersion(cls(**p), grad=True)
+    _check(res, _oracle(num, pos, lat, [True] * 3, kind, p, wrap=True), 16)
+
+
+def test_periodic_2d_slab():
+    _step("2D slab BJ + ATM")
+    num, pos, lat = _cell(22, 10, 9.0)
+    lat[2] = [0.0, 0.0, 30.0]
+    pos[:, 2] = np.random.default_rng(1).uniform(0, 6, len(pos))
+    r = _batch([dict(numbers=num, positions=pos, lattice=lat, periodic=[True, True, False], param=BJ)])[0]
+    _check(r, _oracle(num, pos, lat, [True, True, False], "rational", BJ), 10)
+
+
+def test_batch_mixed_systems():
+    _step("batch of 6 mixed systems")
+    from dftd3 import synthetic
+    systems, refs = [], []
+    for k in range(6):
+        if k % 3 == 2:
+            num, pos, lat = _cell(30 + k, 12, 10.0)
+            s = dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3)
+        else:
+            num, pos = synthetic.molecule_c0(seed=40 + k, n=15 + 9 * k)
+            s = dict(numbers=num, positions=pos)
+        kind = "rational" if k % 2 == 0 else "zero"
+        s.update(kind=kind, param=BJ if kind == "rational" else ZERO)
+        systems.append(s)
+        refs.append(_oracle(s["numbers"], s["positions"], s.get("lattice"), s.get("periodic"), kind, s["param"]))
+    for prec in ("fp64", "mixed"):
+        for r, ref, s in zip(_batch(systems, precision=prec), refs, systems):
+            _check(r, ref, len(s["numbers"]), mixed=prec == "mixed")

@@ -1,0 +1,303 @@
+"""Pins of the CPU oracle (oracle/d3_oracle.c) -- CPU only.
+
+1. Reference golden vectors: the 100-epsilon periodic regressions of the
+   reference's unit tests, the CLI validation JSONs (abs 1e-6) and the
+   Python interface energies (rel 1e-6), see tests/golden/make_fixtures.py.
+2. What the mathematics fixes: dimer closed forms for CN, C6 and the two-body
+   term, the equilateral ATM triangle, finite differences of the energy for
+   gradient and virial, the virial theorem for molecules, translation and
+   rotation invariance, and the coverage of the lattice-image enumeration.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+EPS = np.finfo(float).eps
+TABLES = np.load(O.TABLES)
+
+
+def _kind(name):
+    return O.RATIONAL if name == "rational" else O.ZERO
+
+
+def _run(f, **kw):
+    return O.dispersion(f["numbers"], f["positions"], f["lattice"], f["periodic"], _kind(f["kind"]), f["param"],
+                        f["cutoff"], grad=kw.pop("grad", f["grad"]), wrap_cell=f["wrap"], **kw)
+
+
+TIGHT = ["ta90_3d_bj", "bp86_zero_c4_2d", "bp86_zero_c4_vac3d", "gh185_bj_2d", "gh185_bj_vac3d"]
+LOOSE = ["validation_01-energy-d3-bj", "validation_02-energy-d3-zero", "validation_06-gradient-d3-bj",
+         "validation_07-gradient-d3-zero"]
+
+
+@pytest.mark.parametrize("name", TIGHT)
+def test_tight_goldens(goldens, name):
+    f = goldens[name]
+    r = _run(f)
+    assert abs(r["energy"] - f["expected"]["energy"]) <= f["atol"]
+    if "gradient_norm" in f["expected"]:
+        assert abs(np.linalg.norm(r["gradient"]) - f["expected"]["gradient_norm"]) <= f["atol"]
+
+
+@pytest.mark.parametrize("name", LOOSE)
+def test_validation_goldens(goldens, name):
+    f = goldens[name]
+    r = _run(f)
+    e = f["expected"]
+    assert abs(r["energy"] - e["energy"]) <= f["atol"]
+    if "gradient" in e:
+        np.testing.assert_allclose(r["gradient"], np.array(e["gradient"]), rtol=0, atol=f["atol"])
+        np.testing.assert_allclose(r["virial"], np.array(e["virial"]), rtol=0, atol=f["atol"])
+
+
+@pytest.mark.parametrize("key", ["rational:pbe0", "zero:b3lyp"])
+@pytest.mark.parametrize("atm", [True, False])
+def test_python_interface_energies(python_model, key, atm):
+    import json
+    import os
+    params = json.load(open(os.path.join(os.path.dirname(O.TABLES), "d3_params.json")))
+    kind, method = key.split(":")
+    p = dict(params["defaults"])
+    p.update(params[kind][params["aliases"][method]])
+    p["s9"] = 1.0 if atm else 0.0
+    ref = python_model["energies"][key]["atm" if atm else "noatm"]
+    r = O.dispersion(python_model["numbers"], python_model["positions"], None, None, _kind(kind), p, grad=False)
+    assert r["energy"] == pytest.approx(ref, rel=python_model["rel"])
+
+
+def _c6_single_ref(z1, z2):
+    a, b = max(z1, z2), min(z1, z2)
+    ic = b + a * (a - 1) // 2 - 1
+    return TABLES["c6ref"][ic, 0, 0]
+
+
+def _r4r2(z):
+    return np.sqrt(0.5 * TABLES["r4_over_r2"][z - 1] * np.sqrt(z))
+
+
+def test_cn_dimer_closed_form():
+    r = 2.1
+    cn = O.coordination_number([1, 1], [[0, 0, 0], [0, 0, r]])
+    rc = 2 * 4.0 / 3.0 * 0.32 * O.ORACLE_AATOAU
+    expect = 1.0 / (1.0 + np.exp(-16.0 * (rc / r - 1.0)))
+    np.testing.assert_allclose(cn, [expect, expect], rtol=1e-14)
+
+
+def test_weights_sum_to_one_and_single_reference():
+    num = [1, 6, 7, 8, 14, 16, 2]
+    cn = [0.3, 3.7, 2.2, 1.1, 3.9, 2.0, 0.4]
+    gw, gwd = O.weights(num, cn)
+    np.testing.assert_allclose(gw.sum(axis=1), 1.0, rtol=1e-14)
+    np.testing.assert_allclose(gwd.sum(axis=1), 0.0, atol=1e-13)
+    # He has a single reference: weight one, derivative zero
+    assert gw[6, 0] == pytest.approx(1.0, abs=1e-15) and abs(gwd[6, 0]) < 1e-15
+
+
+def test_weights_exceptional_fallback():
+    # CN far above every reference CN: all Gaussians underflow; the reference
+    # falls back to the highest reference CN (model.f90:260-266)
+    gw, gwd = O.weights([6], [60.0])
+    refcn = TABLES["refcn"][5, :TABLES["nref"][5]]
+    expect = (refcn == refcn.max()).astype(float)
+    np.testing.assert_array_equal(gw[0, :len(expect)], expect)
+    np.testing.assert_array_equal(gwd[0], 0.0)
+
+
+def test_weight_derivative_finite_difference():
+    num = [6, 7, 8, 1, 14]
+    cn = np.array([2.7, 1.4, 0.8, 0.95, 3.1])
+    gw, gwd = O.weights(num, cn)
+    h = 1e-6
+    gp, _ = O.weights(num, cn + h)
+    gm, _ = O.weights(num, cn - h)
+    np.testing.assert_allclose(gwd, (gp - gm) / (2 * h), atol=1e-8)
+
+
+def test_c6_single_reference_elements():
+    # He and Ne have one reference system: C6 is the table entry
+    num = [2, 10]
+    gw, gwd = O.weights(num, [0.0, 0.0])
+    c6, dc6 = O.atomic_c6(num, gw, gwd)
+    assert c6[0, 1] == pytest.approx(_c6_single_ref(2, 10), rel=1e-15)
+    assert c6[0, 0] == pytest.approx(_c6_single_ref(2, 2), rel=1e-15)
+    np.testing.assert_allclose(dc6, 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["rational", "zero"])
+def test_two_body_dimer_closed_form(kind):
+    r = 6.3
+    num = [2, 10]
+    pos = np.array([[0.0, 0.0, 0.0], [0.3, -0.4, r]])
+    d = np.linalg.norm(pos[1] - pos[0])
+    c6 = _c6_single_ref(2, 10)
+    rrij = 3 * _r4r2(2) * _r4r2(10)
+    if kind == "rational":
+        p = dict(s6=0.9, s8=1.3, a1=0.41, a2=4.9, s9=0.0, alp=14.0)
+        r0 = p["a1"] * np.sqrt(rrij) + p["a2"]
+        e = lambda x: -c6 * (p["s6"] / (x ** 6 + r0 ** 6) + p["s8"] * rrij / (x ** 8 + r0 ** 8))
+        de = lambda x: -c6 * (-p["s6"] * 6 * x ** 5 / (x ** 6 + r0 ** 6) ** 2
+                              - p["s8"] * rrij * 8 * x ** 7 / (x ** 8 + r0 ** 8) ** 2)
+    else:
+        p = dict(s6=1.0, s8=0.7, rs6=1.2, rs8=1.0, s9=0.0, alp=14.0)
+        a, b = max(2, 10), min(2, 10)
+        r0 = O.ORACLE_AATOAU * TABLES["vdwrad_aa"][b + a * (a - 1) // 2 - 1]
+
+        def e(x):
+            f6 = 1 / (1 + 6 * (p["rs6"] * r0 / x) ** 14)
+            f8 = 1 / (1 + 6 * (p["rs8"] * r0 / x) ** 16)
+            return -c6 * (p["s6"] * f6 / x ** 6 + p["s8"] * rrij * f8 / x ** 8)
+
+        de = lambda x: (e(x + 1e-6) - e(x - 1e-6)) / 2e-6
+    res = O.dispersion(num, pos, kind=_kind(kind), param=p)
+    assert res["energy"] == pytest.approx(e(d), rel=1e-13)
+    u = (pos[0] - pos[1]) / d
+    np.testing.assert_allclose(res["gradient"][0], de(d) * u, rtol=1e-7, atol=1e-15)
+    np.testing.assert_allclose(res["gradient"][1], -de(d) * u, rtol=1e-7, atol=1e-15)
+
+
+def test_atm_equilateral_closed_form():
+    d = 5.5
+    pos = np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0], [d / 2, d * np.sqrt(3) / 2, 0.0]])
+    p = dict(s6=0.0, s8=0.0, a1=0.4, a2=5.0, s9=1.0, alp=14.0)
+    res = O.dispersion([2, 2, 2], pos, param=p)
+    c6 = _c6_single_ref(2, 2)
+    r0 = (4.0 / 3.0 * O.ORACLE_AATOAU * TABLES["vdwrad_aa"][2 + 2 * 1 // 2 - 1]) ** 3  # pair (2, 2)
+    fdmp = 1 / (1 + 6 * (r0 / d ** 3) ** (16.0 / 3.0))
+    expect = c6 ** 1.5 * (1 + 3 * 0.5 ** 3) / d ** 9 * fdmp
+    assert res["energy"] == pytest.approx(expect, rel=1e-13)
+    np.testing.assert_allclose(res["gradient"].sum(axis=0), 0.0, atol=1e-17)
+
+
+def _small_molecule(seed=7, n=14):
+    rng = np.random.default_rng(seed)
+    num = rng.choice([1, 6, 7, 8, 16], size=n)
+    pos = []
+    while len(pos) < n:
+        x = rng.uniform(0, 9.0, 3)
+        if all(np.linalg.norm(x - y) > 2.0 for y in pos):
+            pos.append(x)
+    return num, np.array(pos)
+
+
+PARAMS = {
+    "rational": dict(s6=1.0, s8=1.2177, a1=0.4145, a2=4.8593, s9=1.0, alp=14.0),
+    "zero": dict(s6=1.0, s8=1.703, rs6=1.261, rs8=1.0, s9=1.0, alp=14.0),
+}
+
+
+@pytest.mark.parametrize("kind", ["rational", "zero"])
+def test_gradient_finite_difference_molecule(kind):
+    num, pos = _small_molecule()
+    p = PARAMS[kind]
+    res = O.dispersion(num, pos, kind=_kind(kind), param=p)
+    h = 1e-4
+    fd = np.zeros_like(pos)
+    for i in range(len(pos)):
+        for c in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, c] += h
+            pm[i, c] -= h
+            fd[i, c] = (O.dispersion(num, pp, kind=_kind(kind), param=p, grad=False)["energy"]
+                        - O.dispersion(num, pm, kind=_kind(kind), param=p, grad=False)["energy"]) / (2 * h)
+    np.testing.assert_allclose(res["gradient"], fd, rtol=0, atol=2e-9)
+    # translation invariance and the virial theorem of a molecule
+    np.testing.assert_allclose(res["gradient"].sum(axis=0), 0.0, atol=1e-14)
+    virial = np.einsum("ia,ib->ab", res["gradient"], pos)  # sum_i g_ia x_ib = sigma(a,b)
+    np.testing.assert_allclose(res["virial"].T, virial, atol=1e-13)
+
+
+def test_rotation_invariance():
+    num, pos = _small_molecule(11)
+    q, _ = np.linalg.qr(np.random.default_rng(3).normal(size=(3, 3)))
+    a = O.dispersion(num, pos, param=PARAMS["rational"])
+    b = O.dispersion(num, pos @ q.T, param=PARAMS["rational"])
+    assert b["energy"] == pytest.approx(a["energy"], rel=1e-13)
+    np.testing.assert_allclose(b["gradient"], a["gradient"] @ q.T, atol=1e-13)
+
+
+def _small_cell(seed=5):
+    rng = np.random.default_rng(seed)
+    lat = np.diag([9.5, 10.2, 8.8]) + rng.uniform(-0.6, 0.6, (3, 3))
+    num = rng.choice([1, 6, 8, 14], size=8)
+    frac = []
+    while len(frac) < 8:
+        f = rng.uniform(0, 1, 3)
+        ok = True
+        for g in frac:
+            d = f - g
+            d -= np.round(d)
+            if np.linalg.norm(d @ lat) < 2.2:
+                ok = False
+        if ok:
+            frac.append(f)
+    return num, np.array(frac) @ lat, lat
+
+
+@pytest.mark.parametrize("kind", ["rational", "zero"])
+def test_periodic_gradient_and_strain_virial(kind):
+    num, pos, lat = _small_cell()
+    p = PARAMS[kind]
+    cut = (20.0, 25.0, 12.0, 0.0, 0.0)  # keeps the FD affordable, same code path
+    res = O.dispersion(num, pos, lat, [True] * 3, _kind(kind), p, cut)
+    e = lambda x, l: O.dispersion(num, x, l, [True] * 3, _kind(kind), p, cut, grad=False)["energy"]
+    h = 1e-4
+    for i, c in [(0, 0), (3, 1), (7, 2)]:
+        pp, pm = pos.copy(), pos.copy()
+        pp[i, c] += h
+        pm[i, c] -= h
+        assert res["gradient"][i, c] == pytest.approx((e(pp, lat) - e(pm, lat)) / (2 * h), abs=2e-9)
+    # sigma_ab = dE/d eps_ab with positions and lattice strained by (1 + eps)
+    # (test_periodic_3d.f90:229-262)
+    sig = np.zeros((3, 3))
+    he = 1e-5
+    for a in range(3):
+        for b in range(3):
+            ep = np.eye(3)
+            ep[a, b] += he
+            em = np.eye(3)
+            em[a, b] -= he
+            sig[a, b] = (e(pos @ ep.T, lat @ ep.T) - e(pos @ em.T, lat @ em.T)) / (2 * he)
+    np.testing.assert_allclose(res["virial"].T, sig, atol=5e-9)
+
+
+def test_lattice_points_cover_cutoff_sphere():
+    rng = np.random.default_rng(0)
+    for trial in range(5):
+        lat = np.diag(rng.uniform(5, 12, 3)) + rng.uniform(-2, 2, (3, 3))
+        rthr = rng.uniform(8, 30)
+        tr = O.lattice_points([1, 1, 1], lat, rthr)
+        n = np.array(np.meshgrid(*[np.arange(-15, 16)] * 3, indexing="ij")).reshape(3, -1).T
+        allt = n @ lat
+        inside = allt[np.linalg.norm(allt, axis=1) <= rthr]
+        have = {tuple(np.round(t, 9)) for t in tr}
+        assert all(tuple(np.round(t, 9)) in have for t in inside)
+        # symmetric and ordered ix outer, iz inner: the first image is -rep
+        np.testing.assert_array_equal(tr[::-1], -tr)
+
+
+def test_lattice_points_molecule_and_2d():
+    assert O.lattice_points([0, 0, 0], np.eye(3), 40.0).tolist() == [[0.0, 0.0, 0.0]]
+    lat = np.array([[4.7, 0.0, 0.0], [-2.3, 4.0, 0.0], [0.0, 0.0, 1.0]])
+    tr = O.lattice_points([1, 1, 0], lat, 30.0)
+    assert np.all(tr[:, 2] == 0.0)
+    n = np.array(np.meshgrid(np.arange(-20, 21), np.arange(-20, 21), indexing="ij")).reshape(2, -1).T
+    allt = n @ lat[:2]
+    inside = allt[np.linalg.norm(allt, axis=1) <= 30.0]
+    have = {tuple(np.round(t, 9)) for t in tr}
+    assert all(tuple(np.round(t, 9)) in have for t in inside)
+
+
+def test_wrap_to_central_cell():
+    lat = np.array([[10.0, 0.0, 0.0], [1.0, 9.0, 0.0], [0.5, 0.5, 8.0]])
+    pos = np.array([[-3.0, 2.0, 1.0], [25.0, -7.0, 19.0]])
+    w = O.wrap(pos, lat, [1, 1, 1])
+    frac = w @ np.linalg.inv(lat)
+    assert np.all(frac >= -1e-12) and np.all(frac < 1.0)
+    shift = (w - pos) @ np.linalg.inv(lat)
+    np.testing.assert_allclose(shift, np.round(shift), atol=1e-12)
+
+
+def test_too_close():
+    assert O.too_close([[0, 0, 0], [0, 0, 1e-10]])
+    assert not O.too_close([[0, 0, 0], [0, 0, 1e-8]])
