@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Benchmark of the D3 energy + gradient + virial path on one MI355X.
+
+Workload (default ``--config c1``): periodic triclinic cell of 2,000 atoms
+(28 A cubic with +-5 % off-diagonal strain, H/C/N/O/Si/S), PBE0-D3(BJ) with
+the ATM three-body term, default cutoffs (CN 40, two-body 60, three-body 40
+bohr).  One step is one complete evaluation of energy, gradient and virial
+(CN, weights, C6, two-body, ATM, dCN/dR chain rule) with the positions
+resident in device memory.  Both arithmetic paths are timed: plain FP64 and
+mixed (FP32 pair/triple arithmetic, FP64 accumulation).
+
+Prints one JSON line.  ``value`` is the mixed path's E+grad pair-terms/s
+(two-body terms of the reference's loops per second of complete
+evaluations); the FP64 path, triple-terms/s, cells/s, the roofline of the
+dominant (ATM) kernel and a CPU baseline (the C oracle on the host cores, on
+a bounded sample of the same workload) are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "simple-dftd3_amd"))
+
+METRIC = "E+grad pair-terms/s & molecules/s: mixed vs FP64 on GPU, |dE|<=1e-8 Eh/atom"
+# Algorithmic operation counts per work unit (mul and add counted separately,
+# an FMA as two; div, rcp, sqrt, exp and pow as one each).
+OPS_PER_TRIPLE = 233    # ATM E+grad+sigma (atm.f90:733-806)
+OPS_PER_PAIR_BJ = 63    # two-body BJ E+grad+sigma (rational.f90:462-497)
+PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload(config):
+    from dftd3 import synthetic
+    if config == "c1":
+        num, pos, lat = synthetic.periodic_c1()
+        sysd = dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3, kind="rational",
+                    param=dict(synthetic.PBE0_BJ))
+        return [sysd], dict(workload="config1-periodic-2000-atoms-pbe0-d3bj-atm", natoms=len(num), cells=1,
+                            lattice="28 A cubic, +-5% off-diagonal strain", elements="H/C/N/O/Si/S")
+    if config == "c0":
+        num, pos = synthetic.molecule_c0()
+        sysd = dict(numbers=num, positions=pos, kind="rational", param=dict(synthetic.PBE0_BJ))
+        return [sysd], dict(workload="config0-molecule-200-atoms-pbe0-d3bj-atm", natoms=len(num), cells=1)
+    raise SystemExit(f"unknown config {config}")
+
+
+def timed_steps(run, steps, sync):
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(systems, counts, stride, threads):
+    """Oracle (FP64 C, OpenMP) on every stride-th outer atom of each loop."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    s = systems[0]
+    t = oracle.time_sampled_dispersion(stride, s["numbers"], s["positions"], s.get("lattice"), s.get("periodic"),
+                                       oracle.RATIONAL, s["param"])
+    t_full = t * stride
+    return dict(value=float(counts[1] / t_full), unit="pair-terms/s", cores=oracle.num_threads(), kind="port",
+                sample=f"oracle E+grad+sigma with the outer atom loops restricted to every {stride}-th atom "
+                       f"({t:.2f} s), scaled by {stride} to one full evaluation ({t_full:.1f} s)",
+                seconds_per_eval_estimate=t_full)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--cpu-stride", type=int, default=10)
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="run one device copy of this size first (reference kernel for FETCH/WRITE_SIZE counters)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has a single MI355X; "
+              "the sharding of this path over several GPUs is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    from dftd3 import library, torch_api
+
+    dev = torch.device("cuda:0")
+    if args.calibration_copy_mib > 0:
+        n = args.calibration_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibration_copy_mib} MiB read + {args.calibration_copy_mib} MiB written")
+        del src, dst
+    systems, config = workload(args.config)
+    spec, pos = torch_api.batch_spec(systems)
+    positions = torch.as_tensor(pos, dtype=torch.float64, device=dev).contiguous()
+    counts = torch_api.count_terms(spec, positions).sum(axis=0)
+    log(f"{config['workload']}: CN terms {counts[0]}, pair terms {counts[1]}, triple terms {counts[2]}")
+    ws = torch_api.workspace(dev)
+    outs = {}
+    res = {}
+    for prec in ("fp64", "mixed"):
+        out = torch_api.run_batch(spec, positions, precision=prec)
+        run = lambda: torch_api.run_batch(spec, positions, precision=prec, out=out)  # noqa: E731
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        library.set_kernel_timing(ws, True)
+        t = timed_steps(run, args.steps, torch.cuda.synchronize)
+        atm_ms, atm_calls = library.get_kernel_time(ws, "atm")
+        all_ms = {k: library.get_kernel_time(ws, k) for k in
+                  ("pack", "cn", "weights", "c6", "disp2", "atm_list", "atm", "cn_chain", "reduce")}
+        library.set_kernel_timing(ws, False)
+        t_plain = timed_steps(run, args.steps, torch.cuda.synchronize)  # without event records
+        outs[prec] = {k: v.clone() for k, v in out.items()}
+        atm_avg = atm_ms / max(atm_calls, 1) / 1e3
+        achieved = counts[2] * OPS_PER_TRIPLE / atm_avg / 1e12 if atm_avg > 0 else 0.0
+        res[prec] = dict(ms_per_step=t_plain * 1e3, ms_per_step_with_kernel_events=t * 1e3,
+                         pair_terms_per_s=counts[1] / t_plain, triple_terms_per_s=counts[2] / t_plain,
+                         cells_per_s=config["cells"] / t_plain, atm_kernel_ms=atm_avg * 1e3,
+                         atm_tflops=achieved, kernel_ms={k: v[0] / max(v[1], 1) for k, v in all_ms.items()})
+        log(f"{prec}: {t_plain * 1e3:.2f} ms/step, ATM kernel {atm_avg * 1e3:.2f} ms ({achieved:.2f} TFLOP/s)")
+    natoms = config["natoms"]
+    de = (outs["mixed"]["energy"] - outs["fp64"]["energy"]).abs().max().item()
+    dg = (outs["mixed"]["gradient"] - outs["fp64"]["gradient"]).abs().max().item()
+    ds = (outs["mixed"]["sigma"] - outs["fp64"]["sigma"]).abs().max().item()
+    mixed, fp64 = res["mixed"], res["fp64"]
+    line = dict(
+        metric=METRIC, value=mixed["pair_terms_per_s"], unit="pair-terms/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(config, pair_terms=int(counts[1]), triple_terms=int(counts[2]), cn_terms=int(counts[0]),
+                    parallelism="single-gpu", accumulation="f64"),
+        molecules_per_s=mixed["cells_per_s"], triple_terms_per_s=mixed["triple_terms_per_s"],
+        fp64=dict(value=fp64["pair_terms_per_s"], unit="pair-terms/s", ms_per_step=fp64["ms_per_step"],
+                  molecules_per_s=fp64["cells_per_s"], triple_terms_per_s=fp64["triple_terms_per_s"],
+                  atm_kernel_ms=fp64["atm_kernel_ms"], atm_tflops=fp64["atm_tflops"],
+                  atm_frac_of_fp64_peak=fp64["atm_tflops"] / PEAK_TFLOPS["fp64"]),
+        mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"],
+        mixed_vs_fp64_error=dict(energy_per_atom=de / natoms, max_gradient=dg, max_sigma=ds),
+        kernel_ms=dict(fp64=fp64["kernel_ms"], mixed=mixed["kernel_ms"]),
+        roofline=dict(kernel="atm_kernel<float,true>", bound="valu", achieved=mixed["atm_tflops"],
+                      peak=PEAK_TFLOPS["mixed"], unit="TFLOP/s", frac=mixed["atm_tflops"] / PEAK_TFLOPS["mixed"],
+                      traffic=None, ops_per_triple=OPS_PER_TRIPLE, triples_per_launch=int(counts[2]),
+                      launch_ms=mixed["atm_kernel_ms"]),
+    )
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: oracle on every {args.cpu_stride}-th atom, {args.cpu_threads} threads")
+        line["cpu_baseline"] = cpu_baseline(systems, counts, args.cpu_stride, args.cpu_threads)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

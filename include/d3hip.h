@@ -124,4 +124,24 @@ D3HIP_API void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspa
  * and stream) of d3hip_get_dispersion_batch. */
 D3HIP_API dftd3_model d3hip_new_workspace(dftd3_error error);
 
+/* Count the work units the reference's loops visit, per system, on the GPU:
+ * counts[nsys][3] (host) = {CN terms (ncoord.f90:118-127), two-body pair
+ * terms (rational.f90:450-464), ATM triple terms (atm.f90:706-742)}.  Same
+ * batch description as d3hip_get_dispersion_batch; triples are counted even
+ * for s9 = 0.  Synchronizes the stream. */
+D3HIP_API void d3hip_count_terms_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
+                                       const int* numbers, const double* lattices, const bool* periodic,
+                                       const double* params, const double* cutoffs, const double* positions,
+                                       long long* counts, void* stream);
+
+/* Time every kernel launch of the model / workspace with HIP events on its
+ * stream (enable != 0; resets the totals).  Timing synchronizes each call. */
+D3HIP_API void d3hip_set_kernel_timing(dftd3_error error, dftd3_model model, int enable);
+
+/* Total milliseconds and launches of one kernel group since timing was
+ * enabled; name is one of pack, cn, weights, c6, disp2, atm_list, atm,
+ * cn_chain, reduce. */
+D3HIP_API void d3hip_get_kernel_time(dftd3_error error, dftd3_model model, const char* name, double* total_ms,
+                                     long long* calls);
+
 #endif /* D3HIP_H */

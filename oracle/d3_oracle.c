@@ -37,6 +37,9 @@
 #define MAX_REF 7
 #define NPAIR (MAX_ELEM * (MAX_ELEM + 1) / 2)
 #define EPS DBL_EPSILON
+/* Element (r, c) of an nat x nat matrix in Fortran (column-major) order, the
+ * storage of the reference's c6(:,:) and dc6dcn(:,:) (disp.f90:135-136). */
+#define FIDX(r, c) ((size_t)(r) + (size_t)(c) * (size_t)nat)
 
 /* ------------------------------------------------------------------ tables */
 static int g_nref[MAX_ELEM];
@@ -46,6 +49,13 @@ static double g_r4r2raw[118];
 static double g_vdwrad_aa[NPAIR];
 static double g_covrad_aa[118];
 static double g_aatoau;
+
+/* Bounded timing samples: only outer atoms iat with iat % stride == 0 are
+ * visited by the CN, two-body, ATM and chain-rule loops (1 = all).  The C6
+ * matrix is always complete: the ATM loop skips triples with a vanishing
+ * C6 (atm.f90:725-726), so a partial matrix would skip real work. */
+static int g_sample_stride = 1;
+void d3o_set_sample_stride(int stride) { g_sample_stride = stride > 0 ? stride : 1; }
 
 /* Copy the element tables (data/d3_tables.npz) into the oracle. */
 void d3o_set_tables(const int *nref, const double *refcn, const double *c6ref,
@@ -261,6 +271,7 @@ void d3o_cn(int nat, const int *num, const double *xyz, int ntr, const double *t
         double *loc = (double *)calloc(nat, sizeof(double));
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
+            if (iat % g_sample_stride) continue; /* bounded timing sample */
             for (int jat = 0; jat <= iat; ++jat) {
                 double rc = d3o_rcov(num[iat]) + d3o_rcov(num[jat]);
                 for (int itr = 0; itr < ntr; ++itr) {
@@ -294,6 +305,7 @@ void d3o_cn_derivs(int nat, const int *num, const double *xyz, int ntr, const do
         double sl[9] = {0};
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
+            if (iat % g_sample_stride) continue; /* bounded timing sample */
             for (int jat = 0; jat <= iat; ++jat) {
                 double rc = d3o_rcov(num[iat]) + d3o_rcov(num[jat]);
                 for (int itr = 0; itr < ntr; ++itr) {
@@ -373,8 +385,8 @@ void d3o_weights(int nat, const int *num, const double *cn, int mref, double *gw
 
 /*
  * model.f90:340-491 get_atomic_c6, first-derivative and plain branches.
- * c6 and dc6dcn are [nat][nat] with c6[i*nat+j] = Fortran c6(i,j) and
- * dc6dcn[i*nat+j] = Fortran dc6dcn(i,j) = dC6_ij / dCN_i.
+ * c6 and dc6dcn are nat x nat in Fortran order: c6[FIDX(i,j)] = c6(i,j)
+ * and dc6dcn[FIDX(i,j)] = dc6dcn(i,j) = dC6_ij / dCN_i.
  */
 void d3o_c6(int nat, const int *num, int mref, const double *gwvec, const double *gwdcn,
             const int *ghost, double *c6, double *dc6dcn)
@@ -398,11 +410,11 @@ void d3o_c6(int nat, const int *num, int mref, const double *gwvec, const double
                         dcj += gwvec[iat * mref + ir] * gwdcn[jat * mref + jr] * refc6;
                     }
                 }
-            c6[(size_t)iat * nat + jat] = dc6;
-            c6[(size_t)jat * nat + iat] = dc6;
+            c6[FIDX(iat, jat)] = dc6;
+            c6[FIDX(jat, iat)] = dc6;
             if (dc6dcn) {
-                dc6dcn[(size_t)iat * nat + jat] = dci;
-                dc6dcn[(size_t)jat * nat + iat] = dcj;
+                dc6dcn[FIDX(iat, jat)] = dci;
+                dc6dcn[FIDX(jat, iat)] = dcj;
             }
         }
     }
@@ -437,11 +449,12 @@ static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz
         double sl[9] = {0};
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
+            if (iat % g_sample_stride) continue; /* bounded timing sample */
             int zi = num[iat];
             for (int jat = 0; jat <= iat; ++jat) {
                 int zj = num[jat];
                 double rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
-                double c6ij = c6[(size_t)jat * nat + iat];
+                double c6ij = c6[FIDX(jat, iat)];
                 double r0ij = 0, r0ij6 = 0, r0ij8 = 0, rs6r0 = 0, rs8r0 = 0;
                 if (p->kind == 0) {
                     r0ij = p->a1 * sqrt(rrij) + p->a2;
@@ -496,10 +509,10 @@ static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz
                     for (int c = 0; c < 3; ++c) dG[c] = -c6ij * gdisp * v[c];
                     for (int a = 0; a < 3; ++a)
                         for (int b = 0; b < 3; ++b) dS[b * 3 + a] = dG[a] * v[b] * 0.5;
-                    dl[iat] -= dc6dcn[(size_t)iat * nat + jat] * edisp;
+                    dl[iat] -= dc6dcn[FIDX(iat, jat)] * edisp;
                     for (int k = 0; k < 9; ++k) sl[k] += dS[k];
                     if (iat != jat) {
-                        dl[jat] -= dc6dcn[(size_t)jat * nat + iat] * edisp;
+                        dl[jat] -= dc6dcn[FIDX(jat, iat)] * edisp;
                         for (int c = 0; c < 3; ++c) {
                             gl[3 * iat + c] += dG[c];
                             gl[3 * jat + c] -= dG[c];
@@ -551,10 +564,11 @@ static void atm(int nat, const int *num, const double *xyz, int ntr, const doubl
         double sl[9] = {0};
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
+            if (iat % g_sample_stride) continue; /* bounded timing sample */
             int zi = num[iat];
             for (int jat = 0; jat <= iat; ++jat) {
                 int zj = num[jat];
-                double c6ij = c6[(size_t)jat * nat + iat];
+                double c6ij = c6[FIDX(jat, iat)];
                 double r0ij = rs9 * d3o_rvdw(zj, zi);
                 for (int jtr = 0; jtr < ntr; ++jtr) {
                     const double *tj = trans + 3 * jtr;
@@ -566,8 +580,8 @@ static void atm(int nat, const int *num, const double *xyz, int ntr, const doubl
                     smooth_cutoff(rij, cutoff, width, &swij, &dswijdr);
                     for (int kat = 0; kat <= jat; ++kat) {
                         int zk = num[kat];
-                        double c6ik = c6[(size_t)kat * nat + iat];
-                        double c6jk = c6[(size_t)kat * nat + jat];
+                        double c6ik = c6[FIDX(kat, iat)];
+                        double c6jk = c6[FIDX(kat, jat)];
                         if (grad && (fabs(c6ij) < EPS || fabs(c6ik) < EPS || fabs(c6jk) < EPS))
                             continue;
                         double c9 = -s9 * sqrt(fabs(c6ij * c6ik * c6jk));
@@ -639,12 +653,12 @@ static void atm(int nat, const int *num, const double *xyz, int ntr, const doubl
                                 for (int b = 0; b < 3; ++b)
                                     sl[b * 3 + a] += (dGij[a] * vij[b] + dGik[a] * vik[b]
                                                       + dGjk[a] * vjk[b]) * triple;
-                            dl[iat] -= dE * 0.5 * (dc6dcn[(size_t)iat * nat + jat] / c6ij
-                                                   + dc6dcn[(size_t)iat * nat + kat] / c6ik);
-                            dl[jat] -= dE * 0.5 * (dc6dcn[(size_t)jat * nat + iat] / c6ij
-                                                   + dc6dcn[(size_t)jat * nat + kat] / c6jk);
-                            dl[kat] -= dE * 0.5 * (dc6dcn[(size_t)kat * nat + iat] / c6ik
-                                                   + dc6dcn[(size_t)kat * nat + jat] / c6jk);
+                            dl[iat] -= dE * 0.5 * (dc6dcn[FIDX(iat, jat)] / c6ij
+                                                   + dc6dcn[FIDX(iat, kat)] / c6ik);
+                            dl[jat] -= dE * 0.5 * (dc6dcn[FIDX(jat, iat)] / c6ij
+                                                   + dc6dcn[FIDX(jat, kat)] / c6jk);
+                            dl[kat] -= dE * 0.5 * (dc6dcn[FIDX(kat, iat)] / c6ik
+                                                   + dc6dcn[FIDX(kat, jat)] / c6jk);
                         }
                     }
                 }

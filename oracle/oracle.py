@@ -144,7 +144,8 @@ def atomic_c6(numbers, gw, gwd=None, ghost=None):
     c6 = np.zeros((n, n))
     dc6 = np.zeros((n, n)) if gwd is not None else None
     L.d3o_c6(n, _ptr(num, _ip), gw.shape[1], _ptr(gw), _ptr(gwd), _ptr(gh, _ip), _ptr(c6), _ptr(dc6))
-    return c6, dc6
+    # the oracle stores Fortran (column-major) matrices; return [i, j] = (i, j)
+    return c6.T.copy(), (None if dc6 is None else dc6.T.copy())
 
 
 RATIONAL, ZERO = 0, 1
@@ -203,3 +204,21 @@ def count_terms(positions, lattice=None, periodic=None, cutoff=DEFAULT_CUTOFF, t
 
 def num_threads():
     return lib().d3o_num_threads()
+
+
+def set_sample_stride(stride: int):
+    """Visit only outer atoms iat % stride == 0 (bounded timing samples)."""
+    lib().d3o_set_sample_stride(int(stride))
+
+
+def time_sampled_dispersion(stride, *args, **kwargs):
+    """Wall time of one sampled E+grad+sigma evaluation (seconds).  The
+    results of a sampled run are partial sums and are not returned."""
+    import time
+    set_sample_stride(stride)
+    try:
+        t0 = time.perf_counter()
+        dispersion(*args, **kwargs)
+        return time.perf_counter() - t0
+    finally:
+        set_sample_stride(1)

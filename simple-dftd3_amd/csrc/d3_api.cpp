@@ -423,6 +423,33 @@ void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model di
   });
 }
 
+namespace {
+std::vector<d3::SystemDesc> batch_systems(int nsys, const int* natoms, const double* lattices, const bool* periodic,
+                                          const double* params, const double* cutoffs) {
+  std::vector<d3::SystemDesc> sys(nsys);
+  for (int k = 0; k < nsys; ++k) {
+    d3::SystemDesc& s = sys[k];
+    s.natoms = natoms[k];
+    if (s.natoms <= 0) throw d3::Error("Invalid molecular structure data provided");
+    if (lattices) std::memcpy(s.lattice, lattices + 9 * k, sizeof s.lattice);
+    if (periodic)
+      for (int c = 0; c < 3; ++c) s.periodic[c] = periodic[3 * k + c] ? 1 : 0;
+    const double* p = params + 7 * k;
+    _dftd3_param prm;
+    prm.kind = static_cast<int>(p[0]);
+    if (prm.kind != d3::kRational && prm.kind != d3::kZero) throw d3::Error("Unknown damping kind");
+    prm.s6 = p[1]; prm.s8 = p[2]; prm.s9 = p[3]; prm.a1 = p[4]; prm.a2 = p[5]; prm.alp = p[6];
+    double cut[5] = {40.0, 60.0, 40.0, 0.0, 0.0};
+    if (cutoffs) {
+      const double* c = cutoffs + 5 * k;
+      cut[0] = c[2]; cut[1] = c[0]; cut[2] = c[1]; cut[3] = c[3]; cut[4] = c[4];
+    }
+    s.param = make_sys_param(&prm, cut);
+  }
+  return sys;
+}
+}  // namespace
+
 void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
                                 const int* numbers, const double* lattices, const bool* periodic,
                                 const double* params, const double* cutoffs, int precision,
@@ -434,30 +461,41 @@ void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspace, int ns
     return fatal(error, "Invalid batch description");
   if (precision != d3::kFP64 && precision != d3::kMixed) return fatal(error, "Unknown precision mode");
   guarded(error, [&] {
-    std::vector<d3::SystemDesc> sys(nsys);
-    for (int k = 0; k < nsys; ++k) {
-      d3::SystemDesc& s = sys[k];
-      s.natoms = natoms[k];
-      if (s.natoms <= 0) throw d3::Error("Invalid molecular structure data provided");
-      if (lattices) std::memcpy(s.lattice, lattices + 9 * k, sizeof s.lattice);
-      if (periodic)
-        for (int c = 0; c < 3; ++c) s.periodic[c] = periodic[3 * k + c] ? 1 : 0;
-      const double* p = params + 7 * k;
-      _dftd3_param prm;
-      prm.kind = static_cast<int>(p[0]);
-      if (prm.kind != d3::kRational && prm.kind != d3::kZero) throw d3::Error("Unknown damping kind");
-      prm.s6 = p[1]; prm.s8 = p[2]; prm.s9 = p[3]; prm.a1 = p[4]; prm.a2 = p[5]; prm.alp = p[6];
-      double cut[5] = {40.0, 60.0, 40.0, 0.0, 0.0};
-      if (cutoffs) {
-        const double* c = cutoffs + 5 * k;
-        cut[0] = c[2]; cut[1] = c[0]; cut[2] = c[1]; cut[3] = c[3]; cut[4] = c[4];
-      }
-      s.param = make_sys_param(&prm, cut);
-    }
+    std::vector<d3::SystemDesc> sys = batch_systems(nsys, natoms, lattices, periodic, params, cutoffs);
     d3::BatchOutput out;
     out.energy = energy;
     out.gradient = gradient;
     out.sigma = sigma;
     workspace->engine->run(sys, numbers, positions, precision, out, static_cast<hipStream_t>(stream));
   });
+}
+
+void d3hip_count_terms_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
+                             const int* numbers, const double* lattices, const bool* periodic,
+                             const double* params, const double* cutoffs, const double* positions,
+                             long long* counts, void* stream) {
+  if (!error) return;
+  if (!workspace) return fatal(error, "Dispersion model is missing");
+  if (nsys <= 0 || !natoms || !numbers || !params || !positions || !counts)
+    return fatal(error, "Invalid batch description");
+  guarded(error, [&] {
+    std::vector<d3::SystemDesc> sys = batch_systems(nsys, natoms, lattices, periodic, params, cutoffs);
+    workspace->engine->count_terms(sys, numbers, positions, counts, static_cast<hipStream_t>(stream));
+  });
+}
+
+void d3hip_set_kernel_timing(dftd3_error error, dftd3_model model, int enable) {
+  if (!error) return;
+  if (!model) return fatal(error, "Dispersion model is missing");
+  model->engine->set_kernel_timing(enable != 0);
+}
+
+void d3hip_get_kernel_time(dftd3_error error, dftd3_model model, const char* name, double* total_ms,
+                           long long* calls) {
+  if (!error) return;
+  if (!model) return fatal(error, "Dispersion model is missing");
+  if (!name) return fatal(error, "Kernel name is missing");
+  auto t = model->engine->kernel_time(name);
+  if (total_ms) *total_ms = t.first;
+  if (calls) *calls = t.second;
 }

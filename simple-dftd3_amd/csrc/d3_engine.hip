@@ -765,6 +765,72 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(DevBatch b, const double
   }
 }
 
+// ------------------------------------------------------------ term counts
+// Work units as the reference's loops visit them: CN terms (iat, jat <= iat,
+// T) with 1e-12 <= r2 <= cn^2 (ncoord.f90:118-127) and two-body terms with
+// eps <= r2 <= disp2^2 (rational.f90:450-464), counted per system.
+__global__ __launch_bounds__(kBlock) void count_pairs_kernel(DevBatch b, unsigned long long* __restrict__ counts) {
+  const int lane = wave_lane();
+  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (i >= b.natoms) return;
+  const int s = b.sys[i];
+  const SysGeom g = b.geom[s];
+  const SysParam p = b.par[s];
+  const double cn2 = p.cut_cn * p.cut_cn, c22 = p.cut2 * p.cut2;
+  const double4 xi = b.pos[i];
+  unsigned long long ncn = 0, n2 = 0;
+  for (int j = g.atom_begin + lane; j <= i; j += kWave) {
+    const double4 xj = b.pos[j];
+    double u0, u1, u2;
+    for (int t = 0; t < g.img_cn_count; ++t) {
+      const double r2 = pair_vec(xi, xj, b.img[g.img_cn_begin + t], false, u0, u1, u2);
+      ncn += (r2 > cn2 || r2 < kCnFloor) ? 0 : 1;
+    }
+    for (int t = 0; t < g.img2_count; ++t) {
+      const double r2 = pair_vec(xi, xj, b.img[g.img2_begin + t], false, u0, u1, u2);
+      n2 += (r2 > c22 || r2 < kEps) ? 0 : 1;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    ncn += __shfl_xor(ncn, off, kWave);
+    n2 += __shfl_xor(n2, off, kWave);
+  }
+  if (lane == 0) {
+    atomicAdd(counts + 3 * s, ncn);
+    atomicAdd(counts + 3 * s + 1, n2);
+  }
+}
+
+// ATM triple terms (atm.f90:706-742): ordered (a, b) of L(i) with
+// atom(b) <= atom(a) and eps <= r2_ab <= cut3^2.
+__global__ __launch_bounds__(kBlock) void count_triples_kernel(DevBatch b, const long long* __restrict__ offset,
+                                                               const AtmEntry* __restrict__ list,
+                                                               unsigned long long* __restrict__ counts) {
+  __shared__ unsigned long long red[kWavesPerBlock];
+  const int i = blockIdx.x;
+  const int s = b.sys[i];
+  const SysParam p = b.par[s];
+  const double cut2 = p.cut3 * p.cut3;
+  const long long L0 = offset[i], L1 = offset[i + 1];
+  unsigned long long n = 0;
+  for (long long a = L0 + threadIdx.x; a < L1; a += blockDim.x) {
+    const AtmEntry ea = list[a];
+    for (long long bb = L0; bb < L0 + ea.hi; ++bb) {
+      double v0, v1, v2;
+      const double r2 = diff_norm2(list[bb], ea, v0, v1, v2);
+      n += (r2 > cut2 || r2 < kEps) ? 0 : 1;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, kWave);
+  if (wave_lane() == 0) red[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += red[w];
+    if (t) atomicAdd(counts + 3 * s + 2, t);
+  }
+}
+
 // ------------------------------------------------------------ device memory
 class DeviceBuffer {
  public:
@@ -877,15 +943,98 @@ class PinnedStaging {
   hipEvent_t event_ = nullptr;
 };
 
+
+// Optional per-kernel timing with HIP events recorded on the launch stream.
+class KernelTimer {
+ public:
+  ~KernelTimer() { clear(); }
+  bool enabled = false;
+  void begin(hipStream_t st) {
+    if (!enabled) return;
+    hipEvent_t a;
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventRecord(a, st), "hipEventRecord");
+    open_ = a;
+  }
+  void end(const char* name, hipStream_t st) {
+    if (!enabled) return;
+    hipEvent_t b;
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    hip_check(hipEventRecord(b, st), "hipEventRecord");
+    pending_.push_back({name, open_, b});
+  }
+  void collect() {
+    for (auto& p : pending_) {
+      hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
+      float ms = 0.0f;
+      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      auto& t = totals_[p.name];
+      t.first += ms;
+      t.second += 1;
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending_.clear();
+  }
+  void clear() {
+    for (auto& p : pending_) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending_.clear();
+    totals_.clear();
+  }
+  std::pair<double, long long> get(const std::string& name) const {
+    auto it = totals_.find(name);
+    return it == totals_.end() ? std::make_pair(0.0, 0LL) : it->second;
+  }
+
+ private:
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  hipEvent_t open_ = nullptr;
+  std::vector<Rec> pending_;
+  std::map<std::string, std::pair<double, long long>> totals_;
+};
+
+// Device description of a prepared batch.
+struct Prepared {
+  DevBatch db;
+  int natoms = 0, nsys = 0, nsp = 0, mref = 0;
+  long long c6_total = 0;
+  bool any_atm = false;
+  const int* ghost = nullptr;
+};
+
 struct Engine::Impl {
-  DeviceBuffer meta, pos, cn, gw, dgw, c6, dc6, dc6t, e_atom, dedcn, grad, sig, cnt, off, list,
-      energy, sigma, ghost;
+  DeviceBuffer meta, pos, cn, gw, dgw, c6, dc6, dc6t, e_atom, dedcn, grad, sig, cnt, off, list, counts;
   std::vector<char> staging;
   PinnedStaging pinned;
+  KernelTimer timer;
+
+  Prepared prepare(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                   hipStream_t stream, const int* ghost_host);
+  long long build_atm_lists(const Prepared& pb, hipStream_t stream, long long** off, AtmEntry** list);
+
+  template <typename F>
+  void timed(const char* name, hipStream_t st, F&& launch) {
+    timer.begin(st);
+    launch();
+    timer.end(name, st);
+  }
 };
 
 Engine::Engine() : impl_(new Impl) {}
 Engine::~Engine() { delete impl_; }
+
+void Engine::set_kernel_timing(bool enable) {
+  impl_->timer.clear();
+  impl_->timer.enabled = enable;
+}
+
+std::pair<double, long long> Engine::kernel_time(const std::string& name) const { return impl_->timer.get(name); }
 
 namespace {
 template <typename T>
@@ -895,19 +1044,21 @@ size_t stage(std::vector<char>& buf, const std::vector<T>& v) {
   if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
   return off;
 }
-
 }  // namespace
 
-void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
-                 int precision, const BatchOutput& out, hipStream_t stream, const BatchDebug* debug,
-                 RunStats* stats, const int* ghost_host) {
+// Host planning of a batch: species of the batch, per-species tables, lattice
+// images per system and cutoff, C6 block offsets; one metadata upload and the
+// packing of the positions.
+Prepared Engine::Impl::prepare(const std::vector<SystemDesc>& systems, const int* numbers,
+                               const double* positions_dev, hipStream_t stream, const int* ghost_host) {
+  Prepared pb;
   const int nsys = static_cast<int>(systems.size());
-  if (nsys == 0) return;
   int natoms = 0;
-  for (const auto& s : systems) natoms += s.natoms;
-  if (natoms <= 0) throw Error("Invalid molecular structure data provided");
+  for (const auto& s : systems) {
+    if (s.natoms <= 0) throw Error("Invalid molecular structure data provided");
+    natoms += s.natoms;
+  }
   const auto& tab = element_tables();
-  // species of the batch in order of first appearance
   std::map<int, int> zmap;
   std::vector<int> zlist;
   std::vector<int> sp(natoms), sysidx(natoms);
@@ -926,8 +1077,8 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
   const int nsp = static_cast<int>(zlist.size());
   if (nsp > kMaxSpecies) throw Error("Too many distinct elements in one batch");
   int mref = 0;
-  std::vector<double> rcov(nsp), r4r2(nsp), refcn(nsp * kMaxRef, -1.0), c6ref(size_t(nsp) * nsp * kMaxRef * kMaxRef, 0.0),
-      rvdw(nsp * nsp);
+  std::vector<double> rcov(nsp), r4r2(nsp), refcn(nsp * kMaxRef, -1.0),
+      c6ref(size_t(nsp) * nsp * kMaxRef * kMaxRef, 0.0), rvdw(nsp * nsp);
   std::vector<int> nref(nsp);
   for (int a = 0; a < nsp; ++a) {
     const int z = zlist[a];
@@ -944,7 +1095,6 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
           c6ref[(size_t(a) * nsp + c) * kMaxRef * kMaxRef + ri * kMaxRef + rj] = reference_c6(ri, rj, z, zc);
     }
   }
-  // systems: images per cutoff and C6 block offsets
   std::vector<SysGeom> geom(nsys);
   std::vector<SysParam> par(nsys);
   std::vector<double4> img;
@@ -957,7 +1107,6 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
     g.natoms = sd.natoms;
     a += sd.natoms;
     par[k] = sd.param;
-    par[k].grad = out.gradient || out.sigma ? 1 : 0;
     auto add = [&](double cut, int& begin, int& count) {
       std::vector<double> tr = lattice_points(sd.periodic, sd.lattice, cut);
       begin = static_cast<int>(img.size());
@@ -971,24 +1120,23 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
     c6_total += static_cast<long long>(sd.natoms) * sd.natoms;
     if (std::fabs(sd.param.s9) >= kEps) any_atm = true;
   }
-  // stage the metadata and upload it in one copy
-  std::vector<char>& sb = impl_->staging;
+  std::vector<char>& sb = staging;
   sb.clear();
   std::vector<int> ghost;
   if (ghost_host) ghost.assign(ghost_host, ghost_host + natoms);
-  size_t o_ghost = stage(sb, ghost);
-  size_t o_sp = stage(sb, sp), o_sys = stage(sb, sysidx), o_geom = stage(sb, geom), o_par = stage(sb, par),
-         o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2), o_nref = stage(sb, nref),
-         o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref), o_rvdw = stage(sb, rvdw);
-  char* dmeta = static_cast<char*>(impl_->meta.get(sb.size()));
-  char* pinned = impl_->pinned.get(sb.size());
-  std::memcpy(pinned, sb.data(), sb.size());
-  hip_check(hipMemcpyAsync(dmeta, pinned, sb.size(), hipMemcpyHostToDevice, stream), "upload");
-  impl_->pinned.record(stream);
+  const size_t o_ghost = stage(sb, ghost), o_sp = stage(sb, sp), o_sys = stage(sb, sysidx), o_geom = stage(sb, geom),
+               o_par = stage(sb, par), o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2),
+               o_nref = stage(sb, nref), o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref),
+               o_rvdw = stage(sb, rvdw);
+  char* dmeta = static_cast<char*>(meta.get(sb.size()));
+  char* host = pinned.get(sb.size());
+  std::memcpy(host, sb.data(), sb.size());
+  hip_check(hipMemcpyAsync(dmeta, host, sb.size(), hipMemcpyHostToDevice, stream), "upload");
+  pinned.record(stream);
 
-  DevBatch db;
+  DevBatch& db = pb.db;
   db.natoms = natoms; db.nsys = nsys; db.nsp = nsp; db.mref = mref;
-  db.pos = impl_->pos.as<double4>(natoms);
+  db.pos = pos.as<double4>(natoms);
   db.sp = reinterpret_cast<const int*>(dmeta + o_sp);
   db.sys = reinterpret_cast<const int*>(dmeta + o_sys);
   db.geom = reinterpret_cast<const SysGeom*>(dmeta + o_geom);
@@ -1000,80 +1148,145 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
   db.refcn = reinterpret_cast<const double*>(dmeta + o_refcn);
   db.c6ref = reinterpret_cast<const double*>(dmeta + o_c6r);
   db.rvdw = reinterpret_cast<const double*>(dmeta + o_rvdw);
+  pb.natoms = natoms; pb.nsys = nsys; pb.nsp = nsp; pb.mref = mref;
+  pb.c6_total = c6_total;
+  pb.any_atm = any_atm;
+  pb.ghost = ghost_host ? reinterpret_cast<const int*>(dmeta + o_ghost) : nullptr;
+  timed("pack", stream, [&] {
+    hipLaunchKernelGGL(pack_positions, dim3(blocks_for(natoms, kBlock)), dim3(kBlock), 0, stream, positions_dev,
+                       natoms, const_cast<double4*>(db.pos));
+  });
+  return pb;
+}
 
+// Three-body neighbour lists: count, scan, one read of the total, fill.
+long long Engine::Impl::build_atm_lists(const Prepared& pb, hipStream_t stream, long long** off_out,
+                                        AtmEntry** list_out) {
+  const int natoms = pb.natoms;
+  const int wave_grid = blocks_for(natoms, kWavesPerBlock);
+  long long* c = cnt.as<long long>(natoms);
+  long long* o = off.as<long long>(natoms + 1);
+  timed("atm_list", stream, [&] {
+    hipLaunchKernelGGL(atm_list_kernel<false>, dim3(wave_grid), dim3(kBlock), 0, stream, pb.db, c,
+                       (const long long*)nullptr, (AtmEntry*)nullptr);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, c, natoms, o);
+  });
+  long long total = 0;
+  hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
+  hip_check(hipStreamSynchronize(stream), "list size sync");
+  AtmEntry* l = list.as<AtmEntry>(total);
+  timed("atm_list", stream, [&] {
+    hipLaunchKernelGGL(atm_list_kernel<true>, dim3(wave_grid), dim3(kBlock), 0, stream, pb.db, c,
+                       (const long long*)o, l);
+  });
+  *off_out = o;
+  *list_out = l;
+  return total;
+}
+
+void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                 int precision, const BatchOutput& out, hipStream_t stream, const BatchDebug* debug,
+                 RunStats* stats, const int* ghost_host) {
+  if (systems.empty()) return;
+  Impl& m = *impl_;
+  const Prepared pb = m.prepare(systems, numbers, positions_dev, stream, ghost_host);
+  const DevBatch& db = pb.db;
+  const int natoms = pb.natoms;
   const bool grad = out.gradient || out.sigma;
-  hipLaunchKernelGGL(pack_positions, dim3(blocks_for(natoms, kBlock)), dim3(kBlock), 0, stream, positions_dev,
-                     natoms, const_cast<double4*>(db.pos));
-  double* cn = debug && debug->cn ? debug->cn : impl_->cn.as<double>(natoms);
-  double* gw = impl_->gw.as<double>(size_t(natoms) * mref);
-  double* dgw = grad ? impl_->dgw.as<double>(size_t(natoms) * mref) : nullptr;
-  double* c6 = impl_->c6.as<double>(c6_total);
-  double* dc6 = grad ? impl_->dc6.as<double>(c6_total) : nullptr;
-  double* dc6t = grad ? impl_->dc6t.as<double>(c6_total) : nullptr;
-  double* e_atom = impl_->e_atom.as<double>(natoms);
-  double* dedcn = debug && debug->dedcn ? debug->dedcn : impl_->dedcn.as<double>(natoms);
-  double* gout = out.gradient ? out.gradient : impl_->grad.as<double>(size_t(natoms) * 3);
-  double* sig = impl_->sig.as<double>(size_t(natoms) * 6);
+  double* cn = debug && debug->cn ? debug->cn : m.cn.as<double>(natoms);
+  double* gw = m.gw.as<double>(size_t(natoms) * pb.mref);
+  double* dgw = grad ? m.dgw.as<double>(size_t(natoms) * pb.mref) : nullptr;
+  double* c6 = m.c6.as<double>(pb.c6_total);
+  double* dc6 = grad ? m.dc6.as<double>(pb.c6_total) : nullptr;
+  double* dc6t = grad ? m.dc6t.as<double>(pb.c6_total) : nullptr;
+  double* e_atom = m.e_atom.as<double>(natoms);
+  double* dedcn = debug && debug->dedcn ? debug->dedcn : m.dedcn.as<double>(natoms);
+  double* gout = out.gradient ? out.gradient : m.grad.as<double>(size_t(natoms) * 3);
+  double* sig = m.sig.as<double>(size_t(natoms) * 6);
   const int wave_grid = blocks_for(natoms, kWavesPerBlock);
   const bool mixed = precision == kMixed;
+  const dim3 blk(kBlock);
 
-  if (mixed) hipLaunchKernelGGL(cn_kernel<float>, dim3(wave_grid), dim3(kBlock), 0, stream, db, cn);
-  else hipLaunchKernelGGL(cn_kernel<double>, dim3(wave_grid), dim3(kBlock), 0, stream, db, cn);
-  hipLaunchKernelGGL(weights_kernel, dim3(blocks_for(natoms, kBlock)), dim3(kBlock), 0, stream, db, cn, gw, dgw);
-  const int* dghost = ghost_host ? reinterpret_cast<const int*>(dmeta + o_ghost) : nullptr;
-  hipLaunchKernelGGL(c6_kernel, dim3(natoms), dim3(kBlock), 0, stream, db, gw, dgw, dghost, c6, dc6, dc6t);
-  if (grad) {
-    if (mixed)
-      hipLaunchKernelGGL((disp2_kernel<float, true>), dim3(wave_grid), dim3(kBlock), 0, stream, db, c6, dc6, e_atom,
-                         dedcn, gout, sig);
+  m.timed("cn", stream, [&] {
+    if (mixed) hipLaunchKernelGGL(cn_kernel<float>, dim3(wave_grid), blk, 0, stream, db, cn);
+    else hipLaunchKernelGGL(cn_kernel<double>, dim3(wave_grid), blk, 0, stream, db, cn);
+  });
+  m.timed("weights", stream, [&] {
+    hipLaunchKernelGGL(weights_kernel, dim3(blocks_for(natoms, kBlock)), blk, 0, stream, db, cn, gw, dgw);
+  });
+  m.timed("c6", stream, [&] {
+    hipLaunchKernelGGL(c6_kernel, dim3(natoms), blk, 0, stream, db, gw, dgw, pb.ghost, c6, dc6, dc6t);
+  });
+  m.timed("disp2", stream, [&] {
+    if (grad && mixed)
+      hipLaunchKernelGGL((disp2_kernel<float, true>), dim3(wave_grid), blk, 0, stream, db, c6, dc6, e_atom, dedcn,
+                         gout, sig);
+    else if (grad)
+      hipLaunchKernelGGL((disp2_kernel<double, true>), dim3(wave_grid), blk, 0, stream, db, c6, dc6, e_atom, dedcn,
+                         gout, sig);
+    else if (mixed)
+      hipLaunchKernelGGL((disp2_kernel<float, false>), dim3(wave_grid), blk, 0, stream, db, c6, dc6, e_atom, dedcn,
+                         gout, sig);
     else
-      hipLaunchKernelGGL((disp2_kernel<double, true>), dim3(wave_grid), dim3(kBlock), 0, stream, db, c6, dc6, e_atom,
-                         dedcn, gout, sig);
-  } else {
-    if (mixed)
-      hipLaunchKernelGGL((disp2_kernel<float, false>), dim3(wave_grid), dim3(kBlock), 0, stream, db, c6, dc6, e_atom,
-                         dedcn, gout, sig);
-    else
-      hipLaunchKernelGGL((disp2_kernel<double, false>), dim3(wave_grid), dim3(kBlock), 0, stream, db, c6, dc6, e_atom,
-                         dedcn, gout, sig);
-  }
-  if (any_atm) {
-    long long* cnt = impl_->cnt.as<long long>(natoms);
-    long long* off = impl_->off.as<long long>(natoms + 1);
-    hipLaunchKernelGGL(atm_list_kernel<false>, dim3(wave_grid), dim3(kBlock), 0, stream, db, cnt,
-                       (const long long*)nullptr, (AtmEntry*)nullptr);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, cnt, natoms, off);
-    long long total = 0;
-    hip_check(hipMemcpyAsync(&total, off + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
-    hip_check(hipStreamSynchronize(stream), "list size sync");
+      hipLaunchKernelGGL((disp2_kernel<double, false>), dim3(wave_grid), blk, 0, stream, db, c6, dc6, e_atom, dedcn,
+                         gout, sig);
+  });
+  if (pb.any_atm) {
+    long long* off = nullptr;
+    AtmEntry* list = nullptr;
+    const long long total = m.build_atm_lists(pb, stream, &off, &list);
     if (stats) stats->atm_list_entries = total;
-    AtmEntry* list = impl_->list.as<AtmEntry>(total);
-    hipLaunchKernelGGL(atm_list_kernel<true>, dim3(wave_grid), dim3(kBlock), 0, stream, db, cnt,
-                       (const long long*)off, list);
-    if (grad) {
-      if (mixed)
-        hipLaunchKernelGGL((atm_kernel<float, true>), dim3(natoms), dim3(kBlock), 0, stream, db, off, list, c6, dc6,
-                           dc6t, e_atom, dedcn, gout, sig);
+    m.timed("atm", stream, [&] {
+      if (grad && mixed)
+        hipLaunchKernelGGL((atm_kernel<float, true>), dim3(natoms), blk, 0, stream, db, off, list, c6, dc6, dc6t,
+                           e_atom, dedcn, gout, sig);
+      else if (grad)
+        hipLaunchKernelGGL((atm_kernel<double, true>), dim3(natoms), blk, 0, stream, db, off, list, c6, dc6, dc6t,
+                           e_atom, dedcn, gout, sig);
+      else if (mixed)
+        hipLaunchKernelGGL((atm_kernel<float, false>), dim3(natoms), blk, 0, stream, db, off, list, c6, dc6, dc6t,
+                           e_atom, dedcn, gout, sig);
       else
-        hipLaunchKernelGGL((atm_kernel<double, true>), dim3(natoms), dim3(kBlock), 0, stream, db, off, list, c6, dc6,
-                           dc6t, e_atom, dedcn, gout, sig);
-    } else {
-      if (mixed)
-        hipLaunchKernelGGL((atm_kernel<float, false>), dim3(natoms), dim3(kBlock), 0, stream, db, off, list, c6, dc6,
-                           dc6t, e_atom, dedcn, gout, sig);
-      else
-        hipLaunchKernelGGL((atm_kernel<double, false>), dim3(natoms), dim3(kBlock), 0, stream, db, off, list, c6,
-                           dc6, dc6t, e_atom, dedcn, gout, sig);
-    }
+        hipLaunchKernelGGL((atm_kernel<double, false>), dim3(natoms), blk, 0, stream, db, off, list, c6, dc6, dc6t,
+                           e_atom, dedcn, gout, sig);
+    });
   }
   if (grad) {
-    if (mixed)
-      hipLaunchKernelGGL(cn_chain_kernel<float>, dim3(wave_grid), dim3(kBlock), 0, stream, db, dedcn, gout, sig);
-    else
-      hipLaunchKernelGGL(cn_chain_kernel<double>, dim3(wave_grid), dim3(kBlock), 0, stream, db, dedcn, gout, sig);
+    m.timed("cn_chain", stream, [&] {
+      if (mixed) hipLaunchKernelGGL(cn_chain_kernel<float>, dim3(wave_grid), blk, 0, stream, db, dedcn, gout, sig);
+      else hipLaunchKernelGGL(cn_chain_kernel<double>, dim3(wave_grid), blk, 0, stream, db, dedcn, gout, sig);
+    });
   }
-  hipLaunchKernelGGL(reduce_kernel, dim3(nsys), dim3(kBlock), 0, stream, db, e_atom, sig, out.energy, out.sigma);
+  m.timed("reduce", stream, [&] {
+    hipLaunchKernelGGL(reduce_kernel, dim3(pb.nsys), blk, 0, stream, db, e_atom, sig, out.energy,
+                       grad ? out.sigma : nullptr);
+  });
   hip_check(hipGetLastError(), "kernel launch");
+  if (m.timer.enabled) m.timer.collect();
+}
+
+void Engine::count_terms(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                         long long* counts_host, hipStream_t stream) {
+  if (systems.empty()) return;
+  Impl& m = *impl_;
+  std::vector<SystemDesc> sys = systems;
+  for (auto& s : sys)
+    if (std::fabs(s.param.s9) < kEps) s.param.s9 = 1.0;  // triples are counted whatever s9 is
+  const Prepared pb = m.prepare(sys, numbers, positions_dev, stream, nullptr);
+  unsigned long long* dc = m.counts.as<unsigned long long>(size_t(pb.nsys) * 3);
+  hip_check(hipMemsetAsync(dc, 0, sizeof(unsigned long long) * 3 * pb.nsys, stream), "hipMemsetAsync");
+  hipLaunchKernelGGL(count_pairs_kernel, dim3(blocks_for(pb.natoms, kWavesPerBlock)), dim3(kBlock), 0, stream,
+                     pb.db, dc);
+  long long* off = nullptr;
+  AtmEntry* list = nullptr;
+  m.build_atm_lists(pb, stream, &off, &list);
+  hipLaunchKernelGGL(count_triples_kernel, dim3(pb.natoms), dim3(kBlock), 0, stream, pb.db, off, list, dc);
+  hip_check(hipGetLastError(), "kernel launch");
+  std::vector<unsigned long long> h(3 * pb.nsys);
+  hip_check(hipMemcpyAsync(h.data(), dc, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, stream),
+            "download counts");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  for (size_t k = 0; k < h.size(); ++k) counts_host[k] = static_cast<long long>(h[k]);
 }
 
 }  // namespace d3

@@ -63,6 +63,18 @@ class Engine {
            hipStream_t stream, const BatchDebug* debug = nullptr, RunStats* stats = nullptr,
            const int* ghost_host = nullptr);
 
+  // Work units of the reference's loops per system: counts_host[3*s + 0..2]
+  // = CN terms, two-body pair terms, ATM triple terms (host array).
+  void count_terms(const std::vector<SystemDesc>& systems, const int* numbers_host,
+                   const double* positions_dev, long long* counts_host, hipStream_t stream);
+
+  // Per-kernel timing with HIP events on the launch stream (resets totals).
+  void set_kernel_timing(bool enable);
+  // Total milliseconds and number of launches of a kernel group since the
+  // timing was enabled: pack, cn, weights, c6, disp2, atm_list, atm,
+  // cn_chain, reduce.
+  std::pair<double, long long> kernel_time(const std::string& name) const;
+
  private:
   struct Impl;
   Impl* impl_;

@@ -55,6 +55,10 @@ _SIGNATURES = {
     "d3hip_new_workspace": (_vp, [_vp]),
     "d3hip_get_dispersion_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, ctypes.c_int,
                                           _vp, _vp, _vp, _vp, _vp]),
+    "d3hip_count_terms_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, _vp,
+                                       ctypes.POINTER(ctypes.c_longlong), _vp]),
+    "d3hip_set_kernel_timing": (None, [_vp, _vp, ctypes.c_int]),
+    "d3hip_get_kernel_time": (None, [_vp, _vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 for _name, (_res, _args) in _SIGNATURES.items():
@@ -208,6 +212,30 @@ def get_dispersion(mol: StructureHandle, disp: ModelHandle, param: ParamHandle, 
     """Evaluate the dispersion energy and its derivatives"""
     return error_check(lib.dftd3_get_dispersion)(mol.handle, disp.handle, param.handle, _cast(_dp, energy),
                                                  _cast(_dp, gradient), _cast(_dp, sigma))
+
+
+def count_terms_batch(workspace: ModelHandle, natoms, numbers, lattices, periodic, params, cutoffs,
+                      positions_ptr: int, stream: Optional[int]) -> np.ndarray:
+    """Work units per system: (nsys, 3) array of CN, pair and triple terms"""
+    counts = np.zeros((len(natoms), 3), dtype=np.int64)
+    error_check(lib.d3hip_count_terms_batch)(
+        workspace.handle, len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices),
+        _cast(_bp, periodic), _cast(_dp, params), _cast(_dp, cutoffs), positions_ptr,
+        counts.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), stream)
+    return counts
+
+
+def set_kernel_timing(disp: ModelHandle, enable: bool) -> None:
+    """Time kernel launches with HIP events (resets the totals)"""
+    return error_check(lib.d3hip_set_kernel_timing)(disp.handle, 1 if enable else 0)
+
+
+def get_kernel_time(disp: ModelHandle, name: str):
+    """(total milliseconds, launches) of a kernel group since timing was enabled"""
+    ms = ctypes.c_double()
+    calls = ctypes.c_longlong()
+    error_check(lib.d3hip_get_kernel_time)(disp.handle, name.encode(), ctypes.byref(ms), ctypes.byref(calls))
+    return ms.value, calls.value
 
 
 def get_dispersion_batch(workspace: ModelHandle, natoms, numbers, lattices, periodic, params, cutoffs,

@@ -37,21 +37,23 @@ def param_row(kind: str, param: dict) -> List[float]:
             param.get("alp", 14.0)]
 
 
-def get_dispersion_batch(systems: Sequence[dict], precision: str = "fp64", grad: bool = True,
-                         device: Optional[torch.device] = None) -> dict:
-    """Evaluate a batch of systems.
+class BatchSpec:
+    """Host metadata of a batch (everything except the positions)."""
 
-    Each system is a dict with ``numbers`` (n,), ``positions`` (n, 3) in bohr
-    (numpy array or tensor), optional ``lattice`` (3, 3) and ``periodic``
-    (3,), ``kind`` ('rational' or 'zero'), ``param`` (dict of damping
-    parameters) and optional ``cutoff`` (dict with disp2, disp3, cn,
-    width2, width3).  Positions are used as given (not wrapped).
+    def __init__(self, natoms, numbers, lattices, periodic, params, cutoffs):
+        self.natoms = np.ascontiguousarray(natoms, dtype=np.int32)
+        self.numbers = np.ascontiguousarray(numbers, dtype=np.int32)
+        self.lattices = None if lattices is None else np.ascontiguousarray(lattices, dtype=float)
+        self.periodic = None if periodic is None else np.ascontiguousarray(periodic, dtype=bool)
+        self.params = np.ascontiguousarray(params, dtype=float)
+        self.cutoffs = None if cutoffs is None else np.ascontiguousarray(cutoffs, dtype=float)
 
-    Returns ``energy`` (nsys,), and with ``grad`` also ``gradient`` (N, 3)
-    over the concatenated atoms and ``sigma`` (nsys, 3, 3) laid out like the
-    reference's ``virial``.
-    """
-    dev = torch.device(device or "cuda")
+    def __len__(self):
+        return len(self.natoms)
+
+
+def batch_spec(systems: Sequence[dict]):
+    """BatchSpec and the concatenated host positions of a list of systems."""
     nsys = len(systems)
     natoms = np.array([len(s["numbers"]) for s in systems], dtype=np.int32)
     numbers = np.ascontiguousarray(np.concatenate([np.asarray(s["numbers"], dtype=np.int32) for s in systems]))
@@ -74,39 +76,66 @@ def get_dispersion_batch(systems: Sequence[dict], precision: str = "fp64", grad:
     pos = []
     for s in systems:
         p = s["positions"]
-        p = p if isinstance(p, torch.Tensor) else torch.as_tensor(np.asarray(p, dtype=float))
-        pos.append(p.to(device=dev, dtype=torch.float64).reshape(-1, 3))
-    positions = torch.cat(pos).contiguous()
-    return get_dispersion_batch_tensors(natoms, numbers, lattices, periodic, params, cutoffs, positions,
-                                        precision=precision, grad=grad)
+        p = p.detach().cpu().numpy() if isinstance(p, torch.Tensor) else np.asarray(p, dtype=float)
+        pos.append(np.asarray(p, dtype=float).reshape(-1, 3))
+    return BatchSpec(natoms, numbers, lattices, periodic, params, cutoffs), np.concatenate(pos)
 
 
-def get_dispersion_batch_tensors(natoms: np.ndarray, numbers: np.ndarray, lattices: Optional[np.ndarray],
-                                 periodic: Optional[np.ndarray], params: np.ndarray, cutoffs: Optional[np.ndarray],
-                                 positions: torch.Tensor, precision: str = "fp64", grad: bool = True,
-                                 out: Optional[dict] = None) -> dict:
-    """Low-level batched call on prepared host metadata and a device tensor
-    of concatenated positions (float64, contiguous).  ``out`` may supply
-    preallocated result tensors (energy, gradient, sigma)."""
+def get_dispersion_batch(systems: Sequence[dict], precision: str = "fp64", grad: bool = True,
+                         device: Optional[torch.device] = None) -> dict:
+    """Evaluate a batch of systems.
+
+    Each system is a dict with ``numbers`` (n,), ``positions`` (n, 3) in bohr
+    (numpy array or tensor), optional ``lattice`` (3, 3) and ``periodic``
+    (3,), ``kind`` ('rational' or 'zero'), ``param`` (dict of damping
+    parameters) and optional ``cutoff`` (dict with disp2, disp3, cn,
+    width2, width3).  Positions are used as given (not wrapped).
+
+    Returns ``energy`` (nsys,), and with ``grad`` also ``gradient`` (N, 3)
+    over the concatenated atoms and ``sigma`` (nsys, 3, 3) laid out like the
+    reference's ``virial``.
+    """
+    dev = torch.device(device or "cuda")
+    spec, pos = batch_spec(systems)
+    positions = torch.as_tensor(pos, dtype=torch.float64, device=dev).contiguous()
+    return run_batch(spec, positions, precision=precision, grad=grad)
+
+
+def _check_positions(positions: torch.Tensor):
     if positions.dtype != torch.float64 or not positions.is_contiguous() or not positions.is_cuda:
         raise ValueError("positions must be a contiguous float64 tensor on a GPU")
-    nsys = len(natoms)
+
+
+def run_batch(spec: BatchSpec, positions: torch.Tensor, precision: str = "fp64", grad: bool = True,
+              out: Optional[dict] = None, ws: Optional[library.ModelHandle] = None) -> dict:
+    """Batched call on prepared host metadata and a device tensor of the
+    concatenated positions (float64, contiguous, bohr).  ``out`` may supply
+    preallocated result tensors (energy, gradient, sigma); the call runs on
+    the current stream."""
+    _check_positions(positions)
+    nsys = len(spec)
     dev = positions.device
     prec = {"fp64": library.PRECISION_FP64, "mixed": library.PRECISION_MIXED}[precision]
     if out is None:
         out = dict(energy=torch.empty(nsys, dtype=torch.float64, device=dev))
         if grad:
-            out["gradient"] = torch.empty(int(np.sum(natoms)), 3, dtype=torch.float64, device=dev)
+            out["gradient"] = torch.empty(int(np.sum(spec.natoms)), 3, dtype=torch.float64, device=dev)
             out["sigma"] = torch.empty(nsys, 3, 3, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
     with torch.cuda.device(dev):
         library.get_dispersion_batch(
-            workspace(dev), np.ascontiguousarray(natoms, dtype=np.int32),
-            np.ascontiguousarray(numbers, dtype=np.int32),
-            None if lattices is None else np.ascontiguousarray(lattices, dtype=float),
-            None if periodic is None else np.ascontiguousarray(periodic, dtype=bool),
-            np.ascontiguousarray(params, dtype=float),
-            None if cutoffs is None else np.ascontiguousarray(cutoffs, dtype=float),
+            ws or workspace(dev), spec.natoms, spec.numbers, spec.lattices, spec.periodic, spec.params, spec.cutoffs,
             prec, positions.data_ptr(), out["energy"].data_ptr(),
             out["gradient"].data_ptr() if grad else None, out["sigma"].data_ptr() if grad else None, stream)
     return out
+
+
+def count_terms(spec: BatchSpec, positions: torch.Tensor) -> np.ndarray:
+    """(nsys, 3) int64 array of CN terms, two-body pair terms and ATM triple
+    terms as the reference's loops visit them (counted on the GPU)."""
+    _check_positions(positions)
+    dev = positions.device
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with torch.cuda.device(dev):
+        return library.count_terms_batch(workspace(dev), spec.natoms, spec.numbers, spec.lattices, spec.periodic,
+                                         spec.params, spec.cutoffs, positions.data_ptr(), stream)

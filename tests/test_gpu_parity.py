@@ -203,6 +203,22 @@ def test_periodic_2d_slab():
     _check(r, _oracle(num, pos, lat, [True, True, False], "rational", BJ), 10)
 
 
+def test_term_counts_match_oracle_enumeration():
+    """GPU work-unit counts (used for pair-/triple-terms/s) equal the
+    oracle's enumeration of the reference's loops."""
+    import torch
+    from dftd3 import synthetic, torch_api
+    _step("term counts")
+    num0, pos0 = synthetic.molecule_c0(seed=13, n=70)
+    num1, pos1, lat1 = _cell(23, 14, 9.5)
+    systems = [dict(numbers=num0, positions=pos0, param=BJ),
+               dict(numbers=num1, positions=pos1, lattice=lat1, periodic=[True] * 3, param=BJ)]
+    spec, pos = torch_api.batch_spec(systems)
+    counts = torch_api.count_terms(spec, torch.as_tensor(pos, device="cuda"))
+    assert tuple(counts[0]) == O.count_terms(pos0)
+    assert tuple(counts[1]) == O.count_terms(pos1, lat1, [True] * 3)
+
+
 def test_batch_mixed_systems():
     _step("batch of 6 mixed systems")
     from dftd3 import synthetic
