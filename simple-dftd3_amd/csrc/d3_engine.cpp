@@ -1,0 +1,445 @@
+// Host side of the batch engine: plans a batch (species tables, lattice
+// images, C6 block offsets), owns the device work buffers and launches the
+// kernels of d3_kernels.hip on one stream.
+//
+// Order of one evaluation (disp.f90:56-185):
+//   pack -> cn -> weights -> pair (C6 and derivatives) -> disp2 ->
+//   atm_list (count, scan, fill) -> atm -> cn_chain -> reduce
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "d3_engine.hpp"
+#include "d3_kernels.hpp"
+
+namespace d3 {
+
+void hip_check(hipError_t err, const char* what) {
+  if (err != hipSuccess) throw Error(std::string("HIP error in ") + what + ": " + hipGetErrorString(err));
+}
+
+namespace {
+
+constexpr double kEps = 2.220446049250313e-16;
+
+// Grow-only device allocation.
+class DeviceBuffer {
+ public:
+  ~DeviceBuffer() {
+    if (ptr_) (void)hipFree(ptr_);
+  }
+  void* get(size_t bytes) {
+    if (bytes > cap_) {
+      if (ptr_) hip_check(hipFree(ptr_), "hipFree");
+      ptr_ = nullptr;
+      cap_ = 0;
+      const size_t want = std::max<size_t>(bytes + bytes / 4, 256);
+      hip_check(hipMalloc(&ptr_, want), "hipMalloc");
+      cap_ = want;
+    }
+    return ptr_;
+  }
+  template <typename T>
+  T* as(size_t count) {
+    return static_cast<T*>(get(std::max<size_t>(count, 1) * sizeof(T)));
+  }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t cap_ = 0;
+};
+
+// Pinned host buffer for the metadata upload; an event recorded after the copy
+// guards its reuse by the next call.
+class PinnedStaging {
+ public:
+  ~PinnedStaging() {
+    if (event_) {
+      (void)hipEventSynchronize(event_);
+      (void)hipEventDestroy(event_);
+    }
+    if (ptr_) (void)hipHostFree(ptr_);
+  }
+  char* get(size_t bytes) {
+    if (event_) hip_check(hipEventSynchronize(event_), "staging reuse");
+    if (bytes > cap_) {
+      if (ptr_) hip_check(hipHostFree(ptr_), "hipHostFree");
+      ptr_ = nullptr;
+      cap_ = 0;
+      const size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+      hip_check(hipHostMalloc(reinterpret_cast<void**>(&ptr_), want, hipHostMallocDefault), "hipHostMalloc");
+      cap_ = want;
+    }
+    return ptr_;
+  }
+  void record(hipStream_t stream) {
+    if (!event_) hip_check(hipEventCreateWithFlags(&event_, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(event_, stream), "hipEventRecord");
+  }
+
+ private:
+  char* ptr_ = nullptr;
+  size_t cap_ = 0;
+  hipEvent_t event_ = nullptr;
+};
+
+// Optional per-kernel timing with HIP events recorded on the launch stream.
+class KernelTimer {
+ public:
+  ~KernelTimer() { clear(); }
+  bool enabled = false;
+  void begin(hipStream_t st) {
+    if (!enabled) return;
+    hip_check(hipEventCreate(&open_), "hipEventCreate");
+    hip_check(hipEventRecord(open_, st), "hipEventRecord");
+  }
+  void end(const char* name, hipStream_t st) {
+    if (!enabled) return;
+    hipEvent_t b;
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    hip_check(hipEventRecord(b, st), "hipEventRecord");
+    pending_.push_back({name, open_, b});
+    open_ = nullptr;
+  }
+  void collect() {
+    for (auto& p : pending_) {
+      hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
+      float ms = 0.0f;
+      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      auto& t = totals_[p.name];
+      t.first += ms;
+      t.second += 1;
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending_.clear();
+  }
+  void clear() {
+    for (auto& p : pending_) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending_.clear();
+    totals_.clear();
+  }
+  std::pair<double, long long> get(const std::string& name) const {
+    auto it = totals_.find(name);
+    return it == totals_.end() ? std::make_pair(0.0, 0LL) : it->second;
+  }
+
+ private:
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  hipEvent_t open_ = nullptr;
+  std::vector<Rec> pending_;
+  std::map<std::string, std::pair<double, long long>> totals_;
+};
+
+struct Prepared {
+  DevBatch db{};
+  long long pair_total = 0;  // sum over systems of n^2
+  bool any_atm = false;
+};
+
+template <typename T>
+size_t stage(std::vector<char>& buf, const std::vector<T>& v) {
+  const size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + v.size() * sizeof(T));
+  if (!v.empty()) std::memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+  return off;
+}
+
+}  // namespace
+
+// cutoff.f90:140-227 get_lattice_points (host, FP64, same operation order).
+std::vector<double> lattice_points(const int periodic[3], const double lattice[9], double rthr) {
+  const int np = periodic[0] + periodic[1] + periodic[2];
+  if (np == 0) return {0.0, 0.0, 0.0};
+  double lat[3][3];
+  std::memcpy(lat, lattice, sizeof lat);
+  auto cross = [](const double* a, const double* c, double* out) {
+    out[0] = a[1] * c[2] - c[1] * a[2];
+    out[1] = a[2] * c[0] - c[2] * a[0];
+    out[2] = a[0] * c[1] - c[0] * a[1];
+  };
+  auto translations = [&](double l[3][3], int rep[3]) {
+    double nrm[3][3];
+    cross(l[1], l[2], nrm[0]);
+    cross(l[2], l[0], nrm[1]);
+    cross(l[0], l[1], nrm[2]);
+    for (int k = 0; k < 3; ++k) {
+      const double len = std::sqrt(nrm[k][0] * nrm[k][0] + nrm[k][1] * nrm[k][1] + nrm[k][2] * nrm[k][2]);
+      for (int c = 0; c < 3; ++c) nrm[k][c] = nrm[k][c] / len;
+    }
+    for (int k = 0; k < 3; ++k) {
+      const double cs = nrm[k][0] * l[k][0] + nrm[k][1] * l[k][1] + nrm[k][2] * l[k][2];
+      rep[k] = static_cast<int>(std::ceil(std::fabs(rthr / cs)));
+    }
+  };
+  int rep[3] = {0, 0, 0};
+  if (np == 3) {
+    translations(lat, rep);
+  } else if (np == 1) {
+    for (int k = 0; k < 3; ++k) {
+      const double len = std::sqrt(lat[k][0] * lat[k][0] + lat[k][1] * lat[k][1] + lat[k][2] * lat[k][2]);
+      rep[k] = periodic[k] ? static_cast<int>(std::ceil(rthr / len)) : 0;
+    }
+  } else {
+    double nrm[3][3];
+    for (int k = 0; k < 3; ++k) cross(lat[(k + 1) % 3], lat[(k + 2) % 3], nrm[k]);
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 3; ++c)
+        if (periodic[c]) nrm[k][c] = lat[k][c];  // where(spread(periodic, 2, 3)) norms = lat
+    translations(nrm, rep);
+    for (int k = 0; k < 3; ++k)
+      if (!periodic[k]) rep[k] = 0;
+  }
+  std::vector<double> out;
+  out.reserve(3 * (2 * rep[0] + 1) * (2 * rep[1] + 1) * (2 * rep[2] + 1));
+  for (int ix = -rep[0]; ix <= rep[0]; ++ix)
+    for (int iy = -rep[1]; iy <= rep[1]; ++iy)
+      for (int iz = -rep[2]; iz <= rep[2]; ++iz)
+        for (int c = 0; c < 3; ++c) out.push_back(lat[0][c] * ix + lat[1][c] * iy + lat[2][c] * iz);
+  return out;
+}
+
+struct Engine::Impl {
+  DeviceBuffer meta, pos, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items, item_off, slots, counts;
+  std::vector<char> staging;
+  PinnedStaging pinned;
+  KernelTimer timer;
+
+  Prepared prepare(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                   hipStream_t stream, const int* ghost_host);
+  // Three-body lists and work items; returns the launch size of the ATM kernel.
+  long long build_atm(const Prepared& pb, const void* pair, bool mixed, hipStream_t stream, long long** offset,
+                      long long** item_off, void** slots, long long* nslots);
+
+  template <typename F>
+  void timed(const char* name, hipStream_t st, F&& launch) {
+    timer.begin(st);
+    launch();
+    timer.end(name, st);
+  }
+};
+
+Engine::Engine() : impl_(new Impl) {}
+Engine::~Engine() { delete impl_; }
+
+void Engine::set_kernel_timing(bool enable) {
+  impl_->timer.clear();
+  impl_->timer.enabled = enable;
+}
+
+std::pair<double, long long> Engine::kernel_time(const std::string& name) const { return impl_->timer.get(name); }
+
+// Host planning of a batch: species of the batch, per-species tables, lattice
+// images per system and cutoff, pair-block offsets; one metadata upload and
+// the packing of the positions.
+Prepared Engine::Impl::prepare(const std::vector<SystemDesc>& systems, const int* numbers,
+                               const double* positions_dev, hipStream_t stream, const int* ghost_host) {
+  Prepared pb;
+  const int nsys = static_cast<int>(systems.size());
+  int natoms = 0;
+  for (const auto& s : systems) {
+    if (s.natoms <= 0) throw Error("Invalid molecular structure data provided");
+    natoms += s.natoms;
+  }
+  const auto& tab = element_tables();
+  std::map<int, int> zmap;
+  std::vector<int> zlist;
+  std::vector<int> sp(natoms), sysidx(natoms);
+  for (int k = 0, a = 0; k < nsys; ++k)
+    for (int q = 0; q < systems[k].natoms; ++q, ++a) {
+      const int z = numbers[a];
+      if (z < 1 || z > kMaxElem) throw Error("Unsupported element with atomic number " + std::to_string(z));
+      auto it = zmap.find(z);
+      if (it == zmap.end()) {
+        it = zmap.emplace(z, static_cast<int>(zlist.size())).first;
+        zlist.push_back(z);
+      }
+      sp[a] = it->second;
+      sysidx[a] = k;
+    }
+  const int nsp = static_cast<int>(zlist.size());
+  if (nsp > kMaxSpecies) throw Error("Too many distinct elements in one batch");
+  int mref = 0;
+  std::vector<double> rcov(nsp), r4r2(nsp), refcn(nsp * kMaxRef, -1.0),
+      c6ref(size_t(nsp) * nsp * kMaxRef * kMaxRef, 0.0), rvdw(nsp * nsp);
+  std::vector<int> nref(nsp);
+  for (int a = 0; a < nsp; ++a) {
+    const int z = zlist[a];
+    rcov[a] = covalent_radius(z);
+    r4r2[a] = r4r2_value(z);
+    nref[a] = tab.nref[z - 1];
+    mref = std::max(mref, nref[a]);
+    for (int r = 0; r < nref[a]; ++r) refcn[a * kMaxRef + r] = tab.refcn[z - 1][r];
+    for (int c = 0; c < nsp; ++c) {
+      const int zc = zlist[c];
+      rvdw[a * nsp + c] = vdw_radius(zc, z);
+      for (int ri = 0; ri < nref[a]; ++ri)
+        for (int rj = 0; rj < tab.nref[zc - 1]; ++rj)
+          c6ref[(size_t(a) * nsp + c) * kMaxRef * kMaxRef + ri * kMaxRef + rj] = reference_c6(ri, rj, z, zc);
+    }
+  }
+  std::vector<SysGeom> geom(nsys);
+  std::vector<SysParam> par(nsys);
+  std::vector<double4> img;
+  long long pair_total = 0;
+  bool any_atm = false;
+  for (int k = 0, a = 0; k < nsys; ++k) {
+    const SystemDesc& sd = systems[k];
+    SysGeom& g = geom[k];
+    g.atom_begin = a;
+    g.natoms = sd.natoms;
+    a += sd.natoms;
+    par[k] = sd.param;
+    auto add = [&](double cut, int& begin, int& count) {
+      const std::vector<double> tr = lattice_points(sd.periodic, sd.lattice, cut);
+      begin = static_cast<int>(img.size());
+      count = static_cast<int>(tr.size() / 3);
+      for (int t = 0; t < count; ++t) img.push_back(make_double4(tr[3 * t], tr[3 * t + 1], tr[3 * t + 2], 0.0));
+    };
+    add(sd.param.cut_cn, g.img_cn_begin, g.img_cn_count);
+    add(sd.param.cut2, g.img2_begin, g.img2_count);
+    add(sd.param.cut3, g.img3_begin, g.img3_count);
+    g.c6_begin = pair_total;
+    pair_total += static_cast<long long>(sd.natoms) * sd.natoms;
+    if (std::fabs(sd.param.s9) >= kEps) any_atm = true;
+  }
+  std::vector<char>& sb = staging;
+  sb.clear();
+  std::vector<int> ghost;
+  if (ghost_host) ghost.assign(ghost_host, ghost_host + natoms);
+  const size_t o_ghost = stage(sb, ghost), o_sp = stage(sb, sp), o_sys = stage(sb, sysidx), o_geom = stage(sb, geom),
+               o_par = stage(sb, par), o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2),
+               o_nref = stage(sb, nref), o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref),
+               o_rvdw = stage(sb, rvdw);
+  char* dmeta = static_cast<char*>(meta.get(sb.size()));
+  char* host = pinned.get(sb.size());
+  std::memcpy(host, sb.data(), sb.size());
+  hip_check(hipMemcpyAsync(dmeta, host, sb.size(), hipMemcpyHostToDevice, stream), "upload");
+  pinned.record(stream);
+
+  DevBatch& db = pb.db;
+  db.natoms = natoms;
+  db.nsys = nsys;
+  db.nsp = nsp;
+  db.mref = mref;
+  db.pos = pos.as<double4>(natoms);
+  db.sp = reinterpret_cast<const int*>(dmeta + o_sp);
+  db.sys = reinterpret_cast<const int*>(dmeta + o_sys);
+  db.geom = reinterpret_cast<const SysGeom*>(dmeta + o_geom);
+  db.par = reinterpret_cast<const SysParam*>(dmeta + o_par);
+  db.img = reinterpret_cast<const double4*>(dmeta + o_img);
+  db.rcov = reinterpret_cast<const double*>(dmeta + o_rcov);
+  db.r4r2 = reinterpret_cast<const double*>(dmeta + o_r4);
+  db.nref = reinterpret_cast<const int*>(dmeta + o_nref);
+  db.refcn = reinterpret_cast<const double*>(dmeta + o_refcn);
+  db.c6ref = reinterpret_cast<const double*>(dmeta + o_c6r);
+  db.rvdw = reinterpret_cast<const double*>(dmeta + o_rvdw);
+  db.ghost = ghost_host ? reinterpret_cast<const int*>(dmeta + o_ghost) : nullptr;
+  pb.pair_total = pair_total;
+  pb.any_atm = any_atm;
+  timed("pack", stream, [&] { launch_pack(positions_dev, natoms, const_cast<double4*>(db.pos), stream); });
+  return pb;
+}
+
+long long Engine::Impl::build_atm(const Prepared& pb, const void* pair_records, bool mixed, hipStream_t stream,
+                                  long long** offset_out, long long** item_off_out, void** slots_out,
+                                  long long* nslots) {
+  const int natoms = pb.db.natoms;
+  long long* c = cnt.as<long long>(natoms);
+  long long* o = off.as<long long>(natoms + 1);
+  long long* it = items.as<long long>(natoms);
+  long long* io = item_off.as<long long>(natoms + 1);
+  timed("atm_list", stream, [&] {
+    launch_atm_count(pb.db, pair_records, mixed, c, stream);
+    launch_scan(c, natoms, o, stream);
+    launch_atm_items(o, natoms, it, stream);
+    launch_scan(it, natoms, io, stream);
+  });
+  long long total = 0;
+  hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
+  hip_check(hipStreamSynchronize(stream), "list size sync");
+  void* s = slots.get(std::max<long long>(total, 1) * slot_record_bytes(mixed));
+  timed("atm_list", stream, [&] { launch_atm_fill(pb.db, pair_records, mixed, o, s, stream); });
+  *offset_out = o;
+  *item_off_out = io;
+  *slots_out = s;
+  *nslots = total;
+  return total / kTile + natoms;  // upper bound of the number of work items
+}
+
+void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                 int precision, const BatchOutput& out, hipStream_t stream, const BatchDebug* debug,
+                 RunStats* stats, const int* ghost_host) {
+  if (systems.empty()) return;
+  Impl& m = *impl_;
+  const Prepared pb = m.prepare(systems, numbers, positions_dev, stream, ghost_host);
+  const DevBatch& db = pb.db;
+  const int natoms = db.natoms;
+  const bool grad = out.gradient || out.sigma;
+  const bool mixed = precision == kMixed;
+  DevWork w{};
+  w.cn = debug && debug->cn ? debug->cn : m.cn.as<double>(natoms);
+  w.gw = m.gw.as<double>(size_t(natoms) * db.mref);
+  w.dgw = grad ? m.dgw.as<double>(size_t(natoms) * db.mref) : nullptr;
+  w.pair = m.pair.get(std::max<long long>(pb.pair_total, 1) * pair_record_bytes(mixed));
+  w.e_atom = m.e_atom.as<double>(natoms);
+  w.dedcn = debug && debug->dedcn ? debug->dedcn : m.dedcn.as<double>(natoms);
+  w.grad = out.gradient ? out.gradient : m.grad.as<double>(size_t(natoms) * 3);
+  w.sig_atom = m.sig.as<double>(size_t(natoms) * 6);
+
+  m.timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
+  m.timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
+  m.timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
+  m.timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, stream); });
+  if (pb.any_atm) {
+    long long *offset = nullptr, *item_off = nullptr, nslots = 0;
+    void* slots = nullptr;
+    const long long max_items = m.build_atm(pb, w.pair, mixed, stream, &offset, &item_off, &slots, &nslots);
+    if (stats) stats->atm_list_entries = nslots;
+    bool smooth = false;
+    for (const auto& s : systems) smooth |= s.param.width3 > 0.0;
+    m.timed("atm", stream,
+            [&] { launch_atm(db, mixed, grad, smooth, item_off, max_items, offset, slots, w, stream); });
+  }
+  if (grad) m.timed("cn_chain", stream, [&] { launch_cn_chain(db, mixed, w, stream); });
+  m.timed("reduce", stream, [&] { launch_reduce(db, w, out.energy, grad ? out.sigma : nullptr, stream); });
+  hip_check(hipGetLastError(), "kernel launch");
+  if (m.timer.enabled) m.timer.collect();
+}
+
+void Engine::count_terms(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                         long long* counts_host, hipStream_t stream) {
+  if (systems.empty()) return;
+  Impl& m = *impl_;
+  std::vector<SystemDesc> sys = systems;
+  for (auto& s : sys)
+    if (std::fabs(s.param.s9) < kEps) s.param.s9 = 1.0;  // triples are counted whatever s9 is
+  const Prepared pb = m.prepare(sys, numbers, positions_dev, stream, nullptr);
+  unsigned long long* dc = m.counts.as<unsigned long long>(size_t(pb.db.nsys) * 3);
+  hip_check(hipMemsetAsync(dc, 0, sizeof(unsigned long long) * 3 * pb.db.nsys, stream), "hipMemsetAsync");
+  launch_count_pairs(pb.db, dc, stream);
+  long long *offset = nullptr, *item_off = nullptr, nslots = 0;
+  void* slots = nullptr;
+  // lists without the C6 test: every pair within the cutoff is a slot
+  m.build_atm(pb, nullptr, false, stream, &offset, &item_off, &slots, &nslots);
+  launch_count_triples(pb.db, false, offset, slots, dc, stream);
+  hip_check(hipGetLastError(), "kernel launch");
+  std::vector<unsigned long long> h(3 * pb.db.nsys);
+  hip_check(hipMemcpyAsync(h.data(), dc, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, stream),
+            "download counts");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  for (size_t k = 0; k < h.size(); ++k) counts_host[k] = static_cast<long long>(h[k]);
+}
+
+}  // namespace d3

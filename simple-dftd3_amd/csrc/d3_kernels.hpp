@@ -1,0 +1,73 @@
+// Device data of a batch and the launch wrappers of the D3 kernels
+// (implemented in d3_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "d3_common.hpp"
+
+namespace d3 {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kTile = 64;  // slots per three-body tile
+
+// Device view of one prepared batch (pointers into device memory).
+struct DevBatch {
+  int natoms, nsys, nsp, mref;
+  const double4* pos;   // [N] (x, y, z, 0)
+  const int* sp;        // [N] species index
+  const int* sys;       // [N] system index
+  const SysGeom* geom;  // [nsys]
+  const SysParam* par;  // [nsys]
+  const double4* img;   // lattice translations of all systems and cutoffs
+  const double* rcov;   // [nsp]
+  const double* r4r2;   // [nsp]
+  const int* nref;      // [nsp]
+  const double* refcn;  // [nsp][kMaxRef]
+  const double* c6ref;  // [nsp*nsp][kMaxRef*kMaxRef], [(si*nsp+sj)][ri*7+rj]
+  const double* rvdw;   // [nsp*nsp]
+  const int* ghost;     // [N] or null
+};
+
+// Per-evaluation device arrays.
+struct DevWork {
+  double* cn;       // [N]
+  double* gw;       // [N][mref]
+  double* dgw;      // [N][mref] or null (energy only)
+  void* pair;       // per system n x n block of R4 {sqrt C6, dC6/dCN_i / C6, dC6/dCN_j / C6, r0^(alp/3)}
+  double* e_atom;   // [N]
+  double* dedcn;    // [N]
+  double* grad;     // [N][3]
+  double* sig_atom; // [N][6]
+};
+
+// Size in bytes of one pair record and one three-body slot.
+size_t pair_record_bytes(bool mixed);
+size_t slot_record_bytes(bool mixed);
+
+// Launchers (all asynchronous on `stream`).
+void launch_pack(const double* xyz, int n, double4* pos, hipStream_t stream);
+void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t stream);
+void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw, hipStream_t stream);
+void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, void* pair, hipStream_t stream);
+void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, hipStream_t stream);
+// Three-body lists: count per atom (pair may be null: no C6 test), fill slots.
+void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, hipStream_t stream);
+void launch_scan(const long long* count, int n, long long* offset, hipStream_t stream);
+void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long long* offset, void* slots,
+                     hipStream_t stream);
+// Work items (atom i, slot block P) of the tiled three-body kernel.
+void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t stream);
+// smooth: some system has a nonzero three-body switching width.
+void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const long long* item_off,
+                long long max_items, const long long* offset, const void* slots, const DevWork& w,
+                hipStream_t stream);
+void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t stream);
+void launch_reduce(const DevBatch& b, const DevWork& w, double* energy, double* sigma, hipStream_t stream);
+void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream_t stream);
+void launch_count_triples(const DevBatch& b, bool mixed, const long long* offset, const void* slots,
+                          unsigned long long* counts, hipStream_t stream);
+
+}  // namespace d3
