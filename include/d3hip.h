@@ -144,4 +144,57 @@ D3HIP_API void d3hip_set_kernel_timing(dftd3_error error, dftd3_model model, int
 D3HIP_API void d3hip_get_kernel_time(dftd3_error error, dftd3_model model, const char* name, double* total_ms,
                                      long long* calls);
 
+/* ------------------------------------------------------------------ plans
+ * A batch planned once and evaluated many times with new coordinates of the
+ * same systems (no counterpart in the reference).  The metadata upload,
+ * buffer sizes and three-body launch shape are fixed by the plan; an
+ * evaluation is a fixed sequence of kernel launches on the stream without
+ * host synchronisation or device allocation, and can be recorded into a HIP
+ * graph (d3hip_plan_capture) and replayed (d3hip_plan_replay).
+ * Three-body lists: molecules get the exact bound; periodic systems get 1.5x
+ * the list size at the planning coordinates (at least 64 more slots per
+ * atom).  An evaluation whose lists do not fit adds no three-body terms and
+ * sets the flag that d3hip_plan_overflowed reports. */
+typedef struct _d3hip_plan* d3hip_plan;
+
+/* Same batch description as d3hip_get_dispersion_batch; grad != 0 plans
+ * energy + gradient + virial, else the energy only; positions (device) are
+ * the planning coordinates.  Synchronizes the stream. */
+D3HIP_API d3hip_plan d3hip_new_plan(dftd3_error error, int nsys, const int* natoms, const int* numbers,
+                                    const double* lattices, const bool* periodic, const double* params,
+                                    const double* cutoffs, int precision, int grad, const double* positions,
+                                    void* stream);
+D3HIP_API void d3hip_delete_plan(d3hip_plan* plan);
+
+/* One evaluation; device outputs as in d3hip_get_dispersion_batch (gradient
+ * and sigma must be given exactly when the plan has grad != 0). */
+D3HIP_API void d3hip_plan_execute(dftd3_error error, d3hip_plan plan, const double* positions, double* energy,
+                                  double* gradient, double* sigma, void* stream);
+
+/* Candidate launch shapes of the three-body kernel: wavefronts per block
+ * (2 or 4) and persistent blocks per compute unit (0 = one block per work
+ * item). */
+D3HIP_API int d3hip_plan_num_shapes(void);
+D3HIP_API void d3hip_plan_candidate_shape(int index, int* waves, int* blocks_per_cu);
+D3HIP_API void d3hip_plan_get_shape(dftd3_error error, d3hip_plan plan, int* waves, int* blocks_per_cu);
+D3HIP_API void d3hip_plan_set_shape(dftd3_error error, d3hip_plan plan, int waves, int blocks_per_cu);
+
+/* Time every candidate shape on this batch (one warm-up and reps timed
+ * evaluations each) and keep the fastest; returns its index, -1 on error.
+ * times_ms[d3hip_plan_num_shapes()] (host, may be NULL) receives the mean
+ * milliseconds per evaluation of each.  Synchronizes the stream. */
+D3HIP_API int d3hip_plan_autotune(dftd3_error error, d3hip_plan plan, const double* positions, double* energy,
+                                  double* gradient, double* sigma, int reps, float* times_ms, void* stream);
+
+/* Record one evaluation with these device pointers into a HIP graph (the
+ * stream must not be capturing already); replay launches it on a stream.
+ * New coordinates go into the same positions buffer before a replay. */
+D3HIP_API void d3hip_plan_capture(dftd3_error error, d3hip_plan plan, const double* positions, double* energy,
+                                  double* gradient, double* sigma, void* stream);
+D3HIP_API void d3hip_plan_replay(dftd3_error error, d3hip_plan plan, void* stream);
+
+/* 1 when an evaluation since the plan was made had more three-body slots than
+ * the plan holds, 0 otherwise, -1 on error.  Synchronizes the stream. */
+D3HIP_API int d3hip_plan_overflowed(dftd3_error error, d3hip_plan plan, void* stream);
+
 #endif /* D3HIP_H */

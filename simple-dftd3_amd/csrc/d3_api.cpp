@@ -499,3 +499,127 @@ void d3hip_get_kernel_time(dftd3_error error, dftd3_model model, const char* nam
   if (total_ms) *total_ms = t.first;
   if (calls) *calls = t.second;
 }
+
+// ------------------------------------------------------------------ plans
+struct _d3hip_plan {
+  std::unique_ptr<d3::Plan> plan;
+};
+
+namespace {
+d3::BatchOutput outputs(double* energy, double* gradient, double* sigma) {
+  d3::BatchOutput out;
+  out.energy = energy;
+  out.gradient = gradient;
+  out.sigma = sigma;
+  return out;
+}
+}  // namespace
+
+d3hip_plan d3hip_new_plan(dftd3_error error, int nsys, const int* natoms, const int* numbers,
+                          const double* lattices, const bool* periodic, const double* params, const double* cutoffs,
+                          int precision, int grad, const double* positions, void* stream) {
+  if (!error) return nullptr;
+  if (nsys <= 0 || !natoms || !numbers || !params || !positions) {
+    fatal(error, "Invalid batch description");
+    return nullptr;
+  }
+  if (precision != d3::kFP64 && precision != d3::kMixed) {
+    fatal(error, "Unknown precision mode");
+    return nullptr;
+  }
+  d3hip_plan h = nullptr;
+  guarded(error, [&] {
+    std::vector<d3::SystemDesc> sys = batch_systems(nsys, natoms, lattices, periodic, params, cutoffs);
+    std::unique_ptr<_d3hip_plan> p(new _d3hip_plan);
+    p->plan.reset(new d3::Plan(sys, numbers, positions, precision, grad != 0, nullptr,
+                               static_cast<hipStream_t>(stream)));
+    h = p.release();
+  });
+  return h;
+}
+
+void d3hip_delete_plan(d3hip_plan* plan) {
+  if (!plan || !*plan) return;
+  delete *plan;
+  *plan = nullptr;
+}
+
+void d3hip_plan_execute(dftd3_error error, d3hip_plan plan, const double* positions, double* energy,
+                        double* gradient, double* sigma, void* stream) {
+  if (!error) return;
+  if (!plan) return fatal(error, "Plan is missing");
+  if (!positions || !energy) return fatal(error, "Invalid batch description");
+  guarded(error, [&] {
+    plan->plan->execute(positions, outputs(energy, gradient, sigma), static_cast<hipStream_t>(stream));
+  });
+}
+
+int d3hip_plan_num_shapes(void) { return static_cast<int>(d3::Plan::candidate_shapes().size()); }
+
+void d3hip_plan_candidate_shape(int index, int* waves, int* blocks_per_cu) {
+  const auto c = d3::Plan::candidate_shapes();
+  if (index < 0 || index >= static_cast<int>(c.size())) return;
+  if (waves) *waves = c[index].waves;
+  if (blocks_per_cu) *blocks_per_cu = c[index].blocks_per_cu;
+}
+
+void d3hip_plan_get_shape(dftd3_error error, d3hip_plan plan, int* waves, int* blocks_per_cu) {
+  if (!error) return;
+  if (!plan) return fatal(error, "Plan is missing");
+  if (waves) *waves = plan->plan->shape().waves;
+  if (blocks_per_cu) *blocks_per_cu = plan->plan->shape().blocks_per_cu;
+}
+
+void d3hip_plan_set_shape(dftd3_error error, d3hip_plan plan, int waves, int blocks_per_cu) {
+  if (!error) return;
+  if (!plan) return fatal(error, "Plan is missing");
+  guarded(error, [&] { plan->plan->set_shape(waves, blocks_per_cu); });
+}
+
+int d3hip_plan_autotune(dftd3_error error, d3hip_plan plan, const double* positions, double* energy,
+                        double* gradient, double* sigma, int reps, float* times_ms, void* stream) {
+  if (!error) return -1;
+  if (!plan) {
+    fatal(error, "Plan is missing");
+    return -1;
+  }
+  if (!positions || !energy) {
+    fatal(error, "Invalid batch description");
+    return -1;
+  }
+  int best = -1;
+  guarded(error, [&] {
+    std::vector<float> t;
+    best = plan->plan->autotune(positions, outputs(energy, gradient, sigma), static_cast<hipStream_t>(stream),
+                                reps, &t);
+    if (times_ms) std::copy(t.begin(), t.end(), times_ms);
+  });
+  return best;
+}
+
+void d3hip_plan_capture(dftd3_error error, d3hip_plan plan, const double* positions, double* energy,
+                        double* gradient, double* sigma, void* stream) {
+  if (!error) return;
+  if (!plan) return fatal(error, "Plan is missing");
+  if (!positions || !energy) return fatal(error, "Invalid batch description");
+  guarded(error, [&] {
+    plan->plan->capture(positions, outputs(energy, gradient, sigma), static_cast<hipStream_t>(stream));
+  });
+}
+
+void d3hip_plan_replay(dftd3_error error, d3hip_plan plan, void* stream) {
+  if (!error) return;
+  if (!plan) return fatal(error, "Plan is missing");
+  guarded(error, [&] { plan->plan->replay(static_cast<hipStream_t>(stream)); });
+}
+
+int d3hip_plan_overflowed(dftd3_error error, d3hip_plan plan, void* stream) {
+  if (!error) return -1;
+  if (!plan) {
+    fatal(error, "Plan is missing");
+    return -1;
+  }
+  int r = -1;
+  guarded(error, [&] { r = plan->plan->overflowed(static_cast<hipStream_t>(stream)) ? 1 : 0; });
+  return r;
+}

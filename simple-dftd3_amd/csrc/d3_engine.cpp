@@ -142,11 +142,19 @@ class KernelTimer {
   std::map<std::string, std::pair<double, long long>> totals_;
 };
 
+}  // namespace
+
 struct Prepared {
   DevBatch db{};
-  long long pair_total = 0;  // sum over systems of n^2
-  bool any_atm = false;
+  long long pair_total = 0;    // sum over systems of n^2
+  bool any_atm = false;        // some system has s9 != 0
+  bool smooth = false;         // some three-body system has a nonzero width3
+  bool periodic_atm = false;   // some three-body system has lattice images
+  long long slot_bound = 0;    // sum over three-body systems of (il + 1) * images
+  long long item_bound = 0;    // work items of those bounds
 };
+
+namespace {
 
 template <typename T>
 size_t stage(std::vector<char>& buf, const std::vector<T>& v) {
@@ -210,17 +218,12 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
   return out;
 }
 
-struct Engine::Impl {
-  DeviceBuffer meta, pos, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items, item_off, slots, counts;
+struct Workspace {
+  DeviceBuffer meta, pos, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items, item_off, slots, counts,
+      counter, flag;
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
-
-  Prepared prepare(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
-                   hipStream_t stream, const int* ghost_host);
-  // Three-body lists and work items; returns the launch size of the ATM kernel.
-  long long build_atm(const Prepared& pb, const void* pair, bool mixed, hipStream_t stream, long long** offset,
-                      long long** item_off, void** slots, long long* nslots);
 
   template <typename F>
   void timed(const char* name, hipStream_t st, F&& launch) {
@@ -228,23 +231,39 @@ struct Engine::Impl {
     launch();
     timer.end(name, st);
   }
+
+  // Host planning of a batch: species of the batch, per-species tables,
+  // lattice images per system and cutoff, pair-block offsets; one metadata
+  // upload (stream-ordered).
+  Prepared upload(const std::vector<SystemDesc>& systems, const int* numbers, hipStream_t stream,
+                  const int* ghost_host);
+  void pack(const Prepared& pb, const double* positions_dev, hipStream_t stream) {
+    timed("pack", stream,
+          [&] { launch_pack(positions_dev, pb.db.natoms, const_cast<double4*>(pb.db.pos), stream); });
+  }
+  DevWork work(const Prepared& pb, bool mixed, bool grad, const BatchOutput& out, const BatchDebug* debug);
+  // count -> scan -> work items -> scan (three-body lists of every atom)
+  void list_offsets(const Prepared& pb, const void* pair, bool mixed, hipStream_t stream);
+  // pack .. disp2; then the three-body lists and kernel; then the chain
+  // rule and the reduction.  capacity < 0: read the list size back and size
+  // the slot buffer to it; otherwise use the fixed capacity and item bound.
+  void evaluate(const Prepared& pb, const double* positions_dev, bool mixed, bool grad, bool smooth,
+                const BatchOutput& out, const BatchDebug* debug, RunStats* stats, long long capacity,
+                long long max_items, const AtmShape& shape, hipStream_t stream);
 };
 
-Engine::Engine() : impl_(new Impl) {}
-Engine::~Engine() { delete impl_; }
+Engine::Engine() : ws_(new Workspace) {}
+Engine::~Engine() { delete ws_; }
 
 void Engine::set_kernel_timing(bool enable) {
-  impl_->timer.clear();
-  impl_->timer.enabled = enable;
+  ws_->timer.clear();
+  ws_->timer.enabled = enable;
 }
 
-std::pair<double, long long> Engine::kernel_time(const std::string& name) const { return impl_->timer.get(name); }
+std::pair<double, long long> Engine::kernel_time(const std::string& name) const { return ws_->timer.get(name); }
 
-// Host planning of a batch: species of the batch, per-species tables, lattice
-// images per system and cutoff, pair-block offsets; one metadata upload and
-// the packing of the positions.
-Prepared Engine::Impl::prepare(const std::vector<SystemDesc>& systems, const int* numbers,
-                               const double* positions_dev, hipStream_t stream, const int* ghost_host) {
+Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* numbers, hipStream_t stream,
+                           const int* ghost_host) {
   Prepared pb;
   const int nsys = static_cast<int>(systems.size());
   int natoms = 0;
@@ -292,8 +311,8 @@ Prepared Engine::Impl::prepare(const std::vector<SystemDesc>& systems, const int
   std::vector<SysGeom> geom(nsys);
   std::vector<SysParam> par(nsys);
   std::vector<double4> img;
-  long long pair_total = 0;
-  bool any_atm = false;
+  long long pair_total = 0, slot_bound = 0, item_bound = 0;
+  bool any_atm = false, smooth = false, periodic_atm = false;
   for (int k = 0, a = 0; k < nsys; ++k) {
     const SystemDesc& sd = systems[k];
     SysGeom& g = geom[k];
@@ -312,7 +331,17 @@ Prepared Engine::Impl::prepare(const std::vector<SystemDesc>& systems, const int
     add(sd.param.cut3, g.img3_begin, g.img3_count);
     g.c6_begin = pair_total;
     pair_total += static_cast<long long>(sd.natoms) * sd.natoms;
-    if (std::fabs(sd.param.s9) >= kEps) any_atm = true;
+    if (std::fabs(sd.param.s9) >= kEps) {
+      any_atm = true;
+      smooth |= sd.param.width3 > 0.0;
+      if (g.img3_count > 1) periodic_atm = true;
+      // slot j <= i of atom i: at most (il + 1) images each
+      for (long long il = 0; il < sd.natoms; ++il) {
+        const long long m = (il + 1) * g.img3_count;
+        slot_bound += m;
+        item_bound += (m + kTile - 1) / kTile;
+      }
+    }
   }
   std::vector<char>& sb = staging;
   sb.clear();
@@ -348,13 +377,30 @@ Prepared Engine::Impl::prepare(const std::vector<SystemDesc>& systems, const int
   db.ghost = ghost_host ? reinterpret_cast<const int*>(dmeta + o_ghost) : nullptr;
   pb.pair_total = pair_total;
   pb.any_atm = any_atm;
-  timed("pack", stream, [&] { launch_pack(positions_dev, natoms, const_cast<double4*>(db.pos), stream); });
+  pb.smooth = smooth;
+  pb.periodic_atm = periodic_atm;
+  pb.slot_bound = slot_bound;
+  pb.item_bound = item_bound;
   return pb;
 }
 
-long long Engine::Impl::build_atm(const Prepared& pb, const void* pair_records, bool mixed, hipStream_t stream,
-                                  long long** offset_out, long long** item_off_out, void** slots_out,
-                                  long long* nslots) {
+DevWork Workspace::work(const Prepared& pb, bool mixed, bool grad, const BatchOutput& out,
+                        const BatchDebug* debug) {
+  const DevBatch& db = pb.db;
+  const int natoms = db.natoms;
+  DevWork w{};
+  w.cn = debug && debug->cn ? debug->cn : cn.as<double>(natoms);
+  w.gw = gw.as<double>(size_t(natoms) * db.mref);
+  w.dgw = grad ? dgw.as<double>(size_t(natoms) * db.mref) : nullptr;
+  w.pair = pair.get(std::max<long long>(pb.pair_total, 1) * pair_record_bytes(mixed));
+  w.e_atom = e_atom.as<double>(natoms);
+  w.dedcn = debug && debug->dedcn ? debug->dedcn : dedcn.as<double>(natoms);
+  w.grad = out.gradient ? out.gradient : this->grad.as<double>(size_t(natoms) * 3);
+  w.sig_atom = sig.as<double>(size_t(natoms) * 6);
+  return w;
+}
+
+void Workspace::list_offsets(const Prepared& pb, const void* pair_records, bool mixed, hipStream_t stream) {
   const int natoms = pb.db.natoms;
   long long* c = cnt.as<long long>(natoms);
   long long* o = off.as<long long>(natoms + 1);
@@ -366,80 +412,266 @@ long long Engine::Impl::build_atm(const Prepared& pb, const void* pair_records, 
     launch_atm_items(o, natoms, it, stream);
     launch_scan(it, natoms, io, stream);
   });
-  long long total = 0;
-  hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
-  hip_check(hipStreamSynchronize(stream), "list size sync");
-  void* s = slots.get(std::max<long long>(total, 1) * slot_record_bytes(mixed));
-  timed("atm_list", stream, [&] { launch_atm_fill(pb.db, pair_records, mixed, o, s, stream); });
-  *offset_out = o;
-  *item_off_out = io;
-  *slots_out = s;
-  *nslots = total;
-  return total / kTile + natoms;  // upper bound of the number of work items
+}
+
+void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool mixed, bool grad, bool smooth,
+                         const BatchOutput& out, const BatchDebug* debug, RunStats* stats, long long capacity,
+                         long long max_items, const AtmShape& shape, hipStream_t stream) {
+  const DevBatch& db = pb.db;
+  const int natoms = db.natoms;
+  pack(pb, positions_dev, stream);
+  const DevWork w = work(pb, mixed, grad, out, debug);
+  timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
+  timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
+  timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
+  timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, stream); });
+  if (pb.any_atm) {
+    list_offsets(pb, w.pair, mixed, stream);
+    long long* o = off.as<long long>(natoms + 1);
+    long long* io = item_off.as<long long>(natoms + 1);
+    void* s = nullptr;
+    if (capacity < 0) {
+      long long total = 0;
+      hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
+      hip_check(hipStreamSynchronize(stream), "list size sync");
+      if (stats) stats->atm_list_entries = total;
+      capacity = total;
+      max_items = total / kTile + natoms;  // bound of the number of work items
+      s = slots.get(std::max<long long>(total, 1) * slot_record_bytes(mixed));
+    } else {
+      s = slots.get(std::max<long long>(capacity, 1) * slot_record_bytes(mixed));
+      launch_atm_guard(o, natoms, capacity, io, flag.as<int>(1), stream);
+    }
+    timed("atm_list", stream, [&] { launch_atm_fill(db, w.pair, mixed, o, s, capacity, stream); });
+    unsigned long long* wc = shape.blocks_per_cu > 0 ? counter.as<unsigned long long>(1) : nullptr;
+    timed("atm", stream,
+          [&] { launch_atm(db, mixed, grad, smooth, shape, io, max_items, o, s, w, wc, stream); });
+  }
+  if (grad) timed("cn_chain", stream, [&] { launch_cn_chain(db, mixed, w, stream); });
+  timed("reduce", stream, [&] { launch_reduce(db, w, out.energy, grad ? out.sigma : nullptr, stream); });
+  hip_check(hipGetLastError(), "kernel launch");
 }
 
 void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
                  int precision, const BatchOutput& out, hipStream_t stream, const BatchDebug* debug,
                  RunStats* stats, const int* ghost_host) {
   if (systems.empty()) return;
-  Impl& m = *impl_;
-  const Prepared pb = m.prepare(systems, numbers, positions_dev, stream, ghost_host);
-  const DevBatch& db = pb.db;
-  const int natoms = db.natoms;
+  Workspace& m = *ws_;
+  const Prepared pb = m.upload(systems, numbers, stream, ghost_host);
   const bool grad = out.gradient || out.sigma;
-  const bool mixed = precision == kMixed;
-  DevWork w{};
-  w.cn = debug && debug->cn ? debug->cn : m.cn.as<double>(natoms);
-  w.gw = m.gw.as<double>(size_t(natoms) * db.mref);
-  w.dgw = grad ? m.dgw.as<double>(size_t(natoms) * db.mref) : nullptr;
-  w.pair = m.pair.get(std::max<long long>(pb.pair_total, 1) * pair_record_bytes(mixed));
-  w.e_atom = m.e_atom.as<double>(natoms);
-  w.dedcn = debug && debug->dedcn ? debug->dedcn : m.dedcn.as<double>(natoms);
-  w.grad = out.gradient ? out.gradient : m.grad.as<double>(size_t(natoms) * 3);
-  w.sig_atom = m.sig.as<double>(size_t(natoms) * 6);
-
-  m.timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
-  m.timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
-  m.timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
-  m.timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, stream); });
-  if (pb.any_atm) {
-    long long *offset = nullptr, *item_off = nullptr, nslots = 0;
-    void* slots = nullptr;
-    const long long max_items = m.build_atm(pb, w.pair, mixed, stream, &offset, &item_off, &slots, &nslots);
-    if (stats) stats->atm_list_entries = nslots;
-    bool smooth = false;
-    for (const auto& s : systems) smooth |= s.param.width3 > 0.0;
-    m.timed("atm", stream,
-            [&] { launch_atm(db, mixed, grad, smooth, item_off, max_items, offset, slots, w, stream); });
-  }
-  if (grad) m.timed("cn_chain", stream, [&] { launch_cn_chain(db, mixed, w, stream); });
-  m.timed("reduce", stream, [&] { launch_reduce(db, w, out.energy, grad ? out.sigma : nullptr, stream); });
-  hip_check(hipGetLastError(), "kernel launch");
+  m.evaluate(pb, positions_dev, precision == kMixed, grad, pb.smooth, out, debug, stats, -1, 0, AtmShape{},
+             stream);
   if (m.timer.enabled) m.timer.collect();
 }
 
 void Engine::count_terms(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
                          long long* counts_host, hipStream_t stream) {
   if (systems.empty()) return;
-  Impl& m = *impl_;
+  Workspace& m = *ws_;
   std::vector<SystemDesc> sys = systems;
   for (auto& s : sys)
     if (std::fabs(s.param.s9) < kEps) s.param.s9 = 1.0;  // triples are counted whatever s9 is
-  const Prepared pb = m.prepare(sys, numbers, positions_dev, stream, nullptr);
+  const Prepared pb = m.upload(sys, numbers, stream, nullptr);
+  m.pack(pb, positions_dev, stream);
+  const int natoms = pb.db.natoms;
   unsigned long long* dc = m.counts.as<unsigned long long>(size_t(pb.db.nsys) * 3);
   hip_check(hipMemsetAsync(dc, 0, sizeof(unsigned long long) * 3 * pb.db.nsys, stream), "hipMemsetAsync");
   launch_count_pairs(pb.db, dc, stream);
-  long long *offset = nullptr, *item_off = nullptr, nslots = 0;
-  void* slots = nullptr;
   // lists without the C6 test: every pair within the cutoff is a slot
-  m.build_atm(pb, nullptr, false, stream, &offset, &item_off, &slots, &nslots);
-  launch_count_triples(pb.db, false, offset, slots, dc, stream);
+  m.list_offsets(pb, nullptr, false, stream);
+  long long* o = m.off.as<long long>(natoms + 1);
+  long long total = 0;
+  hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
+  hip_check(hipStreamSynchronize(stream), "list size sync");
+  void* s = m.slots.get(std::max<long long>(total, 1) * slot_record_bytes(false));
+  launch_atm_fill(pb.db, nullptr, false, o, s, total, stream);
+  launch_count_triples(pb.db, false, o, s, dc, stream);
   hip_check(hipGetLastError(), "kernel launch");
   std::vector<unsigned long long> h(3 * pb.db.nsys);
   hip_check(hipMemcpyAsync(h.data(), dc, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, stream),
             "download counts");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   for (size_t k = 0; k < h.size(); ++k) counts_host[k] = static_cast<long long>(h[k]);
+}
+
+// ------------------------------------------------------------------ Plan
+struct Plan::State {
+  Workspace ws;
+  Prepared pb;
+  bool mixed = false, grad = false;
+  long long capacity = 0, max_items = 0;
+  AtmShape shape{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~State() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  void drop_graph() {
+    if (exec) hip_check(hipGraphExecDestroy(exec), "hipGraphExecDestroy");
+    if (graph) hip_check(hipGraphDestroy(graph), "hipGraphDestroy");
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace {
+int device_cu_count() {
+  int dev = 0, cus = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+  return cus;
+}
+}  // namespace
+
+Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, const double* positions_dev,
+           int precision, bool grad, const int* ghost_host, hipStream_t stream)
+    : st_(new State) {
+  try {
+    if (systems.empty()) throw Error("Invalid molecular structure data provided");
+    State& s = *st_;
+    s.mixed = precision == kMixed;
+    s.grad = grad;
+    s.pb = s.ws.upload(systems, numbers_host, stream, ghost_host);
+    s.shape.num_cus = device_cu_count();
+    const int natoms = s.pb.db.natoms;
+    hip_check(hipMemsetAsync(s.ws.flag.as<int>(1), 0, sizeof(int), stream), "hipMemsetAsync");
+    if (s.pb.any_atm) {
+      long long cap = s.pb.slot_bound;
+      if (s.pb.periodic_atm) {
+        // lists of the initial coordinates, with headroom for later ones
+        s.ws.pack(s.pb, positions_dev, stream);
+        const DevWork w = s.ws.work(s.pb, s.mixed, grad, BatchOutput{}, nullptr);
+        launch_cn(s.pb.db, s.mixed, w.cn, stream);
+        launch_weights(s.pb.db, w.cn, w.gw, w.dgw, stream);
+        launch_pair(s.pb.db, s.mixed, w.gw, w.dgw, w.pair, stream);
+        s.ws.list_offsets(s.pb, w.pair, s.mixed, stream);
+        long long total = 0;
+        hip_check(hipMemcpyAsync(&total, s.ws.off.as<long long>(natoms + 1) + natoms, sizeof total,
+                                 hipMemcpyDeviceToHost, stream),
+                  "list size");
+        hip_check(hipStreamSynchronize(stream), "list size sync");
+        cap = std::min(cap, std::max(total + total / 2, total + 64LL * natoms));
+      }
+      s.capacity = cap;
+      s.max_items = std::min(s.pb.item_bound, cap / kTile + natoms);
+      // size every buffer now, so that an evaluation allocates nothing
+      (void)s.ws.slots.get(std::max<long long>(cap, 1) * slot_record_bytes(s.mixed));
+      (void)s.ws.counter.as<unsigned long long>(1);
+    }
+    (void)s.ws.work(s.pb, s.mixed, grad, BatchOutput{}, nullptr);
+    hip_check(hipStreamSynchronize(stream), "plan sync");
+  } catch (...) {
+    delete st_;
+    throw;
+  }
+}
+
+Plan::~Plan() { delete st_; }
+
+int Plan::natoms() const { return st_->pb.db.natoms; }
+int Plan::nsys() const { return st_->pb.db.nsys; }
+long long Plan::slot_capacity() const { return st_->capacity; }
+const AtmShape& Plan::shape() const { return st_->shape; }
+
+void Plan::set_shape(int waves, int blocks_per_cu) {
+  if (waves != 2 && waves != 4) throw Error("three-body launch: wavefronts per block must be 2 or 4");
+  if (blocks_per_cu < 0 || blocks_per_cu > 64) throw Error("three-body launch: blocks per CU out of range");
+  st_->shape.waves = waves;
+  st_->shape.blocks_per_cu = blocks_per_cu;
+  st_->drop_graph();
+}
+
+void Plan::execute(const double* positions_dev, const BatchOutput& out, hipStream_t stream) {
+  State& s = *st_;
+  if (!out.energy) throw Error("Energy output is missing");
+  if (s.grad != (out.gradient != nullptr || out.sigma != nullptr))
+    throw Error("The plan was made for " + std::string(s.grad ? "energy and gradient" : "the energy only"));
+  s.ws.evaluate(s.pb, positions_dev, s.mixed, s.grad, s.pb.smooth, out, nullptr, nullptr, s.capacity, s.max_items,
+                s.shape, stream);
+}
+
+std::vector<AtmShape> Plan::candidate_shapes() {
+  std::vector<AtmShape> c;
+  for (int waves : {4, 2})
+    for (int bpc : {0, 2, 4, 8}) {
+      AtmShape a;
+      a.waves = waves;
+      a.blocks_per_cu = bpc;
+      c.push_back(a);
+    }
+  return c;
+}
+
+int Plan::autotune(const double* positions_dev, const BatchOutput& out, hipStream_t stream, int reps,
+                   std::vector<float>* times_ms) {
+  State& s = *st_;
+  const std::vector<AtmShape> cand = candidate_shapes();
+  std::vector<float> t(cand.size(), 0.0f);
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  int best = 0;
+  try {
+    for (size_t k = 0; k < cand.size(); ++k) {
+      s.shape.waves = cand[k].waves;
+      s.shape.blocks_per_cu = cand[k].blocks_per_cu;
+      execute(positions_dev, out, stream);  // warm-up
+      hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+      for (int r = 0; r < std::max(reps, 1); ++r) execute(positions_dev, out, stream);
+      hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      hip_check(hipEventElapsedTime(&t[k], e0, e1), "hipEventElapsedTime");
+      t[k] /= float(std::max(reps, 1));
+      if (t[k] < t[best]) best = static_cast<int>(k);
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s.shape.waves = cand[best].waves;
+  s.shape.blocks_per_cu = cand[best].blocks_per_cu;
+  s.drop_graph();
+  if (times_ms) *times_ms = t;
+  return best;
+}
+
+void Plan::capture(const double* positions_dev, const BatchOutput& out, hipStream_t stream) {
+  State& s = *st_;
+  s.drop_graph();
+  const bool timing = s.ws.timer.enabled;
+  s.ws.timer.enabled = false;
+  hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  try {
+    execute(positions_dev, out, stream);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    s.ws.timer.enabled = timing;
+    throw;
+  }
+  hip_check(hipStreamEndCapture(stream, &s.graph), "hipStreamEndCapture");
+  s.ws.timer.enabled = timing;
+  hip_check(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+bool Plan::captured() const { return st_->exec != nullptr; }
+
+void Plan::replay(hipStream_t stream) {
+  if (!st_->exec) throw Error("The plan has no captured graph");
+  hip_check(hipGraphLaunch(st_->exec, stream), "hipGraphLaunch");
+}
+
+bool Plan::overflowed(hipStream_t stream) {
+  int h = 0;
+  hip_check(hipMemcpyAsync(&h, st_->ws.flag.as<int>(1), sizeof h, hipMemcpyDeviceToHost, stream), "overflow flag");
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  return h != 0;
 }
 
 }  // namespace d3

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "d3_common.hpp"
+#include "d3_kernels.hpp"
 
 namespace d3 {
 
@@ -47,6 +48,8 @@ struct RunStats {
 // Lattice points within a cutoff, in the order of cutoff.f90:140-186.
 std::vector<double> lattice_points(const int periodic[3], const double lattice[9], double rthr);
 
+struct Workspace;
+
 class Engine {
  public:
   Engine();
@@ -57,7 +60,8 @@ class Engine {
   // Evaluate a batch.  numbers are host arrays (atomic numbers, 1..103),
   // positions a device array [natoms][3] in bohr.  Systems are consecutive
   // in the atom arrays.  ghost_host (optional, host, [natoms]) marks atoms
-  // whose C6 coefficients vanish (model.f90:166-174, 436-440).
+  // whose C6 coefficients vanish (model.f90:166-174, 436-440).  The size of
+  // the three-body lists is read back once per call.
   void run(const std::vector<SystemDesc>& systems, const int* numbers_host,
            const double* positions_dev, int precision, const BatchOutput& out,
            hipStream_t stream, const BatchDebug* debug = nullptr, RunStats* stats = nullptr,
@@ -76,8 +80,58 @@ class Engine {
   std::pair<double, long long> kernel_time(const std::string& name) const;
 
  private:
-  struct Impl;
-  Impl* impl_;
+  Workspace* ws_;
+};
+
+// A batch planned once and evaluated many times with new coordinates of the
+// same systems: the metadata upload, the buffer sizes and the launch shape
+// are fixed when the plan is made, and an evaluation is a fixed sequence of
+// kernel launches on one stream without host synchronisation or allocation,
+// which can be captured in a HIP graph and replayed.
+//
+// Three-body lists: a batch of molecules gets the exact bound n(n+1)/2 slots
+// per molecule; a periodic system gets 1.5 times the list size of the
+// coordinates the plan is made with (and at least 64 more slots per atom),
+// capped by the exact bound.  An evaluation whose lists do not fit adds no
+// three-body terms and raises the overflow flag that overflowed() reports.
+class Plan {
+ public:
+  Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, const double* positions_dev,
+       int precision, bool grad, const int* ghost_host, hipStream_t stream);
+  ~Plan();
+  Plan(const Plan&) = delete;
+  Plan& operator=(const Plan&) = delete;
+
+  int natoms() const;
+  int nsys() const;
+  long long slot_capacity() const;
+  const AtmShape& shape() const;
+  void set_shape(int waves, int blocks_per_cu);
+
+  // One evaluation (positions: device [natoms][3], bohr).
+  void execute(const double* positions_dev, const BatchOutput& out, hipStream_t stream);
+
+  // Candidate launch shapes of the three-body kernel.
+  static std::vector<AtmShape> candidate_shapes();
+  // Times every candidate shape (one warm-up and `reps` timed evaluations
+  // each, HIP events on `stream`), keeps the fastest and returns its index;
+  // times_ms (optional) receives the mean time per evaluation of each.
+  int autotune(const double* positions_dev, const BatchOutput& out, hipStream_t stream, int reps,
+               std::vector<float>* times_ms);
+
+  // Records one evaluation with these device pointers into a HIP graph;
+  // replay() launches it (new coordinates go into the same positions buffer).
+  void capture(const double* positions_dev, const BatchOutput& out, hipStream_t stream);
+  bool captured() const;
+  void replay(hipStream_t stream);
+
+  // Synchronises `stream`; true when an evaluation since the plan was made
+  // had more three-body slots than the plan holds.
+  bool overflowed(hipStream_t stream);
+
+ private:
+  struct State;
+  State* st_;
 };
 
 }  // namespace d3

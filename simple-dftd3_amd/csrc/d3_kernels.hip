@@ -10,7 +10,9 @@
 // squared distance, so both paths visit exactly the reference's terms.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <stdexcept>
 
 #include "d3_kernels.hpp"
 
@@ -42,9 +44,8 @@ template <typename R>
 struct Slot {
   double vx, vy, vz;  // x_j + T - x_i (atm.f90:711), FP64 for the exact cutoff tests
   R r2;               // |v|^2
-  R rinv;             // 1 / |v|
   R pdmp;             // (r0_ij / r_ij)^(alp/3), r0_ij = 4/3 rvdw_ij, alp = alp6 + 2
-  R sc6;              // sqrt(C6_ij)
+  R sc6h;             // sqrt(C6_ij), halved for an image of atom i itself (j == i)
   R qi;               // (dC6_ij / dCN_i) / C6_ij
   R qx;               // (dC6_ij / dCN_j) / C6_ij
   int atom;           // j (global index)
@@ -124,9 +125,6 @@ template <> __device__ __forceinline__ float d_exp<float>(float x) { return __ex
 template <typename R> __device__ __forceinline__ R d_sqrt(R x);
 template <> __device__ __forceinline__ double d_sqrt<double>(double x) { return sqrt(x); }
 template <> __device__ __forceinline__ float d_sqrt<float>(float x) { return __fsqrt_rn(x); }
-template <typename R> __device__ __forceinline__ R d_rsqrt(R x);
-template <> __device__ __forceinline__ double d_rsqrt<double>(double x) { return rsqrt(x); }
-template <> __device__ __forceinline__ float d_rsqrt<float>(float x) { return __frsqrt_rn(x); }
 template <typename R> __device__ __forceinline__ R d_rcp(R x);
 template <> __device__ __forceinline__ double d_rcp<double>(double x) { return 1.0 / x; }
 template <> __device__ __forceinline__ float d_rcp<float>(float x) { return __frcp_rn(x); }
@@ -134,13 +132,6 @@ template <> __device__ __forceinline__ float d_rcp<float>(float x) { return __fr
 template <typename R> __device__ __forceinline__ R d_pow(R x, R y);
 template <> __device__ __forceinline__ double d_pow<double>(double x, double y) { return pow(x, y); }
 template <> __device__ __forceinline__ float d_pow<float>(float x, float y) { return exp2f(y * __log2f(x)); }
-
-// atm.f90:954-980
-__device__ __forceinline__ double triple_scale(int ii, int jj, int kk) {
-  if (ii == jj) return ii == kk ? 1.0 / 6.0 : 0.5;
-  if (ii != kk && jj != kk) return 1.0;
-  return 0.5;
-}
 
 // Sum v over the block and add the total to *dst (thread 0).  Every thread
 // of the block must call it.
@@ -410,10 +401,11 @@ template <bool kFill, typename R>
 __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R>* __restrict__ pair,
                                                           long long* __restrict__ count,
                                                           const long long* __restrict__ offset,
-                                                          Slot<R>* __restrict__ slots) {
+                                                          Slot<R>* __restrict__ slots, long long capacity) {
   const int lane = wave_lane();
   const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (i >= b.natoms) return;
+  if (kFill && offset[b.natoms] > capacity) return;  // the lists do not fit: no three-body terms
   const int s = b.sys[i];
   const SysGeom g = b.geom[s];
   const SysParam p = b.par[s];
@@ -462,9 +454,8 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
         Slot<R> e;
         e.vx = v0; e.vy = v1; e.vz = v2;
         e.r2 = R(r2);
-        e.rinv = R(1.0 / sqrt(r2));
         e.pdmp = q.w * d_pow(R(r2), malp6);
-        e.sc6 = q.x;
+        e.sc6h = j == i ? R(0.5) * q.x : q.x;
         e.qi = q.y;
         e.qx = q.z;
         e.atom = j;
@@ -511,50 +502,150 @@ __global__ void atm_items_kernel(const long long* __restrict__ offset, int n, lo
 }
 
 // --------------------------------------------------------- three-body term
+constexpr int kQueue = 128;  // entries of the accepted-pair ring of a wavefront
+
+// Slot data of one 64-slot tile in LDS.
 template <typename R>
-struct TileLds {  // b-slots of the current tile of one wavefront
-  double vx[kTile], vy[kTile], vz[kTile];
-  R r2[kTile], rinv[kTile], pdmp[kTile], sc6[kTile];
-  int atom[kTile];
-  double acc[6][kTile];  // F, E, Gx, Gy, Gz, Dq of the b-slots
+struct TileSlots {
+  double vx[kTile], vy[kTile], vz[kTile];  // FP64 vectors for the exact r_jk tests
+  R4<R> g0[kTile];                          // vx, vy, vz, r2
+  R4<R> g1[kTile];                          // pdmp, sc6h, qi, qx
+  int atom[kTile];                          // -1 past the end of the list
 };
+// Accepted slot pair (a, b) of a tile pair.
+template <typename R>
+struct PairEntry {
+  unsigned ab;   // a | b << 8 | (both slots are images of atom i) << 16
+  unsigned rec;  // index of the (j, k) pair record within the system
+  R C;           // r_jk^2
+  R pad;
+};
+template <>
+struct PairEntry<double> {
+  unsigned ab, rec;
+  double C;
+};
+
+// Per-slot sums of a tile pair: M = sum (f_is + f_jk), K = sum f_jk v_partner
+// (three components), D = sum dE (qx_s + q_partner-side).
+constexpr int kAcc = 5;
+
+template <typename R>
+__device__ __forceinline__ void load_tile(TileSlots<R>& T, const Slot<R>* __restrict__ slots, long long base, int idx,
+                                          int m, int l) {
+  if (idx < m) {
+    const Slot<R> s = slots[base + idx];
+    T.vx[l] = s.vx; T.vy[l] = s.vy; T.vz[l] = s.vz;
+    T.g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
+    T.g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
+    T.atom[l] = s.atom;
+  } else {  // fails every distance test
+    T.vx[l] = 1e30; T.vy[l] = 1e30; T.vz[l] = 1e30;
+    T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
+    T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
+    T.atom[l] = -1;
+  }
+}
+
+// Adds the sums of slot s (vector v, atom k) to the gradient and dE/dCN of
+// atom k and to the i-side sums:
+//   grad_k += M v - K,  grad_i -= M v - K,  dEdcn_k -= D / 2,
+//   sigma  += M v v^T - sym(v K^T).
+__device__ __forceinline__ void flush_slot(double M, double Kx, double Ky, double Kz, double D, double vx, double vy,
+                                           double vz, int k, double* __restrict__ grad, double* __restrict__ dedcn,
+                                           double* gi, double* sg) {
+  if (M == 0.0 && D == 0.0 && Kx == 0.0 && Ky == 0.0 && Kz == 0.0) return;
+  const double gx = M * vx - Kx, gy = M * vy - Ky, gz = M * vz - Kz;
+  atomicAdd(grad + 3 * k, gx);
+  atomicAdd(grad + 3 * k + 1, gy);
+  atomicAdd(grad + 3 * k + 2, gz);
+  atomicAdd(dedcn + k, -0.5 * D);
+  gi[0] -= gx; gi[1] -= gy; gi[2] -= gz;
+  sg[0] += M * vx * vx - vx * Kx;
+  sg[1] += M * vy * vy - vy * Ky;
+  sg[2] += M * vz * vz - vz * Kz;
+  sg[3] += M * vx * vy - 0.5 * (vx * Ky + vy * Kx);
+  sg[4] += M * vx * vz - 0.5 * (vx * Kz + vz * Kx);
+  sg[5] += M * vy * vz - 0.5 * (vy * Kz + vz * Ky);
+}
+
+// Three-body hot path: x**y for x > 0, 1/x and 1/sqrt(x) for normal x > 0.
+// FP32 uses the hardware approximations (v_exp_f32, v_log_f32, v_rcp_f32,
+// v_rsq_f32, about 1 ulp); FP64 the device-library functions.
+template <typename R> __device__ __forceinline__ R pow_pos(R x, R y);
+template <> __device__ __forceinline__ double pow_pos<double>(double x, double y) { return exp2(y * log2(x)); }
+template <> __device__ __forceinline__ float pow_pos<float>(float x, float y) {
+  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+template <typename R> __device__ __forceinline__ R rcp_fast(R x);
+template <> __device__ __forceinline__ double rcp_fast<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float rcp_fast<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <typename R> __device__ __forceinline__ R rsqrt_fast(R x);
+template <> __device__ __forceinline__ double rsqrt_fast<double>(double x) { return rsqrt(x); }
+template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // atm.f90:625-825 get_atm_dispersion_derivs (kGrad) and atm.f90:500-621,
 // with rs9 = 4/3 and alp + 2 (rational.f90:563, zero.f90:602).
 //
 // Work item = (atom i, slot block P of L(i)); the four wavefronts of the
-// block take the slot blocks Q >= P in turn.  In a 64 x 64 tile, lane l owns
-// slot a = P*64 + l and at step s pairs it with b = Q*64 + (l + s) mod 64, so
-// the b-slots of a step are distinct: their contributions are added into LDS
-// without conflicts, the a-slot contributions stay in registers.  A diagonal
-// tile (Q == P) runs the steps 1..32 (32 only for l < 32) so that every
-// unordered pair is visited once.
+// block take the slot blocks Q >= P in turn, the A tile (block P) is shared.
+// A tile pair runs in two phases that alternate through a ring of kQueue
+// entries per wavefront:
+//  * test: at step s lane l tests a = l against b = (l + s) mod 64 with the
+//    reference's FP64 r_jk^2 (atm.f90:723-724) and appends accepted pairs
+//    (ballot + mbcnt compaction);
+//  * terms: each lane takes one queued pair, so every lane evaluates a
+//    triple (the pair-record gather of the next 64 entries is issued before
+//    the current ones are evaluated); the per-slot sums go into LDS with
+//    atomics in type R.
+// A diagonal tile (Q == P) runs the steps 1..32 (32 only for l < 32) so that
+// every unordered pair is visited once.  The per-slot sums of a tile pair
+// (at most 64 terms each) are added to double-precision sums at its end.
 //
-// Per triple the contributions are condensed into per-slot scalars: with
-// f_xy the radial factor of pair xy (dG_xy = f_xy v_xy, atm.f90:754-790),
-//   F_s   = sum f_is   over the triples of slot s   -> grad_x += F_s v_s,
-//                                                      grad_i -= F_s v_s,
-//                                                      sigma  += F_s v_s v_s
-//   E_s   = sum dE                                  -> dEdcn_x -= E_s qx_s/2,
-//                                                      dEdcn_i -= E_s qi_s/2
-//   G_s   = sum -/+ f_jk v_jk, Dq_s = sum dE q_jk  -> grad_x += G_s,
-//                                                      dEdcn_x -= Dq_s/2
-// so a triple costs one gather of the (j, k) pair record, one rsqrt, one pow
-// and one reciprocal, and the global atomics happen once per slot and tile.
+// With A = r_ij^2, B = r_ik^2, C = r_jk^2, S = ABC, t1 = A+B-C, t2 = A-B+C,
+// t3 = -A+B+C, P = t1 t2 t3 the angular factor is
+//   ang = S^(-5/2) (3/8 P + S),
+// and 2X d(ang)/dX = S^(-5/2) (3/4 X dP/dX - 15/8 P - 3S) for X = A, B, C
+// with dP/dA = t2t3 + t1t3 - t1t2 (and cyclic), the polynomials of
+// atm.f90:754-760 in factored form.  The radial factor of pair X is
+// f_X = -c9 fdmp S^(-5/2) (3/4 X dP/dX + K1) / X with the damping part
+// K1 = -15/8 P - 3S + 2 alp pp fdmp (3/8 P + S); dG_X = f_X v_X.
+// The per-slot sums M, K, D (see kAcc) give the gradient and virial when the
+// slot is flushed (flush_slot); the weight of a triple, triple_scale times 2
+// for two images of one atom (atm.f90:954-980), is 1/2 per slot that is an
+// image of atom i (folded into sc6h) and 4/3 more when both are.
 // kSmooth adds the switching function of a nonzero width3 (cutoff.f90:73-99,
 // atm.f90:720-790).
-template <typename R, bool kGrad, bool kSmooth>
-__global__ __launch_bounds__(kBlock) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
-                                                     const long long* __restrict__ offset,
-                                                     const Slot<R>* __restrict__ slots,
-                                                     const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
-                                                     double* __restrict__ dedcn, double* __restrict__ grad,
-                                                     double* __restrict__ sig_atom) {
-  __shared__ TileLds<R> lds[kWavesPerBlock];
-  __shared__ double comb[kGrad ? kWavesPerBlock : 1][6][kTile];
-  __shared__ double red[kWavesPerBlock];
-  const long long item = blockIdx.x;
-  if (item >= item_off[b.natoms]) return;  // whole block
+//
+// Work items are taken in reverse order (the longest slot lists first):
+// block k takes item total-1-k, or, with a work counter (persistent launch),
+// the items total-1-c for the counter values c it draws until none is left.
+template <typename R, bool kGrad, bool kSmooth, int kW>
+__global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
+                                                         const long long* __restrict__ offset,
+                                                         const Slot<R>* __restrict__ slots,
+                                                         const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
+                                                         double* __restrict__ dedcn, double* __restrict__ grad,
+                                                         double* __restrict__ sig_atom,
+                                                         unsigned long long* __restrict__ work_counter) {
+  constexpr int kAccLen = kGrad ? kAcc * kTile : 1;
+  __shared__ TileSlots<R> tileA;
+  __shared__ TileSlots<R> tileB[kW];
+  using Acc = double;  // LDS sums of a tile pair (FP32 LDS atomics measured several times slower)
+  __shared__ Acc acc[kW][2][kAccLen];  // a-side and b-side sums of the current tile pair
+  __shared__ PairEntry<R> ring[kW][kQueue];
+  __shared__ double red[kW];
+  __shared__ long long next_item;
+  static_assert(sizeof(tileB) >= sizeof(double) * kW * kAcc * kTile, "a-side combine buffer");
+  const long long total = item_off[b.natoms];
+  long long k_item = blockIdx.x;
+  if (work_counter) {
+    if (threadIdx.x == 0) next_item = static_cast<long long>(atomicAdd(work_counter, 1ull));
+    __syncthreads();
+    k_item = next_item;
+  }
+  for (; k_item < total;) {
+  const long long item = total - 1 - k_item;
   int lo = 0, hi = b.natoms;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -566,175 +657,208 @@ __global__ __launch_bounds__(kBlock) void atm_kernel(DevBatch b, const long long
   const int s = b.sys[i];
   const SysGeom g = b.geom[s];
   const SysParam p = b.par[s];
-  const int n = g.natoms;
+  const int n = g.natoms, ab0 = g.atom_begin;
   const long long L0 = offset[i];
   const int m = static_cast<int>(offset[i + 1] - L0);
   const int nb = (m + kTile - 1) / kTile;
   const double cut2 = p.cut3 * p.cut3;
   const R alp = R(p.alp) + R(2);
+  const R alp2 = R(2) * alp;
   const R malp6 = -alp / R(6);
-  const R s9 = R(p.s9);
-  const int w = threadIdx.x >> 6, l = wave_lane();
-  TileLds<R>& T = lds[w];
-
-  const int a = P * kTile + l;
-  const bool aval = a < m;
-  Slot<R> sa{};
-  if (aval) sa = slots[L0 + a];
-  const int jat = sa.atom;
-  const R4<R>* prow = aval ? pair + g.c6_begin + static_cast<long long>(jat - g.atom_begin) * n - g.atom_begin
-                           : pair;
-  const R ar2 = sa.r2, arinv = sa.rinv, apdmp = sa.pdmp, asc6 = sa.sc6;
-  const R air2 = arinv * arinv;
+  const R ns9 = -R(p.s9);
   const R cutr = R(p.cut3), width = R(p.width3);
-  R swa = R(1), dswa = R(0);  // switching of r_ij and d sw / dr / r
-  if (kSmooth && aval) {
-    smooth_cutoff(R(1) / arinv, cutr, width, swa, dswa);
-    dswa *= arinv;
-  }
-  double ei = 0.0;
-  double Fa = 0.0, Ea = 0.0, Gax = 0.0, Gay = 0.0, Gaz = 0.0, Dqa = 0.0;
-  double gix = 0.0, giy = 0.0, giz = 0.0, di = 0.0;
-  double sxx = 0.0, syy = 0.0, szz = 0.0, sxy = 0.0, sxz = 0.0, syz = 0.0;
+  const R4<R>* __restrict__ prow = pair + g.c6_begin;  // n x n pair records of the system
+  // wavefront index as a uniform value, so the tile loop, the ring counters
+  // and the LDS tile addresses live in scalar registers
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = wave_lane();
 
-  for (int Q = P + w; Q < nb; Q += kWavesPerBlock) {
-    const int bq = Q * kTile + l;
-    Slot<R> sq{};
-    sq.atom = -1;
-    if (bq < m) sq = slots[L0 + bq];
-    T.vx[l] = sq.vx; T.vy[l] = sq.vy; T.vz[l] = sq.vz;
-    T.r2[l] = sq.r2; T.rinv[l] = sq.rinv; T.pdmp[l] = sq.pdmp; T.sc6[l] = sq.sc6;
-    T.atom[l] = sq.atom;
-    if (kGrad)
-      for (int c = 0; c < 6; ++c) T.acc[c][l] = 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    const bool diag = Q == P;
-    const int s0 = diag ? 1 : 0, s1 = diag ? kTile / 2 : kTile - 1;
-    if (aval) {
-      for (int st = s0; st <= s1; ++st) {
-        const int bl = (l + st) & (kTile - 1);
-        const int kat = T.atom[bl];
-        if (kat < 0 || (diag && st == kTile / 2 && l >= kTile / 2)) continue;
-        double vjxd, vjyd, vjzd;
-        const double Cd = diff_norm2(T.vx[bl], T.vy[bl], T.vz[bl], sa.vx, sa.vy, sa.vz, vjxd, vjyd, vjzd);
-        if (Cd > cut2 || Cd < kEps) continue;
-        const R4<R> q = prow[kat];
-        const R C = R(Cd);
-        const R A = ar2, B = T.r2[bl];
-        const R sc = d_rsqrt(C);
-        const R pc = q.w * d_pow(C, malp6);
-        const R ir1 = arinv * T.rinv[bl] * sc;
-        const R ir2 = ir1 * ir1, ir3 = ir1 * ir2, ir5 = ir3 * ir2;
-        const R t1 = A + C - B, t2 = A - C + B, t3 = -A + C + B;
-        const R ang = R(0.375) * t1 * t2 * t3 * ir5 + ir3;
-        const R pp = apdmp * T.pdmp[bl] * pc;
-        const R fd = d_rcp(R(1) + R(6) * pp);
-        const R c9 = -s9 * asc6 * T.sc6[bl] * q.x;
-        const R wgt = R(triple_scale(i, jat, kat) * (jat == kat ? 2.0 : 1.0));
-        const R cw = c9 * wgt;
-        R sw = R(1), swb = R(1), dswb = R(0), swc = R(1), dswc = R(0);
-        if (kSmooth) {
-          smooth_cutoff(R(1) / T.rinv[bl], cutr, width, swb, dswb);
-          dswb *= T.rinv[bl];
-          smooth_cutoff(C * sc, cutr, width, swc, dswc);
-          dswc *= sc;
-          sw = swa * swb * swc;
-        }
-        const R dE = ang * fd * cw * sw;
-        ei -= double(dE);
-        if (kGrad) {
-          const R dfd = R(-2) * alp * pp * fd * fd;
-          const R A2 = A * A, B2 = B * B, C2 = C * C;
-          const R dang_ij = R(-0.375) * (A2 * A + A2 * (C + B) + A * (R(3) * C2 + R(2) * C * B + R(3) * B2)
-                                         - R(5) * (C - B) * (C - B) * (C + B)) * ir5;
-          const R dang_ik = R(-0.375) * (B2 * B + B2 * (C + A) + B * (R(3) * C2 + R(2) * C * A + R(3) * A2)
-                                         - R(5) * (C - A) * (C - A) * (C + A)) * ir5;
-          const R dang_jk = R(-0.375) * (C2 * C + C2 * (B + A) + C * (R(3) * B2 + R(2) * B * A + R(3) * A2)
-                                         - R(5) * (B - A) * (B - A) * (B + A)) * ir5;
-          const R angd = ang * dfd;
-          const R brinv = T.rinv[bl];
-          R fij = cw * (angd - dang_ij * fd) * air2;
-          R fik = cw * (angd - dang_ik * fd) * (brinv * brinv);
-          R fjk = cw * (angd - dang_jk * fd) * (sc * sc);
-          if (kSmooth) {  // atm.f90:754-790, switching-function terms
-            const R e0 = ang * fd * cw;
-            fij = sw * fij - e0 * dswa * swb * swc;
-            fik = sw * fik - e0 * dswb * swa * swc;
-            fjk = sw * fjk - e0 * dswc * swa * swb;
-          }
-          const R vjx = R(vjxd), vjy = R(vjyd), vjz = R(vjzd);
-          const R gx = fjk * vjx, gy = fjk * vjy, gz = fjk * vjz;
-          Fa += double(fij);
-          Ea += double(dE);
-          Gax -= double(gx); Gay -= double(gy); Gaz -= double(gz);
-          Dqa += double(dE * q.y);
-          atomicAdd(&T.acc[0][bl], double(fik));
-          atomicAdd(&T.acc[1][bl], double(dE));
-          atomicAdd(&T.acc[2][bl], double(gx));
-          atomicAdd(&T.acc[3][bl], double(gy));
-          atomicAdd(&T.acc[4][bl], double(gz));
-          atomicAdd(&T.acc[5][bl], double(dE * q.z));
-          sxx += double(gx * vjx); syy += double(gy * vjy); szz += double(gz * vjz);
-          sxy += double(gx * vjy); sxz += double(gx * vjz); syz += double(gy * vjz);
-        }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    if (kGrad && bq < m) {
-      const double F = T.acc[0][l], E = T.acc[1][l];
-      if (F != 0.0 || E != 0.0) {
-        const int k = sq.atom;
-        const double vx = sq.vx, vy = sq.vy, vz = sq.vz;
-        atomicAdd(grad + 3 * k, F * vx + T.acc[2][l]);
-        atomicAdd(grad + 3 * k + 1, F * vy + T.acc[3][l]);
-        atomicAdd(grad + 3 * k + 2, F * vz + T.acc[4][l]);
-        atomicAdd(dedcn + k, -0.5 * (E * double(sq.qx) + T.acc[5][l]));
-        gix -= F * vx; giy -= F * vy; giz -= F * vz;
-        di -= 0.5 * E * double(sq.qi);
-        sxx += F * vx * vx; syy += F * vy * vy; szz += F * vz * vz;
-        sxy += F * vx * vy; sxz += F * vx * vz; syz += F * vy * vz;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (kGrad) {
-    comb[w][0][l] = Fa; comb[w][1][l] = Ea; comb[w][2][l] = Gax;
-    comb[w][3][l] = Gay; comb[w][4][l] = Gaz; comb[w][5][l] = Dqa;
-  }
+  // A tile (shared)
+  if (w == 0) load_tile(tileA, slots, L0, P * kTile + l, m, l);
+  Acc* __restrict__ accA = acc[w][0];
+  Acc* __restrict__ accB = acc[w][1];
+  if (kGrad)
+    for (int c = 0; c < kAcc; ++c) accA[c * kTile + l] = accB[c * kTile + l] = Acc(0);
   __syncthreads();
-  if (kGrad && w == 0 && aval) {
-    double F = 0, E = 0, Gx = 0, Gy = 0, Gz = 0, Dq = 0;
-    for (int v = 0; v < kWavesPerBlock; ++v) {
-      F += comb[v][0][l]; E += comb[v][1][l]; Gx += comb[v][2][l];
-      Gy += comb[v][3][l]; Gz += comb[v][4][l]; Dq += comb[v][5][l];
+  const bool aval = P * kTile + l < m;
+  const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
+  const int jat = tileA.atom[l];
+  const unsigned arow = aval ? static_cast<unsigned>(jat - ab0) * static_cast<unsigned>(n) : 0u;
+  const unsigned aself = aval && jat == i ? 1u << 16 : 0u;
+
+  double ei = 0.0, di = 0.0;
+  double aM = 0.0, aKx = 0.0, aKy = 0.0, aKz = 0.0, aD = 0.0;  // a-slot l, this wavefront
+  double gi[3] = {0.0, 0.0, 0.0};
+  double sg[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  PairEntry<R>* __restrict__ q_ring = ring[w];
+
+  for (int Q = P + w; Q < nb; Q += kW) {
+    const bool diag = Q == P;
+    TileSlots<R>& TB = diag ? tileA : tileB[w];
+    Acc* __restrict__ accb = diag ? accA : accB;  // b-slots of a diagonal tile are a-slots
+    if (!diag) load_tile(TB, slots, L0, Q * kTile + l, m, l);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const int s0 = diag ? 1 : 0, s1 = diag ? kTile / 2 : kTile - 1;
+    int st = s0, head = 0, tail = 0;
+    while (true) {
+      // test phase: append the accepted pairs of the next steps
+      for (; st <= s1 && tail - head <= kQueue - kWave; ++st) {
+        const int bl = (l + st) & (kTile - 1);
+        double v0, v1, v2;
+        const double Cd = diff_norm2(TB.vx[bl], TB.vy[bl], TB.vz[bl], ax, ay, az, v0, v1, v2);
+        const bool ok = aval && (!diag || st < kTile / 2 || l < kTile / 2) && Cd <= cut2 && Cd >= kEps;
+        const unsigned long long bal = __ballot(ok);
+        if (ok) {
+          const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                     static_cast<unsigned>(bal >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
+          const int kat = TB.atom[bl];
+          PairEntry<R> e;
+          e.ab = static_cast<unsigned>(l | (bl << 8)) | (kat == i ? aself : 0u);
+          e.rec = arow + static_cast<unsigned>(kat - ab0);
+          e.C = R(Cd);
+          q_ring[pos & (kQueue - 1)] = e;
+        }
+        tail += __popcll(bal);
+      }
+      const bool done = st > s1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      // term phase: one queued pair per lane; the entries and pair records of
+      // the next 64 entries are fetched before the current ones are used
+      PairEntry<R> en{};
+      R4<R> qn{};
+      bool pf = false;
+      while (tail - head >= kWave || (done && tail > head)) {
+        const bool act = head + l < tail;
+        PairEntry<R> e = en;
+        R4<R> q = qn;
+        if (!pf && act) {
+          e = q_ring[(head + l) & (kQueue - 1)];
+          q = prow[e.rec];
+        }
+        const int nh = head + kWave;
+        pf = tail - nh >= kWave || (done && tail > nh);
+        if (pf && nh + l < tail) {
+          en = q_ring[(nh + l) & (kQueue - 1)];
+          qn = prow[en.rec];
+        }
+        if (act) {
+          const int a = e.ab & 0xff, bb = (e.ab >> 8) & 0xff;
+          const R C = e.C;
+          const R4<R> a0 = tileA.g0[a], a1 = tileA.g1[a];
+          const R4<R> b0 = TB.g0[bb], b1 = TB.g1[bb];
+          const R A = a0.w, B = b0.w;
+          const R AB = A * B;
+          const R S = AB * C;
+          const R ir1 = rsqrt_fast(S);
+          const R ir2 = ir1 * ir1;
+          const R ir5 = ir2 * ir2 * ir1;
+          const R dab = A - B;
+          const R t1 = (A + B) - C, t2 = dab + C, t3 = C - dab;
+          const R u = t2 * t3, v = t1 * t3, wv = t1 * t2;
+          const R Pt = t1 * u;
+          const R core = R(0.375) * Pt + S;
+          const R ang = ir5 * core;
+          const R pp = a1.x * b1.x * (q.w * pow_pos(C, malp6));
+          const R fd = rcp_fast(R(1) + R(6) * pp);
+          R c9 = ns9 * a1.y * b1.y * q.x;
+          if (e.ab >> 16) c9 *= R(4.0 / 3.0);  // j == k == i
+          const R cwf = c9 * fd;
+          R dE = ang * cwf;
+          R sw = R(1), swa = R(1), dswa = R(0), swb = R(1), dswb = R(0), swc = R(1), dswc = R(0);
+          if (kSmooth) {
+            const R ra = d_sqrt(A), rb = d_sqrt(B), rc = d_sqrt(C);
+            smooth_cutoff(ra, cutr, width, swa, dswa);
+            smooth_cutoff(rb, cutr, width, swb, dswb);
+            smooth_cutoff(rc, cutr, width, swc, dswc);
+            dswa /= ra; dswb /= rb; dswc /= rc;
+            sw = swa * swb * swc;
+          }
+          if (kGrad) {
+            const R Y = alp2 * pp * fd;
+            const R sgm = u + v + wv;
+            const R K1 = R(-1.875) * Pt - R(3) * S + Y * core;
+            const R hA = R(0.75) * A * (sgm - R(2) * wv) + K1;
+            const R hB = R(0.75) * B * (sgm - R(2) * v) + K1;
+            const R hC = R(0.75) * C * (sgm - R(2) * u) + K1;
+            const R om = -cwf * ir5 * ir2;  // 1/X = (S/X) / S
+            R fA = om * hA * (B * C);
+            R fB = om * hB * (A * C);
+            R fC = om * hC * AB;
+            if (kSmooth) {  // atm.f90:754-790, switching-function terms
+              fA = sw * fA - dE * dswa * swb * swc;
+              fB = sw * fB - dE * dswb * swa * swc;
+              fC = sw * fC - dE * dswc * swa * swb;
+              dE *= sw;
+            }
+            di += double(dE * (a1.z + b1.z));
+            atomicAdd(&accA[0 * kTile + a], Acc(fA + fC));
+            atomicAdd(&accA[1 * kTile + a], Acc(fC * b0.x));
+            atomicAdd(&accA[2 * kTile + a], Acc(fC * b0.y));
+            atomicAdd(&accA[3 * kTile + a], Acc(fC * b0.z));
+            atomicAdd(&accA[4 * kTile + a], Acc(dE * (a1.w + q.y)));
+            atomicAdd(&accb[0 * kTile + bb], Acc(fB + fC));
+            atomicAdd(&accb[1 * kTile + bb], Acc(fC * a0.x));
+            atomicAdd(&accb[2 * kTile + bb], Acc(fC * a0.y));
+            atomicAdd(&accb[3 * kTile + bb], Acc(fC * a0.z));
+            atomicAdd(&accb[4 * kTile + bb], Acc(dE * (b1.w + q.z)));
+          } else if (kSmooth) {
+            dE *= sw;
+          }
+          ei -= double(dE);
+        }
+        head = nh;
+      }
+      // the next test phase may overwrite the ring entries read above
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      if (done) break;
     }
-    if (F != 0.0 || E != 0.0) {
-      const double vx = sa.vx, vy = sa.vy, vz = sa.vz;
-      atomicAdd(grad + 3 * jat, F * vx + Gx);
-      atomicAdd(grad + 3 * jat + 1, F * vy + Gy);
-      atomicAdd(grad + 3 * jat + 2, F * vz + Gz);
-      atomicAdd(dedcn + jat, -0.5 * (E * double(sa.qx) + Dq));
-      gix -= F * vx; giy -= F * vy; giz -= F * vz;
-      di -= 0.5 * E * double(sa.qi);
-      sxx += F * vx * vx; syy += F * vy * vy; szz += F * vz * vz;
-      sxy += F * vx * vy; sxz += F * vx * vz; syz += F * vy * vz;
+    if (kGrad) {
+      // a-slot l: add this tile pair's sums in double; flush the b-slots
+      aM += double(accA[0 * kTile + l]); aKx += double(accA[1 * kTile + l]);
+      aKy += double(accA[2 * kTile + l]); aKz += double(accA[3 * kTile + l]);
+      aD += double(accA[4 * kTile + l]);
+      for (int c = 0; c < kAcc; ++c) accA[c * kTile + l] = Acc(0);
+      if (!diag) {
+        const double M = accB[0 * kTile + l], Kx = accB[1 * kTile + l], Ky = accB[2 * kTile + l];
+        const double Kz = accB[3 * kTile + l], D = accB[4 * kTile + l];
+        for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = Acc(0);
+        if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], TB.atom[l], grad, dedcn, gi, sg);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (kGrad) {  // combine the a-slot sums of the wavefronts (tileB is free now)
+    __syncthreads();
+    double* comb = reinterpret_cast<double*>(&tileB[0]);
+    double* cw = comb + w * kAcc * kTile;
+    cw[0 * kTile + l] = aM; cw[1 * kTile + l] = aKx; cw[2 * kTile + l] = aKy;
+    cw[3 * kTile + l] = aKz; cw[4 * kTile + l] = aD;
+    __syncthreads();
+    if (w == 0 && aval) {
+      double t[kAcc] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int v = 0; v < kW; ++v)
+        for (int c = 0; c < kAcc; ++c) t[c] += comb[(v * kAcc + c) * kTile + l];
+      flush_slot(t[0], t[1], t[2], t[3], t[4], ax, ay, az, jat, grad, dedcn, gi, sg);
     }
   }
   block_add(e_atom + i, ei, red);
   if (kGrad) {
-    block_add(grad + 3 * i, gix, red);
-    block_add(grad + 3 * i + 1, giy, red);
-    block_add(grad + 3 * i + 2, giz, red);
-    block_add(dedcn + i, di, red);
-    block_add(sig_atom + 6 * i + 0, sxx, red);
-    block_add(sig_atom + 6 * i + 1, syy, red);
-    block_add(sig_atom + 6 * i + 2, szz, red);
-    block_add(sig_atom + 6 * i + 3, sxy, red);
-    block_add(sig_atom + 6 * i + 4, sxz, red);
-    block_add(sig_atom + 6 * i + 5, syz, red);
+    block_add(grad + 3 * i, gi[0], red);
+    block_add(grad + 3 * i + 1, gi[1], red);
+    block_add(grad + 3 * i + 2, gi[2], red);
+    block_add(dedcn + i, -0.5 * di, red);
+    for (int c = 0; c < 6; ++c) block_add(sig_atom + 6 * i + c, sg[c], red);
+  }
+  if (!work_counter) break;
+  // block_add ended with a barrier: every wavefront is done with the LDS
+  if (threadIdx.x == 0) next_item = static_cast<long long>(atomicAdd(work_counter, 1ull));
+  __syncthreads();
+  k_item = next_item;
+  __syncthreads();  // next_item is rewritten only after every thread has read it
   }
 }
 
@@ -945,10 +1069,10 @@ void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long
   const dim3 grid(blocks_for(b.natoms, kWavesPerBlock)), blk(kBlock);
   if (mixed)
     hipLaunchKernelGGL((atm_list_kernel<false, float>), grid, blk, 0, st, b, static_cast<const float4*>(pair), count,
-                       (const long long*)nullptr, (Slot<float>*)nullptr);
+                       (const long long*)nullptr, (Slot<float>*)nullptr, 0LL);
   else
     hipLaunchKernelGGL((atm_list_kernel<false, double>), grid, blk, 0, st, b, static_cast<const double4*>(pair),
-                       count, (const long long*)nullptr, (Slot<double>*)nullptr);
+                       count, (const long long*)nullptr, (Slot<double>*)nullptr, 0LL);
 }
 
 void launch_scan(const long long* count, int n, long long* offset, hipStream_t st) {
@@ -956,14 +1080,29 @@ void launch_scan(const long long* count, int n, long long* offset, hipStream_t s
 }
 
 void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long long* offset, void* slots,
-                     hipStream_t st) {
+                     long long capacity, hipStream_t st) {
   const dim3 grid(blocks_for(b.natoms, kWavesPerBlock)), blk(kBlock);
   if (mixed)
     hipLaunchKernelGGL((atm_list_kernel<true, float>), grid, blk, 0, st, b, static_cast<const float4*>(pair),
-                       (long long*)nullptr, offset, static_cast<Slot<float>*>(slots));
+                       (long long*)nullptr, offset, static_cast<Slot<float>*>(slots), capacity);
   else
     hipLaunchKernelGGL((atm_list_kernel<true, double>), grid, blk, 0, st, b, static_cast<const double4*>(pair),
-                       (long long*)nullptr, offset, static_cast<Slot<double>*>(slots));
+                       (long long*)nullptr, offset, static_cast<Slot<double>*>(slots), capacity);
+}
+
+namespace {
+__global__ void atm_guard_kernel(const long long* __restrict__ offset, int n, long long capacity,
+                                 long long* __restrict__ item_off, int* __restrict__ overflow) {
+  if (offset[n] > capacity) {
+    *overflow = 1;
+    item_off[n] = 0;
+  }
+}
+}  // namespace
+
+void launch_atm_guard(const long long* offset, int n, long long capacity, long long* item_off, int* overflow,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(atm_guard_kernel, dim3(1), dim3(1), 0, st, offset, n, capacity, item_off, overflow);
 }
 
 void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t st) {
@@ -971,25 +1110,52 @@ void launch_atm_items(const long long* offset, int n, long long* item_count, hip
 }
 
 namespace {
-template <typename R, bool kGrad, bool kSmooth>
+__global__ void zero_counter(unsigned long long* c) { *c = 0ull; }
+
+template <typename R, bool kGrad, bool kSmooth, int kW>
 void launch_atm_t(const DevBatch& b, const long long* item_off, long long max_items, const long long* offset,
-                  const void* slots, const DevWork& w, hipStream_t st) {
-  hipLaunchKernelGGL((atm_kernel<R, kGrad, kSmooth>), dim3(static_cast<unsigned>(max_items)), dim3(kBlock), 0, st,
-                     b, item_off, offset, static_cast<const Slot<R>*>(slots), static_cast<const R4<R>*>(w.pair),
-                     w.e_atom, w.dedcn, w.grad, w.sig_atom);
+                  const void* slots, const DevWork& w, const AtmShape& shape, unsigned long long* counter,
+                  hipStream_t st) {
+  long long grid = max_items;
+  unsigned long long* wc = nullptr;
+  if (shape.blocks_per_cu > 0) {
+    grid = std::min<long long>(max_items, static_cast<long long>(shape.blocks_per_cu) * shape.num_cus);
+    wc = counter;
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, st, wc);
+  }
+  hipLaunchKernelGGL((atm_kernel<R, kGrad, kSmooth, kW>), dim3(static_cast<unsigned>(grid)), dim3(kW * kWave), 0,
+                     st, b, item_off, offset, static_cast<const Slot<R>*>(slots),
+                     static_cast<const R4<R>*>(w.pair), w.e_atom, w.dedcn, w.grad, w.sig_atom, wc);
+}
+
+template <int kW>
+void launch_atm_w(const DevBatch& b, bool mixed, bool grad, bool smooth, const long long* item_off,
+                  long long max_items, const long long* offset, const void* slots, const DevWork& w,
+                  const AtmShape& shape, unsigned long long* counter, hipStream_t st) {
+  using L = void (*)(const DevBatch&, const long long*, long long, const long long*, const void*, const DevWork&,
+                     const AtmShape&, unsigned long long*, hipStream_t);
+  static const L table[8] = {
+      launch_atm_t<double, false, false, kW>, launch_atm_t<double, false, true, kW>,
+      launch_atm_t<double, true, false, kW>,  launch_atm_t<double, true, true, kW>,
+      launch_atm_t<float, false, false, kW>,  launch_atm_t<float, false, true, kW>,
+      launch_atm_t<float, true, false, kW>,   launch_atm_t<float, true, true, kW>};
+  table[(mixed ? 4 : 0) + (grad ? 2 : 0) + (smooth ? 1 : 0)](b, item_off, max_items, offset, slots, w, shape,
+                                                             counter, st);
 }
 }  // namespace
 
-void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const long long* item_off,
-                long long max_items, const long long* offset, const void* slots, const DevWork& w, hipStream_t st) {
+void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const AtmShape& shape,
+                const long long* item_off, long long max_items, const long long* offset, const void* slots,
+                const DevWork& w, unsigned long long* counter, hipStream_t st) {
   if (max_items <= 0) return;
-  using L = void (*)(const DevBatch&, const long long*, long long, const long long*, const void*, const DevWork&,
-                     hipStream_t);
-  static const L table[8] = {
-      launch_atm_t<double, false, false>, launch_atm_t<double, false, true>, launch_atm_t<double, true, false>,
-      launch_atm_t<double, true, true>,   launch_atm_t<float, false, false>, launch_atm_t<float, false, true>,
-      launch_atm_t<float, true, false>,   launch_atm_t<float, true, true>};
-  table[(mixed ? 4 : 0) + (grad ? 2 : 0) + (smooth ? 1 : 0)](b, item_off, max_items, offset, slots, w, st);
+  if (shape.blocks_per_cu > 0 && (!counter || shape.num_cus <= 0))
+    throw std::invalid_argument("persistent three-body launch needs a work counter and the CU count");
+  if (shape.waves == 2)
+    launch_atm_w<2>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
+  else if (shape.waves == 4)
+    launch_atm_w<4>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
+  else
+    throw std::invalid_argument("three-body launch: wavefronts per block must be 2 or 4");
 }
 
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t st) {

@@ -56,14 +56,31 @@ void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, hi
 // Three-body lists: count per atom (pair may be null: no C6 test), fill slots.
 void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, hipStream_t stream);
 void launch_scan(const long long* count, int n, long long* offset, hipStream_t stream);
+// Writes nothing when the lists hold more than `capacity` slots.
 void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long long* offset, void* slots,
-                     hipStream_t stream);
+                     long long capacity, hipStream_t stream);
+// When the lists hold more than `capacity` slots: sets *overflow = 1 and
+// empties the work-item range (item_off[n] = 0), so no triple is evaluated.
+void launch_atm_guard(const long long* offset, int n, long long capacity, long long* item_off, int* overflow,
+                      hipStream_t stream);
 // Work items (atom i, slot block P) of the tiled three-body kernel.
 void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t stream);
-// smooth: some system has a nonzero three-body switching width.
-void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const long long* item_off,
-                long long max_items, const long long* offset, const void* slots, const DevWork& w,
-                hipStream_t stream);
+
+// Launch shape of the three-body kernel: wavefronts per block (2 or 4, the
+// block shares one 64-slot tile of a list) and either one block per work
+// item (blocks_per_cu = 0) or a persistent grid of blocks_per_cu blocks per
+// compute unit drawing items from a work counter.
+struct AtmShape {
+  int waves = 4;
+  int blocks_per_cu = 0;
+  int num_cus = 0;
+};
+// smooth: some system has a nonzero three-body switching width.  max_items
+// bounds the number of work items (item_off[natoms]); counter is device
+// memory for the work counter of a persistent shape.
+void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const AtmShape& shape,
+                const long long* item_off, long long max_items, const long long* offset, const void* slots,
+                const DevWork& w, unsigned long long* counter, hipStream_t stream);
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t stream);
 void launch_reduce(const DevBatch& b, const DevWork& w, double* energy, double* sigma, hipStream_t stream);
 void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream_t stream);

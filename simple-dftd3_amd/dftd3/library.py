@@ -59,6 +59,18 @@ _SIGNATURES = {
                                        ctypes.POINTER(ctypes.c_longlong), _vp]),
     "d3hip_set_kernel_timing": (None, [_vp, _vp, ctypes.c_int]),
     "d3hip_get_kernel_time": (None, [_vp, _vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "d3hip_new_plan": (_vp, [_vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "d3hip_delete_plan": (None, [ctypes.POINTER(_vp)]),
+    "d3hip_plan_execute": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d3hip_plan_num_shapes": (ctypes.c_int, []),
+    "d3hip_plan_candidate_shape": (None, [ctypes.c_int, _ip, _ip]),
+    "d3hip_plan_get_shape": (None, [_vp, _vp, _ip, _ip]),
+    "d3hip_plan_set_shape": (None, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "d3hip_plan_autotune": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_float), _vp]),
+    "d3hip_plan_capture": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d3hip_plan_replay": (None, [_vp, _vp, _vp]),
+    "d3hip_plan_overflowed": (ctypes.c_int, [_vp, _vp, _vp]),
 }
 
 for _name, (_res, _args) in _SIGNATURES.items():
@@ -109,6 +121,10 @@ class ModelHandle(Handle):
 
 class ParamHandle(Handle):
     _deleter = "dftd3_delete_param"
+
+
+class PlanHandle(Handle):
+    _deleter = "d3hip_delete_plan"
 
 
 class _Error:
@@ -246,3 +262,64 @@ def get_dispersion_batch(workspace: ModelHandle, natoms, numbers, lattices, peri
         workspace.handle, len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices),
         _cast(_bp, periodic), _cast(_dp, params), _cast(_dp, cutoffs), precision, positions_ptr, energy_ptr,
         gradient_ptr, sigma_ptr, stream)
+
+
+def new_plan(natoms, numbers, lattices, periodic, params, cutoffs, precision: int, grad: bool,
+             positions_ptr: int, stream: Optional[int]) -> PlanHandle:
+    """Plan a batch once for repeated evaluations (include/d3hip.h, plans)"""
+    return PlanHandle.with_gc(error_check(lib.d3hip_new_plan)(
+        len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices), _cast(_bp, periodic),
+        _cast(_dp, params), _cast(_dp, cutoffs), precision, 1 if grad else 0, positions_ptr, stream))
+
+
+def plan_execute(plan: PlanHandle, positions_ptr: int, energy_ptr: int, gradient_ptr: Optional[int],
+                 sigma_ptr: Optional[int], stream: Optional[int]) -> None:
+    """One evaluation of a plan on device pointers"""
+    return error_check(lib.d3hip_plan_execute)(plan.handle, positions_ptr, energy_ptr, gradient_ptr, sigma_ptr,
+                                               stream)
+
+
+def plan_candidate_shapes():
+    """[(wavefronts per block, persistent blocks per CU)] of the three-body kernel"""
+    out = []
+    for k in range(lib.d3hip_plan_num_shapes()):
+        w, b = ctypes.c_int(), ctypes.c_int()
+        lib.d3hip_plan_candidate_shape(k, ctypes.byref(w), ctypes.byref(b))
+        out.append((w.value, b.value))
+    return out
+
+
+def plan_get_shape(plan: PlanHandle):
+    w, b = ctypes.c_int(), ctypes.c_int()
+    error_check(lib.d3hip_plan_get_shape)(plan.handle, ctypes.byref(w), ctypes.byref(b))
+    return w.value, b.value
+
+
+def plan_set_shape(plan: PlanHandle, waves: int, blocks_per_cu: int) -> None:
+    return error_check(lib.d3hip_plan_set_shape)(plan.handle, waves, blocks_per_cu)
+
+
+def plan_autotune(plan: PlanHandle, positions_ptr: int, energy_ptr: int, gradient_ptr: Optional[int],
+                  sigma_ptr: Optional[int], reps: int, stream: Optional[int]):
+    """(index of the fastest shape, mean milliseconds of every candidate)"""
+    times = (ctypes.c_float * lib.d3hip_plan_num_shapes())()
+    best = error_check(lib.d3hip_plan_autotune)(plan.handle, positions_ptr, energy_ptr, gradient_ptr, sigma_ptr,
+                                                reps, times, stream)
+    return best, list(times)
+
+
+def plan_capture(plan: PlanHandle, positions_ptr: int, energy_ptr: int, gradient_ptr: Optional[int],
+                 sigma_ptr: Optional[int], stream: Optional[int]) -> None:
+    """Record one evaluation of the plan into a HIP graph"""
+    return error_check(lib.d3hip_plan_capture)(plan.handle, positions_ptr, energy_ptr, gradient_ptr, sigma_ptr,
+                                               stream)
+
+
+def plan_replay(plan: PlanHandle, stream: Optional[int]) -> None:
+    """Launch the captured graph of a plan"""
+    return error_check(lib.d3hip_plan_replay)(plan.handle, stream)
+
+
+def plan_overflowed(plan: PlanHandle, stream: Optional[int]) -> bool:
+    """True when an evaluation had more three-body slots than the plan holds"""
+    return bool(error_check(lib.d3hip_plan_overflowed)(plan.handle, stream))

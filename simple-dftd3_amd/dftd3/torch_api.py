@@ -139,3 +139,101 @@ def count_terms(spec: BatchSpec, positions: torch.Tensor) -> np.ndarray:
     with torch.cuda.device(dev):
         return library.count_terms_batch(workspace(dev), spec.natoms, spec.numbers, spec.lattices, spec.periodic,
                                          spec.params, spec.cutoffs, positions.data_ptr(), stream)
+
+
+class BatchPlan:
+    """A batch planned once and evaluated many times with new coordinates of
+    the same systems (include/d3hip.h, plans).
+
+    The plan owns a positions tensor and the output tensors; ``capture()``
+    records one evaluation of them into a HIP graph and ``replay()`` launches
+    it, so a caller writes new coordinates into ``plan.positions`` and
+    replays::
+
+        plan = BatchPlan(spec, positions, precision="mixed")
+        plan.autotune()            # three-body launch shape for this batch
+        plan.capture()
+        plan.positions.copy_(new_positions)
+        out = plan.replay()        # dict of energy, gradient, sigma tensors
+
+    ``execute(positions)`` runs one evaluation without a graph.  Every call
+    runs on the current stream of the plan's device; the capture uses a
+    stream of its own.
+    """
+
+    def __init__(self, spec: BatchSpec, positions: torch.Tensor, precision: str = "mixed", grad: bool = True):
+        _check_positions(positions)
+        self.spec = spec
+        self.grad = grad
+        self.precision = precision
+        self.device = positions.device
+        self.natoms = int(np.sum(spec.natoms))
+        prec = {"fp64": library.PRECISION_FP64, "mixed": library.PRECISION_MIXED}[precision]
+        self.positions = positions.clone()
+        nsys = len(spec)
+        self.out = dict(energy=torch.zeros(nsys, dtype=torch.float64, device=self.device))
+        if grad:
+            self.out["gradient"] = torch.zeros(self.natoms, 3, dtype=torch.float64, device=self.device)
+            self.out["sigma"] = torch.zeros(nsys, 3, 3, dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            self.handle = library.new_plan(spec.natoms, spec.numbers, spec.lattices, spec.periodic, spec.params,
+                                           spec.cutoffs, prec, grad, self.positions.data_ptr(), self._stream())
+        self._capture_stream = None
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _ptrs(self, out):
+        return (out["energy"].data_ptr(), out["gradient"].data_ptr() if self.grad else None,
+                out["sigma"].data_ptr() if self.grad else None)
+
+    def execute(self, positions: Optional[torch.Tensor] = None, out: Optional[dict] = None) -> dict:
+        """One evaluation (default: the plan's own positions and outputs)."""
+        pos = self.positions if positions is None else positions
+        _check_positions(pos)
+        if pos.shape[0] != self.natoms:
+            raise ValueError("positions do not match the planned batch")
+        out = self.out if out is None else out
+        with torch.cuda.device(self.device):
+            library.plan_execute(self.handle, pos.data_ptr(), *self._ptrs(out), self._stream())
+        return out
+
+    @property
+    def shape(self):
+        """(wavefronts per block, persistent blocks per CU) of the three-body kernel."""
+        return library.plan_get_shape(self.handle)
+
+    def set_shape(self, waves: int, blocks_per_cu: int) -> None:
+        library.plan_set_shape(self.handle, waves, blocks_per_cu)
+
+    def autotune(self, reps: int = 3):
+        """Time the candidate shapes on the plan's positions and keep the
+        fastest; returns (shape, {shape: mean ms per evaluation})."""
+        with torch.cuda.device(self.device):
+            best, times = library.plan_autotune(self.handle, self.positions.data_ptr(), *self._ptrs(self.out),
+                                                reps, self._stream())
+        shapes = library.plan_candidate_shapes()
+        return shapes[best], dict(zip(shapes, times))
+
+    def capture(self) -> None:
+        """Record one evaluation of ``positions`` -> ``out`` into a HIP graph."""
+        if self._capture_stream is None:
+            self._capture_stream = torch.cuda.Stream(device=self.device)
+        cur = torch.cuda.current_stream(self.device)
+        self._capture_stream.wait_stream(cur)
+        with torch.cuda.device(self.device):
+            library.plan_capture(self.handle, self.positions.data_ptr(), *self._ptrs(self.out),
+                                 self._capture_stream.cuda_stream)
+        cur.wait_stream(self._capture_stream)
+
+    def replay(self) -> dict:
+        """Launch the captured graph on the current stream; returns ``out``."""
+        with torch.cuda.device(self.device):
+            library.plan_replay(self.handle, self._stream())
+        return self.out
+
+    def overflowed(self) -> bool:
+        """True when an evaluation had more three-body slots than the plan
+        holds (its three-body terms were skipped).  Synchronizes."""
+        with torch.cuda.device(self.device):
+            return library.plan_overflowed(self.handle, self._stream())

@@ -171,3 +171,24 @@ def test_synthetic_workloads_are_seeded():
     d = np.linalg.norm(a[1][:, None] - a[1][None], axis=-1) + np.eye(200) * 1e9
     assert d.min() >= 1.0 * synthetic.AATOAU - 1e-12
     assert sorted(set(a[0].tolist())) == [1, 6, 7, 8]
+
+
+def test_plan_api_errors_without_a_gpu():
+    """The plan entry points validate their arguments before any GPU work."""
+    from dftd3 import library as L
+    shapes = L.plan_candidate_shapes()
+    assert (4, 0) in shapes and all(w in (2, 4) and b >= 0 for w, b in shapes)
+    buf = ctypes.create_string_buffer(128)
+    err = L._Error()
+    assert L.lib.d3hip_new_plan(err.handle, 0, None, None, None, None, None, None, 0, 1, None, None) is None
+    L.lib.dftd3_get_error(err.handle, buf, None)
+    assert buf.value.decode() == "Invalid batch description"
+    err = L._Error()
+    L.lib.d3hip_plan_execute(err.handle, None, None, None, None, None, None)
+    L.lib.dftd3_get_error(err.handle, buf, None)
+    assert buf.value.decode() == "Plan is missing"
+    err = L._Error()
+    assert L.lib.d3hip_plan_overflowed(err.handle, None, None) == -1
+    h = ctypes.c_void_p(None)
+    L.lib.d3hip_delete_plan(ctypes.byref(h))  # NULL is accepted
+    L.lib.d3hip_delete_plan(None)

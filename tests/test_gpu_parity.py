@@ -12,14 +12,12 @@ import numpy as np
 import pytest
 
 import oracle as O
+from helpers import BJ, ZERO, cell as _cell
 
 pytestmark = pytest.mark.gpu
 
 E_REL, G_ABS, S_ABS = 1e-10, 1e-9, 1e-9
 MIX_E_PER_ATOM, MIX_G = 1e-8, 1e-7
-
-BJ = dict(s6=1.0, s8=1.2177, a1=0.4145, a2=4.8593, s9=1.0, alp=14.0)
-ZERO = dict(s6=1.0, s8=1.703, rs6=1.261, rs8=1.0, s9=1.0, alp=14.0)
 
 
 def _step(msg):
@@ -163,25 +161,6 @@ def test_zero_damping_atm_ghost_and_smooth_cutoff():
     _check(res, ref, 60)
 
 
-def _cell(seed, n, edge):
-    rng = np.random.default_rng(seed)
-    lat = np.eye(3) * edge + rng.uniform(-0.08, 0.08, (3, 3)) * edge
-    frac = []
-    while len(frac) < n:
-        f = rng.uniform(0, 1, 3)
-        ok = True
-        for g in frac:
-            d = f - g
-            d -= np.round(d)
-            if np.linalg.norm(d @ lat) < 1.9:
-                ok = False
-                break
-        if ok:
-            frac.append(f)
-    num = rng.choice([1, 6, 7, 8, 14, 16], size=n)
-    return num, np.array(frac) @ lat, lat
-
-
 @pytest.mark.parametrize("kind", ["rational", "zero"])
 def test_periodic_cell_default_cutoffs(kind):
     from dftd3.interface import DispersionModel, RationalDampingParam, ZeroDampingParam
@@ -192,6 +171,26 @@ def test_periodic_cell_default_cutoffs(kind):
     cls = RationalDampingParam if kind == "rational" else ZeroDampingParam
     res = model.get_dispersion(cls(**p), grad=True)
     _check(res, _oracle(num, pos, lat, [True] * 3, kind, p, wrap=True), 16)
+
+
+@pytest.mark.parametrize("natoms,edge", [(1, 6.5), (2, 7.5)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_small_cell_self_image_triples(natoms, edge, precision):
+    """Cells much smaller than the three-body cutoff: most triples are built
+    from images of one atom (weights 1/6, 1/2 of the reference's
+    triple_scale) and long slot lists (many 64-slot tiles per atom)."""
+    _step(f"{natoms}-atom cell, edge {edge}, {precision}")
+    num, pos, lat = _cell(40 + natoms, natoms, edge)
+    r = _batch([dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3, param=BJ)], precision=precision)[0]
+    _check(r, _oracle(num, pos, lat, [True] * 3, "rational", BJ), natoms, mixed=precision == "mixed")
+
+
+def test_periodic_mixed_long_lists():
+    """Mixed path on a dense periodic cell (about 80 tiles per slot list)."""
+    _step("32-atom cell, mixed, BJ + ATM")
+    num, pos, lat = _cell(51, 32, 12.0)
+    r = _batch([dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3, param=BJ)], precision="mixed")[0]
+    _check(r, _oracle(num, pos, lat, [True] * 3, "rational", BJ), 32, mixed=True)
 
 
 def test_periodic_2d_slab():
