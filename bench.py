@@ -50,6 +50,18 @@ def workload(config):
         num, pos = synthetic.molecule_c0()
         sysd = dict(numbers=num, positions=pos, kind="rational", param=dict(synthetic.PBE0_BJ))
         return [sysd], dict(workload="config0-molecule-200-atoms-pbe0-d3bj-atm", natoms=len(num), cells=1)
+    if config == "c2":
+        num, pos = synthetic.ellipsoid_c2()
+        sysd = dict(numbers=num, positions=pos, kind="rational", param=dict(synthetic.PBE0_BJ))
+        return [sysd], dict(workload="config2-ellipsoid-20000-atoms-pbe0-d3bj-atm", natoms=len(num), cells=1,
+                            elements="H/C/N/O/S protein-like")
+    if config == "c3":
+        systems = []
+        for num, pos, param, kind in synthetic.batch_c3():
+            systems.append(dict(numbers=num, positions=pos, kind="rational" if kind == 0 else "zero", param=param))
+        return systems, dict(workload="config3-batch-4096-molecules-10-300-atoms-bj-zero-atm",
+                             natoms=int(sum(len(s["numbers"]) for s in systems)), cells=len(systems),
+                             elements="H/B/C/N/O/F/P/S/Cl")
     raise SystemExit(f"unknown config {config}")
 
 
@@ -75,6 +87,73 @@ def cpu_baseline(systems, counts, stride, threads):
                 sample=f"oracle E+grad+sigma with the outer atom loops restricted to every {stride}-th atom "
                        f"({t:.2f} s), scaled by {stride} to one full evaluation ({t_full:.1f} s)",
                 seconds_per_eval_estimate=t_full)
+
+
+def bench_batched(args, systems, config, spec, positions, counts):
+    """Config 3: the whole batch as one planned evaluation (HIP graph of the
+    kernel sequence, three-body launch shape autotuned), FP64 and mixed,
+    against one FP64 evaluation per molecule."""
+    import torch
+    from dftd3 import torch_api
+    nmol = len(systems)
+    res, outs = {}, {}
+    for prec in ("fp64", "mixed"):
+        plan = torch_api.BatchPlan(spec, positions, precision=prec)
+        best, times = plan.autotune(reps=2)
+        log(f"{prec}: three-body shape {best} chosen; " +
+            ", ".join(f"{w}w/{b}bpc {t:.2f} ms" for (w, b), t in times.items()))
+        plan.capture()
+        for _ in range(args.warmup):
+            plan.replay()
+        t = timed_steps(plan.replay, args.steps, torch.cuda.synchronize)
+        if plan.overflowed():
+            raise RuntimeError("three-body lists exceeded the planned capacity")
+        outs[prec] = {k: v.clone() for k, v in plan.out.items()}
+        res[prec] = dict(ms_per_step=t * 1e3, molecules_per_s=nmol / t, pair_terms_per_s=counts[1] / t,
+                         triple_terms_per_s=counts[2] / t, shape=list(best),
+                         autotune_ms={f"{w}x{b}": round(v, 4) for (w, b), v in times.items()})
+        log(f"{prec}: batched graph replay {t * 1e3:.2f} ms ({nmol / t:.0f} molecules/s)")
+        del plan
+    # one FP64 evaluation per molecule (each call plans, launches and reads
+    # back its list size)
+    singles = []
+    for s in systems:
+        sp, p = torch_api.batch_spec([s])
+        pt = torch.as_tensor(p, dtype=torch.float64, device=positions.device).contiguous()
+        singles.append((sp, pt, torch_api.run_batch(sp, pt, precision="fp64")))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sp, pt, out in singles:
+        torch_api.run_batch(sp, pt, precision="fp64", out=out)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    log(f"fp64 per-molecule launches: {t_single * 1e3:.1f} ms for {nmol} molecules")
+    e_single = torch.cat([o["energy"] for _, _, o in singles])
+    natoms = spec.natoms.astype(float)
+    de = ((outs["mixed"]["energy"] - outs["fp64"]["energy"]).abs().cpu().numpy() / natoms).max()
+    dg = (outs["mixed"]["gradient"] - outs["fp64"]["gradient"]).abs().max().item()
+    ds = (outs["mixed"]["sigma"] - outs["fp64"]["sigma"]).abs().max().item()
+    d_single = ((outs["fp64"]["energy"] - e_single).abs().cpu().numpy() / natoms).max()
+    mixed, fp64 = res["mixed"], res["fp64"]
+    line = dict(
+        metric=METRIC, value=mixed["pair_terms_per_s"], unit="pair-terms/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(config, pair_terms=int(counts[1]), triple_terms=int(counts[2]), cn_terms=int(counts[0]),
+                    parallelism="single-gpu", accumulation="f64", launch="hip-graph replay of a planned batch"),
+        molecules_per_s=mixed["molecules_per_s"], triple_terms_per_s=mixed["triple_terms_per_s"],
+        fp64_batched=fp64,
+        fp64_per_molecule=dict(molecules_per_s=nmol / t_single, ms_total=t_single * 1e3,
+                               pair_terms_per_s=counts[1] / t_single),
+        mixed_batched_over_fp64_per_molecule=(nmol / mixed["ms_per_step"] * 1e3) / (nmol / t_single),
+        mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"], mixed_shape=mixed["shape"],
+        mixed_autotune_ms=mixed["autotune_ms"],
+        mixed_vs_fp64_error=dict(energy_per_atom=float(de), max_gradient=dg, max_sigma=ds),
+        fp64_batched_vs_per_molecule_energy_per_atom=float(d_single),
+        roofline=None,
+    )
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def main():
@@ -111,6 +190,8 @@ def main():
     positions = torch.as_tensor(pos, dtype=torch.float64, device=dev).contiguous()
     counts = torch_api.count_terms(spec, positions).sum(axis=0)
     log(f"{config['workload']}: CN terms {counts[0]}, pair terms {counts[1]}, triple terms {counts[2]}")
+    if args.config == "c3":
+        return bench_batched(args, systems, config, spec, positions, counts)
     ws = torch_api.workspace(dev)
     outs = {}
     res = {}
@@ -154,7 +235,7 @@ def main():
         mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"],
         mixed_vs_fp64_error=dict(energy_per_atom=de / natoms, max_gradient=dg, max_sigma=ds),
         kernel_ms=dict(fp64=fp64["kernel_ms"], mixed=mixed["kernel_ms"]),
-        roofline=dict(kernel="atm_kernel<float,true>", bound="valu", achieved=mixed["atm_tflops"],
+        roofline=dict(kernel="atm_kernel<float, true, false, 4>", bound="valu", achieved=mixed["atm_tflops"],
                       peak=PEAK_TFLOPS["mixed"], unit="TFLOP/s", frac=mixed["atm_tflops"] / PEAK_TFLOPS["mixed"],
                       traffic=None, ops_per_triple=OPS_PER_TRIPLE, triples_per_launch=int(counts[2]),
                       launch_ms=mixed["atm_kernel_ms"]),
