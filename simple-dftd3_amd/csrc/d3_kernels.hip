@@ -631,12 +631,13 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   constexpr int kAccLen = kGrad ? kAcc * kTile : 1;
   __shared__ TileSlots<R> tileA;
   __shared__ TileSlots<R> tileB[kW];
-  using Acc = double;  // LDS sums of a tile pair (FP32 LDS atomics measured several times slower)
-  __shared__ Acc acc[kW][2][kAccLen];  // a-side and b-side sums of the current tile pair
+  // per-slot sums in double (FP32 LDS atomics measured several times slower):
+  // the A tile's for the whole item, each wavefront's B tile for one tile pair
+  __shared__ double accA[kAccLen];
+  __shared__ double accBw[kW][kAccLen];
   __shared__ PairEntry<R> ring[kW][kQueue];
   __shared__ double red[kW];
   __shared__ long long next_item;
-  static_assert(sizeof(tileB) >= sizeof(double) * kW * kAcc * kTile, "a-side combine buffer");
   const long long total = item_off[b.natoms];
   long long k_item = blockIdx.x;
   if (work_counter) {
@@ -673,11 +674,14 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = wave_lane();
 
   // A tile (shared)
-  if (w == 0) load_tile(tileA, slots, L0, P * kTile + l, m, l);
-  Acc* __restrict__ accA = acc[w][0];
-  Acc* __restrict__ accB = acc[w][1];
+  double* __restrict__ accB = accBw[w];
+  if (w == 0) {
+    load_tile(tileA, slots, L0, P * kTile + l, m, l);
+    if (kGrad)
+      for (int c = 0; c < kAcc; ++c) accA[c * kTile + l] = 0.0;
+  }
   if (kGrad)
-    for (int c = 0; c < kAcc; ++c) accA[c * kTile + l] = accB[c * kTile + l] = Acc(0);
+    for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
   __syncthreads();
   const bool aval = P * kTile + l < m;
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
@@ -686,7 +690,6 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   const unsigned aself = aval && jat == i ? 1u << 16 : 0u;
 
   double ei = 0.0, di = 0.0;
-  double aM = 0.0, aKx = 0.0, aKy = 0.0, aKz = 0.0, aD = 0.0;  // a-slot l, this wavefront
   double gi[3] = {0.0, 0.0, 0.0};
   double sg[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   PairEntry<R>* __restrict__ q_ring = ring[w];
@@ -694,7 +697,7 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   for (int Q = P + w; Q < nb; Q += kW) {
     const bool diag = Q == P;
     TileSlots<R>& TB = diag ? tileA : tileB[w];
-    Acc* __restrict__ accb = diag ? accA : accB;  // b-slots of a diagonal tile are a-slots
+    double* __restrict__ accb = diag ? accA : accB;  // b-slots of a diagonal tile are a-slots
     if (!diag) load_tile(TB, slots, L0, Q * kTile + l, m, l);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -726,24 +729,10 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
       __builtin_amdgcn_wave_barrier();
       // term phase: one queued pair per lane; the entries and pair records of
       // the next 64 entries are fetched before the current ones are used
-      PairEntry<R> en{};
-      R4<R> qn{};
-      bool pf = false;
-      while (tail - head >= kWave || (done && tail > head)) {
-        const bool act = head + l < tail;
-        PairEntry<R> e = en;
-        R4<R> q = qn;
-        if (!pf && act) {
-          e = q_ring[(head + l) & (kQueue - 1)];
-          q = prow[e.rec];
-        }
-        const int nh = head + kWave;
-        pf = tail - nh >= kWave || (done && tail > nh);
-        if (pf && nh + l < tail) {
-          en = q_ring[(nh + l) & (kQueue - 1)];
-          qn = prow[en.rec];
-        }
-        if (act) {
+      for (; tail - head >= kWave || (done && tail > head); head += kWave) {
+        if (head + l < tail) {
+          const PairEntry<R> e = q_ring[(head + l) & (kQueue - 1)];
+          const R4<R> q = prow[e.rec];
           const int a = e.ab & 0xff, bb = (e.ab >> 8) & 0xff;
           const R C = e.C;
           const R4<R> a0 = tileA.g0[a], a1 = tileA.g1[a];
@@ -793,58 +782,40 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
               dE *= sw;
             }
             di += double(dE * (a1.z + b1.z));
-            atomicAdd(&accA[0 * kTile + a], Acc(fA + fC));
-            atomicAdd(&accA[1 * kTile + a], Acc(fC * b0.x));
-            atomicAdd(&accA[2 * kTile + a], Acc(fC * b0.y));
-            atomicAdd(&accA[3 * kTile + a], Acc(fC * b0.z));
-            atomicAdd(&accA[4 * kTile + a], Acc(dE * (a1.w + q.y)));
-            atomicAdd(&accb[0 * kTile + bb], Acc(fB + fC));
-            atomicAdd(&accb[1 * kTile + bb], Acc(fC * a0.x));
-            atomicAdd(&accb[2 * kTile + bb], Acc(fC * a0.y));
-            atomicAdd(&accb[3 * kTile + bb], Acc(fC * a0.z));
-            atomicAdd(&accb[4 * kTile + bb], Acc(dE * (b1.w + q.z)));
+            atomicAdd(&accA[0 * kTile + a], double(fA + fC));
+            atomicAdd(&accA[1 * kTile + a], double(fC * b0.x));
+            atomicAdd(&accA[2 * kTile + a], double(fC * b0.y));
+            atomicAdd(&accA[3 * kTile + a], double(fC * b0.z));
+            atomicAdd(&accA[4 * kTile + a], double(dE * (a1.w + q.y)));
+            atomicAdd(&accb[0 * kTile + bb], double(fB + fC));
+            atomicAdd(&accb[1 * kTile + bb], double(fC * a0.x));
+            atomicAdd(&accb[2 * kTile + bb], double(fC * a0.y));
+            atomicAdd(&accb[3 * kTile + bb], double(fC * a0.z));
+            atomicAdd(&accb[4 * kTile + bb], double(dE * (b1.w + q.z)));
           } else if (kSmooth) {
             dE *= sw;
           }
           ei -= double(dE);
         }
-        head = nh;
       }
       // the next test phase may overwrite the ring entries read above
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       __builtin_amdgcn_wave_barrier();
       if (done) break;
     }
-    if (kGrad) {
-      // a-slot l: add this tile pair's sums in double; flush the b-slots
-      aM += double(accA[0 * kTile + l]); aKx += double(accA[1 * kTile + l]);
-      aKy += double(accA[2 * kTile + l]); aKz += double(accA[3 * kTile + l]);
-      aD += double(accA[4 * kTile + l]);
-      for (int c = 0; c < kAcc; ++c) accA[c * kTile + l] = Acc(0);
-      if (!diag) {
-        const double M = accB[0 * kTile + l], Kx = accB[1 * kTile + l], Ky = accB[2 * kTile + l];
-        const double Kz = accB[3 * kTile + l], D = accB[4 * kTile + l];
-        for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = Acc(0);
-        if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], TB.atom[l], grad, dedcn, gi, sg);
-      }
+    if (kGrad && !diag) {  // flush the b-slots of this tile pair
+      const double M = accB[0 * kTile + l], Kx = accB[1 * kTile + l], Ky = accB[2 * kTile + l];
+      const double Kz = accB[3 * kTile + l], D = accB[4 * kTile + l];
+      for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
+      if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], TB.atom[l], grad, dedcn, gi, sg);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
   }
-  if (kGrad) {  // combine the a-slot sums of the wavefronts (tileB is free now)
-    __syncthreads();
-    double* comb = reinterpret_cast<double*>(&tileB[0]);
-    double* cw = comb + w * kAcc * kTile;
-    cw[0 * kTile + l] = aM; cw[1 * kTile + l] = aKx; cw[2 * kTile + l] = aKy;
-    cw[3 * kTile + l] = aKz; cw[4 * kTile + l] = aD;
-    __syncthreads();
-    if (w == 0 && aval) {
-      double t[kAcc] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int v = 0; v < kW; ++v)
-        for (int c = 0; c < kAcc; ++c) t[c] += comb[(v * kAcc + c) * kTile + l];
-      flush_slot(t[0], t[1], t[2], t[3], t[4], ax, ay, az, jat, grad, dedcn, gi, sg);
-    }
-  }
+  __syncthreads();
+  if (kGrad && w == 0 && aval)
+    flush_slot(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l],
+               accA[4 * kTile + l], ax, ay, az, jat, grad, dedcn, gi, sg);
   block_add(e_atom + i, ei, red);
   if (kGrad) {
     block_add(grad + 3 * i, gi[0], red);

@@ -39,6 +39,19 @@ def _spec_with_lattice(spec: torch_api.BatchSpec, lattice: Optional[torch.Tensor
     return torch_api.BatchSpec(spec.natoms, spec.numbers, lat, periodic, spec.params, spec.cutoffs)
 
 
+def lattice_adjoint(gradient: torch.Tensor, sigma: torch.Tensor, positions: torch.Tensor, lattice: torch.Tensor,
+                    owner: torch.Tensor) -> torch.Tensor:
+    """dE/dL (nsys, 3, 3) from the gradient G (N, 3), the virial sigma
+    (nsys, 3, 3) as the library returns it (sigma(a, b) at [b][a]), the
+    positions X (N, 3), the lattices L (nsys, 3, 3, rows = vectors) and the
+    system index of every atom: dE/dL = L^{-T} (sigma - G^T X)^T."""
+    sig = sigma.transpose(1, 2)
+    sig = 0.5 * (sig + sig.transpose(1, 2))
+    gx = torch.zeros_like(sig)
+    gx.index_add_(0, owner, gradient.unsqueeze(2) * positions.unsqueeze(1))  # sum_i G_ia x_ib
+    return torch.linalg.solve(lattice.transpose(1, 2), (sig - gx).transpose(1, 2))
+
+
 class _Dispersion(torch.autograd.Function):
     @staticmethod
     def forward(ctx, positions, lattice, spec, precision):
@@ -63,14 +76,7 @@ class _Dispersion(torch.autograd.Function):
         grad_pos = (g * ge[owner].unsqueeze(1)).to(positions.dtype) if ctx.needs_input_grad[0] else None
         grad_lat = None
         if ctx.has_lattice and ctx.needs_input_grad[1]:
-            # sigma(a, b) = sum_i G_ia x_ib + sum_k D_ka L_kb per system
-            sig = out["sigma"].transpose(1, 2)             # (nsys, a, b)
-            sig = 0.5 * (sig + sig.transpose(1, 2))
-            gx = torch.zeros_like(sig)
-            gx.index_add_(0, owner, g.unsqueeze(2) * pos64.unsqueeze(1))  # sum_i G_ia x_ib
-            lat64 = lattice.detach().to(torch.float64)
-            rhs = (sig - gx).transpose(1, 2)                 # (sigma - G^T X)^T
-            d = torch.linalg.solve(lat64.transpose(1, 2), rhs)  # L^{-T} rhs
+            d = lattice_adjoint(g, out["sigma"], pos64, lattice.detach().to(torch.float64), owner)
             grad_lat = (d * ge.view(-1, 1, 1)).to(lattice.dtype)
         return grad_pos, grad_lat, None, None
 

@@ -3,7 +3,8 @@ position and lattice adjoints from the backward pass.
 
 Checks: energies and position adjoints against the oracle's energy and
 analytic gradient; lattice adjoints against central finite differences of
-the energy (h = 1e-4 bohr, FP64) with the Cartesian positions held fixed; the
+the energy (h = 1e-5 bohr, FP64, cutoffs placed in gaps of the pair
+distances) with the Cartesian positions held fixed; the
 virial rebuilt from the adjoints (G^T X + (dE/dL)^T L) against the oracle's
 virial; float32 inputs (mixed precision) against the FP64 results within the
 mixed-precision bars.
@@ -65,15 +66,43 @@ def test_energy_and_position_adjoint_against_oracle():
         off += n
 
 
+def _gap_cutoff(systems, target, margin=2e-4):
+    """A cutoff near `target` with no image-pair distance of any system within
+    `margin`: the hard cutoffs make the energy jump when a pair crosses them,
+    and a lattice step h moves an image pair by up to |n| h."""
+    d = []
+    for s in systems:
+        x = np.asarray(s["positions"], dtype=float)
+        tr = O.lattice_points(s["periodic"], s["lattice"], target + 1.0)
+        v = x[:, None, None, :] + tr[None, None, :, :] - x[None, :, None, :]
+        d.append(np.sqrt((v ** 2).sum(-1)).ravel())
+    d = np.sort(np.concatenate(d))
+    for step in range(2000):
+        for c in (target - step * 1e-3, target + step * 1e-3):
+            k = np.searchsorted(d, c)
+            lo = d[k - 1] if k > 0 else -np.inf
+            hi = d[k] if k < len(d) else np.inf
+            if c - lo > margin and hi - c > margin:
+                return c
+    raise AssertionError("no gap in the pair distances")
+
+
 def test_lattice_adjoint_against_finite_differences():
     from dftd3 import torch_api
     from dftd3.autograd import dispersion_energy
     _step("fp64 lattice adjoint vs central differences")
-    systems = _cells(seed=31)[:2]
+    systems = []
+    for k, (n, edge) in enumerate([(4, 11.0), (5, 12.0)]):
+        num, xyz, lt = cell(31 + k, n, edge)
+        systems.append(dict(numbers=num, positions=xyz, lattice=lt, periodic=[True] * 3, kind="rational",
+                            param=BJ))
+    cut = dict(cn=_gap_cutoff(systems, 40.0), disp2=_gap_cutoff(systems, 60.0), disp3=_gap_cutoff(systems, 40.0))
+    for s in systems:
+        s["cutoff"] = cut
     spec, pos, lat = _inputs(systems, torch.float64)
     e = dispersion_energy(spec, pos, lat)
     glat, = torch.autograd.grad(e.sum(), lat)
-    h = 1e-4
+    h = 1e-5  # moves an image pair by at most about 1e-4 bohr, inside the cutoff gaps
     fd = np.zeros((len(systems), 3, 3))
     base = lat.detach().cpu().numpy()
     p64 = pos.detach().contiguous()

@@ -301,3 +301,30 @@ def test_wrap_to_central_cell():
 def test_too_close():
     assert O.too_close([[0, 0, 0], [0, 0, 1e-10]])
     assert not O.too_close([[0, 0, 0], [0, 0, 1e-8]])
+
+
+def test_lattice_adjoint_identity_on_the_oracle():
+    """dftd3.autograd.lattice_adjoint turns the oracle's gradient and virial
+    into dE/dL (Cartesian positions fixed); central differences of the
+    oracle energy in the lattice (h = 1e-5 bohr) agree."""
+    import torch
+    from dftd3.autograd import lattice_adjoint
+    from helpers import BJ, cell
+    for k, (n, edge) in enumerate([(4, 11.0), (5, 12.0)]):
+        num, x, lat = cell(31 + k, n, edge)
+        r = O.dispersion(num, x, lat, [True] * 3, O.RATIONAL, BJ, O.DEFAULT_CUTOFF)
+        sigma_lib = torch.as_tensor(r["virial"].T.copy()).reshape(1, 3, 3).transpose(1, 2)  # [b][a] layout
+        d = lattice_adjoint(torch.as_tensor(r["gradient"]), sigma_lib, torch.as_tensor(x),
+                            torch.as_tensor(lat).reshape(1, 3, 3), torch.zeros(n, dtype=torch.long))[0].numpy()
+        h = 1e-5
+        fd = np.zeros((3, 3))
+        for a in range(3):
+            for b in range(3):
+                e = []
+                for sgn in (1.0, -1.0):
+                    lt = lat.copy()
+                    lt[a, b] += sgn * h
+                    e.append(O.dispersion(num, x, lt, [True] * 3, O.RATIONAL, BJ, O.DEFAULT_CUTOFF,
+                                          grad=False)["energy"])
+                fd[a, b] = (e[0] - e[1]) / (2 * h)
+        assert np.abs(d - fd).max() <= 1e-9 * np.abs(fd).max() + 1e-12
