@@ -32,6 +32,12 @@ METRIC = "E+grad pair-terms/s & molecules/s: mixed vs FP64 on GPU, |dE|<=1e-8 Eh
 OPS_PER_TRIPLE = 233    # ATM E+grad+sigma (atm.f90:733-806)
 OPS_PER_PAIR_BJ = 63    # two-body BJ E+grad+sigma (rational.f90:462-497)
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
+# HBM bytes per launch of the mixed ATM kernel on config 1, measured by two
+# rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch) scaled by a 256 MiB device copy (FETCH x2.00, WRITE x1.00);
+# commands and figures in profiles/atm_kernel__config1_periodic_2000.md.
+ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4>", "fetch_kib": 12_320_000, "write_kib": 6_878_000,
+                  "fetch_scale": 2.0, "write_scale": 1.0}
 
 
 def log(msg):
@@ -221,6 +227,10 @@ def main():
     dg = (outs["mixed"]["gradient"] - outs["fp64"]["gradient"]).abs().max().item()
     ds = (outs["mixed"]["sigma"] - outs["fp64"]["sigma"]).abs().max().item()
     mixed, fp64 = res["mixed"], res["fp64"]
+    traffic = None
+    if args.config == "c1":
+        t = ATM_TRAFFIC_C1
+        traffic = (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
     line = dict(
         metric=METRIC, value=mixed["pair_terms_per_s"], unit="pair-terms/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak",
@@ -237,8 +247,8 @@ def main():
         kernel_ms=dict(fp64=fp64["kernel_ms"], mixed=mixed["kernel_ms"]),
         roofline=dict(kernel="atm_kernel<float, true, false, 4>", bound="valu", achieved=mixed["atm_tflops"],
                       peak=PEAK_TFLOPS["mixed"], unit="TFLOP/s", frac=mixed["atm_tflops"] / PEAK_TFLOPS["mixed"],
-                      traffic=None, ops_per_triple=OPS_PER_TRIPLE, triples_per_launch=int(counts[2]),
-                      launch_ms=mixed["atm_kernel_ms"]),
+                      traffic=traffic, ops_per_triple=OPS_PER_TRIPLE, triples_per_launch=int(counts[2]),
+                      launch_ms=mixed["atm_kernel_ms"], traffic_unit="bytes per launch (HBM read + write)"),
     )
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on every {args.cpu_stride}-th atom, {args.cpu_threads} threads")
