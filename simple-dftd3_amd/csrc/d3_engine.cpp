@@ -219,8 +219,8 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
 }
 
 struct Workspace {
-  DeviceBuffer meta, pos, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items, item_off, slots, counts,
-      counter, flag;
+  DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
+      item_off, slots, counts, counter, flag;
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
@@ -237,9 +237,18 @@ struct Workspace {
   // upload (stream-ordered).
   Prepared upload(const std::vector<SystemDesc>& systems, const int* numbers, hipStream_t stream,
                   const int* ghost_host);
-  void pack(const Prepared& pb, const double* positions_dev, hipStream_t stream) {
+  // Packs the positions; with order_atoms also (re)computes the spatial order
+  // of the atoms (the storage order of pair records and three-body lists).
+  void pack(const Prepared& pb, const double* positions_dev, hipStream_t stream, bool order_atoms) {
     timed("pack", stream,
           [&] { launch_pack(positions_dev, pb.db.natoms, const_cast<double4*>(pb.db.pos), stream); });
+    if (order_atoms) {
+      const size_t bytes = spatial_order_temp_bytes(pb.db.natoms);
+      void* t = sort_tmp.get(bytes);
+      timed("order", stream, [&] {
+        launch_spatial_order(pb.db, t, bytes, const_cast<int*>(pb.db.rank), const_cast<int*>(pb.db.order), stream);
+      });
+    }
   }
   DevWork work(const Prepared& pb, bool mixed, bool grad, const BatchOutput& out, const BatchDebug* debug);
   // count -> scan -> work items -> scan (three-body lists of every atom)
@@ -247,8 +256,8 @@ struct Workspace {
   // pack .. disp2; then the three-body lists and kernel; then the chain
   // rule and the reduction.  capacity < 0: read the list size back and size
   // the slot buffer to it; otherwise use the fixed capacity and item bound.
-  void evaluate(const Prepared& pb, const double* positions_dev, bool mixed, bool grad, bool smooth,
-                const BatchOutput& out, const BatchDebug* debug, RunStats* stats, long long capacity,
+  void evaluate(const Prepared& pb, const double* positions_dev, bool order_atoms, bool mixed, bool grad,
+                bool smooth, const BatchOutput& out, const BatchDebug* debug, RunStats* stats, long long capacity,
                 long long max_items, const AtmShape& shape, hipStream_t stream);
 };
 
@@ -375,6 +384,8 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   db.c6ref = reinterpret_cast<const double*>(dmeta + o_c6r);
   db.rvdw = reinterpret_cast<const double*>(dmeta + o_rvdw);
   db.ghost = ghost_host ? reinterpret_cast<const int*>(dmeta + o_ghost) : nullptr;
+  db.rank = rank.as<int>(natoms);
+  db.order = order.as<int>(natoms);
   pb.pair_total = pair_total;
   pb.any_atm = any_atm;
   pb.smooth = smooth;
@@ -414,12 +425,12 @@ void Workspace::list_offsets(const Prepared& pb, const void* pair_records, bool 
   });
 }
 
-void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool mixed, bool grad, bool smooth,
-                         const BatchOutput& out, const BatchDebug* debug, RunStats* stats, long long capacity,
-                         long long max_items, const AtmShape& shape, hipStream_t stream) {
+void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool order_atoms, bool mixed, bool grad,
+                         bool smooth, const BatchOutput& out, const BatchDebug* debug, RunStats* stats,
+                         long long capacity, long long max_items, const AtmShape& shape, hipStream_t stream) {
   const DevBatch& db = pb.db;
   const int natoms = db.natoms;
-  pack(pb, positions_dev, stream);
+  pack(pb, positions_dev, stream, order_atoms);
   const DevWork w = work(pb, mixed, grad, out, debug);
   timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
   timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
@@ -459,7 +470,7 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
   Workspace& m = *ws_;
   const Prepared pb = m.upload(systems, numbers, stream, ghost_host);
   const bool grad = out.gradient || out.sigma;
-  m.evaluate(pb, positions_dev, precision == kMixed, grad, pb.smooth, out, debug, stats, -1, 0, AtmShape{},
+  m.evaluate(pb, positions_dev, true, precision == kMixed, grad, pb.smooth, out, debug, stats, -1, 0, AtmShape{},
              stream);
   if (m.timer.enabled) m.timer.collect();
 }
@@ -472,7 +483,7 @@ void Engine::count_terms(const std::vector<SystemDesc>& systems, const int* numb
   for (auto& s : sys)
     if (std::fabs(s.param.s9) < kEps) s.param.s9 = 1.0;  // triples are counted whatever s9 is
   const Prepared pb = m.upload(sys, numbers, stream, nullptr);
-  m.pack(pb, positions_dev, stream);
+  m.pack(pb, positions_dev, stream, true);
   const int natoms = pb.db.natoms;
   unsigned long long* dc = m.counts.as<unsigned long long>(size_t(pb.db.nsys) * 3);
   hip_check(hipMemsetAsync(dc, 0, sizeof(unsigned long long) * 3 * pb.db.nsys, stream), "hipMemsetAsync");
@@ -537,11 +548,12 @@ Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, cons
     s.shape.num_cus = device_cu_count();
     const int natoms = s.pb.db.natoms;
     hip_check(hipMemsetAsync(s.ws.flag.as<int>(1), 0, sizeof(int), stream), "hipMemsetAsync");
+    // the spatial order of the planning coordinates is kept for every evaluation
+    s.ws.pack(s.pb, positions_dev, stream, true);
     if (s.pb.any_atm) {
       long long cap = s.pb.slot_bound;
       if (s.pb.periodic_atm) {
         // lists of the initial coordinates, with headroom for later ones
-        s.ws.pack(s.pb, positions_dev, stream);
         const DevWork w = s.ws.work(s.pb, s.mixed, grad, BatchOutput{}, nullptr);
         launch_cn(s.pb.db, s.mixed, w.cn, stream);
         launch_weights(s.pb.db, w.cn, w.gw, w.dgw, stream);
@@ -588,8 +600,8 @@ void Plan::execute(const double* positions_dev, const BatchOutput& out, hipStrea
   if (!out.energy) throw Error("Energy output is missing");
   if (s.grad != (out.gradient != nullptr || out.sigma != nullptr))
     throw Error("The plan was made for " + std::string(s.grad ? "energy and gradient" : "the energy only"));
-  s.ws.evaluate(s.pb, positions_dev, s.mixed, s.grad, s.pb.smooth, out, nullptr, nullptr, s.capacity, s.max_items,
-                s.shape, stream);
+  s.ws.evaluate(s.pb, positions_dev, false, s.mixed, s.grad, s.pb.smooth, out, nullptr, nullptr, s.capacity,
+                s.max_items, s.shape, stream);
 }
 
 std::vector<AtmShape> Plan::candidate_shapes() {

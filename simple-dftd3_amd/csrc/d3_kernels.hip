@@ -10,9 +10,12 @@
 // squared distance, so both paths visit exactly the reference's terms.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <string>
 
 #include "d3_kernels.hpp"
 
@@ -49,7 +52,7 @@ struct Slot {
   R qi;               // (dC6_ij / dCN_i) / C6_ij
   R qx;               // (dC6_ij / dCN_j) / C6_ij
   int atom;           // j (global index)
-  int pad;
+  int rank;           // position of j in the spatial order of its system
 };
 
 __device__ __forceinline__ int wave_lane() { return threadIdx.x & (kWave - 1); }
@@ -150,10 +153,55 @@ __device__ __forceinline__ void block_add(double* __restrict__ dst, double v, do
 
 int blocks_for(long long n, int per_block) { return static_cast<int>((n + per_block - 1) / per_block); }
 
+void hip_check_sort(hipError_t e) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("hipcub radix sort: ") + hipGetErrorString(e));
+}
+
 // --------------------------------------------------------------- packing
 __global__ void pack_positions(const double* __restrict__ xyz, int n, double4* __restrict__ pos) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) pos[i] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0);
+}
+
+// ----------------------------------------------------------- spatial order
+// Molecules: Morton code of the 4-bohr cell of every atom (10 bits per axis,
+// coordinates wrapped modulo 4096 bohr).  Systems with lattice images keep the
+// input order (their lists hold several images of an atom, which a cell-based
+// order does not keep together; measured slower).  Keyed by the system index
+// so that one sort orders every system.
+__device__ __forceinline__ unsigned spread_bits3(unsigned v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ void spatial_key_kernel(DevBatch b, unsigned long long* __restrict__ keys, int* __restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.natoms) return;
+  const int s = b.sys[i];
+  const SysGeom g = b.geom[s];
+  unsigned code;
+  if (g.img_cn_count > 1 || g.img2_count > 1 || g.img3_count > 1) {
+    code = static_cast<unsigned>(i - g.atom_begin);
+  } else {
+    const double4 x = b.pos[i];
+    const unsigned qx = static_cast<unsigned>(static_cast<int>(floor(x.x * 0.25)));
+    const unsigned qy = static_cast<unsigned>(static_cast<int>(floor(x.y * 0.25)));
+    const unsigned qz = static_cast<unsigned>(static_cast<int>(floor(x.z * 0.25)));
+    code = spread_bits3(qx) | (spread_bits3(qy) << 1) | (spread_bits3(qz) << 2);
+  }
+  keys[i] = (static_cast<unsigned long long>(s) << 30) | code;
+  idx[i] = i;
+}
+
+__global__ void spatial_rank_kernel(DevBatch b, const int* __restrict__ order, int* __restrict__ rank) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.natoms) return;
+  const int a = order[p];
+  rank[a] = p - b.geom[b.sys[a]].atom_begin;
 }
 
 // ------------------------------------------------------- coordination number
@@ -228,7 +276,8 @@ __global__ void weights_kernel(DevBatch b, const double* __restrict__ cn, double
 // model.f90:432-462: C6_ij = sum_kl w_ik w_jl C6ref_kl and its derivatives,
 // stored per ordered pair as {sqrt(C6_ij), (dC6_ij/dCN_i)/C6_ij,
 // (dC6_ij/dCN_j)/C6_ij, (4/3 rvdw_ij)^(alp/3)} (the last two feed the ATM
-// term, atm.f90:667-668,811-816).  One block per row i of the system block.
+// term, atm.f90:667-668,811-816).  One block per row i of the system block;
+// rows and columns are laid out in the spatial order of the atoms.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* __restrict__ gw,
                                                       const double* __restrict__ dgw, R4<R>* __restrict__ pair) {
@@ -236,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
   const int s = b.sys[i];
   const SysGeom g = b.geom[s];
   const double alp3 = (b.par[s].alp + 2.0) / 3.0;
-  const int n = g.natoms, il = i - g.atom_begin;
+  const int n = g.natoms;
   const int spi = b.sp[i];
   const int mref = b.mref;
   const int nri = b.nref[spi];
@@ -245,30 +294,42 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
     gi[r] = r < nri ? gw[i * mref + r] : 0.0;
     dgi[r] = (r < nri && dgw) ? dgw[i * mref + r] : 0.0;
   }
+  // per species s of the batch: h[s][r] = sum_ri w_i[ri] C6ref[ri][r] and the
+  // same with dw_i/dCN_i, so that a record costs three nref_j-term dots; and
+  // (4/3 rvdw_is)^(alp/3)
+  __shared__ double h[kMaxSpecies][kMaxRef], dh[kMaxSpecies][kMaxRef], r0p_s[kMaxSpecies];
+  for (int t = threadIdx.x; t < b.nsp * kMaxRef; t += blockDim.x) {
+    const int sj = t / kMaxRef, rj = t % kMaxRef;
+    const double* ref = b.c6ref + (spi * b.nsp + sj) * (kMaxRef * kMaxRef);
+    double a = 0.0, d = 0.0;
+    for (int ri = 0; ri < nri; ++ri) {
+      a += gi[ri] * ref[ri * kMaxRef + rj];
+      d += dgi[ri] * ref[ri * kMaxRef + rj];
+    }
+    h[sj][rj] = a;
+    dh[sj][rj] = d;
+  }
+  for (int t = threadIdx.x; t < b.nsp; t += blockDim.x) r0p_s[t] = pow(kRs9 * b.rvdw[t * b.nsp + spi], alp3);
+  __syncthreads();
   const bool gi_ghost = b.ghost && b.ghost[i];
-  R4<R>* row = pair + g.c6_begin + static_cast<long long>(il) * n;
-  for (int jl = threadIdx.x; jl < n; jl += blockDim.x) {
-    const int j = g.atom_begin + jl;
+  R4<R>* row = pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n;
+  for (int jr = threadIdx.x; jr < n; jr += blockDim.x) {
+    const int j = b.order[g.atom_begin + jr];
     const int spj = b.sp[j];
     const int nrj = b.nref[spj];
-    const double* ref = b.c6ref + (spi * b.nsp + spj) * (kMaxRef * kMaxRef);
     double cc = 0.0, di = 0.0, dj = 0.0;
     if (!(gi_ghost || (b.ghost && b.ghost[j]))) {  // model.f90:436-440
-      for (int ri = 0; ri < nri; ++ri) {
-        for (int rj = 0; rj < nrj; ++rj) {
-          const double w = gw[j * mref + rj];
-          const double rc = ref[ri * kMaxRef + rj];
-          cc += gi[ri] * w * rc;
-          if (dgw) {
-            di += dgi[ri] * w * rc;
-            dj += gi[ri] * dgw[j * mref + rj] * rc;
-          }
+      for (int rj = 0; rj < nrj; ++rj) {
+        const double w = gw[j * mref + rj];
+        cc += w * h[spj][rj];
+        if (dgw) {
+          di += w * dh[spj][rj];
+          dj += dgw[j * mref + rj] * h[spj][rj];
         }
       }
     }
     const double ic = cc > 0.0 ? 1.0 / cc : 0.0;
-    const double r0p = pow(kRs9 * b.rvdw[spj * b.nsp + spi], alp3);
-    row[jl] = make_r4<R>(R(sqrt(cc)), R(di * ic), R(dj * ic), R(r0p));
+    row[jr] = make_r4<R>(R(sqrt(cc)), R(di * ic), R(dj * ic), R(r0p_s[spj]));
   }
 }
 
@@ -286,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
   const int s = b.sys[i];
   const SysGeom g = b.geom[s];
   const SysParam p = b.par[s];
-  const int n = g.natoms, il = i - g.atom_begin;
+  const int n = g.natoms;
   const double cut2 = p.cut2 * p.cut2;
   const double4 xi = b.pos[i];
   const int spi = b.sp[i];
@@ -295,14 +356,14 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
   double sxx = 0.0, syy = 0.0, szz = 0.0, sxy = 0.0, sxz = 0.0, syz = 0.0;
   const bool active = !(fabs(p.s6) < kEps && fabs(p.s8) < kEps);  // rational.f90:267
   const R s6 = R(p.s6), s8 = R(p.s8), cutr = R(p.cut2), width = R(p.width2);
-  const R4<R>* row = pair + g.c6_begin + static_cast<long long>(il) * n;
+  const R4<R>* row = pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n;
   if (active) {
-    for (int jl = lane; jl < n; jl += kWave) {
-      const int j = g.atom_begin + jl;
+    for (int jr = lane; jr < n; jr += kWave) {
+      const int j = b.order[g.atom_begin + jr];
       const double4 xj = b.pos[j];
       const int spj = b.sp[j];
       const R rrij = R(3) * r4r2i * R(b.r4r2[spj]);
-      const R4<R> q = row[jl];
+      const R4<R> q = row[jr];
       const R c6ij = q.x * q.x;
       const R dc6i = q.y * c6ij;  // dC6_ij/dCN_i
       R r0a = R(0), r0b = R(0);   // BJ: r0^6, r0^8; zero: rs6*R0, rs8*R0
@@ -391,8 +452,8 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
 // ------------------------------------------------------ three-body lists
 // L(i) = {(j, T): j <= i, eps <= |x_j + T - x_i|^2 <= cut3^2, C6_ij > 0}
 // (atm.f90:706-716; pairs with a vanishing C6 -- ghost atoms -- contribute
-// nothing to either branch of the reference, atm.f90:725-726), in (j, T)
-// order.  The triples of atom i are the unordered pairs {a, b} of distinct
+// nothing to either branch of the reference, atm.f90:725-726), with the
+// partners j in the spatial order of the system and images T in order.  The triples of atom i are the unordered pairs {a, b} of distinct
 // slots of L(i) with eps <= |v_b - v_a|^2 <= cut3^2 (atm.f90:717-742): a pair
 // of different atoms is the reference's triple with the larger index as j, a
 // pair of two images of the same atom stands for the two triples (j, Tj, j,
@@ -416,17 +477,18 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
   const double cut2 = p.cut3 * p.cut3;
   const R malp6 = -(R(p.alp) + R(2)) / R(6);  // exponent of r2 in (r0/r)^(alp/3)
   const double4 xi = b.pos[i];
-  const int n = g.natoms, il = i - g.atom_begin;
-  const R4<R>* row = pair ? pair + g.c6_begin + static_cast<long long>(il) * n : nullptr;
+  const int n = g.natoms;
+  const R4<R>* row = pair ? pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n : nullptr;
   Slot<R>* out = kFill ? slots + offset[i] : nullptr;
   long long total = 0;
-  for (int j0 = g.atom_begin; j0 <= i; j0 += kWave) {
-    const int j = j0 + lane;
+  for (int r0 = 0; r0 < n; r0 += kWave) {  // partners in spatial order
+    const int r = r0 + lane;
+    const int j = r < n ? b.order[g.atom_begin + r] : i + 1;
     int cnt = 0;
     bool valid = j <= i;
     R4<R> q{};
     if (valid && row) {
-      q = row[j - g.atom_begin];
+      q = row[r];
       valid = !(double(q.x) * double(q.x) < kEps);
     }
     const double4 xj = valid ? b.pos[j] : xi;
@@ -459,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
         e.qi = q.y;
         e.qx = q.z;
         e.atom = j;
-        e.pad = 0;
+        e.rank = r;
         out[w++] = e;
       }
     }
@@ -510,7 +572,7 @@ struct TileSlots {
   double vx[kTile], vy[kTile], vz[kTile];  // FP64 vectors for the exact r_jk tests
   R4<R> g0[kTile];                          // vx, vy, vz, r2
   R4<R> g1[kTile];                          // pdmp, sc6h, qi, qx
-  int atom[kTile];                          // -1 past the end of the list
+  int2 key[kTile];                          // {atom (-1 past the end of the list), rank}
 };
 // Accepted slot pair (a, b) of a tile pair.
 template <typename R>
@@ -518,7 +580,6 @@ struct PairEntry {
   unsigned ab;   // a | b << 8 | (both slots are images of atom i) << 16
   unsigned rec;  // index of the (j, k) pair record within the system
   R C;           // r_jk^2
-  R pad;
 };
 template <>
 struct PairEntry<double> {
@@ -538,12 +599,12 @@ __device__ __forceinline__ void load_tile(TileSlots<R>& T, const Slot<R>* __rest
     T.vx[l] = s.vx; T.vy[l] = s.vy; T.vz[l] = s.vz;
     T.g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
     T.g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
-    T.atom[l] = s.atom;
+    T.key[l] = make_int2(s.atom, s.rank);
   } else {  // fails every distance test
     T.vx[l] = 1e30; T.vy[l] = 1e30; T.vz[l] = 1e30;
     T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
     T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
-    T.atom[l] = -1;
+    T.key[l] = make_int2(-1, 0);
   }
 }
 
@@ -658,7 +719,7 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   const int s = b.sys[i];
   const SysGeom g = b.geom[s];
   const SysParam p = b.par[s];
-  const int n = g.natoms, ab0 = g.atom_begin;
+  const int n = g.natoms;
   const long long L0 = offset[i];
   const int m = static_cast<int>(offset[i + 1] - L0);
   const int nb = (m + kTile - 1) / kTile;
@@ -685,8 +746,8 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   __syncthreads();
   const bool aval = P * kTile + l < m;
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
-  const int jat = tileA.atom[l];
-  const unsigned arow = aval ? static_cast<unsigned>(jat - ab0) * static_cast<unsigned>(n) : 0u;
+  const int jat = tileA.key[l].x;
+  const unsigned arow = aval ? static_cast<unsigned>(tileA.key[l].y) * static_cast<unsigned>(n) : 0u;
   const unsigned aself = aval && jat == i ? 1u << 16 : 0u;
 
   double ei = 0.0, di = 0.0;
@@ -715,10 +776,11 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
           const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
                                      static_cast<unsigned>(bal >> 32),
                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
-          const int kat = TB.atom[bl];
+          const int2 kb = TB.key[bl];
+          const int kat = kb.x;
           PairEntry<R> e;
           e.ab = static_cast<unsigned>(l | (bl << 8)) | (kat == i ? aself : 0u);
-          e.rec = arow + static_cast<unsigned>(kat - ab0);
+          e.rec = arow + static_cast<unsigned>(kb.y);
           e.C = R(Cd);
           q_ring[pos & (kQueue - 1)] = e;
         }
@@ -807,7 +869,7 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
       const double M = accB[0 * kTile + l], Kx = accB[1 * kTile + l], Ky = accB[2 * kTile + l];
       const double Kz = accB[3 * kTile + l], D = accB[4 * kTile + l];
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
-      if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], TB.atom[l], grad, dedcn, gi, sg);
+      if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], TB.key[l].x, grad, dedcn, gi, sg);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -1000,6 +1062,39 @@ size_t slot_record_bytes(bool mixed) { return mixed ? sizeof(Slot<float>) : size
 
 void launch_pack(const double* xyz, int n, double4* pos, hipStream_t st) {
   hipLaunchKernelGGL(pack_positions, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, xyz, n, pos);
+}
+
+namespace {
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+size_t radix_sort_bytes(int n) {
+  size_t bytes = 0;
+  hip_check_sort(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<unsigned long long*>(nullptr),
+                                                    static_cast<unsigned long long*>(nullptr),
+                                                    static_cast<int*>(nullptr), static_cast<int*>(nullptr), n));
+  return bytes;
+}
+}  // namespace
+
+size_t spatial_order_temp_bytes(int n) {
+  return 2 * align256(sizeof(unsigned long long) * n) + align256(sizeof(int) * n) + radix_sort_bytes(n);
+}
+
+void launch_spatial_order(const DevBatch& b, void* temp, size_t temp_bytes, int* rank, int* order, hipStream_t st) {
+  const int n = b.natoms;
+  char* t = static_cast<char*>(temp);
+  auto* keys_in = reinterpret_cast<unsigned long long*>(t);
+  t += align256(sizeof(unsigned long long) * n);
+  auto* keys_out = reinterpret_cast<unsigned long long*>(t);
+  t += align256(sizeof(unsigned long long) * n);
+  int* idx = reinterpret_cast<int*>(t);
+  t += align256(sizeof(int) * n);
+  size_t sort_bytes = temp_bytes - static_cast<size_t>(t - static_cast<char*>(temp));
+  int sys_bits = 0;
+  while ((1LL << sys_bits) < b.nsys) ++sys_bits;
+  hipLaunchKernelGGL(spatial_key_kernel, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, b, keys_in, idx);
+  hip_check_sort(hipcub::DeviceRadixSort::SortPairs(t, sort_bytes, keys_in, keys_out, idx, order, n, 0,
+                                                    30 + sys_bits, st));
+  hipLaunchKernelGGL(spatial_rank_kernel, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, b, order, rank);
 }
 
 void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t st) {

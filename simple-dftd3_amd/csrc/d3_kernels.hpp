@@ -29,6 +29,10 @@ struct DevBatch {
   const double* c6ref;  // [nsp*nsp][kMaxRef*kMaxRef], [(si*nsp+sj)][ri*7+rj]
   const double* rvdw;   // [nsp*nsp]
   const int* ghost;     // [N] or null
+  // Spatial (Morton) order of the atoms of each system, the storage order of
+  // the pair records and of the three-body lists (no result depends on it):
+  const int* rank;      // [N] position of the atom in its system's order
+  const int* order;     // [N] atom at position atom_begin + r of that order
 };
 
 // Per-evaluation device arrays.
@@ -36,7 +40,8 @@ struct DevWork {
   double* cn;       // [N]
   double* gw;       // [N][mref]
   double* dgw;      // [N][mref] or null (energy only)
-  void* pair;       // per system n x n block of R4 {sqrt C6, dC6/dCN_i / C6, dC6/dCN_j / C6, r0^(alp/3)}
+  void* pair;       // per system n x n block of R4 {sqrt C6, dC6/dCN_i / C6, dC6/dCN_j / C6, r0^(alp/3)},
+                    // record (i, j) at [rank_i][rank_j]
   double* e_atom;   // [N]
   double* dedcn;    // [N]
   double* grad;     // [N][3]
@@ -49,6 +54,12 @@ size_t slot_record_bytes(bool mixed);
 
 // Launchers (all asynchronous on `stream`).
 void launch_pack(const double* xyz, int n, double4* pos, hipStream_t stream);
+// Spatial order (rank, order of DevBatch) from the packed positions: Morton
+// codes of 4-bohr cells within each system, sorted on the device.  temp is
+// scratch of spatial_order_temp_bytes(n) bytes.
+size_t spatial_order_temp_bytes(int n);
+void launch_spatial_order(const DevBatch& b, void* temp, size_t temp_bytes, int* rank, int* order,
+                          hipStream_t stream);
 void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t stream);
 void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw, hipStream_t stream);
 void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, void* pair, hipStream_t stream);

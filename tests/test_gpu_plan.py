@@ -129,10 +129,15 @@ def test_graph_replays_follow_new_coordinates(grad):
         got = {k: v.clone() for k, v in out.items()}
         # eager work between the replays
         scratch = torch.randn(1 << 16, device="cuda").square().sum()
+        eager = plan.execute(new.contiguous(), out={k: torch.zeros_like(v) for k, v in out.items()})
         ref = torch_api.run_batch(spec, new.contiguous(), precision="mixed", grad=grad)
         torch.cuda.synchronize()
         assert torch.isfinite(scratch)
-        _close(got, ref, grad=grad)
+        # the replay is the plan's own evaluation (same storage order)
+        _close(got, eager, grad=grad)
+        # the batch entry point orders the atoms of the new coordinates afresh,
+        # which changes the FP32 rounding of single triples (~1 ulp)
+        _close(got, ref, grad=grad, rtol=1e-7, atol=1e-10)
     assert not plan.overflowed()
 
 
