@@ -31,6 +31,7 @@ METRIC = "E+grad pair-terms/s & molecules/s: mixed vs FP64 on GPU, |dE|<=1e-8 Eh
 # an FMA as two; div, rcp, sqrt, exp and pow as one each).
 OPS_PER_TRIPLE = 233    # ATM E+grad+sigma (atm.f90:733-806)
 OPS_PER_PAIR_BJ = 63    # two-body BJ E+grad+sigma (rational.f90:462-497)
+KERNEL_GROUPS = ("pack", "order", "cn", "weights", "c6", "disp2", "atm_list", "atm", "cn_chain", "reduce")
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
 # HBM bytes per launch of the mixed ATM kernel on config 1, measured by two
 # rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
@@ -61,6 +62,14 @@ def workload(config):
         sysd = dict(numbers=num, positions=pos, kind="rational", param=dict(synthetic.PBE0_BJ))
         return [sysd], dict(workload="config2-ellipsoid-20000-atoms-pbe0-d3bj-atm", natoms=len(num), cells=1,
                             elements="H/C/N/O/S protein-like")
+    if config == "c4":
+        systems = []
+        for num, pos, lat in synthetic.cells_c4():
+            systems.append(dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3, kind="rational",
+                                param=dict(synthetic.PBE0_BJ)))
+        return systems, dict(workload="config4-autograd-256-cells-64-512-atoms-pbe0-d3bj-atm",
+                             natoms=int(sum(len(s["numbers"]) for s in systems)), cells=len(systems),
+                             elements="H/C/N/O/Si")
     if config == "c3":
         systems = []
         for num, pos, param, kind in synthetic.batch_c3():
@@ -100,7 +109,7 @@ def bench_batched(args, systems, config, spec, positions, counts):
     kernel sequence, three-body launch shape autotuned), FP64 and mixed,
     against one FP64 evaluation per molecule."""
     import torch
-    from dftd3 import torch_api
+    from dftd3 import library, torch_api
     nmol = len(systems)
     res, outs = {}, {}
     for prec in ("fp64", "mixed"):
@@ -120,6 +129,17 @@ def bench_batched(args, systems, config, spec, positions, counts):
                          autotune_ms={f"{w}x{b}": round(v, 4) for (w, b), v in times.items()})
         log(f"{prec}: batched graph replay {t * 1e3:.2f} ms ({nmol / t:.0f} molecules/s)")
         del plan
+    # kernel breakdown of one batched evaluation (dynamic entry point, HIP events)
+    ws = torch_api.workspace(positions.device)
+    kernel_ms = {}
+    for prec in ("fp64", "mixed"):
+        out = torch_api.run_batch(spec, positions, precision=prec)
+        torch.cuda.synchronize()
+        library.set_kernel_timing(ws, True)
+        torch_api.run_batch(spec, positions, precision=prec, out=out)
+        torch.cuda.synchronize()
+        kernel_ms[prec] = {k: library.get_kernel_time(ws, k)[0] for k in KERNEL_GROUPS}
+        library.set_kernel_timing(ws, False)
     # one FP64 evaluation per molecule (each call plans, launches and reads
     # back its list size)
     singles = []
@@ -153,9 +173,62 @@ def bench_batched(args, systems, config, spec, positions, counts):
                                pair_terms_per_s=counts[1] / t_single),
         mixed_batched_over_fp64_per_molecule=(nmol / mixed["ms_per_step"] * 1e3) / (nmol / t_single),
         mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"], mixed_shape=mixed["shape"],
-        mixed_autotune_ms=mixed["autotune_ms"],
+        mixed_autotune_ms=mixed["autotune_ms"], kernel_ms=kernel_ms,
         mixed_vs_fp64_error=dict(energy_per_atom=float(de), max_gradient=dg, max_sigma=ds),
         fp64_batched_vs_per_molecule_energy_per_atom=float(d_single),
+        roofline=None,
+    )
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_autograd(args, systems, config, spec, positions, counts):
+    """Config 4: energies of 256 cells from the autograd forward pass and
+    position + lattice adjoints from its backward pass (float32 inputs, mixed
+    kernels), against the same with float64 inputs (FP64 kernels)."""
+    import torch
+    from dftd3.autograd import dispersion_energy
+    lat = torch.as_tensor(np.stack([s["lattice"] for s in systems]), device=positions.device)
+    res, grads = {}, {}
+    # both runs see the float32-rounded coordinates, so that the difference is
+    # that of the arithmetic
+    pos32, lat32 = positions.to(torch.float32), lat.to(torch.float32)
+    for prec, dtype in (("mixed", torch.float32), ("fp64", torch.float64)):
+        pos = pos32.to(dtype).detach().requires_grad_(True)
+        lt = lat32.to(dtype).detach().requires_grad_(True)
+
+        def step():
+            e = dispersion_energy(spec, pos, lt)
+            gp, gl = torch.autograd.grad(e.sum(), (pos, lt))
+            return e, gp, gl
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            e, gp, gl = step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        grads[prec] = (e.double(), gp.double(), gl.double())
+        res[prec] = dict(ms_per_step=t * 1e3, cells_per_s=len(systems) / t, pair_terms_per_s=counts[1] / t,
+                         triple_terms_per_s=counts[2] / t)
+        log(f"{prec}: forward + backward {t * 1e3:.2f} ms ({len(systems) / t:.0f} cells/s)")
+    natoms = torch.as_tensor(spec.natoms, dtype=torch.float64, device=positions.device)
+    de = ((grads["mixed"][0] - grads["fp64"][0]).abs() / natoms).max().item()
+    dg = (grads["mixed"][1] - grads["fp64"][1]).abs().max().item()
+    dl = (grads["mixed"][2] - grads["fp64"][2]).abs().max().item()
+    mixed, fp64 = res["mixed"], res["fp64"]
+    line = dict(
+        metric=METRIC, value=mixed["pair_terms_per_s"], unit="pair-terms/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(config, pair_terms=int(counts[1]), triple_terms=int(counts[2]), cn_terms=int(counts[0]),
+                    parallelism="single-gpu", accumulation="f64",
+                    step="autograd forward (energy kernels) + backward (gradient kernels, adjoints)"),
+        molecules_per_s=mixed["cells_per_s"], triple_terms_per_s=mixed["triple_terms_per_s"], fp64=fp64,
+        mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"],
+        mixed_vs_fp64_error=dict(energy_per_atom=de, max_position_adjoint=dg, max_lattice_adjoint=dl),
         roofline=None,
     )
     print(json.dumps(line), flush=True)
@@ -198,6 +271,8 @@ def main():
     log(f"{config['workload']}: CN terms {counts[0]}, pair terms {counts[1]}, triple terms {counts[2]}")
     if args.config == "c3":
         return bench_batched(args, systems, config, spec, positions, counts)
+    if args.config == "c4":
+        return bench_autograd(args, systems, config, spec, positions, counts)
     ws = torch_api.workspace(dev)
     outs = {}
     res = {}
@@ -210,8 +285,7 @@ def main():
         library.set_kernel_timing(ws, True)
         t = timed_steps(run, args.steps, torch.cuda.synchronize)
         atm_ms, atm_calls = library.get_kernel_time(ws, "atm")
-        all_ms = {k: library.get_kernel_time(ws, k) for k in
-                  ("pack", "cn", "weights", "c6", "disp2", "atm_list", "atm", "cn_chain", "reduce")}
+        all_ms = {k: library.get_kernel_time(ws, k) for k in KERNEL_GROUPS}
         library.set_kernel_timing(ws, False)
         t_plain = timed_steps(run, args.steps, torch.cuda.synchronize)  # without event records
         outs[prec] = {k: v.clone() for k, v in out.items()}

@@ -139,7 +139,7 @@ D3HIP_API void d3hip_count_terms_batch(dftd3_error error, dftd3_model workspace,
 D3HIP_API void d3hip_set_kernel_timing(dftd3_error error, dftd3_model model, int enable);
 
 /* Total milliseconds and launches of one kernel group since timing was
- * enabled; name is one of pack, cn, weights, c6, disp2, atm_list, atm,
+ * enabled; name is one of pack, order, cn, weights, c6, disp2, atm_list, atm,
  * cn_chain, reduce. */
 D3HIP_API void d3hip_get_kernel_time(dftd3_error error, dftd3_model model, const char* name, double* total_ms,
                                      long long* calls);

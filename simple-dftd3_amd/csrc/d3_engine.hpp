@@ -75,7 +75,7 @@ class Engine {
   // Per-kernel timing with HIP events on the launch stream (resets totals).
   void set_kernel_timing(bool enable);
   // Total milliseconds and number of launches of a kernel group since the
-  // timing was enabled: pack, cn, weights, c6, disp2, atm_list, atm,
+  // timing was enabled: pack, order, cn, weights, c6, disp2, atm_list, atm,
   // cn_chain, reduce.
   std::pair<double, long long> kernel_time(const std::string& name) const;
 

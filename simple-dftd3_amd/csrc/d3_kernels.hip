@@ -572,7 +572,8 @@ struct TileSlots {
   double vx[kTile], vy[kTile], vz[kTile];  // FP64 vectors for the exact r_jk tests
   R4<R> g0[kTile];                          // vx, vy, vz, r2
   R4<R> g1[kTile];                          // pdmp, sc6h, qi, qx
-  int2 key[kTile];                          // {atom (-1 past the end of the list), rank}
+  int2 key[kTile];                          // {queue tag of slot l as b, rank}: tag = l << 8, plus 1 << 16
+                                            // when the slot is an image of atom i
 };
 // Accepted slot pair (a, b) of a tile pair.
 template <typename R>
@@ -591,21 +592,25 @@ struct PairEntry<double> {
 // (three components), D = sum dE (qx_s + q_partner-side).
 constexpr int kAcc = 5;
 
+// Loads slot idx of the list of atom i into position l of a tile; returns
+// the slot's atom (-1 past the end of the list).
 template <typename R>
-__device__ __forceinline__ void load_tile(TileSlots<R>& T, const Slot<R>* __restrict__ slots, long long base, int idx,
-                                          int m, int l) {
+__device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restrict__ slots, long long base, int idx,
+                                         int m, int l, int i) {
   if (idx < m) {
     const Slot<R> s = slots[base + idx];
     T.vx[l] = s.vx; T.vy[l] = s.vy; T.vz[l] = s.vz;
     T.g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
     T.g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
-    T.key[l] = make_int2(s.atom, s.rank);
-  } else {  // fails every distance test
-    T.vx[l] = 1e30; T.vy[l] = 1e30; T.vz[l] = 1e30;
-    T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
-    T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
-    T.key[l] = make_int2(-1, 0);
+    T.key[l] = make_int2((l << 8) | (s.atom == i ? 1 << 16 : 0), s.rank);
+    return s.atom;
   }
+  // fails every distance test
+  T.vx[l] = 1e30; T.vy[l] = 1e30; T.vz[l] = 1e30;
+  T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
+  T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
+  T.key[l] = make_int2(l << 8, 0);
+  return -1;
 }
 
 // Adds the sums of slot s (vector v, atom k) to the gradient and dE/dCN of
@@ -736,8 +741,9 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
 
   // A tile (shared)
   double* __restrict__ accB = accBw[w];
+  int jat = -1;  // atom of a-slot l
   if (w == 0) {
-    load_tile(tileA, slots, L0, P * kTile + l, m, l);
+    jat = load_tile(tileA, slots, L0, P * kTile + l, m, l, i);
     if (kGrad)
       for (int c = 0; c < kAcc; ++c) accA[c * kTile + l] = 0.0;
   }
@@ -745,10 +751,12 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
     for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
   __syncthreads();
   const bool aval = P * kTile + l < m;
+  const bool aval_half = aval && l < kTile / 2;  // step 32 of a diagonal tile
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
-  const int jat = tileA.key[l].x;
-  const unsigned arow = aval ? static_cast<unsigned>(tileA.key[l].y) * static_cast<unsigned>(n) : 0u;
-  const unsigned aself = aval && jat == i ? 1u << 16 : 0u;
+  const int2 akey = tileA.key[l];
+  const unsigned arow = aval ? static_cast<unsigned>(akey.y) * static_cast<unsigned>(n) : 0u;
+  // entry tag of the b-slot, keeping the self flag only when a is also an image of i
+  const unsigned amask = (akey.x & (1 << 16)) ? 0x1FF00u : 0xFF00u;
 
   double ei = 0.0, di = 0.0;
   double gi[3] = {0.0, 0.0, 0.0};
@@ -759,7 +767,8 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
     const bool diag = Q == P;
     TileSlots<R>& TB = diag ? tileA : tileB[w];
     double* __restrict__ accb = diag ? accA : accB;  // b-slots of a diagonal tile are a-slots
-    if (!diag) load_tile(TB, slots, L0, Q * kTile + l, m, l);
+    int katl = -1;  // atom of b-slot l of this tile
+    if (!diag) katl = load_tile(TB, slots, L0, Q * kTile + l, m, l, i);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const int s0 = diag ? 1 : 0, s1 = diag ? kTile / 2 : kTile - 1;
@@ -770,16 +779,16 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
         const int bl = (l + st) & (kTile - 1);
         double v0, v1, v2;
         const double Cd = diff_norm2(TB.vx[bl], TB.vy[bl], TB.vz[bl], ax, ay, az, v0, v1, v2);
-        const bool ok = aval && (!diag || st < kTile / 2 || l < kTile / 2) && Cd <= cut2 && Cd >= kEps;
+        const bool lane_ok = (diag && st == kTile / 2) ? aval_half : aval;
+        const bool ok = lane_ok & (Cd <= cut2) & (Cd >= kEps);
         const unsigned long long bal = __ballot(ok);
         if (ok) {
           const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
                                      static_cast<unsigned>(bal >> 32),
                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
           const int2 kb = TB.key[bl];
-          const int kat = kb.x;
           PairEntry<R> e;
-          e.ab = static_cast<unsigned>(l | (bl << 8)) | (kat == i ? aself : 0u);
+          e.ab = (static_cast<unsigned>(kb.x) & amask) | static_cast<unsigned>(l);
           e.rec = arow + static_cast<unsigned>(kb.y);
           e.C = R(Cd);
           q_ring[pos & (kQueue - 1)] = e;
@@ -869,7 +878,7 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
       const double M = accB[0 * kTile + l], Kx = accB[1 * kTile + l], Ky = accB[2 * kTile + l];
       const double Kz = accB[3 * kTile + l], D = accB[4 * kTile + l];
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
-      if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], TB.key[l].x, grad, dedcn, gi, sg);
+      if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, grad, dedcn, gi, sg);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
