@@ -93,6 +93,13 @@ D3HIP_API void dftd3_delete_param(dftd3_param* param);
 D3HIP_API void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model disp, dftd3_param param,
                                     double* energy, double* gradient, double* sigma);
 
+/* s-dftd3.h:390-397 dftd3_get_pairwise_dispersion (api.f90:1180-1233,
+ * disp.f90:403-457): pair_energy2[n][n] receives the additive (two-body) and
+ * pair_energy3[n][n] the non-additive (three-body) energy of every pair of
+ * atoms (host arrays; both matrices are symmetric). */
+D3HIP_API void dftd3_get_pairwise_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model disp,
+                                             dftd3_param param, double* pair_energy2, double* pair_energy3);
+
 /* ------------------------------------------------------------------ Part 2
  * No counterpart in the reference: device-pointer entry points. */
 
@@ -119,6 +126,17 @@ D3HIP_API void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspa
                                           const double* params, const double* cutoffs, int precision,
                                           const double* positions, double* energy, double* gradient,
                                           double* sigma, void* stream);
+
+/* Pairwise-resolved energies of a batch (device arrays; same batch
+ * description as d3hip_get_dispersion_batch): pair_energy2 and pair_energy3
+ * hold per system an n x n block in atom order, the blocks one after another
+ * (system s at offset sum of n^2 of the systems before it).  Asynchronous
+ * except for one device-to-host read of the three-body list size. */
+D3HIP_API void d3hip_get_pairwise_dispersion_batch(dftd3_error error, dftd3_model workspace, int nsys,
+                                                   const int* natoms, const int* numbers, const double* lattices,
+                                                   const bool* periodic, const double* params,
+                                                   const double* cutoffs, int precision, const double* positions,
+                                                   double* pair_energy2, double* pair_energy3, void* stream);
 
 /* A model without a structure, usable only as the workspace (device buffers
  * and stream) of d3hip_get_dispersion_batch. */

@@ -677,6 +677,145 @@ static void atm(int nat, const int *num, const double *xyz, int ntr, const doubl
     }
 }
 
+/* -------------------------------------------------------- pairwise */
+/*
+ * rational.f90:623-705 / zero.f90:662-755 get_pairwise_dispersion2: the
+ * two-body energy of every pair (iat, jat <= iat) summed over the images,
+ * half of each term on (jat, iat) and half on (iat, jat) (once on the
+ * diagonal).  energy is nat x nat in Fortran order (FIDX).
+ */
+static void disp2_pairwise(const d3o_param *p, int nat, const int *num, const double *xyz, int ntr,
+                           const double *trans, double cutoff, double width, const double *c6,
+                           double *energy)
+{
+    double cutoff2 = cutoff * cutoff;
+#pragma omp parallel
+    {
+        double *el = (double *)calloc((size_t)nat * nat, sizeof(double));
+#pragma omp for schedule(static)
+        for (int iat = 0; iat < nat; ++iat) {
+            int zi = num[iat];
+            for (int jat = 0; jat <= iat; ++jat) {
+                int zj = num[jat];
+                double rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
+                double c6ij = c6[FIDX(jat, iat)];
+                double r0ij6 = 0, r0ij8 = 0, rs6r0 = 0, rs8r0 = 0;
+                if (p->kind == 0) {
+                    double r0ij = p->a1 * sqrt(rrij) + p->a2;
+                    double r0ij2 = r0ij * r0ij;
+                    r0ij6 = r0ij2 * r0ij2 * r0ij2;
+                    r0ij8 = r0ij6 * r0ij2;
+                } else {
+                    double r0ij = d3o_rvdw(zj, zi);
+                    rs6r0 = p->a1 * r0ij;
+                    rs8r0 = p->a2 * r0ij;
+                }
+                for (int jtr = 0; jtr < ntr; ++jtr) {
+                    const double *t = trans + 3 * jtr;
+                    double v[3];
+                    for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + t[c]);
+                    double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    if (r2 > cutoff2 || r2 < EPS) continue;
+                    double r1 = sqrt(r2), sw, dswdr;
+                    smooth_cutoff(r1, cutoff, width, &sw, &dswdr);
+                    double edisp;
+                    if (p->kind == 0) {
+                        double t6 = 1.0 / (pow(r2, 3) + r0ij6);
+                        double t8 = 1.0 / (pow(r2, 4) + r0ij8);
+                        edisp = sw * (p->s6 * t6 + p->s8 * rrij * t8);
+                    } else {
+                        double r6 = r2 * r2 * r2, r8 = r6 * r2;
+                        double t6 = pow(rs6r0 / r1, p->alp);
+                        double t8 = pow(rs8r0 / r1, p->alp + 2.0);
+                        double f6 = 1.0 / (1.0 + 6.0 * t6);
+                        double f8 = 1.0 / (1.0 + 6.0 * t8);
+                        edisp = sw * (p->s6 * f6 / r6 + p->s8 * rrij * f8 / r8);
+                    }
+                    double dE = -c6ij * edisp * 0.5;
+                    el[FIDX(jat, iat)] += dE;
+                    if (iat != jat) el[FIDX(iat, jat)] += dE;
+                }
+            }
+        }
+#pragma omp critical(d3o_disp2_pairwise_)
+        for (size_t k = 0; k < (size_t)nat * nat; ++k) energy[k] += el[k];
+        free(el);
+    }
+}
+
+/*
+ * atm.f90:829-950 get_atm_pairwise_dispersion: every triple term, divided
+ * by six, on the six ordered pairs of its atoms.  No C6 screening (unlike
+ * the gradient branch).
+ */
+static void atm_pairwise(int nat, const int *num, const double *xyz, int ntr, const double *trans,
+                         double cutoff, double width, double s9, double rs9, double alp,
+                         const double *c6, double *energy)
+{
+    if (fabs(s9) < EPS) return;
+    double cutoff2 = cutoff * cutoff, alp3 = alp / 3.0;
+#pragma omp parallel
+    {
+        double *el = (double *)calloc((size_t)nat * nat, sizeof(double));
+#pragma omp for schedule(static)
+        for (int iat = 0; iat < nat; ++iat) {
+            int zi = num[iat];
+            for (int jat = 0; jat <= iat; ++jat) {
+                int zj = num[jat];
+                double c6ij = c6[FIDX(jat, iat)];
+                double r0ij = rs9 * d3o_rvdw(zj, zi);
+                for (int jtr = 0; jtr < ntr; ++jtr) {
+                    const double *tj = trans + 3 * jtr;
+                    double vij[3];
+                    for (int c = 0; c < 3; ++c) vij[c] = xyz[3 * jat + c] + tj[c] - xyz[3 * iat + c];
+                    double r2ij = vij[0] * vij[0] + vij[1] * vij[1] + vij[2] * vij[2];
+                    if (r2ij > cutoff2 || r2ij < EPS) continue;
+                    double rij = sqrt(r2ij), swij, dsw;
+                    smooth_cutoff(rij, cutoff, width, &swij, &dsw);
+                    for (int kat = 0; kat <= jat; ++kat) {
+                        int zk = num[kat];
+                        double c6ik = c6[FIDX(kat, iat)];
+                        double c6jk = c6[FIDX(kat, jat)];
+                        double c9 = -s9 * sqrt(fabs(c6ij * c6ik * c6jk));
+                        double r0 = r0ij * (rs9 * d3o_rvdw(zk, zi)) * (rs9 * d3o_rvdw(zk, zj));
+                        double triple = triple_scale(iat, jat, kat);
+                        for (int ktr = 0; ktr < ntr; ++ktr) {
+                            const double *tk = trans + 3 * ktr;
+                            double vik[3], vjk[3];
+                            for (int c = 0; c < 3; ++c) vik[c] = xyz[3 * kat + c] + tk[c] - xyz[3 * iat + c];
+                            double r2ik = vik[0] * vik[0] + vik[1] * vik[1] + vik[2] * vik[2];
+                            if (r2ik > cutoff2 || r2ik < EPS) continue;
+                            double rik = sqrt(r2ik), swik;
+                            smooth_cutoff(rik, cutoff, width, &swik, &dsw);
+                            for (int c = 0; c < 3; ++c) vjk[c] = vik[c] - vij[c];
+                            double r2jk = vjk[0] * vjk[0] + vjk[1] * vjk[1] + vjk[2] * vjk[2];
+                            if (r2jk > cutoff2 || r2jk < EPS) continue;
+                            double rjk = sqrt(r2jk), swjk;
+                            smooth_cutoff(rjk, cutoff, width, &swjk, &dsw);
+                            double sw = swij * swik * swjk;
+                            double r2 = r2ij * r2ik * r2jk;
+                            double r1 = sqrt(r2), r3 = r2 * r1, r5 = r3 * r2;
+                            double fdmp = 1.0 / (1.0 + 6.0 * pow(r0 / r1, alp3));
+                            double ang = 0.375 * (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik)
+                                       * (-r2ij + r2jk + r2ik) / r5 + 1.0 / r3;
+                            double dE = ang * fdmp * c9 * triple * sw / 6.0;
+                            el[FIDX(jat, iat)] -= dE;
+                            el[FIDX(kat, iat)] -= dE;
+                            el[FIDX(iat, jat)] -= dE;
+                            el[FIDX(kat, jat)] -= dE;
+                            el[FIDX(iat, kat)] -= dE;
+                            el[FIDX(jat, kat)] -= dE;
+                        }
+                    }
+                }
+            }
+        }
+#pragma omp critical(d3o_atm_pairwise_)
+        for (size_t k = 0; k < (size_t)nat * nat; ++k) energy[k] += el[k];
+        free(el);
+    }
+}
+
 /* ------------------------------------------------------- orchestrator */
 static int lattice_alloc(const int *periodic, const double *lat, double rthr, double **trans)
 {
@@ -748,6 +887,46 @@ int d3o_get_dispersion(int nat, const int *num, const double *xyz, const double 
     if (dEdcn_out && grad) memcpy(dEdcn_out, dEdcn, sizeof(double) * nat);
     if (energies_out) memcpy(energies_out, en, sizeof(double) * nat);
     free(cn); free(gw); free(gwd); free(c6); free(dc6); free(en); free(dEdcn); free(gl);
+    return 0;
+}
+
+/*
+ * disp.f90:403-457 get_pairwise_dispersion_v2: CN, weights and C6 as for the
+ * energy, then the pairwise two-body and three-body energies.  pair2 and
+ * pair3 are nat x nat (Fortran order; both are symmetric).
+ */
+int d3o_get_pairwise_dispersion(int nat, const int *num, const double *xyz, const double *lattice,
+                                const int *periodic, int kind, const double *par, const double *cut,
+                                const int *ghost, double *pair2, double *pair3)
+{
+    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5]};
+    int mref = 0;
+    for (int i = 0; i < nat; ++i)
+        if (g_nref[num[i] - 1] > mref) mref = g_nref[num[i] - 1];
+    double zero_lat[9] = {0};
+    int zero_pbc[3] = {0, 0, 0};
+    if (!lattice) lattice = zero_lat;
+    if (!periodic) periodic = zero_pbc;
+
+    double *cn = (double *)calloc(nat, sizeof(double));
+    double *trans;
+    int ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+    d3o_cn(nat, num, xyz, ntr, trans, cut[0], cn);
+    free(trans);
+    double *gw = (double *)calloc((size_t)nat * mref, sizeof(double));
+    d3o_weights(nat, num, cn, mref, gw, NULL);
+    double *c6 = (double *)calloc((size_t)nat * nat, sizeof(double));
+    d3o_c6(nat, num, mref, gw, NULL, ghost, c6, NULL);
+
+    memset(pair2, 0, sizeof(double) * nat * nat);
+    memset(pair3, 0, sizeof(double) * nat * nat);
+    ntr = lattice_alloc(periodic, lattice, cut[1], &trans);
+    disp2_pairwise(&p, nat, num, xyz, ntr, trans, cut[1], cut[3], c6, pair2);
+    free(trans);
+    ntr = lattice_alloc(periodic, lattice, cut[2], &trans);
+    atm_pairwise(nat, num, xyz, ntr, trans, cut[2], cut[4], p.s9, 4.0 / 3.0, p.alp + 2.0, c6, pair3);
+    free(trans);
+    free(cn); free(gw); free(c6);
     return 0;
 }
 

@@ -190,6 +190,30 @@ def dispersion(numbers, positions, lattice=None, periodic=None, kind=RATIONAL,
     return out
 
 
+def pairwise(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, param=None,
+             cutoff=DEFAULT_CUTOFF, ghost=None):
+    """FP64 oracle of get_pairwise_dispersion (disp.f90:403-457): returns the
+    additive (two-body) and non-additive (three-body) pair energy matrices
+    (n, n), [i, j] = energy2(i, j)."""
+    L = lib()
+    num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
+    p = dict(param)
+    s6 = p.get("s6", 1.0)
+    if kind == RATIONAL:
+        par = [s6, p["s8"], p.get("s9", 1.0), p["a1"], p["a2"], p.get("alp", 14.0)]
+    else:
+        par = [s6, p["s8"], p.get("s9", 1.0), p["rs6"], p.get("rs8", 1.0), p.get("alp", 14.0)]
+    par = np.array(par, dtype=np.float64)
+    cut = np.array(cutoff, dtype=np.float64)
+    n = len(num)
+    gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
+    e2 = np.zeros((n, n))
+    e3 = np.zeros((n, n))
+    L.d3o_get_pairwise_dispersion(n, _ptr(num, _ip), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip), kind,
+                                  _ptr(par), _ptr(cut), _ptr(gh, _ip), _ptr(e2), _ptr(e3))
+    return e2.T.copy(), e3.T.copy()
+
+
 def count_terms(positions, lattice=None, periodic=None, cutoff=DEFAULT_CUTOFF, triples=True):
     """Exact counts of CN, two-body and ATM terms the reference loops visit."""
     L = lib()

@@ -48,9 +48,12 @@ struct _dftd3_model {
   double* d_xyz = nullptr;
   double* d_out = nullptr;  // energy | sigma[9] | gradient[nat][3]
   size_t cap_atoms = 0;
+  double* d_pairs = nullptr;  // pairwise energies: pair2[nat][nat] | pair3[nat][nat]
+  size_t cap_pairs = 0;
   ~_dftd3_model() {
     if (d_xyz) (void)hipFree(d_xyz);
     if (d_out) (void)hipFree(d_out);
+    if (d_pairs) (void)hipFree(d_pairs);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -423,6 +426,48 @@ void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model di
   });
 }
 
+void dftd3_get_pairwise_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model disp, dftd3_param param,
+                                   double* pair_energy2, double* pair_energy3) {
+  // api.f90:1180-1233
+  if (!error) return;
+  if (!mol) return fatal(error, "Molecular structure data is missing");
+  if (!disp) return fatal(error, "Dispersion model is missing");
+  if (!param) return fatal(error, "Damping parameters are missing");
+  if (!disp->has_structure || disp->nat != mol->nat)
+    return fatal(error, "Dispersion model does not match the molecular structure");
+  if (!pair_energy2 || !pair_energy3) return fatal(error, "Pairwise energy arrays are missing");
+  guarded(error, [&] {
+    const int nat = mol->nat;
+    ensure_model_buffers(disp, nat);
+    const size_t nn = static_cast<size_t>(nat) * static_cast<size_t>(nat);
+    if (nn > disp->cap_pairs || !disp->d_pairs) {
+      if (disp->d_pairs) d3::hip_check(hipFree(disp->d_pairs), "hipFree");
+      disp->d_pairs = nullptr;
+      d3::hip_check(hipMalloc(&disp->d_pairs, sizeof(double) * 2 * nn), "hipMalloc");
+      disp->cap_pairs = nn;
+    }
+    std::vector<d3::SystemDesc> sys(1);
+    sys[0].natoms = nat;
+    std::memcpy(sys[0].lattice, mol->lattice, sizeof mol->lattice);
+    std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
+    sys[0].param = make_sys_param(param, disp->cutoff);
+    d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
+                                 disp->stream),
+                  "upload positions");
+    bool any_ghost = false;
+    for (int g : disp->ghost) any_ghost |= g != 0;
+    disp->engine->run_pairwise(sys, mol->num.data(), disp->d_xyz, disp->precision, disp->d_pairs,
+                               disp->d_pairs + nn, disp->stream, any_ghost ? disp->ghost.data() : nullptr);
+    d3::hip_check(hipMemcpyAsync(pair_energy2, disp->d_pairs, sizeof(double) * nn, hipMemcpyDeviceToHost,
+                                 disp->stream),
+                  "download results");
+    d3::hip_check(hipMemcpyAsync(pair_energy3, disp->d_pairs + nn, sizeof(double) * nn, hipMemcpyDeviceToHost,
+                                 disp->stream),
+                  "download results");
+    d3::hip_check(hipStreamSynchronize(disp->stream), "hipStreamSynchronize");
+  });
+}
+
 namespace {
 std::vector<d3::SystemDesc> batch_systems(int nsys, const int* natoms, const double* lattices, const bool* periodic,
                                           const double* params, const double* cutoffs) {
@@ -467,6 +512,23 @@ void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspace, int ns
     out.gradient = gradient;
     out.sigma = sigma;
     workspace->engine->run(sys, numbers, positions, precision, out, static_cast<hipStream_t>(stream));
+  });
+}
+
+void d3hip_get_pairwise_dispersion_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
+                                         const int* numbers, const double* lattices, const bool* periodic,
+                                         const double* params, const double* cutoffs, int precision,
+                                         const double* positions, double* pair_energy2, double* pair_energy3,
+                                         void* stream) {
+  if (!error) return;
+  if (!workspace) return fatal(error, "Dispersion model is missing");
+  if (nsys <= 0 || !natoms || !numbers || !params || !positions || !pair_energy2 || !pair_energy3)
+    return fatal(error, "Invalid batch description");
+  if (precision != d3::kFP64 && precision != d3::kMixed) return fatal(error, "Unknown precision mode");
+  guarded(error, [&] {
+    std::vector<d3::SystemDesc> sys = batch_systems(nsys, natoms, lattices, periodic, params, cutoffs);
+    workspace->engine->run_pairwise(sys, numbers, positions, precision, pair_energy2, pair_energy3,
+                                    static_cast<hipStream_t>(stream));
   });
 }
 

@@ -220,7 +220,7 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
-      item_off, slots, counts, counter, flag;
+      item_off, slots, counts, counter, flag, pair3s;
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
@@ -435,7 +435,7 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
   timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
   timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
   timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
-  timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, stream); });
+  timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, nullptr, stream); });
   if (pb.any_atm) {
     list_offsets(pb, w.pair, mixed, stream);
     long long* o = off.as<long long>(natoms + 1);
@@ -472,6 +472,45 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
   const bool grad = out.gradient || out.sigma;
   m.evaluate(pb, positions_dev, true, precision == kMixed, grad, pb.smooth, out, debug, stats, -1, 0, AtmShape{},
              stream);
+  if (m.timer.enabled) m.timer.collect();
+}
+
+void Engine::run_pairwise(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                          int precision, double* pair2, double* pair3, hipStream_t stream, const int* ghost_host) {
+  // disp.f90:403-457: CN, weights and C6 as for the energy (no derivatives),
+  // then the pairwise two-body and three-body energies
+  if (systems.empty()) return;
+  Workspace& m = *ws_;
+  const Prepared pb = m.upload(systems, numbers, stream, ghost_host);
+  const DevBatch& db = pb.db;
+  const int natoms = db.natoms;
+  const bool mixed = precision == kMixed;
+  m.pack(pb, positions_dev, stream, true);
+  const DevWork w = m.work(pb, mixed, false, BatchOutput{}, nullptr);
+  m.timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
+  m.timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, nullptr, stream); });
+  m.timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, nullptr, w.pair, stream); });
+  m.timed("disp2", stream, [&] { launch_disp2(db, mixed, false, w, pair2, stream); });
+  const size_t bytes = sizeof(double) * static_cast<size_t>(pb.pair_total);
+  if (!pb.any_atm) {
+    hip_check(hipMemsetAsync(pair3, 0, bytes, stream), "hipMemsetAsync");
+  } else {
+    double* S = m.pair3s.as<double>(static_cast<size_t>(pb.pair_total));
+    hip_check(hipMemsetAsync(S, 0, bytes, stream), "hipMemsetAsync");
+    m.list_offsets(pb, w.pair, mixed, stream);
+    long long* o = m.off.as<long long>(natoms + 1);
+    long long* io = m.item_off.as<long long>(natoms + 1);
+    long long total = 0;
+    hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
+    hip_check(hipStreamSynchronize(stream), "list size sync");
+    void* sl = m.slots.get(std::max<long long>(total, 1) * slot_record_bytes(mixed));
+    m.timed("atm_list", stream, [&] { launch_atm_fill(db, w.pair, mixed, o, sl, total, stream); });
+    m.timed("atm", stream, [&] {
+      launch_atm_pairs(db, mixed, pb.smooth, io, total / kTile + natoms, o, sl, w, S, stream);
+      launch_pair3_finalize(db, S, pair3, stream);
+    });
+  }
+  hip_check(hipGetLastError(), "kernel launch");
   if (m.timer.enabled) m.timer.collect();
 }
 

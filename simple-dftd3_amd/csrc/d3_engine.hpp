@@ -67,6 +67,16 @@ class Engine {
            hipStream_t stream, const BatchDebug* debug = nullptr, RunStats* stats = nullptr,
            const int* ghost_host = nullptr);
 
+  // Pairwise-resolved energies (disp.f90:403-457) of a batch: pair2 and
+  // pair3 (device) hold per system an n x n block in atom order, the systems
+  // one after another (offset sum of n^2 of the systems before); pair2 is the
+  // two-body (additive) and pair3 the three-body (non-additive) energy of
+  // every ordered pair.  Both blocks are symmetric.  The size of the
+  // three-body lists is read back once per call.
+  void run_pairwise(const std::vector<SystemDesc>& systems, const int* numbers_host, const double* positions_dev,
+                    int precision, double* pair2, double* pair3, hipStream_t stream,
+                    const int* ghost_host = nullptr);
+
   // Work units of the reference's loops per system: counts_host[3*s + 0..2]
   // = CN terms, two-body pair terms, ATM triple terms (host array).
   void count_terms(const std::vector<SystemDesc>& systems, const int* numbers_host,

@@ -337,10 +337,17 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
 // rational.f90:378-514 (BJ) and zero.f90:409-553 (zero damping), E+grad+sigma;
 // rational.f90:282-374 / zero.f90:306-405 when kGrad is false.  One wavefront
 // per atom i; atom i collects the ordered pairs (i, j, T) it takes part in.
-template <typename R, bool kGrad>
+// kPairs (energy only) instead evaluates the pairs j <= i of atom i once and
+// writes their image sums, the pairwise two-body energy of
+// rational.f90:623-705 / zero.f90:662-755, to both pair2 entries (i, j) and
+// (j, i) of the system's n x n block (atom order, offset c6_begin), so the
+// matrix is exactly symmetric like the reference's; e_atom is then only
+// scratch.
+template <typename R, bool kGrad, bool kPairs>
 __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* __restrict__ pair,
                                                        double* __restrict__ e_atom, double* __restrict__ dedcn,
-                                                       double* __restrict__ grad, double* __restrict__ sig_atom) {
+                                                       double* __restrict__ grad, double* __restrict__ sig_atom,
+                                                       double* __restrict__ pair2) {
   const int lane = wave_lane();
   const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (i >= b.natoms) return;
@@ -357,9 +364,11 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
   const bool active = !(fabs(p.s6) < kEps && fabs(p.s8) < kEps);  // rational.f90:267
   const R s6 = R(p.s6), s8 = R(p.s8), cutr = R(p.cut2), width = R(p.width2);
   const R4<R>* row = pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n;
-  if (active) {
+  if (active || kPairs) {  // the pairwise energies are written (as zeros) for s6 = s8 = 0 too
     for (int jr = lane; jr < n; jr += kWave) {
       const int j = b.order[g.atom_begin + jr];
+      if (kPairs && j > i) continue;
+      double ej = 0.0;
       const double4 xj = b.pos[j];
       const int spj = b.sp[j];
       const R rrij = R(3) * r4r2i * R(b.r4r2[spj]);
@@ -415,7 +424,9 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
           }
         }
         const R edisp = sw * edisp0;
-        acc_e += double(-c6ij * edisp * R(0.5));
+        const double de = double(-c6ij * edisp * R(0.5));
+        acc_e += de;
+        if (kPairs) ej += de;
         if (kGrad) {
           const R gdisp = sw * gdisp0 + dswdr * edisp0 / r1;
           const R f = -c6ij * gdisp;  // dG = f * vec
@@ -428,6 +439,11 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
           sxx += double(h * v0 * v0); syy += double(h * v1 * v1); szz += double(h * v2 * v2);
           sxy += double(h * v0 * v1); sxz += double(h * v0 * v2); syz += double(h * v1 * v2);
         }
+      }
+      if (kPairs) {
+        const long long il = i - g.atom_begin, jl = j - g.atom_begin;
+        pair2[g.c6_begin + il * n + jl] = ej;
+        pair2[g.c6_begin + jl * n + il] = ej;
       }
     }
   }
@@ -686,15 +702,28 @@ template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return
 // Work items are taken in reverse order (the longest slot lists first):
 // block k takes item total-1-k, or, with a work counter (persistent launch),
 // the items total-1-c for the counter values c it draws until none is left.
-template <typename R, bool kGrad, bool kSmooth, int kW>
-__global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
-                                                         const long long* __restrict__ offset,
-                                                         const Slot<R>* __restrict__ slots,
-                                                         const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
-                                                         double* __restrict__ dedcn, double* __restrict__ grad,
-                                                         double* __restrict__ sig_atom,
-                                                         unsigned long long* __restrict__ work_counter) {
-  constexpr int kAccLen = kGrad ? kAcc * kTile : 1;
+//
+// kPairs (energy only) distributes every triple term over the pairs of its
+// atoms, the pairwise three-body energy of atm.f90:829-950: a sixth of the
+// term, t, goes to each ordered pair, i.e. t to each of the three unordered
+// pairs {i, j}, {i, k}, {j, k} in both orders.  The pair sums are collected
+// in pair3s, an n x n block per system in storage order (rank_x, rank_y),
+// each unordered pair in one of its two orders: {j, k} at the queue entry's
+// pair-record index, {i, s} per slot s at (rank_i, rank_s) when the slot is
+// flushed.  The pair energy of (x, y) is then pair3s(rx, ry) + pair3s(ry, rx)
+// (pair3_finalize_kernel).
+constexpr int kAccPairs = 1;  // per-slot sums of the pairwise mode: sum t
+
+template <typename R, bool kGrad, bool kSmooth, int kW, bool kPairs>
+__device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict__ item_off,
+                                         const long long* __restrict__ offset, const Slot<R>* __restrict__ slots,
+                                         const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
+                                         double* __restrict__ dedcn, double* __restrict__ grad,
+                                         double* __restrict__ sig_atom, unsigned long long* __restrict__ work_counter,
+                                         double* __restrict__ pair3s) {
+  static_assert(!(kGrad && kPairs), "the pairwise mode is energy only");
+  constexpr int kNAcc = kGrad ? kAcc : (kPairs ? kAccPairs : 0);
+  constexpr int kAccLen = kNAcc > 0 ? kNAcc * kTile : 1;
   __shared__ TileSlots<R> tileA;
   __shared__ TileSlots<R> tileB[kW];
   // per-slot sums in double (FP32 LDS atomics measured several times slower):
@@ -744,11 +773,9 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   int jat = -1;  // atom of a-slot l
   if (w == 0) {
     jat = load_tile(tileA, slots, L0, P * kTile + l, m, l, i);
-    if (kGrad)
-      for (int c = 0; c < kAcc; ++c) accA[c * kTile + l] = 0.0;
+    for (int c = 0; c < kNAcc; ++c) accA[c * kTile + l] = 0.0;
   }
-  if (kGrad)
-    for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
+  for (int c = 0; c < kNAcc; ++c) accB[c * kTile + l] = 0.0;
   __syncthreads();
   const bool aval = P * kTile + l < m;
   const bool aval_half = aval && l < kTile / 2;  // step 32 of a diagonal tile
@@ -757,6 +784,7 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   const unsigned arow = aval ? static_cast<unsigned>(akey.y) * static_cast<unsigned>(n) : 0u;
   // entry tag of the b-slot, keeping the self flag only when a is also an image of i
   const unsigned amask = (akey.x & (1 << 16)) ? 0x1FF00u : 0xFF00u;
+  const long long irow = kPairs ? static_cast<long long>(b.rank[i]) * n : 0;  // pairs {i, s} in pair3s
 
   double ei = 0.0, di = 0.0;
   double gi[3] = {0.0, 0.0, 0.0};
@@ -867,6 +895,12 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
             dE *= sw;
           }
           ei -= double(dE);
+          if (kPairs) {
+            const double t = double(dE) * (-1.0 / 6.0);
+            atomicAdd(&accA[a], t);
+            atomicAdd(&accb[bb], t);
+            atomicAdd(pair3s + g.c6_begin + e.rec, t);
+          }
         }
       }
       // the next test phase may overwrite the ring entries read above
@@ -880,6 +914,11 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
       if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, grad, dedcn, gi, sg);
     }
+    if (kPairs && !diag) {  // pair {i, b-slot atom}
+      const double t = accB[l];
+      accB[l] = 0.0;
+      if (Q * kTile + l < m && t != 0.0) atomicAdd(pair3s + g.c6_begin + irow + TB.key[l].y, t);
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
   }
@@ -887,6 +926,8 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   if (kGrad && w == 0 && aval)
     flush_slot(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l],
                accA[4 * kTile + l], ax, ay, az, jat, grad, dedcn, gi, sg);
+  if (kPairs && w == 0 && aval && accA[l] != 0.0)  // pair {i, a-slot atom}
+    atomicAdd(pair3s + g.c6_begin + irow + akey.y, accA[l]);
   block_add(e_atom + i, ei, red);
   if (kGrad) {
     block_add(grad + 3 * i, gi[0], red);
@@ -901,6 +942,48 @@ __global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long 
   __syncthreads();
   k_item = next_item;
   __syncthreads();  // next_item is rewritten only after every thread has read it
+  }
+}
+
+template <typename R, bool kGrad, bool kSmooth, int kW>
+__global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
+                                                         const long long* __restrict__ offset,
+                                                         const Slot<R>* __restrict__ slots,
+                                                         const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
+                                                         double* __restrict__ dedcn, double* __restrict__ grad,
+                                                         double* __restrict__ sig_atom,
+                                                         unsigned long long* __restrict__ work_counter) {
+  atm_body<R, kGrad, kSmooth, kW, false>(b, item_off, offset, slots, pair, e_atom, dedcn, grad, sig_atom,
+                                         work_counter, nullptr);
+}
+
+// Pairwise three-body energies (one block of four wavefronts per work item).
+template <typename R, bool kSmooth>
+__global__ __launch_bounds__(4 * kWave) void atm_pairs_kernel(DevBatch b, const long long* __restrict__ item_off,
+                                                              const long long* __restrict__ offset,
+                                                              const Slot<R>* __restrict__ slots,
+                                                              const R4<R>* __restrict__ pair,
+                                                              double* __restrict__ e_atom,
+                                                              double* __restrict__ pair3s) {
+  atm_body<R, false, kSmooth, 4, true>(b, item_off, offset, slots, pair, e_atom, nullptr, nullptr, nullptr, nullptr,
+                                       pair3s);
+}
+
+// Pair energies in atom order from the storage-order sums of atm_pairs_kernel:
+// out[c6_begin + x * n + y] = pair3s(rx, ry) + pair3s(ry, rx), one wavefront
+// per atom x (row x of its system's block).
+__global__ __launch_bounds__(kBlock) void pair3_finalize_kernel(DevBatch b, const double* __restrict__ pair3s,
+                                                                double* __restrict__ out) {
+  const int x = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (x >= b.natoms) return;
+  const SysGeom g = b.geom[b.sys[x]];
+  const int n = g.natoms;
+  const long long rx = b.rank[x];
+  const double* __restrict__ S = pair3s + g.c6_begin;
+  double* __restrict__ row = out + g.c6_begin + static_cast<long long>(x - g.atom_begin) * n;
+  for (int y = wave_lane(); y < n; y += kWave) {
+    const long long ry = b.rank[g.atom_begin + y];
+    row[y] = S[rx * n + ry] + S[ry * n + rx];
   }
 }
 
@@ -1125,19 +1208,24 @@ void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* 
                        static_cast<double4*>(pair));
 }
 
-void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, hipStream_t st) {
+template <typename R, bool kGrad, bool kPairs>
+void launch_disp2_t(const DevBatch& b, const DevWork& w, double* pair2, hipStream_t st) {
   const dim3 grid(blocks_for(b.natoms, kWavesPerBlock)), blk(kBlock);
-  const float4* pf = static_cast<const float4*>(w.pair);
-  const double4* pd = static_cast<const double4*>(w.pair);
-  if (mixed && grad)
-    hipLaunchKernelGGL((disp2_kernel<float, true>), grid, blk, 0, st, b, pf, w.e_atom, w.dedcn, w.grad, w.sig_atom);
-  else if (mixed)
-    hipLaunchKernelGGL((disp2_kernel<float, false>), grid, blk, 0, st, b, pf, w.e_atom, w.dedcn, w.grad, w.sig_atom);
-  else if (grad)
-    hipLaunchKernelGGL((disp2_kernel<double, true>), grid, blk, 0, st, b, pd, w.e_atom, w.dedcn, w.grad, w.sig_atom);
-  else
-    hipLaunchKernelGGL((disp2_kernel<double, false>), grid, blk, 0, st, b, pd, w.e_atom, w.dedcn, w.grad,
-                       w.sig_atom);
+  hipLaunchKernelGGL((disp2_kernel<R, kGrad, kPairs>), grid, blk, 0, st, b, static_cast<const R4<R>*>(w.pair),
+                     w.e_atom, w.dedcn, w.grad, w.sig_atom, pair2);
+}
+
+void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, double* pair2, hipStream_t st) {
+  if (pair2) {
+    if (mixed) launch_disp2_t<float, false, true>(b, w, pair2, st);
+    else launch_disp2_t<double, false, true>(b, w, pair2, st);
+  } else if (mixed) {
+    if (grad) launch_disp2_t<float, true, false>(b, w, nullptr, st);
+    else launch_disp2_t<float, false, false>(b, w, nullptr, st);
+  } else {
+    if (grad) launch_disp2_t<double, true, false>(b, w, nullptr, st);
+    else launch_disp2_t<double, false, false>(b, w, nullptr, st);
+  }
 }
 
 void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, hipStream_t st) {
@@ -1231,6 +1319,33 @@ void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const Atm
     launch_atm_w<4>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
   else
     throw std::invalid_argument("three-body launch: wavefronts per block must be 2 or 4");
+}
+
+namespace {
+template <typename R, bool kSmooth>
+void launch_atm_pairs_t(const DevBatch& b, const long long* item_off, long long max_items, const long long* offset,
+                        const void* slots, const DevWork& w, double* pair3s, hipStream_t st) {
+  hipLaunchKernelGGL((atm_pairs_kernel<R, kSmooth>), dim3(static_cast<unsigned>(max_items)), dim3(4 * kWave), 0, st,
+                     b, item_off, offset, static_cast<const Slot<R>*>(slots), static_cast<const R4<R>*>(w.pair),
+                     w.e_atom, pair3s);
+}
+}  // namespace
+
+void launch_atm_pairs(const DevBatch& b, bool mixed, bool smooth, const long long* item_off, long long max_items,
+                      const long long* offset, const void* slots, const DevWork& w, double* pair3s, hipStream_t st) {
+  if (max_items <= 0) return;
+  if (mixed) {
+    if (smooth) launch_atm_pairs_t<float, true>(b, item_off, max_items, offset, slots, w, pair3s, st);
+    else launch_atm_pairs_t<float, false>(b, item_off, max_items, offset, slots, w, pair3s, st);
+  } else {
+    if (smooth) launch_atm_pairs_t<double, true>(b, item_off, max_items, offset, slots, w, pair3s, st);
+    else launch_atm_pairs_t<double, false>(b, item_off, max_items, offset, slots, w, pair3s, st);
+  }
+}
+
+void launch_pair3_finalize(const DevBatch& b, const double* pair3s, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(pair3_finalize_kernel, dim3(blocks_for(b.natoms, kWavesPerBlock)), dim3(kBlock), 0, st, b,
+                     pair3s, out);
 }
 
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t st) {

@@ -63,7 +63,9 @@ void launch_spatial_order(const DevBatch& b, void* temp, size_t temp_bytes, int*
 void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t stream);
 void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw, hipStream_t stream);
 void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, void* pair, hipStream_t stream);
-void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, hipStream_t stream);
+// pair2 (energy only, may be null): per system an n x n block (offset
+// c6_begin, atom order) of pairwise two-body energies.
+void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, double* pair2, hipStream_t stream);
 // Three-body lists: count per atom (pair may be null: no C6 test), fill slots.
 void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, hipStream_t stream);
 void launch_scan(const long long* count, int n, long long* offset, hipStream_t stream);
@@ -92,6 +94,13 @@ struct AtmShape {
 void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const AtmShape& shape,
                 const long long* item_off, long long max_items, const long long* offset, const void* slots,
                 const DevWork& w, unsigned long long* counter, hipStream_t stream);
+// Pairwise three-body energies: adds the pair sums of every triple term to
+// pair3s (per system an n x n block in storage order, zeroed by the caller);
+// pair3_finalize writes the symmetric pair energies in atom order to out.
+void launch_atm_pairs(const DevBatch& b, bool mixed, bool smooth, const long long* item_off, long long max_items,
+                      const long long* offset, const void* slots, const DevWork& w, double* pair3s,
+                      hipStream_t stream);
+void launch_pair3_finalize(const DevBatch& b, const double* pair3s, double* out, hipStream_t stream);
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t stream);
 void launch_reduce(const DevBatch& b, const DevWork& w, double* energy, double* sigma, hipStream_t stream);
 void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream_t stream);

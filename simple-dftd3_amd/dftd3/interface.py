@@ -12,12 +12,14 @@ this library implements:
   ``set_realspace_cutoff``, ``set_ghost_atoms`` and
   ``get_dispersion(param, grad)`` returning ``energy`` (0-d array),
   ``gradient`` (n, 3) and ``virial`` (3, 3) in Hartree and bohr
-  (interface.py:505-616).
+  (interface.py:505-616), and ``get_pairwise_dispersion(param)`` returning
+  the ``additive pairwise energy`` and ``non-additive pairwise energy``
+  (n, n) matrices (interface.py:618-636).
 
 All evaluation happens on the GPU through libd3hip.so.  Additions beyond the
 reference: ``DispersionModel.set_precision("fp64" | "mixed")``.  The damping
-families, Hessian, pairwise energies, Ewald summation and counterpoise
-correction of the reference are not part of this library; their classes
+families, Hessian, Ewald summation and counterpoise correction of the
+reference are not part of this library; their classes
 exist so that imports keep working, and raise ``NotImplementedError`` when
 used.
 """
@@ -219,7 +221,16 @@ class DispersionModel(Structure):
         raise NotImplementedError("Work partitions are not available in this library")
 
     def get_pairwise_dispersion(self, param: DampingParam) -> dict:
-        raise NotImplementedError("Pairwise dispersion energies are not available in this library")
+        """Evaluate pairwise representation of the dispersion energy
+        (interface.py:618-636): the additive (two-body) and non-additive
+        (three-body) energy of every pair of atoms, (n, n) arrays."""
+        _pair_disp2 = np.zeros((len(self), len(self)))
+        _pair_disp3 = np.zeros((len(self), len(self)))
+        library.get_pairwise_dispersion(self._mol, self._disp, param._param, _pair_disp2, _pair_disp3)
+        return {
+            "additive pairwise energy": _pair_disp2,
+            "non-additive pairwise energy": _pair_disp3,
+        }
 
     def get_hessian(self, param: DampingParam) -> dict:
         raise NotImplementedError("The dispersion Hessian is not available in this library")

@@ -51,10 +51,13 @@ _SIGNATURES = {
     "dftd3_load_rational_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
     "dftd3_delete_param": (None, [ctypes.POINTER(_vp)]),
     "dftd3_get_dispersion": (None, [_vp, _vp, _vp, _vp, _dp, _dp, _dp]),
+    "dftd3_get_pairwise_dispersion": (None, [_vp, _vp, _vp, _vp, _dp, _dp]),
     "d3hip_set_model_precision": (None, [_vp, _vp, ctypes.c_int]),
     "d3hip_new_workspace": (_vp, [_vp]),
     "d3hip_get_dispersion_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, ctypes.c_int,
                                           _vp, _vp, _vp, _vp, _vp]),
+    "d3hip_get_pairwise_dispersion_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp,
+                                                   ctypes.c_int, _vp, _vp, _vp, _vp]),
     "d3hip_count_terms_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, _vp,
                                        ctypes.POINTER(ctypes.c_longlong), _vp]),
     "d3hip_set_kernel_timing": (None, [_vp, _vp, ctypes.c_int]),
@@ -228,6 +231,23 @@ def get_dispersion(mol: StructureHandle, disp: ModelHandle, param: ParamHandle, 
     """Evaluate the dispersion energy and its derivatives"""
     return error_check(lib.dftd3_get_dispersion)(mol.handle, disp.handle, param.handle, _cast(_dp, energy),
                                                  _cast(_dp, gradient), _cast(_dp, sigma))
+
+
+def get_pairwise_dispersion(mol: StructureHandle, disp: ModelHandle, param: ParamHandle, energy2: np.ndarray,
+                            energy3: np.ndarray) -> None:
+    """Evaluate the pairwise representation of the dispersion energy"""
+    return error_check(lib.dftd3_get_pairwise_dispersion)(mol.handle, disp.handle, param.handle,
+                                                          _cast(_dp, energy2), _cast(_dp, energy3))
+
+
+def get_pairwise_dispersion_batch(workspace: ModelHandle, natoms, numbers, lattices, periodic, params, cutoffs,
+                                  precision: int, positions_ptr: int, pair2_ptr: int, pair3_ptr: int,
+                                  stream: Optional[int]) -> None:
+    """Batched pair energies on device pointers (include/d3hip.h, Part 2)"""
+    return error_check(lib.d3hip_get_pairwise_dispersion_batch)(
+        workspace.handle, len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices),
+        _cast(_bp, periodic), _cast(_dp, params), _cast(_dp, cutoffs), precision, positions_ptr, pair2_ptr,
+        pair3_ptr, stream)
 
 
 def count_terms_batch(workspace: ModelHandle, natoms, numbers, lattices, periodic, params, cutoffs,

@@ -130,6 +130,38 @@ def run_batch(spec: BatchSpec, positions: torch.Tensor, precision: str = "fp64",
     return out
 
 
+def run_pairwise(spec: BatchSpec, positions: torch.Tensor, precision: str = "fp64",
+                 ws: Optional[library.ModelHandle] = None) -> dict:
+    """Pairwise-resolved energies of a batch (disp.f90:403-457) on the
+    current stream: ``pair2`` (two-body) and ``pair3`` (three-body) are flat
+    float64 tensors holding per system an (n, n) block in atom order, the
+    blocks one after another; ``blocks(out, key)`` splits them into (n, n)
+    views."""
+    _check_positions(positions)
+    dev = positions.device
+    prec = {"fp64": library.PRECISION_FP64, "mixed": library.PRECISION_MIXED}[precision]
+    total = int(np.sum(spec.natoms.astype(np.int64) ** 2))
+    out = dict(pair2=torch.empty(total, dtype=torch.float64, device=dev),
+               pair3=torch.empty(total, dtype=torch.float64, device=dev))
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with torch.cuda.device(dev):
+        library.get_pairwise_dispersion_batch(
+            ws or workspace(dev), spec.natoms, spec.numbers, spec.lattices, spec.periodic, spec.params, spec.cutoffs,
+            prec, positions.data_ptr(), out["pair2"].data_ptr(), out["pair3"].data_ptr(), stream)
+    out["natoms"] = spec.natoms
+    return out
+
+
+def blocks(out: dict, key: str):
+    """(n, n) views of the per-system blocks of ``run_pairwise``'s ``key``."""
+    views, off = [], 0
+    for n in out["natoms"]:
+        n = int(n)
+        views.append(out[key][off:off + n * n].view(n, n))
+        off += n * n
+    return views
+
+
 def count_terms(spec: BatchSpec, positions: torch.Tensor) -> np.ndarray:
     """(nsys, 3) int64 array of CN terms, two-body pair terms and ATM triple
     terms as the reference's loops visit them (counted on the GPU)."""

@@ -25,5 +25,10 @@ def goldens():
 
 
 @pytest.fixture(scope="session")
+def pairwise_goldens():
+    return {f["name"]: f for f in load_golden("pairwise_goldens.json")}
+
+
+@pytest.fixture(scope="session")
 def python_model():
     return load_golden("python_interface_model.json")

@@ -12,6 +12,8 @@ Sources:
   test/unit/test_periodic_2d.f90:78-211    BP86-D3(0) 2D and gh185 2D cells
   test/validation/01,02,06,07 (*.resp + geometry + *.json)  CLI goldens
   python/dftd3/test_interface.py:78-100,236-356  16-atom model energies
+  test/validation/11-13 (*.resp + geometry + *.json)  pair energy matrices
+  python/dftd3/test_interface.py:569-762   S8 pair energy matrices
 """
 import argparse
 import json
@@ -176,6 +178,65 @@ def validation_fixtures(ref, params):
     return out
 
 
+def pairwise_fixtures(ref):
+    """test/validation/11-13 (s-dftd3 --pair-resolved): the pair energy
+    matrices and the total energy.  Parameters as the CLI builds them from the
+    .resp arguments (app/cli.f90:400-427,478-492,530-547): --bj-param /
+    --bjm-param / --zero-param give s6 s8 a1|rs6 [a2]; s9 is 0 (param.f90:39)
+    unless --atm (1) or --atm-scale sets it; rs8 = 1, alp = 14.  --bjm-param
+    builds the rational damping (app/driver.f90:144-165).  Cases 14 and 15 use
+    the modified zero and optimized power damping and are not extracted."""
+    vdir = os.path.join(ref, "test/validation")
+    cases = [
+        ("11-pairwise-d3-bj", "11-cf3br-benzene.xyz", read_xyz, "rational",
+         dict(s6=1.0, s8=2.9491, a1=0.4466, a2=6.1742, s9=0.0, alp=14.0)),
+        ("12-pairwise-d3-zero", "12-butane.gen", read_gen, "zero",
+         dict(s6=1.0, s8=1.764, rs6=0.806, rs8=1.0, s9=1.0, alp=14.0)),
+        ("13-pairwise-d3-bjm", "13-c60.coord", read_tmol, "rational",
+         dict(s6=1.0, s8=1.206988, a1=0.240184, a2=3.864426, s9=2.0, alp=14.0)),
+    ]
+    out = []
+    for name, geo, reader, kind, par in cases:
+        nums, pos, lat, pbc = reader(os.path.join(vdir, geo))
+        ref_json = json.load(open(os.path.join(vdir, name + ".json")))
+        n = len(nums)
+        # Fortran (n, n) matrices written column by column; both are symmetric
+        e2 = ref_json["additive pairwise energy"]
+        e3 = ref_json["non-additive pairwise energy"]
+        out.append(dict(
+            name="validation_" + name, source=f"test/validation/{name}.json,{name}.resp,{geo}",
+            numbers=nums, positions=pos, lattice=lat, periodic=pbc, kind=kind, param=par,
+            cutoff=[40.0, 60.0, 40.0, 0.0, 0.0],
+            expected=dict(energy=ref_json["energy"],
+                          pair2=[[e2[c * n + r] for c in range(n)] for r in range(n)],
+                          pair3=[[e3[c * n + r] for c in range(n)] for r in range(n)]),
+            atol=1.0e-6))
+    return out
+
+
+def python_pairwise_fixture(ref):
+    """python/dftd3/test_interface.py:569-762 test_pair_resolved (S8 ring,
+    bohr, RationalDampingParam(a1=0.5545, s8=2.2609, a2=3.2297) with the
+    Python defaults s6 = 1, s9 = 1, alp = 14; abs tolerance 1e-8)."""
+    text = open(os.path.join(ref, "python/dftd3/test_interface.py")).read()
+    body = re.search(r"def test_pair_resolved\(\).*?(?=\ndef |\Z)", text, re.S).group(0)
+    nums = json.loads(re.search(r"numbers = np.array\((\[.*?\])\)", body, re.S).group(1))
+
+    def block(name):
+        m = re.search(name + r" = np.array\(\s*\[(.*?)\]\s*\)\n", body, re.S)
+        rows = re.findall(r"\[([^\[\]]*)\]", m.group(1))
+        return [[float(v) for v in re.findall(r"[+-]?\d+\.\d+(?:[eE][+-]?\d+)?", r)] for r in rows]
+
+    par = re.search(r"RationalDampingParam\((.*?)\)", body, re.S).group(1)
+    par = {k: float(v) for k, v in re.findall(r"(\w+)=([-\d.]+)", par)}
+    par.update(s6=1.0, s9=1.0, alp=14.0)
+    thr = float(re.search(r"thr = ([-\d.eE]+)", body).group(1))
+    return dict(name="python_pair_resolved", source="python/dftd3/test_interface.py:569-762",
+                numbers=nums, positions=block("positions"), lattice=None, periodic=None,
+                kind="rational", param=par, cutoff=[40.0, 60.0, 40.0, 0.0, 0.0],
+                expected=dict(pair2=block("pair_disp2"), pair3=block("pair_disp3")), atol=thr)
+
+
 def python_fixtures(ref):
     text = open(os.path.join(ref, "python/dftd3/test_interface.py")).read()
     m = re.search(r"def model\(\).*?numbers = np.array\((\[.*?\])\).*?positions = np.array\(\s*\[(.*?)\]\s*\)", text, re.S)
@@ -204,6 +265,9 @@ def main():
         json.dump(fix, fh, indent=1)
     with open(os.path.join(args.out, "python_interface_model.json"), "w") as fh:
         json.dump(python_fixtures(args.reference), fh, indent=1)
+    pw = pairwise_fixtures(args.reference) + [python_pairwise_fixture(args.reference)]
+    with open(os.path.join(args.out, "pairwise_goldens.json"), "w") as fh:
+        json.dump(pw, fh)
     print("wrote", len(fix), "golden cases")
 
 

@@ -144,6 +144,21 @@ def test_c_abi_dispersion_missing_pieces(numbers, positions):
         L.get_dispersion(mol, disp, L.ParamHandle.null(), energy, None, None)
 
 
+def test_c_abi_pairwise_missing_pieces(numbers, positions):
+    """dftd3_get_pairwise_dispersion checks its handles like api.f90:1197-1221."""
+    from dftd3 import library as L
+    n = len(numbers)
+    mol = L.new_structure(n, np.ascontiguousarray(numbers, dtype="i4"), np.ascontiguousarray(positions), None, None)
+    disp = L.new_d3_model(mol)
+    e2, e3 = np.zeros((n, n)), np.zeros((n, n))
+    with pytest.raises(RuntimeError, match="Molecular structure data is missing"):
+        L.get_pairwise_dispersion(L.StructureHandle.null(), disp, L.ParamHandle.null(), e2, e3)
+    with pytest.raises(RuntimeError, match="Dispersion model is missing"):
+        L.get_pairwise_dispersion(mol, L.ModelHandle.null(), L.ParamHandle.null(), e2, e3)
+    with pytest.raises(RuntimeError, match="Damping parameters are missing"):
+        L.get_pairwise_dispersion(mol, disp, L.ParamHandle.null(), e2, e3)
+
+
 def test_unsupported_element_rejected():
     from dftd3.interface import DispersionModel
     with pytest.raises(RuntimeError, match="Unsupported element"):
@@ -159,6 +174,8 @@ def test_no_cpu_fallback(numbers, positions):
     model = DispersionModel(numbers, positions)
     with pytest.raises(RuntimeError, match="HIP"):
         model.get_dispersion(RationalDampingParam(method="pbe0"), grad=True)
+    with pytest.raises(RuntimeError, match="HIP"):
+        model.get_pairwise_dispersion(RationalDampingParam(method="pbe0"))
 
 
 def test_synthetic_workloads_are_seeded():

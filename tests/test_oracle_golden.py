@@ -328,3 +328,71 @@ def test_lattice_adjoint_identity_on_the_oracle():
                                           grad=False)["energy"])
                 fd[a, b] = (e[0] - e[1]) / (2 * h)
         assert np.abs(d - fd).max() <= 1e-9 * np.abs(fd).max() + 1e-12
+
+
+# ------------------------------------------------------------------ pairwise
+PAIRWISE = ["validation_11-pairwise-d3-bj", "validation_12-pairwise-d3-zero", "validation_13-pairwise-d3-bjm",
+            "python_pair_resolved"]
+
+
+@pytest.mark.parametrize("name", PAIRWISE)
+def test_pairwise_goldens(pairwise_goldens, name):
+    """Pair energy matrices of the CLI validation (abs 1e-6) and of the
+    Python interface test (abs 1e-8); the total energy of the CLI cases."""
+    f = pairwise_goldens[name]
+    e2, e3 = O.pairwise(f["numbers"], f["positions"], f["lattice"], f["periodic"], _kind(f["kind"]), f["param"],
+                        f["cutoff"])
+    np.testing.assert_allclose(e2, np.array(f["expected"]["pair2"]), rtol=0, atol=f["atol"])
+    np.testing.assert_allclose(e3, np.array(f["expected"]["pair3"]), rtol=0, atol=f["atol"])
+    if "energy" in f["expected"]:
+        assert abs(e2.sum() + e3.sum() - f["expected"]["energy"]) <= f["atol"]
+
+
+def test_pairwise_ta90_sum(goldens):
+    """The pair energies of the Ta90 cell (3D, 729 images of the 60-bohr
+    cutoff, self-image pairs on the diagonal) sum to the 100-epsilon energy."""
+    f = goldens["ta90_3d_bj"]
+    e2, e3 = O.pairwise(f["numbers"], f["positions"], f["lattice"], f["periodic"], _kind(f["kind"]), f["param"],
+                        f["cutoff"])
+    assert abs(e2.sum() + e3.sum() - f["expected"]["energy"]) <= f["atol"]
+    assert not e3.any()
+
+
+@pytest.mark.parametrize("kind", ["rational", "zero"])
+def test_pairwise_rows_are_atomic_energies(kind):
+    """Row x of the pair matrices sums to the atomic energy of x of the energy
+    branch (atm.f90:500-621, rational.f90:282-374): each two-body term is
+    split in halves between the pair's atoms and each triple term in thirds
+    between its atoms, so a wrong weight on the diagonal (self images), on
+    repeated indices or on one of the three pairs of a triple fails here.
+    Small periodic cells (self images, two atoms of the same index in a
+    triple) with smooth cutoff widths, and a 2D cell."""
+    from helpers import BJ, ZERO, cell
+    par = BJ if kind == "rational" else ZERO
+    cut = (40.0, 60.0, 40.0, 3.0, 4.0)
+    cases = [cell(41, 4, 9.0) + ([True] * 3,), cell(42, 5, 10.0) + ([True, True, False],)]
+    for num, x, lat, pbc in cases:
+        e2, e3 = O.pairwise(num, x, lat, pbc, _kind(kind), par, cut)
+        r = O.dispersion(num, x, lat, pbc, _kind(kind), par, cut, grad=False)
+        p0 = dict(par, s9=0.0)
+        r2 = O.dispersion(num, x, lat, pbc, _kind(kind), p0, cut, grad=False)
+        scale = np.abs(r["energies"]).max()
+        np.testing.assert_allclose(e2.sum(axis=1), r2["energies"], rtol=0, atol=1e-13 * scale)
+        np.testing.assert_allclose(e3.sum(axis=1), r["energies"] - r2["energies"], rtol=0, atol=1e-13 * scale)
+        np.testing.assert_array_equal(e2, e2.T)
+        np.testing.assert_array_equal(e3, e3.T)
+        assert np.abs(np.diag(e2)).min() > 0 and np.abs(np.diag(e3)).min() > 0  # self images present
+
+
+def test_pairwise_ghost_atoms():
+    """A ghost atom (model.f90:166-174) has no pair energy with anything."""
+    from helpers import BJ
+    from dftd3 import synthetic
+    num, x = synthetic.molecule_c0(seed=5, n=12)
+    ghost = np.zeros(12, dtype=np.int32)
+    ghost[[2, 7]] = 1
+    e2, e3 = O.pairwise(num, x, None, None, O.RATIONAL, BJ, O.DEFAULT_CUTOFF, ghost=ghost)
+    for g in (2, 7):
+        assert not e2[g].any() and not e2[:, g].any() and not e3[g].any() and not e3[:, g].any()
+    r = O.dispersion(num, x, None, None, O.RATIONAL, BJ, O.DEFAULT_CUTOFF, grad=False, ghost=ghost)
+    assert abs(e2.sum() + e3.sum() - r["energy"]) <= 1e-14 * abs(r["energy"])
