@@ -31,6 +31,7 @@ METRIC = "E+grad pair-terms/s & molecules/s: mixed vs FP64 on GPU, |dE|<=1e-8 Eh
 # an FMA as two; div, rcp, sqrt, exp and pow as one each).
 OPS_PER_TRIPLE = 233    # ATM E+grad+sigma (atm.f90:733-806)
 OPS_PER_PAIR_BJ = 63    # two-body BJ E+grad+sigma (rational.f90:462-497)
+OPS_PER_TRIPLE_E = 60   # ATM energy only (atm.f90:500-621), the arithmetic of a pairwise triple term
 KERNEL_GROUPS = ("pack", "order", "cn", "weights", "c6", "disp2", "atm_list", "atm", "cn_chain", "reduce")
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
 # HBM bytes per launch of the mixed ATM kernel on config 1, measured by two
@@ -235,6 +236,81 @@ def bench_autograd(args, systems, config, spec, positions, counts):
     return 0
 
 
+def bench_pairwise(args, systems, config, spec, positions, counts):
+    """Pairwise-resolved energies (get_pairwise_dispersion, disp.f90:403-457)
+    of the workload: the two-body and three-body pair energy matrices, FP64
+    and mixed, timed per kernel with HIP events, against the oracle's
+    pairwise loops on the host cores (sampled outer atoms, scaled)."""
+    import torch
+    from dftd3 import library, torch_api
+    ws = torch_api.workspace(positions.device)
+    res, outs = {}, {}
+    for prec in ("fp64", "mixed"):
+        run = lambda: torch_api.run_pairwise(spec, positions, precision=prec, ws=ws)  # noqa: E731
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        library.set_kernel_timing(ws, True)
+        t = timed_steps(run, args.steps, torch.cuda.synchronize)
+        all_ms = {k: library.get_kernel_time(ws, k) for k in KERNEL_GROUPS}
+        library.set_kernel_timing(ws, False)
+        t_plain = timed_steps(run, args.steps, torch.cuda.synchronize)
+        outs[prec] = run()
+        torch.cuda.synchronize()
+        km = {k: v[0] / max(v[1], 1) for k, v in all_ms.items() if v[1]}
+        atm_s = km.get("atm", 0.0) / 1e3
+        res[prec] = dict(ms_per_step=t_plain * 1e3, ms_per_step_with_kernel_events=t * 1e3,
+                         pair_terms_per_s=counts[1] / t_plain, triple_terms_per_s=counts[2] / t_plain,
+                         atm_pairs_ms=atm_s * 1e3,
+                         atm_pairs_tflops=counts[2] * OPS_PER_TRIPLE_E / atm_s / 1e12 if atm_s > 0 else 0.0,
+                         kernel_ms=km)
+        log(f"{prec}: pairwise {t_plain * 1e3:.2f} ms/step, three-body pair kernel + finalize {atm_s * 1e3:.2f} ms")
+    natoms = config["natoms"]
+    d2 = (outs["mixed"]["pair2"] - outs["fp64"]["pair2"]).abs().max().item()
+    d3 = (outs["mixed"]["pair3"] - outs["fp64"]["pair3"]).abs().max().item()
+    rows = [torch_api.blocks(outs[p], "pair2")[0].sum(dim=1) + torch_api.blocks(outs[p], "pair3")[0].sum(dim=1)
+            for p in ("mixed", "fp64")]
+    drow = (rows[0] - rows[1]).abs().max().item()
+    mixed, fp64 = res["mixed"], res["fp64"]
+    line = dict(
+        metric=METRIC, value=mixed["pair_terms_per_s"], unit="pair-terms/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f32", data="synthetic",
+        config=dict(config, pair_terms=int(counts[1]), triple_terms=int(counts[2]), cn_terms=int(counts[0]),
+                    parallelism="single-gpu", accumulation="f64",
+                    step="pairwise-resolved energies: n x n two-body and three-body pair energy matrices"),
+        molecules_per_s=config["cells"] / (mixed["ms_per_step"] / 1e3),
+        triple_terms_per_s=mixed["triple_terms_per_s"],
+        fp64=dict(value=fp64["pair_terms_per_s"], unit="pair-terms/s", ms_per_step=fp64["ms_per_step"],
+                  triple_terms_per_s=fp64["triple_terms_per_s"], atm_pairs_ms=fp64["atm_pairs_ms"],
+                  atm_pairs_tflops=fp64["atm_pairs_tflops"]),
+        mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"],
+        mixed_vs_fp64_error=dict(max_pair2=d2, max_pair3=d3, max_row_sum_per_atom=drow),
+        kernel_ms=dict(fp64=fp64["kernel_ms"], mixed=mixed["kernel_ms"]),
+        roofline=dict(kernel="atm_pairs_kernel<float, false> + pair3_finalize_kernel", bound="valu",
+                      achieved=mixed["atm_pairs_tflops"], peak=PEAK_TFLOPS["mixed"], unit="TFLOP/s",
+                      frac=mixed["atm_pairs_tflops"] / PEAK_TFLOPS["mixed"], traffic=None,
+                      ops_per_triple=OPS_PER_TRIPLE_E, triples_per_launch=int(counts[2]),
+                      launch_ms=mixed["atm_pairs_ms"]),
+    )
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: oracle pairwise loops on every {args.cpu_stride}-th atom, {args.cpu_threads} threads")
+        os.environ["OMP_NUM_THREADS"] = str(args.cpu_threads)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        s = systems[0]
+        t = oracle.time_sampled_pairwise(args.cpu_stride, s["numbers"], s["positions"], s.get("lattice"),
+                                         s.get("periodic"), oracle.RATIONAL, s["param"])
+        t_full = t * args.cpu_stride
+        line["cpu_baseline"] = dict(
+            value=float(counts[1] / t_full), unit="pair-terms/s", cores=oracle.num_threads(), kind="port",
+            sample=f"oracle pairwise loops with the outer atom loops restricted to every {args.cpu_stride}-th "
+                   f"atom ({t:.2f} s), scaled by {args.cpu_stride} to one full evaluation ({t_full:.1f} s)",
+            seconds_per_eval_estimate=t_full)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +322,8 @@ def main():
                     help="run one device copy of this size first (reference kernel for FETCH/WRITE_SIZE counters)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pairwise", action="store_true",
+                    help="time the pairwise-resolved energies (pair energy matrices) of the workload instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has a single MI355X; "
@@ -273,6 +351,8 @@ def main():
         return bench_batched(args, systems, config, spec, positions, counts)
     if args.config == "c4":
         return bench_autograd(args, systems, config, spec, positions, counts)
+    if args.pairwise:
+        return bench_pairwise(args, systems, config, spec, positions, counts)
     ws = torch_api.workspace(dev)
     outs = {}
     res = {}

@@ -51,7 +51,8 @@ static double g_covrad_aa[118];
 static double g_aatoau;
 
 /* Bounded timing samples: only outer atoms iat with iat % stride == 0 are
- * visited by the CN, two-body, ATM and chain-rule loops (1 = all).  The C6
+ * visited by the CN, two-body, ATM and chain-rule loops and the pairwise
+ * two-body and ATM loops (1 = all).  The C6
  * matrix is always complete: the ATM loop skips triples with a vanishing
  * C6 (atm.f90:725-726), so a partial matrix would skip real work. */
 static int g_sample_stride = 1;
@@ -694,6 +695,7 @@ static void disp2_pairwise(const d3o_param *p, int nat, const int *num, const do
         double *el = (double *)calloc((size_t)nat * nat, sizeof(double));
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
+            if (iat % g_sample_stride) continue; /* bounded timing sample */
             int zi = num[iat];
             for (int jat = 0; jat <= iat; ++jat) {
                 int zj = num[jat];
@@ -759,6 +761,7 @@ static void atm_pairwise(int nat, const int *num, const double *xyz, int ntr, co
         double *el = (double *)calloc((size_t)nat * nat, sizeof(double));
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
+            if (iat % g_sample_stride) continue; /* bounded timing sample */
             int zi = num[iat];
             for (int jat = 0; jat <= iat; ++jat) {
                 int zj = num[jat];

@@ -246,3 +246,16 @@ def time_sampled_dispersion(stride, *args, **kwargs):
         return time.perf_counter() - t0
     finally:
         set_sample_stride(1)
+
+
+def time_sampled_pairwise(stride, *args, **kwargs):
+    """Wall time of one sampled pairwise evaluation (seconds); the results of
+    a sampled run are partial sums and are not returned."""
+    import time
+    set_sample_stride(stride)
+    try:
+        t0 = time.perf_counter()
+        pairwise(*args, **kwargs)
+        return time.perf_counter() - t0
+    finally:
+        set_sample_stride(1)
