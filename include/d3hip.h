@@ -85,6 +85,38 @@ D3HIP_API dftd3_param dftd3_new_rational_damping(dftd3_error error, double s6, d
                                                  double a2, double alp);
 D3HIP_API dftd3_param dftd3_load_rational_damping(dftd3_error error, char* method, bool atm);
 
+/* s-dftd3.h:248-262 modified zero damping (api.f90:748-813, mzero.f90):
+ * zero damping with the offset bet. */
+D3HIP_API dftd3_param dftd3_new_mzero_damping(dftd3_error error, double s6, double s8, double s9, double rs6,
+                                              double rs8, double alp, double bet);
+D3HIP_API dftd3_param dftd3_load_mzero_damping(dftd3_error error, char* method, bool atm);
+
+/* s-dftd3.h:265-278 modified rational damping (api.f90:694-744): the rational
+ * damping function with the parameters of param.f90:1091-1136. */
+D3HIP_API dftd3_param dftd3_new_mrational_damping(dftd3_error error, double s6, double s8, double s9, double a1,
+                                                  double a2, double alp);
+D3HIP_API dftd3_param dftd3_load_mrational_damping(dftd3_error error, char* method, bool atm);
+
+/* s-dftd3.h:281-295 optimized power damping (api.f90:817-882,
+ * optimizedpower.f90). */
+D3HIP_API dftd3_param dftd3_new_optimizedpower_damping(dftd3_error error, double s6, double s8, double s9,
+                                                       double a1, double a2, double alp, double bet);
+D3HIP_API dftd3_param dftd3_load_optimizedpower_damping(dftd3_error error, char* method, bool atm);
+
+/* s-dftd3.h:298-312 CSO damping (api.f90:886-951, cso.f90): C6-only with a
+ * sigmoidal scaling. */
+D3HIP_API dftd3_param dftd3_new_cso_damping(dftd3_error error, double s6, double s9, double a1, double a2,
+                                            double a3, double a4, double alp);
+D3HIP_API dftd3_param dftd3_load_cso_damping(dftd3_error error, char* method, bool atm);
+
+/* s-dftd3.h:315-327 Z damping (api.f90:955-1018, z.f90): r0 = a1 / (id_i +
+ * id_j) with the species identifiers of the structure (elements numbered by
+ * first appearance), damping C6 r0.  The loader has no entries
+ * (param.f90:1297-1326): every method name is an error. */
+D3HIP_API dftd3_param dftd3_new_z_damping(dftd3_error error, double s6, double s8, double s9, double a1,
+                                          double alp);
+D3HIP_API dftd3_param dftd3_load_z_damping(dftd3_error error, char* method, bool atm);
+
 /* s-dftd3.h:330-331 */
 D3HIP_API void dftd3_delete_param(dftd3_param* param);
 
@@ -113,9 +145,12 @@ D3HIP_API void d3hip_set_model_precision(dftd3_error error, dftd3_model model, i
 /* Evaluate a batch of independent systems in one pass.  Host arrays:
  *   natoms[nsys], numbers[sum natoms], lattices[nsys][3][3] (may be NULL
  *   when no system is periodic), periodic[nsys][3] (may be NULL),
- *   params[nsys][7] = {kind (0 rational, 1 zero), s6, s8, s9, a1|rs6,
- *   a2|rs8, alp}, cutoffs[nsys][5] = {disp2, disp3, cn, width2, width3}
- *   (NULL selects the defaults 60, 40, 40, 0, 0).
+ *   params[nsys][10] = {kind, s6, s8, s9, a1|rs6, a2|rs8, alp, bet, a3, a4}
+ *   with kind 0 rational (also the modified rational parameters), 1 zero,
+ *   2 modified zero, 3 optimized power, 4 CSO, 5 Z damping (rs6, rs8 in the
+ *   a1, a2 places for zero and modified zero; unused entries are ignored),
+ *   cutoffs[nsys][5] = {disp2, disp3, cn, width2, width3} (NULL selects the
+ *   defaults 60, 40, 40, 0, 0).
  * Device arrays: positions[sum natoms][3] (not wrapped by this call),
  *   energy[nsys], gradient[sum natoms][3] (may be NULL), sigma[nsys][3][3]
  *   (may be NULL).  stream is a hipStream_t (NULL = default stream).

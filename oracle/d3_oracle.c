@@ -422,26 +422,213 @@ void d3o_c6(int nat, const int *num, int mref, const double *gwvec, const double
 }
 
 /* --------------------------------------------------------- two-body */
+/* Damping families (kind): 0 rational (BJ, also the modified rational
+ * parameters), 1 zero, 2 modified zero, 3 optimized power, 4 CSO, 5 Z. */
+enum { K_RATIONAL = 0, K_ZERO = 1, K_MZERO = 2, K_OP = 3, K_CSO = 4, K_Z = 5 };
 typedef struct {
-    int kind;          /* 0 = rational (BJ), 1 = zero (CHG) */
+    int kind;
     double s6, s8, s9; /* scaling */
-    double a1, a2;     /* rational: a1, a2 ; zero: rs6, rs8 */
+    double a1, a2;     /* rational, optimized power, CSO, Z: a1, a2; zero, modified zero: rs6, rs8 */
     double alp;
+    double bet;        /* modified zero, optimized power */
+    double a3, a4;     /* CSO */
 } d3o_param;
 
 /*
- * rational.f90:220-514 get_dispersion2 (BJ) and zero.f90:243-553 (zero).
- * grad != 0 selects the E+grad+sigma branch (rational.f90:378-514,
- * zero.f90:409-553), otherwise the energy branch (rational.f90:282-374,
- * zero.f90:306-405).
+ * Species identifiers of the structure (mctc-lib structure_type%id): the
+ * elements numbered 1, 2, ... in the order of their first appearance.  Only
+ * the Z damping uses them (z.f90: r0ij = a1 / (izp + jzp)).
+ */
+static void species_ids(int nat, const int *num, int *id)
+{
+    int nid = 0;
+    for (int i = 0; i < nat; ++i) {
+        id[i] = 0;
+        for (int k = 0; k < i; ++k)
+            if (num[k] == num[i]) { id[i] = id[k]; break; }
+        if (!id[i]) id[i] = ++nid;
+    }
+}
+
+/* The per-pair constants of one damping family (everything the image loop
+ * of the reference hoists out of it). */
+typedef struct {
+    double rrij, c6ij;
+    double r0ij, r0ij6, r0ij8;     /* rational, optimized power (with ab), CSO (d6^6 in r0ij6) */
+    double rs6r0, rs8r0;           /* zero: rs6 R0, rs8 R0; modified zero: 1/(rs6 R0), 1/(rs8 R0) */
+    double betr0, betr02rs6, betr02rs8;
+    double ab;                     /* optimized power: r0ij**bet */
+    double a2r0ij;                 /* CSO */
+    double zr0;                    /* Z: a1 / (id_i + id_j) */
+} pair_const;
+
+static pair_const pair_setup(const d3o_param *p, int zi, int zj, int idi, int idj, double c6ij)
+{
+    pair_const c;
+    memset(&c, 0, sizeof c);
+    c.rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
+    c.c6ij = c6ij;
+    switch (p->kind) {
+    case K_RATIONAL: case K_OP: {  /* rational.f90:452-458, optimizedpower.f90 */
+        c.r0ij = p->a1 * sqrt(c.rrij) + p->a2;
+        double r0ij2 = c.r0ij * c.r0ij;
+        c.r0ij6 = r0ij2 * r0ij2 * r0ij2;
+        c.r0ij8 = c.r0ij6 * r0ij2;
+        if (p->kind == K_OP) c.ab = pow(c.r0ij, p->bet);
+        break;
+    }
+    case K_ZERO: {  /* zero.f90:484-488 */
+        double r0 = d3o_rvdw(zj, zi);
+        c.rs6r0 = p->a1 * r0;
+        c.rs8r0 = p->a2 * r0;
+        break;
+    }
+    case K_MZERO: {  /* mzero.f90:441-447 */
+        double r0 = d3o_rvdw(zj, zi);
+        c.r0ij = r0;
+        c.rs6r0 = 1.0 / (p->a1 * r0);
+        c.rs8r0 = 1.0 / (p->a2 * r0);
+        c.betr0 = p->bet * r0;
+        c.betr02rs6 = p->a1 * p->bet * r0 * r0;
+        c.betr02rs8 = p->a2 * p->bet * r0 * r0;
+        break;
+    }
+    case K_CSO: {  /* cso.f90: d6 = a3 sqrt(rrij) + a4 */
+        c.r0ij = sqrt(c.rrij);
+        double d6 = p->a3 * c.r0ij + p->a4;
+        c.r0ij6 = pow(d6, 6);
+        c.a2r0ij = p->a2 * c.r0ij;
+        break;
+    }
+    case K_Z:  /* z.f90: r0ij = a1 / (izp + jzp) */
+        c.zr0 = p->a1 / (double)(idi + idj);
+        break;
+    }
+    return c;
+}
+
+/*
+ * One pair term of the damped two-body energy without the switching
+ * function: returns edisp0 (E = -c6 sw edisp0 / 2 per image); with grad
+ * also gdisp0 (dE/dr / r, dG = -c6 gdisp vec) and, for the Z damping whose
+ * damping depends on C6, dedc6 = d(c6 edisp0)/dc6 (z.f90 dEdc6 / -sw).
+ * rational.f90:466-481, zero.f90:494-512, mzero.f90:456-470,
+ * optimizedpower.f90 (t6/t8/d6/d8), cso.f90 (ef/sf/sig/t6), z.f90.
+ */
+static double pair_term(const d3o_param *p, const pair_const *c, double r2, double r1, int grad,
+                        double *gdisp0, double *dedc6)
+{
+    double e0 = 0.0, g0 = 0.0, r4 = r2 * r2;
+    *dedc6 = 0.0;
+    switch (p->kind) {
+    case K_RATIONAL: {
+        double t6 = 1.0 / (r4 * r2 + c->r0ij6);
+        double t8 = 1.0 / (r4 * r4 + c->r0ij8);
+        e0 = p->s6 * t6 + p->s8 * c->rrij * t8;
+        if (grad) {
+            double d6 = -6 * r4 * t6 * t6;
+            double d8 = -8 * r4 * r2 * t8 * t8;
+            g0 = p->s6 * d6 + p->s8 * c->rrij * d8;
+        }
+        break;
+    }
+    case K_ZERO: {
+        double alp6 = p->alp, alp8 = p->alp + 2.0;
+        double r6 = r2 * r2 * r2, r8 = r6 * r2;
+        double t6 = pow(c->rs6r0 / r1, alp6);
+        double t8 = pow(c->rs8r0 / r1, alp8);
+        double f6 = 1.0 / (1.0 + 6.0 * t6);
+        double f8 = 1.0 / (1.0 + 6.0 * t8);
+        e0 = p->s6 * f6 / r6 + p->s8 * c->rrij * f8 / r8;
+        if (grad) {
+            double d6 = -6.0 * f6 / r2 + 6.0 * alp6 * t6 * f6 * f6 / r2;
+            double d8 = -8.0 * f8 / r2 + 6.0 * alp8 * t8 * f8 * f8 / r2;
+            g0 = p->s6 * d6 / r6 + p->s8 * c->rrij * d8 / r8;
+        }
+        break;
+    }
+    case K_MZERO: {
+        double alp6 = p->alp, alp8 = p->alp + 2.0;
+        double r6 = r2 * r2 * r2, r8 = r6 * r2;
+        double t6 = pow(r1 * c->rs6r0 + c->betr0, -alp6);
+        double t8 = pow(r1 * c->rs8r0 + c->betr0, -alp8);
+        double f6 = 1.0 / (1.0 + 6.0 * t6);
+        double f8 = 1.0 / (1.0 + 6.0 * t8);
+        e0 = p->s6 * f6 / r6 + p->s8 * c->rrij * f8 / r8;
+        if (grad) {
+            double d6 = -6.0 * f6 / r2 + 6.0 * alp6 * t6 * f6 * f6 / (r2 + c->betr02rs6 * r1);
+            double d8 = -8.0 * f8 / r2 + 6.0 * alp8 * t8 * f8 * f8 / (r2 + c->betr02rs8 * r1);
+            g0 = p->s6 * d6 / r6 + p->s8 * c->rrij * d8 / r8;
+        }
+        break;
+    }
+    case K_OP: {
+        double abr0ij6 = c->ab * c->r0ij6, abr0ij8 = c->ab * c->r0ij8;
+        double rb = pow(r2, p->bet * 0.5);
+        double t6 = rb / (rb * r4 * r2 + abr0ij6);
+        double t8 = rb / (rb * r4 * r4 + abr0ij8);
+        e0 = p->s6 * t6 + p->s8 * c->rrij * t8;
+        if (grad) {
+            double d6 = -6 * r4 * t6 * t6 + p->bet * abr0ij6 * t6 * t6 / (rb * r2);
+            double d8 = -8 * r4 * r2 * t8 * t8 + p->bet * abr0ij8 * t8 * t8 / (rb * r2);
+            g0 = p->s6 * d6 + p->s8 * c->rrij * d8;
+        }
+        break;
+    }
+    case K_CSO: {
+        double ef = exp(r1 - c->a2r0ij);
+        double sf = 1.0 / (1.0 + ef);
+        double sig = p->s6 + p->a1 * sf;
+        double t6 = 1.0 / (r4 * r2 + c->r0ij6);
+        e0 = sig * t6;
+        if (grad) {
+            double dsig = -p->a1 * sf * (1.0 - sf) / r1;
+            double dt6 = -6 * r4 * t6 * t6;
+            g0 = dsig * t6 + sig * dt6;
+        }
+        break;
+    }
+    case K_Z: {
+        double t6 = 1.0 / (r4 * r2 + c->zr0 * c->c6ij);
+        double t8 = 1.0 / (r4 * r4 + c->zr0 * c->c6ij * c->rrij);
+        e0 = p->s6 * t6 + p->s8 * c->rrij * t8;
+        if (grad) {
+            double d6 = -6 * r4 * t6 * t6;
+            double d8 = -8 * r4 * r2 * t8 * t8;
+            g0 = p->s6 * d6 + p->s8 * c->rrij * d8;
+            *dedc6 = p->s6 * r4 * r2 * t6 * t6 + p->s8 * c->rrij * r4 * r4 * t8 * t8;
+        }
+        break;
+    }
+    }
+    *gdisp0 = g0;
+    return e0;
+}
+
+/* get_dispersion2 returns early without any term (rational.f90:267 and the
+ * same line of every family; CSO tests s6 and a1). */
+static int two_body_active(const d3o_param *p)
+{
+    if (p->kind == K_CSO) return !(fabs(p->s6) < EPS && fabs(p->a1) < EPS);
+    return !(fabs(p->s6) < EPS && fabs(p->s8) < EPS);
+}
+
+/*
+ * get_dispersion2 of every family: rational.f90:220-514 (BJ), zero.f90:243-553,
+ * mzero.f90:194-508, optimizedpower.f90, cso.f90, z.f90.  grad != 0 selects
+ * the E+grad+sigma branch (get_dispersion_derivs), otherwise the energy
+ * branch (get_dispersion_energy).  dE/dCN is -dC6/dCN edisp, for the Z
+ * damping dC6/dCN * dEdc6 with dEdc6 = -sw d(c6 edisp0)/dc6 (z.f90).
  */
 static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz, int ntr,
                   const double *trans, double cutoff, double width, const double *c6,
                   const double *dc6dcn, double *energy, double *dEdcn, double *gradient,
                   double *sigma, int grad)
 {
-    if (fabs(p->s6) < EPS && fabs(p->s8) < EPS) return;
+    if (!two_body_active(p)) return;
     double cutoff2 = cutoff * cutoff;
+    int *id = (int *)malloc(sizeof(int) * (nat > 0 ? nat : 1));
+    species_ids(nat, num, id);
 #pragma omp parallel
     {
         double *el = (double *)calloc(nat, sizeof(double));
@@ -451,55 +638,18 @@ static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
             if (iat % g_sample_stride) continue; /* bounded timing sample */
-            int zi = num[iat];
             for (int jat = 0; jat <= iat; ++jat) {
-                int zj = num[jat];
-                double rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
-                double c6ij = c6[FIDX(jat, iat)];
-                double r0ij = 0, r0ij6 = 0, r0ij8 = 0, rs6r0 = 0, rs8r0 = 0;
-                if (p->kind == 0) {
-                    r0ij = p->a1 * sqrt(rrij) + p->a2;
-                    double r0ij2 = r0ij * r0ij;
-                    r0ij6 = r0ij2 * r0ij2 * r0ij2;
-                    r0ij8 = r0ij6 * r0ij2;
-                } else {
-                    r0ij = d3o_rvdw(zj, zi);
-                    rs6r0 = p->a1 * r0ij;
-                    rs8r0 = p->a2 * r0ij;
-                }
+                pair_const pc = pair_setup(p, num[iat], num[jat], id[iat], id[jat], c6[FIDX(jat, iat)]);
+                double c6ij = pc.c6ij;
                 for (int jtr = 0; jtr < ntr; ++jtr) {
                     const double *t = trans + 3 * jtr;
                     double v[3];
                     for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + t[c]);
                     double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
                     if (r2 > cutoff2 || r2 < EPS) continue;
-                    double r1 = sqrt(r2), sw, dswdr;
+                    double r1 = sqrt(r2), sw, dswdr, gdisp0, dedc6;
                     smooth_cutoff(r1, cutoff, width, &sw, &dswdr);
-                    double edisp0, gdisp0 = 0.0;
-                    if (p->kind == 0) {
-                        double r4 = r2 * r2;
-                        double t6 = 1.0 / (r4 * r2 + r0ij6);
-                        double t8 = 1.0 / (r4 * r4 + r0ij8);
-                        edisp0 = p->s6 * t6 + p->s8 * rrij * t8;
-                        if (grad) {
-                            double d6 = -6 * r4 * t6 * t6;
-                            double d8 = -8 * r4 * r2 * t8 * t8;
-                            gdisp0 = p->s6 * d6 + p->s8 * rrij * d8;
-                        }
-                    } else {
-                        double alp6 = p->alp, alp8 = p->alp + 2.0;
-                        double r6 = r2 * r2 * r2, r8 = r6 * r2;
-                        double t6 = pow(rs6r0 / r1, alp6);
-                        double t8 = pow(rs8r0 / r1, alp8);
-                        double f6 = 1.0 / (1.0 + 6.0 * t6);
-                        double f8 = 1.0 / (1.0 + 6.0 * t8);
-                        edisp0 = p->s6 * f6 / r6 + p->s8 * rrij * f8 / r8;
-                        if (grad) {
-                            double d6 = -6.0 * f6 / r2 + 6.0 * alp6 * t6 * f6 * f6 / r2;
-                            double d8 = -8.0 * f8 / r2 + 6.0 * alp8 * t8 * f8 * f8 / r2;
-                            gdisp0 = p->s6 * d6 / r6 + p->s8 * rrij * d8 / r8;
-                        }
-                    }
+                    double edisp0 = pair_term(p, &pc, r2, r1, grad, &gdisp0, &dedc6);
                     double edisp = sw * edisp0;
                     double dE = -c6ij * edisp * 0.5;
                     el[iat] += dE;
@@ -510,10 +660,12 @@ static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz
                     for (int c = 0; c < 3; ++c) dG[c] = -c6ij * gdisp * v[c];
                     for (int a = 0; a < 3; ++a)
                         for (int b = 0; b < 3; ++b) dS[b * 3 + a] = dG[a] * v[b] * 0.5;
-                    dl[iat] -= dc6dcn[FIDX(iat, jat)] * edisp;
+                    /* dE/dC6 of the pair (z.f90 dEdc6; -edisp for the other families) */
+                    double dEdc6 = p->kind == K_Z ? -sw * dedc6 : -edisp;
+                    dl[iat] += dc6dcn[FIDX(iat, jat)] * dEdc6;
                     for (int k = 0; k < 9; ++k) sl[k] += dS[k];
                     if (iat != jat) {
-                        dl[jat] -= dc6dcn[FIDX(jat, iat)] * edisp;
+                        dl[jat] += dc6dcn[FIDX(jat, iat)] * dEdc6;
                         for (int c = 0; c < 3; ++c) {
                             gl[3 * iat + c] += dG[c];
                             gl[3 * jat + c] -= dG[c];
@@ -534,6 +686,7 @@ static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz
         }
         free(el); free(dl); free(gl);
     }
+    free(id);
 }
 
 /* atm.f90:954-980 triple_scale (0-based indices). */
@@ -680,60 +833,38 @@ static void atm(int nat, const int *num, const double *xyz, int ntr, const doubl
 
 /* -------------------------------------------------------- pairwise */
 /*
- * rational.f90:623-705 / zero.f90:662-755 get_pairwise_dispersion2: the
- * two-body energy of every pair (iat, jat <= iat) summed over the images,
+ * get_pairwise_dispersion2 of every family (rational.f90:623-705,
+ * zero.f90:662-755, mzero.f90:617-707, optimizedpower.f90, cso.f90, z.f90):
+ * the two-body energy of every pair (iat, jat <= iat) summed over the images,
  * half of each term on (jat, iat) and half on (iat, jat) (once on the
- * diagonal).  energy is nat x nat in Fortran order (FIDX).
+ * diagonal).  energy is nat x nat in Fortran order (FIDX).  Unlike
+ * get_dispersion2 there is no early return for vanishing scalings.
  */
 static void disp2_pairwise(const d3o_param *p, int nat, const int *num, const double *xyz, int ntr,
                            const double *trans, double cutoff, double width, const double *c6,
                            double *energy)
 {
     double cutoff2 = cutoff * cutoff;
+    int *id = (int *)malloc(sizeof(int) * (nat > 0 ? nat : 1));
+    species_ids(nat, num, id);
 #pragma omp parallel
     {
         double *el = (double *)calloc((size_t)nat * nat, sizeof(double));
 #pragma omp for schedule(static)
         for (int iat = 0; iat < nat; ++iat) {
             if (iat % g_sample_stride) continue; /* bounded timing sample */
-            int zi = num[iat];
             for (int jat = 0; jat <= iat; ++jat) {
-                int zj = num[jat];
-                double rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
-                double c6ij = c6[FIDX(jat, iat)];
-                double r0ij6 = 0, r0ij8 = 0, rs6r0 = 0, rs8r0 = 0;
-                if (p->kind == 0) {
-                    double r0ij = p->a1 * sqrt(rrij) + p->a2;
-                    double r0ij2 = r0ij * r0ij;
-                    r0ij6 = r0ij2 * r0ij2 * r0ij2;
-                    r0ij8 = r0ij6 * r0ij2;
-                } else {
-                    double r0ij = d3o_rvdw(zj, zi);
-                    rs6r0 = p->a1 * r0ij;
-                    rs8r0 = p->a2 * r0ij;
-                }
+                pair_const pc = pair_setup(p, num[iat], num[jat], id[iat], id[jat], c6[FIDX(jat, iat)]);
                 for (int jtr = 0; jtr < ntr; ++jtr) {
                     const double *t = trans + 3 * jtr;
                     double v[3];
                     for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + t[c]);
                     double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
                     if (r2 > cutoff2 || r2 < EPS) continue;
-                    double r1 = sqrt(r2), sw, dswdr;
+                    double r1 = sqrt(r2), sw, dswdr, g0, d0;
                     smooth_cutoff(r1, cutoff, width, &sw, &dswdr);
-                    double edisp;
-                    if (p->kind == 0) {
-                        double t6 = 1.0 / (pow(r2, 3) + r0ij6);
-                        double t8 = 1.0 / (pow(r2, 4) + r0ij8);
-                        edisp = sw * (p->s6 * t6 + p->s8 * rrij * t8);
-                    } else {
-                        double r6 = r2 * r2 * r2, r8 = r6 * r2;
-                        double t6 = pow(rs6r0 / r1, p->alp);
-                        double t8 = pow(rs8r0 / r1, p->alp + 2.0);
-                        double f6 = 1.0 / (1.0 + 6.0 * t6);
-                        double f8 = 1.0 / (1.0 + 6.0 * t8);
-                        edisp = sw * (p->s6 * f6 / r6 + p->s8 * rrij * f8 / r8);
-                    }
-                    double dE = -c6ij * edisp * 0.5;
+                    double edisp = sw * pair_term(p, &pc, r2, r1, 0, &g0, &d0);
+                    double dE = -pc.c6ij * edisp * 0.5;
                     el[FIDX(jat, iat)] += dE;
                     if (iat != jat) el[FIDX(iat, jat)] += dE;
                 }
@@ -743,6 +874,7 @@ static void disp2_pairwise(const d3o_param *p, int nat, const int *num, const do
         for (size_t k = 0; k < (size_t)nat * nat; ++k) energy[k] += el[k];
         free(el);
     }
+    free(id);
 }
 
 /*
@@ -830,7 +962,8 @@ static int lattice_alloc(const int *periodic, const double *lat, double rthr, do
 
 /*
  * disp.f90:56-185 get_dispersion_atomic_v2 followed by disp.f90:352-399
- * (energy = sum of atomic energies).  par = {s6, s8, s9, a1|rs6, a2|rs8, alp},
+ * (energy = sum of atomic energies).  par = {s6, s8, s9, a1|rs6, a2|rs8, alp,
+ * bet, a3, a4} (see d3o_param),
  * cut = {cn, disp2, disp3, width2, width3}.  gradient / sigma may be NULL
  * (grad = present(gradient) .or. present(sigma), disp.f90:102).
  * Optional intermediates (may be NULL): cn_out[nat], dEdcn_out[nat],
@@ -841,7 +974,7 @@ int d3o_get_dispersion(int nat, const int *num, const double *xyz, const double 
                        const int *ghost, double *energy, double *gradient, double *sigma,
                        double *cn_out, double *dEdcn_out, double *energies_out)
 {
-    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5]};
+    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5], par[6], par[7], par[8]};
     int grad = (gradient != NULL) || (sigma != NULL);
     int mref = 0;
     for (int i = 0; i < nat; ++i)
@@ -902,7 +1035,7 @@ int d3o_get_pairwise_dispersion(int nat, const int *num, const double *xyz, cons
                                 const int *periodic, int kind, const double *par, const double *cut,
                                 const int *ghost, double *pair2, double *pair3)
 {
-    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5]};
+    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5], par[6], par[7], par[8]};
     int mref = 0;
     for (int i = 0; i < nat; ++i)
         if (g_nref[num[i] - 1] > mref) mref = g_nref[num[i] - 1];

@@ -148,7 +148,25 @@ def atomic_c6(numbers, gw, gwd=None, ghost=None):
     return c6.T.copy(), (None if dc6 is None else dc6.T.copy())
 
 
-RATIONAL, ZERO = 0, 1
+RATIONAL, ZERO, MZERO, OPTIMIZEDPOWER, CSO, ZDAMP = 0, 1, 2, 3, 4, 5
+
+
+def param_vector(kind, param):
+    """{s6, s8, s9, a1|rs6, a2|rs8, alp, bet, a3, a4} of the C oracle from a
+    parameter dict with the names of the reference's constructors."""
+    p = dict(param)
+    s6, s9, alp = p.get("s6", 1.0), p.get("s9", 1.0), p.get("alp", 14.0)
+    if kind in (RATIONAL, OPTIMIZEDPOWER):
+        v = [s6, p["s8"], s9, p["a1"], p["a2"], alp, p.get("bet", 0.0), 0.0, 0.0]
+    elif kind in (ZERO, MZERO):
+        v = [s6, p["s8"], s9, p["rs6"], p.get("rs8", 1.0), alp, p.get("bet", 0.0), 0.0, 0.0]
+    elif kind == CSO:
+        v = [s6, 0.0, s9, p["a1"], p.get("a2", 2.5), alp, 0.0, p.get("a3", 0.0), p.get("a4", 6.25)]
+    elif kind == ZDAMP:
+        v = [s6, p.get("s8", 1.0), s9, p["a1"], 0.0, alp, 0.0, 0.0, 0.0]
+    else:
+        raise ValueError(f"unknown damping kind {kind}")
+    return np.array(v, dtype=np.float64)
 DEFAULT_CUTOFF = (40.0, 60.0, 40.0, 0.0, 0.0)  # cn, disp2, disp3, width2, width3
 
 
@@ -156,7 +174,8 @@ def dispersion(numbers, positions, lattice=None, periodic=None, kind=RATIONAL,
                param=None, cutoff=DEFAULT_CUTOFF, grad=True, ghost=None, wrap_cell=False):
     """FP64 oracle of get_dispersion.
 
-    param: dict with s6, s8, s9 and a1, a2 (rational) or rs6, rs8 (zero), alp.
+    param: dict of damping parameters named as the reference's constructors
+    (s6, s8, s9, alp and a1, a2 / rs6, rs8 / bet / a3, a4 by family).
     Returns a dict with energy, gradient (n,3), sigma (3,3 Fortran order as
     returned by the C API, i.e. sigma[b][a] = sigma(a,b)), virial (= the
     Python interface's array), cn, dEdcn and energies (atomic).
@@ -165,13 +184,7 @@ def dispersion(numbers, positions, lattice=None, periodic=None, kind=RATIONAL,
     num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
     if wrap_cell:
         xyz = wrap(xyz, lat, pbc)
-    p = dict(param)
-    s6 = p.get("s6", 1.0)
-    if kind == RATIONAL:
-        par = [s6, p["s8"], p.get("s9", 1.0), p["a1"], p["a2"], p.get("alp", 14.0)]
-    else:
-        par = [s6, p["s8"], p.get("s9", 1.0), p["rs6"], p.get("rs8", 1.0), p.get("alp", 14.0)]
-    par = np.array(par, dtype=np.float64)
+    par = param_vector(kind, param)
     cut = np.array(cutoff, dtype=np.float64)
     n = len(num)
     gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
@@ -197,13 +210,7 @@ def pairwise(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, par
     (n, n), [i, j] = energy2(i, j)."""
     L = lib()
     num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
-    p = dict(param)
-    s6 = p.get("s6", 1.0)
-    if kind == RATIONAL:
-        par = [s6, p["s8"], p.get("s9", 1.0), p["a1"], p["a2"], p.get("alp", 14.0)]
-    else:
-        par = [s6, p["s8"], p.get("s9", 1.0), p["rs6"], p.get("rs8", 1.0), p.get("alp", 14.0)]
-    par = np.array(par, dtype=np.float64)
+    par = param_vector(kind, param)
     cut = np.array(cutoff, dtype=np.float64)
     n = len(num)
     gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)

@@ -60,7 +60,9 @@ struct _dftd3_model {
 
 struct _dftd3_param {
   int kind = d3::kRational;
-  double s6 = 1.0, s8 = 1.0, s9 = 0.0, a1 = 0.4, a2 = 5.0, alp = 14.0;  // param.f90:38-48 defaults
+  // param.f90:38-48 defaults; a1/a2 hold rs6/rs8 for the (modified) zero
+  // damping, a3/a4 are the CSO parameters (rs6/rs8 of d3_param)
+  double s6 = 1.0, s8 = 1.0, s9 = 0.0, a1 = 0.4, a2 = 5.0, alp = 14.0, bet = 0.0, a3 = 1.0, a4 = 1.0;
 };
 
 namespace {
@@ -183,6 +185,7 @@ d3::SysParam make_sys_param(const _dftd3_param* p, const double* cutoff) {
   d3::SysParam sp{};
   sp.kind = p->kind;
   sp.s6 = p->s6; sp.s8 = p->s8; sp.s9 = p->s9; sp.a1 = p->a1; sp.a2 = p->a2; sp.alp = p->alp;
+  sp.bet = p->bet; sp.a3 = p->a3; sp.a4 = p->a4;
   sp.cut_cn = cutoff[0]; sp.cut2 = cutoff[1]; sp.cut3 = cutoff[2];
   sp.width2 = cutoff[3]; sp.width3 = cutoff[4];
   return sp;
@@ -344,38 +347,123 @@ dftd3_param dftd3_new_zero_damping(dftd3_error error, double s6, double s8, doub
   return p;
 }
 
+// api.f90:748-777 (rs6, rs8 in a1, a2)
+dftd3_param dftd3_new_mzero_damping(dftd3_error error, double s6, double s8, double s9, double rs6, double rs8,
+                                    double alp, double bet) {
+  if (!error) return nullptr;
+  auto* p = new _dftd3_param();
+  p->kind = d3::kModifiedZero;
+  p->s6 = s6; p->s8 = s8; p->s9 = s9; p->a1 = rs6; p->a2 = rs8; p->alp = alp; p->bet = bet;
+  return p;
+}
+
+// api.f90:694-708: the rational damping with other parameters
+dftd3_param dftd3_new_mrational_damping(dftd3_error error, double s6, double s8, double s9, double a1, double a2,
+                                        double alp) {
+  return dftd3_new_rational_damping(error, s6, s8, s9, a1, a2, alp);
+}
+
+// api.f90:817-846
+dftd3_param dftd3_new_optimizedpower_damping(dftd3_error error, double s6, double s8, double s9, double a1,
+                                             double a2, double alp, double bet) {
+  if (!error) return nullptr;
+  auto* p = new _dftd3_param();
+  p->kind = d3::kOptimizedPower;
+  p->s6 = s6; p->s8 = s8; p->s9 = s9; p->a1 = a1; p->a2 = a2; p->alp = alp; p->bet = bet;
+  return p;
+}
+
+// api.f90:886-915 (cso.f90:74-90: a3, a4 are d3_param's rs6, rs8; no s8)
+dftd3_param dftd3_new_cso_damping(dftd3_error error, double s6, double s9, double a1, double a2, double a3,
+                                  double a4, double alp) {
+  if (!error) return nullptr;
+  auto* p = new _dftd3_param();
+  p->kind = d3::kCSO;
+  p->s6 = s6; p->s9 = s9; p->a1 = a1; p->a2 = a2; p->a3 = a3; p->a4 = a4; p->alp = alp;
+  return p;
+}
+
+// api.f90:955-982
+dftd3_param dftd3_new_z_damping(dftd3_error error, double s6, double s8, double s9, double a1, double alp) {
+  if (!error) return nullptr;
+  auto* p = new _dftd3_param();
+  p->kind = d3::kZDamping;
+  p->s6 = s6; p->s8 = s8; p->s9 = s9; p->a1 = a1; p->alp = alp;
+  return p;
+}
+
 namespace {
-dftd3_param load_damping(dftd3_error error, const char* method, bool atm, int kind) {
+// Parameter tables of param.f90, one per loader.
+enum class Family { kRational, kZero, kMRational, kMZero, kOptimizedPower, kCSO, kZ };
+
+dftd3_param load_damping(dftd3_error error, const char* method, bool atm, Family family) {
   if (!error) return nullptr;
   if (!method) {
     fatal(error, "No entry for '' present");
     return nullptr;
   }
   const char* id = method_id(method);
-  const ParamRecord* rec = kind == d3::kRational ? find_record(kRationalParams, kNumRational, id)
-                                                 : find_record(kZeroParams, kNumZero, id);
+  const ParamRecord* rec = nullptr;
+  switch (family) {
+    case Family::kRational: rec = find_record(kRationalParams, kNumRational, id); break;
+    case Family::kZero: rec = find_record(kZeroParams, kNumZero, id); break;
+    case Family::kMRational: rec = find_record(kMRationalParams, kNumMRational, id); break;
+    case Family::kMZero: rec = find_record(kMZeroParams, kNumMZero, id); break;
+    case Family::kOptimizedPower: rec = find_record(kOptimizedPowerParams, kNumOptimizedPower, id); break;
+    case Family::kCSO: rec = find_record(kCSOParams, kNumCSO, id); break;
+    case Family::kZ: rec = find_record(kZParams, kNumZ, id); break;
+  }
   if (!rec) {
     fatal(error, std::string("No entry for '") + method + "' present");
     return nullptr;
   }
   auto* p = new _dftd3_param();
-  p->kind = kind;
   p->s6 = rec->s6;
   p->s8 = rec->s8;
   p->s9 = atm ? 1.0 : rec->s9;  // api.f90:606 (atm -> s9 = 1), param.f90:798-800
-  p->a1 = kind == d3::kRational ? rec->a1 : rec->rs6;
-  p->a2 = kind == d3::kRational ? rec->a2 : rec->rs8;
   p->alp = rec->alp;
+  p->bet = rec->bet;
+  p->a1 = rec->a1;
+  p->a2 = rec->a2;
+  switch (family) {
+    case Family::kRational:
+    case Family::kMRational: p->kind = d3::kRational; break;
+    case Family::kZero: p->kind = d3::kZero; p->a1 = rec->rs6; p->a2 = rec->rs8; break;
+    case Family::kMZero: p->kind = d3::kModifiedZero; p->a1 = rec->rs6; p->a2 = rec->rs8; break;
+    case Family::kOptimizedPower: p->kind = d3::kOptimizedPower; break;
+    case Family::kCSO: p->kind = d3::kCSO; p->a3 = rec->rs6; p->a4 = rec->rs8; break;
+    case Family::kZ: p->kind = d3::kZDamping; break;
+  }
   return p;
 }
 }  // namespace
 
 dftd3_param dftd3_load_rational_damping(dftd3_error error, char* method, bool atm) {
-  return load_damping(error, method, atm, d3::kRational);
+  return load_damping(error, method, atm, Family::kRational);
 }
 
 dftd3_param dftd3_load_zero_damping(dftd3_error error, char* method, bool atm) {
-  return load_damping(error, method, atm, d3::kZero);
+  return load_damping(error, method, atm, Family::kZero);
+}
+
+dftd3_param dftd3_load_mzero_damping(dftd3_error error, char* method, bool atm) {
+  return load_damping(error, method, atm, Family::kMZero);
+}
+
+dftd3_param dftd3_load_mrational_damping(dftd3_error error, char* method, bool atm) {
+  return load_damping(error, method, atm, Family::kMRational);
+}
+
+dftd3_param dftd3_load_optimizedpower_damping(dftd3_error error, char* method, bool atm) {
+  return load_damping(error, method, atm, Family::kOptimizedPower);
+}
+
+dftd3_param dftd3_load_cso_damping(dftd3_error error, char* method, bool atm) {
+  return load_damping(error, method, atm, Family::kCSO);
+}
+
+dftd3_param dftd3_load_z_damping(dftd3_error error, char* method, bool atm) {
+  return load_damping(error, method, atm, Family::kZ);
 }
 
 void dftd3_delete_param(dftd3_param* param) {
@@ -469,6 +557,8 @@ void dftd3_get_pairwise_dispersion(dftd3_error error, dftd3_structure mol, dftd3
 }
 
 namespace {
+constexpr int kParamStride = 10;  // {kind, s6, s8, s9, a1|rs6, a2|rs8, alp, bet, a3, a4}
+
 std::vector<d3::SystemDesc> batch_systems(int nsys, const int* natoms, const double* lattices, const bool* periodic,
                                           const double* params, const double* cutoffs) {
   std::vector<d3::SystemDesc> sys(nsys);
@@ -479,11 +569,13 @@ std::vector<d3::SystemDesc> batch_systems(int nsys, const int* natoms, const dou
     if (lattices) std::memcpy(s.lattice, lattices + 9 * k, sizeof s.lattice);
     if (periodic)
       for (int c = 0; c < 3; ++c) s.periodic[c] = periodic[3 * k + c] ? 1 : 0;
-    const double* p = params + 7 * k;
+    const double* p = params + kParamStride * k;
     _dftd3_param prm;
     prm.kind = static_cast<int>(p[0]);
-    if (prm.kind != d3::kRational && prm.kind != d3::kZero) throw d3::Error("Unknown damping kind");
+    if (prm.kind < d3::kRational || prm.kind > d3::kZDamping || p[0] != static_cast<double>(prm.kind))
+      throw d3::Error("Unknown damping kind");
     prm.s6 = p[1]; prm.s8 = p[2]; prm.s9 = p[3]; prm.a1 = p[4]; prm.a2 = p[5]; prm.alp = p[6];
+    prm.bet = p[7]; prm.a3 = p[8]; prm.a4 = p[9];
     double cut[5] = {40.0, 60.0, 40.0, 0.0, 0.0};
     if (cutoffs) {
       const double* c = cutoffs + 5 * k;

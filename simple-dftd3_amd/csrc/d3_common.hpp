@@ -12,15 +12,23 @@ constexpr int kMaxRef = 7;      // reference systems per element (reference.f90:
 constexpr int kNumPairs = kMaxElem * (kMaxElem + 1) / 2;
 constexpr int kMaxSpecies = 24; // distinct elements in one batch (LDS tables)
 
-enum DampingKind : int { kRational = 0, kZero = 1 };
+// Damping families of the two-body term (the three-body term is the same
+// for all): rational (Becke-Johnson; the modified rational parameters use it
+// too), zero, modified zero, optimized power, CSO and Z damping.
+enum DampingKind : int {
+  kRational = 0, kZero = 1, kModifiedZero = 2, kOptimizedPower = 3, kCSO = 4, kZDamping = 5
+};
 enum Precision : int { kFP64 = 0, kMixed = 1 };
 
-// Damping parameters and cutoffs of one system (rational.f90:34-70,
-// zero.f90:34-70, cutoff.f90:42-66).  For zero damping a1/a2 hold rs6/rs8.
+// Damping parameters and cutoffs of one system (param.f90:38-48 d3_param,
+// the *_damping_param types of damping/*.f90, cutoff.f90:42-66).  For the
+// zero and modified zero damping a1/a2 hold rs6/rs8; bet is the modified
+// zero / optimized power exponent, a3/a4 the CSO parameters (rs6/rs8 of
+// d3_param).
 struct SysParam {
   int kind;
   int grad;
-  double s6, s8, s9, a1, a2, alp;
+  double s6, s8, s9, a1, a2, alp, bet, a3, a4;
   double cut_cn, cut2, cut3, width2, width3;
 };
 

@@ -283,8 +283,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   const auto& tab = element_tables();
   std::map<int, int> zmap;
   std::vector<int> zlist;
-  std::vector<int> sp(natoms), sysidx(natoms);
-  for (int k = 0, a = 0; k < nsys; ++k)
+  std::vector<int> sp(natoms), sysidx(natoms), sid(natoms);
+  for (int k = 0, a = 0; k < nsys; ++k) {
+    std::map<int, int> local;  // element -> species identifier within system k
     for (int q = 0; q < systems[k].natoms; ++q, ++a) {
       const int z = numbers[a];
       if (z < 1 || z > kMaxElem) throw Error("Unsupported element with atomic number " + std::to_string(z));
@@ -295,7 +296,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
       }
       sp[a] = it->second;
       sysidx[a] = k;
+      sid[a] = local.emplace(z, static_cast<int>(local.size()) + 1).first->second;
     }
+  }
   const int nsp = static_cast<int>(zlist.size());
   if (nsp > kMaxSpecies) throw Error("Too many distinct elements in one batch");
   int mref = 0;
@@ -356,7 +359,8 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   sb.clear();
   std::vector<int> ghost;
   if (ghost_host) ghost.assign(ghost_host, ghost_host + natoms);
-  const size_t o_ghost = stage(sb, ghost), o_sp = stage(sb, sp), o_sys = stage(sb, sysidx), o_geom = stage(sb, geom),
+  const size_t o_ghost = stage(sb, ghost), o_sp = stage(sb, sp), o_sys = stage(sb, sysidx), o_sid = stage(sb, sid),
+               o_geom = stage(sb, geom),
                o_par = stage(sb, par), o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2),
                o_nref = stage(sb, nref), o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref),
                o_rvdw = stage(sb, rvdw);
@@ -374,6 +378,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   db.pos = pos.as<double4>(natoms);
   db.sp = reinterpret_cast<const int*>(dmeta + o_sp);
   db.sys = reinterpret_cast<const int*>(dmeta + o_sys);
+  db.sid = reinterpret_cast<const int*>(dmeta + o_sid);
   db.geom = reinterpret_cast<const SysGeom*>(dmeta + o_geom);
   db.par = reinterpret_cast<const SysParam*>(dmeta + o_par);
   db.img = reinterpret_cast<const double4*>(dmeta + o_img);

@@ -333,9 +333,189 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
   }
 }
 
+// ------------------------------------------------- two-body damping families
+// Per-pair constants of the damping (what the reference hoists out of the
+// image loop) and one pair term without the switching function:
+// edisp0 (E = -C6 sw edisp0 / 2 per image), gdisp0 (dE/dr / r) and, for the
+// Z damping whose damping depends on C6, dedc6 = d(C6 edisp0)/dC6.
+//   rational   rational.f90:452-481   k0 = r0^6, k1 = r0^8, r0 = a1 sqrt(rrij) + a2
+//   zero       zero.f90:484-512       k0 = rs6 R0, k1 = rs8 R0
+//   mzero      mzero.f90:441-470      k0 = 1/(rs6 R0), k1 = 1/(rs8 R0), k2 = bet R0,
+//                                     k3 = rs6 bet R0^2, k4 = rs8 bet R0^2
+//   opt. power optimizedpower.f90     k0 = r0^bet r0^6, k1 = r0^bet r0^8
+//   CSO        cso.f90                k0 = (a3 sqrt(rrij) + a4)^6, k1 = a2 sqrt(rrij)
+//   Z          z.f90                  k0 = a1/(id_i + id_j) C6, k1 = k0 rrij
+template <typename R>
+struct DampPair {
+  R rrij, k0, k1, k2, k3, k4;
+};
+
+template <typename R>
+__device__ __forceinline__ DampPair<R> damp_setup(const SysParam& p, R rrij, double rvdw, R c6ij, int idsum) {
+  DampPair<R> d{rrij, R(0), R(0), R(0), R(0), R(0)};
+  switch (p.kind) {
+    case kRational:
+    case kOptimizedPower: {
+      const R r0 = R(p.a1) * d_sqrt(rrij) + R(p.a2);
+      const R r02 = r0 * r0;
+      d.k0 = r02 * r02 * r02;
+      d.k1 = d.k0 * r02;
+      if (p.kind == kOptimizedPower) {
+        const R ab = d_pow(r0, R(p.bet));
+        d.k0 *= ab;
+        d.k1 *= ab;
+      }
+      break;
+    }
+    case kZero: {
+      const R r0 = R(rvdw);
+      d.k0 = R(p.a1) * r0;
+      d.k1 = R(p.a2) * r0;
+      break;
+    }
+    case kModifiedZero: {
+      const R r0 = R(rvdw);
+      d.k0 = d_rcp(R(p.a1) * r0);
+      d.k1 = d_rcp(R(p.a2) * r0);
+      d.k2 = R(p.bet) * r0;
+      d.k3 = R(p.a1) * R(p.bet) * r0 * r0;
+      d.k4 = R(p.a2) * R(p.bet) * r0 * r0;
+      break;
+    }
+    case kCSO: {
+      const R r0 = d_sqrt(rrij);
+      const R d6 = R(p.a3) * r0 + R(p.a4);
+      const R d3 = d6 * d6 * d6;
+      d.k0 = d3 * d3;
+      d.k1 = R(p.a2) * r0;
+      break;
+    }
+    case kZDamping: {
+      d.k0 = R(p.a1) / R(idsum) * c6ij;
+      d.k1 = d.k0 * rrij;
+      break;
+    }
+    default:
+      break;
+  }
+  return d;
+}
+
+template <typename R, bool kGrad>
+__device__ __forceinline__ R damp_term(const SysParam& p, const DampPair<R>& d, R r2, R r1, R& gdisp0, R& dedc6) {
+  const R s6 = R(p.s6), s8 = R(p.s8);
+  R e0 = R(0);
+  gdisp0 = R(0);
+  dedc6 = R(0);
+  switch (p.kind) {
+    case kRational: {
+      const R r4 = r2 * r2;
+      const R t6 = d_rcp(r4 * r2 + d.k0);
+      const R t8 = d_rcp(r4 * r4 + d.k1);
+      e0 = s6 * t6 + s8 * d.rrij * t8;
+      if (kGrad) {
+        const R d6 = R(-6) * r4 * t6 * t6;
+        const R d8 = R(-8) * r4 * r2 * t8 * t8;
+        gdisp0 = s6 * d6 + s8 * d.rrij * d8;
+      }
+      break;
+    }
+    case kZero: {
+      const R alp6 = R(p.alp), alp8 = R(p.alp) + R(2);
+      const R r6 = r2 * r2 * r2, r8 = r6 * r2;
+      const R ir1 = d_rcp(r1);
+      const R t6 = d_pow(d.k0 * ir1, alp6);
+      const R t8 = d_pow(d.k1 * ir1, alp8);
+      const R f6 = d_rcp(R(1) + R(6) * t6);
+      const R f8 = d_rcp(R(1) + R(6) * t8);
+      const R ir6 = d_rcp(r6), ir8 = d_rcp(r8);
+      e0 = s6 * f6 * ir6 + s8 * d.rrij * f8 * ir8;
+      if (kGrad) {
+        const R ir2 = d_rcp(r2);
+        const R d6 = R(-6) * f6 * ir2 + R(6) * alp6 * t6 * f6 * f6 * ir2;
+        const R d8 = R(-8) * f8 * ir2 + R(6) * alp8 * t8 * f8 * f8 * ir2;
+        gdisp0 = s6 * d6 * ir6 + s8 * d.rrij * d8 * ir8;
+      }
+      break;
+    }
+    case kModifiedZero: {
+      const R alp6 = R(p.alp), alp8 = R(p.alp) + R(2);
+      const R r6 = r2 * r2 * r2, r8 = r6 * r2;
+      const R t6 = d_pow(r1 * d.k0 + d.k2, -alp6);
+      const R t8 = d_pow(r1 * d.k1 + d.k2, -alp8);
+      const R f6 = d_rcp(R(1) + R(6) * t6);
+      const R f8 = d_rcp(R(1) + R(6) * t8);
+      const R ir6 = d_rcp(r6), ir8 = d_rcp(r8);
+      e0 = s6 * f6 * ir6 + s8 * d.rrij * f8 * ir8;
+      if (kGrad) {
+        const R ir2 = d_rcp(r2);
+        const R d6 = R(-6) * f6 * ir2 + R(6) * alp6 * t6 * f6 * f6 * d_rcp(r2 + d.k3 * r1);
+        const R d8 = R(-8) * f8 * ir2 + R(6) * alp8 * t8 * f8 * f8 * d_rcp(r2 + d.k4 * r1);
+        gdisp0 = s6 * d6 * ir6 + s8 * d.rrij * d8 * ir8;
+      }
+      break;
+    }
+    case kOptimizedPower: {
+      const R bet = R(p.bet);
+      const R r4 = r2 * r2;
+      const R rb = d_pow(r2, bet * R(0.5));
+      const R t6 = rb * d_rcp(rb * r4 * r2 + d.k0);
+      const R t8 = rb * d_rcp(rb * r4 * r4 + d.k1);
+      e0 = s6 * t6 + s8 * d.rrij * t8;
+      if (kGrad) {
+        const R irb2 = d_rcp(rb * r2);
+        const R d6 = R(-6) * r4 * t6 * t6 + bet * d.k0 * t6 * t6 * irb2;
+        const R d8 = R(-8) * r4 * r2 * t8 * t8 + bet * d.k1 * t8 * t8 * irb2;
+        gdisp0 = s6 * d6 + s8 * d.rrij * d8;
+      }
+      break;
+    }
+    case kCSO: {
+      const R a1 = R(p.a1);
+      const R sf = d_rcp(R(1) + d_exp(r1 - d.k1));
+      const R sig = s6 + a1 * sf;
+      const R r4 = r2 * r2;
+      const R t6 = d_rcp(r4 * r2 + d.k0);
+      e0 = sig * t6;
+      if (kGrad) {
+        const R dsig = -a1 * sf * (R(1) - sf) * d_rcp(r1);
+        const R dt6 = R(-6) * r4 * t6 * t6;
+        gdisp0 = dsig * t6 + sig * dt6;
+      }
+      break;
+    }
+    case kZDamping: {
+      const R r4 = r2 * r2;
+      const R t6 = d_rcp(r4 * r2 + d.k0);
+      const R t8 = d_rcp(r4 * r4 + d.k1);
+      e0 = s6 * t6 + s8 * d.rrij * t8;
+      if (kGrad) {
+        const R d6 = R(-6) * r4 * t6 * t6;
+        const R d8 = R(-8) * r4 * r2 * t8 * t8;
+        gdisp0 = s6 * d6 + s8 * d.rrij * d8;
+        dedc6 = s6 * r4 * r2 * t6 * t6 + s8 * d.rrij * r4 * r4 * t8 * t8;
+      }
+      break;
+    }
+    default:
+      break;
+  }
+  return e0;
+}
+
+// get_dispersion2 returns without any term when both scalings vanish
+// (rational.f90:267 and the same line of every family; CSO tests s6 and a1).
+__device__ __forceinline__ bool two_body_active(const SysParam& p) {
+  if (p.kind == kCSO) return !(fabs(p.s6) < kEps && fabs(p.a1) < kEps);
+  return !(fabs(p.s6) < kEps && fabs(p.s8) < kEps);
+}
+
 // --------------------------------------------------------------- two-body
-// rational.f90:378-514 (BJ) and zero.f90:409-553 (zero damping), E+grad+sigma;
-// rational.f90:282-374 / zero.f90:306-405 when kGrad is false.  One wavefront
+// get_dispersion_derivs of every damping family (rational.f90:378-514,
+// zero.f90:409-553, mzero.f90:361-508, optimizedpower.f90, cso.f90, z.f90),
+// E+grad+sigma; get_dispersion_energy (rational.f90:282-374, ...) when kGrad
+// is false.  dE/dCN_i gets -dC6/dCN_i edisp, for the Z damping
+// dC6/dCN_i dEdc6 with dEdc6 = -sw d(C6 edisp0)/dC6 (z.f90).  One wavefront
 // per atom i; atom i collects the ordered pairs (i, j, T) it takes part in.
 // kPairs (energy only) instead evaluates the pairs j <= i of atom i once and
 // writes their image sums, the pairwise two-body energy of
@@ -361,8 +541,9 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
   const R r4r2i = R(b.r4r2[spi]);
   double acc_e = 0.0, acc_d = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
   double sxx = 0.0, syy = 0.0, szz = 0.0, sxy = 0.0, sxz = 0.0, syz = 0.0;
-  const bool active = !(fabs(p.s6) < kEps && fabs(p.s8) < kEps);  // rational.f90:267
-  const R s6 = R(p.s6), s8 = R(p.s8), cutr = R(p.cut2), width = R(p.width2);
+  const bool active = two_body_active(p);
+  const R cutr = R(p.cut2), width = R(p.width2);
+  const int idi = b.sid[i];
   const R4<R>* row = pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n;
   if (active || kPairs) {  // the pairwise energies are written (as zeros) for s6 = s8 = 0 too
     for (int jr = lane; jr < n; jr += kWave) {
@@ -375,17 +556,7 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
       const R4<R> q = row[jr];
       const R c6ij = q.x * q.x;
       const R dc6i = q.y * c6ij;  // dC6_ij/dCN_i
-      R r0a = R(0), r0b = R(0);   // BJ: r0^6, r0^8; zero: rs6*R0, rs8*R0
-      if (p.kind == kRational) {
-        const R r0 = R(p.a1) * d_sqrt(rrij) + R(p.a2);
-        const R r02 = r0 * r0;
-        r0a = r02 * r02 * r02;
-        r0b = r0a * r02;
-      } else {
-        const R r0 = R(b.rvdw[spj * b.nsp + spi]);
-        r0a = R(p.a1) * r0;
-        r0b = R(p.a2) * r0;
-      }
+      const DampPair<R> dp = damp_setup<R>(p, rrij, b.rvdw[spj * b.nsp + spi], c6ij, idi + b.sid[j]);
       const bool swap = j > i;
       for (int t = 0; t < g.img2_count; ++t) {
         double u0, u1, u2;
@@ -395,34 +566,8 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
         const R r1 = d_sqrt(r2);
         R sw, dswdr;
         smooth_cutoff(r1, cutr, width, sw, dswdr);
-        R edisp0, gdisp0 = R(0);
-        if (p.kind == kRational) {
-          const R r4 = r2 * r2;
-          const R t6 = d_rcp(r4 * r2 + r0a);
-          const R t8 = d_rcp(r4 * r4 + r0b);
-          edisp0 = s6 * t6 + s8 * rrij * t8;
-          if (kGrad) {
-            const R d6 = R(-6) * r4 * t6 * t6;
-            const R d8 = R(-8) * r4 * r2 * t8 * t8;
-            gdisp0 = s6 * d6 + s8 * rrij * d8;
-          }
-        } else {
-          const R alp6 = R(p.alp), alp8 = R(p.alp) + R(2);
-          const R r6 = r2 * r2 * r2, r8 = r6 * r2;
-          const R ir1 = d_rcp(r1);
-          const R t6 = d_pow(r0a * ir1, alp6);
-          const R t8 = d_pow(r0b * ir1, alp8);
-          const R f6 = d_rcp(R(1) + R(6) * t6);
-          const R f8 = d_rcp(R(1) + R(6) * t8);
-          const R ir6 = d_rcp(r6), ir8 = d_rcp(r8);
-          edisp0 = s6 * f6 * ir6 + s8 * rrij * f8 * ir8;
-          if (kGrad) {
-            const R ir2 = d_rcp(r2);
-            const R d6 = R(-6) * f6 * ir2 + R(6) * alp6 * t6 * f6 * f6 * ir2;
-            const R d8 = R(-8) * f8 * ir2 + R(6) * alp8 * t8 * f8 * f8 * ir2;
-            gdisp0 = s6 * d6 * ir6 + s8 * rrij * d8 * ir8;
-          }
-        }
+        R gdisp0, dedc6;
+        const R edisp0 = damp_term<R, kGrad>(p, dp, r2, r1, gdisp0, dedc6);
         const R edisp = sw * edisp0;
         const double de = double(-c6ij * edisp * R(0.5));
         acc_e += de;
@@ -430,7 +575,8 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
         if (kGrad) {
           const R gdisp = sw * gdisp0 + dswdr * edisp0 / r1;
           const R f = -c6ij * gdisp;  // dG = f * vec
-          acc_d -= double(dc6i * edisp);
+          if (p.kind == kZDamping) acc_d -= double(dc6i * sw * dedc6);
+          else acc_d -= double(dc6i * edisp);
           const R v0 = R(u0), v1 = R(u1), v2 = R(u2);
           if (j != i) {
             g0 += double(f * v0); g1 += double(f * v1); g2 += double(f * v2);

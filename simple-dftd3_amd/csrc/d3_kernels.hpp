@@ -29,6 +29,8 @@ struct DevBatch {
   const double* c6ref;  // [nsp*nsp][kMaxRef*kMaxRef], [(si*nsp+sj)][ri*7+rj]
   const double* rvdw;   // [nsp*nsp]
   const int* ghost;     // [N] or null
+  const int* sid;       // [N] species identifier within the atom's system: its element numbered
+                        // 1, 2, ... by first appearance (structure_type%id; Z damping)
   // Spatial (Morton) order of the atoms of each system, the storage order of
   // the pair records and of the three-body lists (no result depends on it):
   const int* rank;      // [N] position of the atom in its system's order

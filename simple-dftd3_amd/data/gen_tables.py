@@ -8,7 +8,7 @@ the numbers from the Fortran sources of simple-dftd3 v1.6.0 and writes them
 into two data files that are committed next to it:
 
   d3_tables.npz   numeric element tables (numpy arrays, no pickles)
-  d3_params.json  method-name -> rational / zero damping parameters
+  d3_params.json  method-name -> damping parameters of every family
 
 It only runs where the reference checkout exists (``--reference`` option,
 default ``/root/reference``); everything else loads the committed files.
@@ -22,6 +22,9 @@ Sources read (reference file:line):
   src/dftd3/param.f90:96-300        get_method_id (aliases)
   src/dftd3/param.f90:302-800       get_rational_damping
   src/dftd3/param.f90:805-1090      get_zero_damping
+  src/dftd3/param.f90:1091-1326     get_mrational_damping, get_mzero_damping,
+                                    get_optimizedpower_damping, get_cso_damping,
+                                    get_z_damping (no entries)
 
 The covalent radii come from mctc-lib (grimme-lab/mctc-lib, rev 06db4d8,
 ``mctc_data_covrad``), which is not part of the reference checkout.  They are
@@ -160,7 +163,11 @@ def parse_params(src):
                     table[c] = kw
         return table
 
-    return aliases, damping("get_rational_damping"), damping("get_zero_damping")
+    families = {"rational": "get_rational_damping", "zero": "get_zero_damping",
+                "mrational": "get_mrational_damping", "mzero": "get_mzero_damping",
+                "optimizedpower": "get_optimizedpower_damping", "cso": "get_cso_damping",
+                "z": "get_z_damping"}
+    return aliases, {fam: damping(fn) for fam, fn in families.items()}
 
 
 def main():
@@ -176,18 +183,18 @@ def main():
     out = os.path.join(args.out, "d3_tables.npz")
     np.savez_compressed(out, nref=nref, refcn=refcn, c6ref=c6, r4_over_r2=r4r2,
                         vdwrad_aa=vdwrad, covrad_2009_aa=rcov)
-    aliases, rational, zero = parse_params(src)
+    aliases, tables = parse_params(src)
     pout = os.path.join(args.out, "d3_params.json")
     with open(pout, "w") as fh:
-        json.dump({"source": "simple-dftd3 v1.6.0 src/dftd3/param.f90",
-                   "defaults": {"s6": 1.0, "s8": 1.0, "s9": 0.0, "rs6": 1.0, "rs8": 1.0,
-                                "a1": 0.4, "a2": 5.0, "alp": 14.0},
-                   "aliases": aliases, "rational": rational, "zero": zero},
+        json.dump(dict({"source": "simple-dftd3 v1.6.0 src/dftd3/param.f90",
+                        "defaults": {"s6": 1.0, "s8": 1.0, "s9": 0.0, "rs6": 1.0, "rs8": 1.0,
+                                     "a1": 0.4, "a2": 5.0, "alp": 14.0, "bet": 0.0},
+                        "aliases": aliases}, **tables),
                   fh, indent=1, sort_keys=True)
     for path in (out, pout):
         digest = hashlib.sha256(open(path, "rb").read()).hexdigest()
         print(f"{os.path.basename(path)}: sha256 {digest}")
-    print(f"aliases {len(aliases)}, rational {len(rational)}, zero {len(zero)}")
+    print(f"aliases {len(aliases)}, " + ", ".join(f"{k} {len(v)}" for k, v in tables.items()))
 
 
 if __name__ == "__main__":

@@ -6,8 +6,11 @@ this library implements:
 
 * ``Structure(numbers, positions, lattice=None, periodic=None)`` and
   ``Structure.update(positions, lattice=None)`` (interface.py:63-152),
-* ``RationalDampingParam`` / ``ZeroDampingParam`` built from explicit
-  parameters or loaded by method name (interface.py:155-287),
+* the damping parameter classes ``RationalDampingParam``,
+  ``ZeroDampingParam``, ``ModifiedRationalDampingParam``,
+  ``ModifiedZeroDampingParam``, ``OptimizedPowerDampingParam``,
+  ``CSODampingParam`` and ``ZDampingParam``, built from explicit parameters
+  or loaded by method name (interface.py:155-502),
 * ``DispersionModel(numbers, positions, lattice, periodic)`` with
   ``set_realspace_cutoff``, ``set_ghost_atoms`` and
   ``get_dispersion(param, grad)`` returning ``energy`` (0-d array),
@@ -17,9 +20,9 @@ this library implements:
   (n, n) matrices (interface.py:618-636).
 
 All evaluation happens on the GPU through libd3hip.so.  Additions beyond the
-reference: ``DispersionModel.set_precision("fp64" | "mixed")``.  The damping
-families, Hessian, Ewald summation and counterpoise correction of the
-reference are not part of this library; their classes
+reference: ``DispersionModel.set_precision("fp64" | "mixed")``.  The Hessian,
+Ewald summation and counterpoise correction of the reference are not part of
+this library; their classes
 exist so that imports keep working, and raise ``NotImplementedError`` when
 used.
 """
@@ -144,31 +147,98 @@ class ZeroDampingParam(DampingParam):
         return library.new_zero_damping(s6, s8, s9, rs6, rs8, alp)
 
 
-class _Unavailable(DampingParam):
-    """Damping family of the reference that this library does not provide."""
+class ModifiedRationalDampingParam(DampingParam):
+    """Rational damping function with the modified parameters of Smith et al.
+    (interface.py:290-329); a full parameter set is the same as
+    ``RationalDampingParam``."""
 
     def __init__(self, **kwargs):
-        raise NotImplementedError(f"{type(self).__name__} is not available in this library")
+        _rename_kwargs(kwargs, "alpha6", "alp")
+        DampingParam.__init__(self, **kwargs)
+
+    @staticmethod
+    def load_param(method: str, atm: bool = False) -> library.ParamHandle:
+        return library.load_mrational_damping(method, atm)
+
+    @staticmethod
+    def new_param(*, s6: float = 1.0, s8: float, s9: float = 1.0, a1: float, a2: float,
+                  alp: float = 14.0) -> library.ParamHandle:
+        return library.new_mrational_damping(s6, s8, s9, a1, a2, alp)
 
 
-class ModifiedRationalDampingParam(_Unavailable):
-    pass
+class ModifiedZeroDampingParam(DampingParam):
+    """Modified zero damping with the offset ``bet`` (interface.py:332-376);
+    identical to zero damping for ``bet=0``."""
+
+    def __init__(self, **kwargs):
+        _rename_kwargs(kwargs, "sr6", "rs6")
+        _rename_kwargs(kwargs, "sr8", "rs8")
+        _rename_kwargs(kwargs, "alpha6", "alp")
+        _rename_kwargs(kwargs, "beta", "bet")
+        DampingParam.__init__(self, **kwargs)
+
+    @staticmethod
+    def load_param(method: str, atm: bool = False) -> library.ParamHandle:
+        return library.load_mzero_damping(method, atm)
+
+    @staticmethod
+    def new_param(*, s6: float = 1.0, s8: float, s9: float = 1.0, rs6: float, rs8: float = 1.0,
+                  alp: float = 14.0, bet: float) -> library.ParamHandle:
+        return library.new_mzero_damping(s6, s8, s9, rs6, rs8, alp, bet)
 
 
-class ModifiedZeroDampingParam(_Unavailable):
-    pass
+class OptimizedPowerDampingParam(DampingParam):
+    """Optimized power damping (interface.py:379-421); ``bet=0`` is the
+    rational damping."""
+
+    def __init__(self, **kwargs):
+        _rename_kwargs(kwargs, "alpha6", "alp")
+        _rename_kwargs(kwargs, "beta", "bet")
+        DampingParam.__init__(self, **kwargs)
+
+    @staticmethod
+    def load_param(method: str, atm: bool = False) -> library.ParamHandle:
+        return library.load_optimizedpower_damping(method, atm)
+
+    @staticmethod
+    def new_param(*, s6: float = 1.0, s8: float, s9: float = 1.0, a1: float, a2: float, alp: float = 14.0,
+                  bet) -> library.ParamHandle:
+        return library.new_optimizedpower_damping(s6, s8, s9, a1, a2, alp, bet)
 
 
-class OptimizedPowerDampingParam(_Unavailable):
-    pass
+class CSODampingParam(DampingParam):
+    """CSO (C6-scaled only) damping (interface.py:424-464)."""
+
+    def __init__(self, **kwargs):
+        _rename_kwargs(kwargs, "alpha6", "alp")
+        DampingParam.__init__(self, **kwargs)
+
+    @staticmethod
+    def load_param(method: str, atm: bool = False) -> library.ParamHandle:
+        return library.load_cso_damping(method, atm)
+
+    @staticmethod
+    def new_param(*, s6: float = 1.0, s9: float = 1.0, a1: float, a2: float = 2.5, a3: float = 0.0,
+                  a4: float = 6.25, alp: float = 14.0) -> library.ParamHandle:
+        return library.new_cso_damping(s6, s9, a1, a2, a3, a4, alp)
 
 
-class CSODampingParam(_Unavailable):
-    pass
+class ZDampingParam(DampingParam):
+    """Z damping (interface.py:467-502); the parameter loader has no
+    entries."""
 
+    def __init__(self, **kwargs):
+        _rename_kwargs(kwargs, "alpha6", "alp")
+        DampingParam.__init__(self, **kwargs)
 
-class ZDampingParam(_Unavailable):
-    pass
+    @staticmethod
+    def load_param(method: str, atm: bool = False) -> library.ParamHandle:
+        return library.load_z_damping(method, atm)
+
+    @staticmethod
+    def new_param(*, s6: float = 1.0, s8: float = 1.0, s9: float = 1.0, a1: float,
+                  alp: float = 14.0) -> library.ParamHandle:
+        return library.new_z_damping(s6, s8, s9, a1, alp)
 
 
 class DispersionModel(Structure):

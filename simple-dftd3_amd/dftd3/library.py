@@ -49,6 +49,16 @@ _SIGNATURES = {
     "dftd3_load_zero_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
     "dftd3_new_rational_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d]),
     "dftd3_load_rational_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
+    "dftd3_new_mzero_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d, _d]),
+    "dftd3_load_mzero_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
+    "dftd3_new_mrational_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d]),
+    "dftd3_load_mrational_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
+    "dftd3_new_optimizedpower_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d, _d]),
+    "dftd3_load_optimizedpower_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
+    "dftd3_new_cso_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d, _d]),
+    "dftd3_load_cso_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
+    "dftd3_new_z_damping": (_vp, [_vp, _d, _d, _d, _d, _d]),
+    "dftd3_load_z_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
     "dftd3_delete_param": (None, [ctypes.POINTER(_vp)]),
     "dftd3_get_dispersion": (None, [_vp, _vp, _vp, _vp, _dp, _dp, _dp]),
     "dftd3_get_pairwise_dispersion": (None, [_vp, _vp, _vp, _vp, _dp, _dp]),
@@ -224,6 +234,58 @@ def new_rational_damping(s6: float, s8: float, s9: float, a1: float, a2: float, 
 def load_rational_damping(method: str, atm: bool) -> ParamHandle:
     """Load rational damping parameters from internal storage"""
     return ParamHandle.with_gc(error_check(lib.dftd3_load_rational_damping)(_char(method), atm))
+
+
+def new_mzero_damping(s6: float, s8: float, s9: float, rs6: float, rs8: float, alp: float,
+                      bet: float) -> ParamHandle:
+    """Create new modified zero damping parameters"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_new_mzero_damping)(s6, s8, s9, rs6, rs8, alp, bet))
+
+
+def load_mzero_damping(method: str, atm: bool) -> ParamHandle:
+    """Load modified zero damping parameters from internal storage"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_load_mzero_damping)(_char(method), atm))
+
+
+def new_mrational_damping(s6: float, s8: float, s9: float, a1: float, a2: float, alp: float) -> ParamHandle:
+    """Create new modified rational damping parameters"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_new_mrational_damping)(s6, s8, s9, a1, a2, alp))
+
+
+def load_mrational_damping(method: str, atm: bool) -> ParamHandle:
+    """Load modified rational damping parameters from internal storage"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_load_mrational_damping)(_char(method), atm))
+
+
+def new_optimizedpower_damping(s6: float, s8: float, s9: float, a1: float, a2: float, alp: float,
+                               bet: float) -> ParamHandle:
+    """Create new optimized power damping parameters"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_new_optimizedpower_damping)(s6, s8, s9, a1, a2, alp, bet))
+
+
+def load_optimizedpower_damping(method: str, atm: bool) -> ParamHandle:
+    """Load optimized power damping parameters from internal storage"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_load_optimizedpower_damping)(_char(method), atm))
+
+
+def new_cso_damping(s6: float, s9: float, a1: float, a2: float, a3: float, a4: float, alp: float) -> ParamHandle:
+    """Create new CSO damping parameters"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_new_cso_damping)(s6, s9, a1, a2, a3, a4, alp))
+
+
+def load_cso_damping(method: str, atm: bool) -> ParamHandle:
+    """Load CSO damping parameters from internal storage"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_load_cso_damping)(_char(method), atm))
+
+
+def new_z_damping(s6: float, s8: float, s9: float, a1: float, alp: float) -> ParamHandle:
+    """Create new Z damping parameters"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_new_z_damping)(s6, s8, s9, a1, alp))
+
+
+def load_z_damping(method: str, atm: bool) -> ParamHandle:
+    """Load Z damping parameters from internal storage"""
+    return ParamHandle.with_gc(error_check(lib.dftd3_load_z_damping)(_char(method), atm))
 
 
 def get_dispersion(mol: StructureHandle, disp: ModelHandle, param: ParamHandle, energy: np.ndarray,

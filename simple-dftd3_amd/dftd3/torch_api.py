@@ -13,7 +13,8 @@ import torch
 
 from . import library
 
-_KIND = {"rational": 0, "bj": 0, "zero": 1}
+_KIND = {"rational": 0, "bj": 0, "mrational": 0, "bjm": 0, "zero": 1, "mzero": 2, "zerom": 2,
+         "optimizedpower": 3, "op": 3, "cso": 4, "z": 5}
 _workspaces: Dict[int, library.ModelHandle] = {}
 
 
@@ -28,13 +29,19 @@ def workspace(device: Optional[torch.device] = None) -> library.ModelHandle:
 
 
 def param_row(kind: str, param: dict) -> List[float]:
-    """[kind, s6, s8, s9, a1|rs6, a2|rs8, alp] of the C ABI."""
+    """[kind, s6, s8, s9, a1|rs6, a2|rs8, alp, bet, a3, a4] of the C ABI, from
+    the keyword names of the damping parameter classes."""
     k = _KIND[kind]
-    if k == 0:
-        return [0.0, param.get("s6", 1.0), param["s8"], param.get("s9", 1.0), param["a1"], param["a2"],
-                param.get("alp", 14.0)]
-    return [1.0, param.get("s6", 1.0), param["s8"], param.get("s9", 1.0), param["rs6"], param.get("rs8", 1.0),
-            param.get("alp", 14.0)]
+    s6, s9, alp = param.get("s6", 1.0), param.get("s9", 1.0), param.get("alp", 14.0)
+    if k in (0, 3):  # rational, optimized power
+        row = [param["s8"], s9, param["a1"], param["a2"], alp, param.get("bet", 0.0), 0.0, 0.0]
+    elif k in (1, 2):  # zero, modified zero
+        row = [param["s8"], s9, param["rs6"], param.get("rs8", 1.0), alp, param.get("bet", 0.0), 0.0, 0.0]
+    elif k == 4:  # CSO
+        row = [0.0, s9, param["a1"], param.get("a2", 2.5), alp, 0.0, param.get("a3", 0.0), param.get("a4", 6.25)]
+    else:  # Z
+        row = [param.get("s8", 1.0), s9, param["a1"], 0.0, alp, 0.0, 0.0, 0.0]
+    return [float(k), s6] + [float(v) for v in row]
 
 
 class BatchSpec:
@@ -87,8 +94,8 @@ def get_dispersion_batch(systems: Sequence[dict], precision: str = "fp64", grad:
 
     Each system is a dict with ``numbers`` (n,), ``positions`` (n, 3) in bohr
     (numpy array or tensor), optional ``lattice`` (3, 3) and ``periodic``
-    (3,), ``kind`` ('rational' or 'zero'), ``param`` (dict of damping
-    parameters) and optional ``cutoff`` (dict with disp2, disp3, cn,
+    (3,), ``kind`` ('rational', 'zero', 'mzero', 'optimizedpower', 'cso' or
+    'z'), ``param`` (dict of damping parameters) and optional ``cutoff`` (dict with disp2, disp3, cn,
     width2, width3).  Positions are used as given (not wrapped).
 
     Returns ``energy`` (nsys,), and with ``grad`` also ``gradient`` (N, 3)

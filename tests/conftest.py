@@ -30,5 +30,10 @@ def pairwise_goldens():
 
 
 @pytest.fixture(scope="session")
+def family_goldens():
+    return {f["name"]: f for f in load_golden("family_goldens.json")}
+
+
+@pytest.fixture(scope="session")
 def python_model():
     return load_golden("python_interface_model.json")

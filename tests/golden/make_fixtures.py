@@ -14,6 +14,8 @@ Sources:
   python/dftd3/test_interface.py:78-100,236-356  16-atom model energies
   test/validation/11-13 (*.resp + geometry + *.json)  pair energy matrices
   python/dftd3/test_interface.py:569-762   S8 pair energy matrices
+  test/validation/03-05,08-10,14-20         modified rational / zero, optimized
+                                            power damping and parameter files
 """
 import argparse
 import json
@@ -147,6 +149,89 @@ def read_tmol(path):
     return nums, pos, None, None
 
 
+def read_molfile(path):
+    """MDL molfile / SD file (V2000): counts line 4, atom block x y z symbol."""
+    lines = open(path).read().splitlines()
+    n = int(lines[3][0:3])
+    nums, pos = [], []
+    for line in lines[4:4 + n]:
+        pos.append([float(line[0:10]) * AATOAU, float(line[10:20]) * AATOAU, float(line[20:30]) * AATOAU])
+        nums.append(to_number(line[31:34]))
+    return nums, pos, None, None
+
+
+def read_pdb(path):
+    """PDB ATOM/HETATM records: coordinates in columns 31-54, element 77-78."""
+    nums, pos = [], []
+    for line in open(path).read().splitlines():
+        if line.startswith(("ATOM", "HETATM")):
+            pos.append([float(line[30:38]) * AATOAU, float(line[38:46]) * AATOAU, float(line[46:54]) * AATOAU])
+            nums.append(to_number(line[76:78]))
+    return nums, pos, None, None
+
+
+def family_fixtures(ref, params):
+    """test/validation cases of the modified rational (bjm), modified zero
+    (zerom) and optimized power (op) damping: energies (03-05), energies,
+    gradients and virials (08-10), pair energy matrices (14, 15) and the
+    parameter-database cases (16-20).  Parameters: loaded by method name as
+    the CLI does (param.f90:1091-1244, s9 = 0 without --atm), given on the
+    command line (app/cli.f90: --zerom-param s6 s8 rs6 bet, --op-param s6 s8 a1
+    a2 bet, --atm-scale s9), or, for 16-20, the damping parameters the JSON
+    records (assets/parameters.toml; 17 lists none: pkzb zero damping,
+    parameters.toml:418-419 with the file's defaults s9 = 1, rs8 = 1)."""
+    vdir = os.path.join(ref, "test/validation")
+
+    def loaded(family, method):
+        p = dict(params["defaults"])
+        p.update(params[family][params["aliases"][method]])
+        return p
+
+    cases = [
+        ("03-energy-d3-bjm", "03-met-xag.coord", read_tmol, "rational", loaded("mrational", "b97d")),
+        ("04-energy-d3-zerom", "04-val-xad.mol", read_molfile, "mzero", loaded("mzero", "bp")),
+        ("05-energy-d3-op", "05-cys-xag.sdf", read_molfile, "optimizedpower", loaded("optimizedpower", "tpss")),
+        ("08-gradient-d3-bjm", "08-teh2-hbr.mol", read_molfile, "rational", loaded("mrational", "pbe0")),
+        ("09-gradient-d3-zerom", "09-0a.pdb", read_pdb, "mzero", loaded("mzero", "lcwpbe")),
+        ("10-gradient-d3-op", "10-but14diol.log", read_xyz, "optimizedpower", loaded("optimizedpower", "revpbe0")),
+        ("14-pairwise-d3-zerom", "14-anthracene2.xyz", read_xyz, "mzero",
+         dict(params["defaults"], s6=1.0, s8=1.841686, rs6=1.279637, bet=0.014370, s9=0.5)),
+        ("15-pairwise-d3-op", "15-pxylene.mol", read_molfile, "optimizedpower",
+         dict(params["defaults"], s6=1.0, s8=0.27632, a1=0.700, a2=2.500, bet=8.0)),
+        ("16-db-d3-bj", "16-i10e.gen", read_gen, "rational", None),
+        ("17-db-d3-zero", "17-li2-ch4.sdf", read_molfile, "zero",
+         dict(params["defaults"], rs6=0.6327, s8=0.0, s9=1.0, rs8=1.0)),
+        ("18-db-d3-bjm", "18-c1.pdb", read_pdb, "rational", None),
+        ("19-db-d3-zerom", "19-g1.mol", read_molfile, "mzero", None),
+        ("20-db-d3-op", "20-i21p.tmol", read_tmol, "optimizedpower", None),
+    ]
+    keys = dict(rational=("s6", "s8", "s9", "a1", "a2", "alp"), zero=("s6", "s8", "s9", "rs6", "rs8", "alp"),
+                mzero=("s6", "s8", "s9", "rs6", "rs8", "alp", "bet"),
+                optimizedpower=("s6", "s8", "s9", "a1", "a2", "alp", "bet"))
+    out = []
+    for name, geo, reader, kind, par in cases:
+        nums, pos, lat, pbc = reader(os.path.join(vdir, geo))
+        ref_json = json.load(open(os.path.join(vdir, name + ".json")))
+        if par is None:
+            par = dict(params["defaults"])
+            par.update({k: v for k, v in ref_json["damping parameters"].items() if k != "damping"})
+        par = {k: par[k] for k in keys[kind]}
+        exp = dict(energy=ref_json["energy"])
+        if "gradient" in ref_json:
+            exp["gradient"] = _triples(ref_json["gradient"])
+            exp["virial"] = _triples(ref_json["virial"])
+        n = len(nums)
+        if "additive pairwise energy" in ref_json:
+            e2, e3 = ref_json["additive pairwise energy"], ref_json["non-additive pairwise energy"]
+            exp["pair2"] = [[e2[c * n + r] for c in range(n)] for r in range(n)]
+            exp["pair3"] = [[e3[c * n + r] for c in range(n)] for r in range(n)]
+        out.append(dict(name="validation_" + name, source=f"test/validation/{name}.json,{name}.resp,{geo}",
+                        numbers=nums, positions=pos, lattice=lat, periodic=pbc, wrap=True, kind=kind, param=par,
+                        cutoff=[40.0, 60.0, 40.0, 0.0, 0.0], grad="gradient" in ref_json, expected=exp,
+                        atol=1.0e-6, version=ref_json["version"]))
+    return out
+
+
 def validation_fixtures(ref, params):
     vdir = os.path.join(ref, "test/validation")
     cases = [
@@ -268,6 +353,8 @@ def main():
     pw = pairwise_fixtures(args.reference) + [python_pairwise_fixture(args.reference)]
     with open(os.path.join(args.out, "pairwise_goldens.json"), "w") as fh:
         json.dump(pw, fh)
+    with open(os.path.join(args.out, "family_goldens.json"), "w") as fh:
+        json.dump(family_fixtures(args.reference, params), fh)
     print("wrote", len(fix), "golden cases")
 
 

@@ -75,10 +75,45 @@ def test_damping_param_loading():
         ZeroDampingParam(method="not-a-method")
 
 
-def test_unavailable_damping_families():
-    from dftd3.interface import ModifiedRationalDampingParam
-    with pytest.raises(NotImplementedError):
-        ModifiedRationalDampingParam(method="pbe")
+def test_damping_family_missing_arguments():
+    """Python-side TypeError for missing keywords, as test_interface.py:132-183."""
+    from dftd3.interface import (CSODampingParam, ModifiedRationalDampingParam, ModifiedZeroDampingParam,
+                                 OptimizedPowerDampingParam, ZDampingParam)
+    with pytest.raises(TypeError):
+        ModifiedRationalDampingParam()
+    with pytest.raises(TypeError, match="s8"):
+        ModifiedRationalDampingParam(a1=0.4, a2=5.0)
+    with pytest.raises(TypeError, match="a2"):
+        ModifiedRationalDampingParam(s8=1.0, a1=0.4)
+    with pytest.raises(TypeError, match="rs6"):
+        ModifiedZeroDampingParam(s8=1.0, bet=1.0)
+    with pytest.raises(TypeError, match="bet"):
+        ModifiedZeroDampingParam(s8=1.0, rs6=1.2)
+    with pytest.raises(TypeError):
+        ModifiedZeroDampingParam(s8=1.0, rs6=1.2, bet=1.0, method="abc")
+    with pytest.raises(TypeError, match="bet"):
+        OptimizedPowerDampingParam(s8=1.0, a1=0.3, a2=4.2)
+    with pytest.raises(TypeError, match="a1"):
+        CSODampingParam(s6=1.0)
+    with pytest.raises(TypeError, match="a1"):
+        ZDampingParam(s8=1.0)
+    # renamed keywords
+    ModifiedZeroDampingParam(s8=1.0, sr6=1.2, beta=0.1, alpha6=14.0)
+    OptimizedPowerDampingParam(s8=1.0, a1=0.3, a2=4.2, beta=1.0)
+
+
+def test_damping_family_loading():
+    """Method-name loaders of param.f90:1091-1326 (atm sets s9 = 1)."""
+    from dftd3.interface import (CSODampingParam, ModifiedRationalDampingParam, ModifiedZeroDampingParam,
+                                 OptimizedPowerDampingParam, ZDampingParam)
+    for cls, method in ((ModifiedRationalDampingParam, "b97d"), (ModifiedZeroDampingParam, "bp"),
+                        (OptimizedPowerDampingParam, "tpss"), (CSODampingParam, "pbe0")):
+        cls(method=method)
+        cls(method=method, atm=True)
+    with pytest.raises(RuntimeError, match="No entry for 'pbe' present"):
+        ZDampingParam(method="pbe")
+    with pytest.raises(RuntimeError, match="No entry for 'scan' present"):
+        CSODampingParam(method="scan")
 
 
 @pytest.mark.parametrize("cls_name", ["Structure", "DispersionModel"])

@@ -17,8 +17,12 @@ EPS = np.finfo(float).eps
 TABLES = np.load(O.TABLES)
 
 
+KINDS = {"rational": O.RATIONAL, "zero": O.ZERO, "mzero": O.MZERO, "optimizedpower": O.OPTIMIZEDPOWER,
+         "cso": O.CSO, "z": O.ZDAMP}
+
+
 def _kind(name):
-    return O.RATIONAL if name == "rational" else O.ZERO
+    return KINDS[name]
 
 
 def _run(f, **kw):
@@ -183,10 +187,17 @@ def _small_molecule(seed=7, n=14):
 PARAMS = {
     "rational": dict(s6=1.0, s8=1.2177, a1=0.4145, a2=4.8593, s9=1.0, alp=14.0),
     "zero": dict(s6=1.0, s8=1.703, rs6=1.261, rs8=1.0, s9=1.0, alp=14.0),
+    # param.f90:1168-1169 (blyp), 1219-1220 (tpss), 1266-1267 (pbe0 with a nonzero sigmoid), z as
+    # test_d3z_mb01 of test/unit/test_param.f90
+    "mzero": dict(s6=1.0, s8=1.841686, rs6=1.279637, rs8=1.0, bet=0.014370, s9=1.0, alp=14.0),
+    "optimizedpower": dict(s6=1.0, s8=0.51581, a1=0.575, a2=3.0, bet=8.0, s9=1.0, alp=14.0),
+    "cso": dict(s6=1.0, a1=0.2, a2=2.5, a3=0.0, a4=6.25, s9=1.0, alp=14.0),
+    "z": dict(s6=1.0, s8=1.0, a1=200770.0, s9=1.0, alp=14.0),
 }
+ALL_KINDS = list(PARAMS)
 
 
-@pytest.mark.parametrize("kind", ["rational", "zero"])
+@pytest.mark.parametrize("kind", ALL_KINDS)
 def test_gradient_finite_difference_molecule(kind):
     num, pos = _small_molecule()
     p = PARAMS[kind]
@@ -234,7 +245,7 @@ def _small_cell(seed=5):
     return num, np.array(frac) @ lat, lat
 
 
-@pytest.mark.parametrize("kind", ["rational", "zero"])
+@pytest.mark.parametrize("kind", ALL_KINDS)
 def test_periodic_gradient_and_strain_virial(kind):
     num, pos, lat = _small_cell()
     p = PARAMS[kind]
@@ -396,3 +407,88 @@ def test_pairwise_ghost_atoms():
         assert not e2[g].any() and not e2[:, g].any() and not e3[g].any() and not e3[:, g].any()
     r = O.dispersion(num, x, None, None, O.RATIONAL, BJ, O.DEFAULT_CUTOFF, grad=False, ghost=ghost)
     assert abs(e2.sum() + e3.sum() - r["energy"]) <= 1e-14 * abs(r["energy"])
+
+
+# ------------------------------------------------------- damping families
+FAMILY = ["validation_03-energy-d3-bjm", "validation_04-energy-d3-zerom", "validation_05-energy-d3-op",
+          "validation_08-gradient-d3-bjm", "validation_09-gradient-d3-zerom", "validation_10-gradient-d3-op",
+          "validation_14-pairwise-d3-zerom", "validation_15-pairwise-d3-op", "validation_16-db-d3-bj",
+          "validation_17-db-d3-zero", "validation_18-db-d3-bjm", "validation_19-db-d3-zerom",
+          "validation_20-db-d3-op"]
+
+
+@pytest.mark.parametrize("name", FAMILY)
+def test_family_goldens(family_goldens, name):
+    """Modified rational, modified zero and optimized power damping (with ATM
+    in 16-20) against the CLI validation set at its 1e-6; the cases written
+    by v1.2.1 (16, 18-20) agree to rounding and are held to 1e-12."""
+    f = family_goldens[name]
+    e = f["expected"]
+    atol = 1e-12 if f["version"] == "1.2.1" else f["atol"]
+    r = _run(f)
+    assert abs(r["energy"] - e["energy"]) <= atol
+    if "gradient" in e:
+        np.testing.assert_allclose(r["gradient"], np.array(e["gradient"]), rtol=0, atol=f["atol"])
+        np.testing.assert_allclose(r["virial"], np.array(e["virial"]), rtol=0, atol=f["atol"])
+    if "pair2" in e:
+        e2, e3 = O.pairwise(f["numbers"], f["positions"], f["lattice"], f["periodic"], _kind(f["kind"]),
+                            f["param"], f["cutoff"])
+        np.testing.assert_allclose(e2, np.array(e["pair2"]), rtol=0, atol=f["atol"])
+        np.testing.assert_allclose(e3, np.array(e["pair3"]), rtol=0, atol=f["atol"])
+
+
+REDUCTIONS = [
+    # modified zero with bet = 0 is the zero damping (interface.py:336-340)
+    ("mzero", dict(PARAMS["mzero"], bet=0.0), "zero", dict(PARAMS["mzero"], bet=0.0)),
+    # optimized power with bet = 0 is the rational damping (interface.py:386-388)
+    ("optimizedpower", dict(PARAMS["optimizedpower"], bet=0.0), "rational", dict(PARAMS["optimizedpower"])),
+    # CSO with a1 = 0: sig = s6 and t6 = 1/(r^6 + (a3 sqrt(rrij) + a4)^6), the rational
+    # damping with s8 = 0, a1 = a3, a2 = a4
+    ("cso", dict(PARAMS["cso"], a1=0.0, a3=0.3, a4=4.0), "rational",
+     dict(s6=1.0, s8=0.0, a1=0.3, a2=4.0, s9=1.0, alp=14.0)),
+    # Z with a1 = 0: no damping, the rational damping with a1 = a2 = 0
+    ("z", dict(PARAMS["z"], a1=0.0), "rational", dict(s6=1.0, s8=1.0, a1=0.0, a2=0.0, s9=1.0, alp=14.0)),
+]
+
+
+@pytest.mark.parametrize("kind,par,ref_kind,ref_par", REDUCTIONS)
+def test_family_reductions(kind, par, ref_kind, ref_par):
+    """Special cases that reduce a family to a pinned one: energies,
+    gradients and virials agree to rounding (molecule and periodic cell)."""
+    num, pos = _small_molecule(5)
+    cases = [(num, pos, None, None, O.DEFAULT_CUTOFF)]
+    cnum, cpos, lat = _small_cell(9)
+    cases.append((cnum, cpos, lat, [True] * 3, (20.0, 25.0, 12.0, 0.0, 0.0)))
+    for n_, x, l, pbc, cut in cases:
+        a = O.dispersion(n_, x, l, pbc, _kind(kind), par, cut)
+        b = O.dispersion(n_, x, l, pbc, _kind(ref_kind), ref_par, cut)
+        assert a["energy"] == pytest.approx(b["energy"], rel=1e-12)
+        np.testing.assert_allclose(a["gradient"], b["gradient"], rtol=0, atol=1e-12 * np.abs(b["gradient"]).max())
+        np.testing.assert_allclose(a["virial"], b["virial"], rtol=0, atol=1e-12 * np.abs(b["virial"]).max())
+
+
+def test_z_damping_species_ids():
+    """The Z damping's r0 = a1 / (id_i + id_j) uses the species identifiers of
+    the structure (elements numbered by first appearance), not atomic
+    numbers: reordering the atoms so that another element appears first
+    changes the energy, and relabelling that keeps the first-appearance
+    order does not."""
+    num, pos = _small_molecule(13)
+    p = PARAMS["z"]
+    e = lambda n_, x: O.dispersion(n_, x, None, None, O.ZDAMP, p, O.DEFAULT_CUTOFF, grad=False)["energy"]
+    base = e(num, pos)
+    # same structure, atoms listed in reverse: the identifiers change unless the
+    # first-appearance order of the elements is unchanged
+    rev = e(num[::-1].copy(), pos[::-1].copy())
+    first = list(dict.fromkeys(num.tolist()))
+    first_rev = list(dict.fromkeys(num[::-1].tolist()))
+    if first != first_rev:
+        assert abs(rev - base) > 1e-10 * abs(base)
+    # a permutation that keeps the first atom of every element in place keeps the ids
+    perm = np.arange(len(num))
+    last = len(num) - 1
+    for k in range(len(num) - 2, 0, -1):
+        if num[k] == num[last] and num[:k].tolist().count(num[k]) > 0:
+            perm[[k, last]] = perm[[last, k]]
+            break
+    assert e(num[perm], pos[perm]) == pytest.approx(base, rel=1e-13)
