@@ -132,6 +132,13 @@ D3HIP_API void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd
 D3HIP_API void dftd3_get_pairwise_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model disp,
                                              dftd3_param param, double* pair_energy2, double* pair_energy3);
 
+/* s-dftd3.h:399-407 dftd3_get_dispersion_hessian (api.f90:1115-1176,
+ * disp.f90:236-308): the energy and the analytic second derivatives
+ * hessian[3n][3n] (host; symmetric, index 3*i + c for component c of atom
+ * i), in Hartree / bohr^2.  Always evaluated in FP64. */
+D3HIP_API void dftd3_get_dispersion_hessian(dftd3_error error, dftd3_structure mol, dftd3_model disp,
+                                            dftd3_param param, double* energy, double* hessian);
+
 /* ------------------------------------------------------------------ Part 2
  * No counterpart in the reference: device-pointer entry points. */
 
@@ -173,6 +180,18 @@ D3HIP_API void d3hip_get_pairwise_dispersion_batch(dftd3_error error, dftd3_mode
                                                    const double* cutoffs, int precision, const double* positions,
                                                    double* pair_energy2, double* pair_energy3, void* stream);
 
+/* Energies and analytic Hessians of a batch, FP64 (device arrays; same batch
+ * description as d3hip_get_dispersion_batch): energy[nsys] (may be NULL) and
+ * hessian holding per system a 3n x 3n block, the blocks one after another
+ * (system s at offset 9 times the sum of n^2 of the systems before it).
+ * Asynchronous except for one device-to-host read of the three-body list
+ * size. */
+D3HIP_API void d3hip_get_dispersion_hessian_batch(dftd3_error error, dftd3_model workspace, int nsys,
+                                                  const int* natoms, const int* numbers, const double* lattices,
+                                                  const bool* periodic, const double* params, const double* cutoffs,
+                                                  const double* positions, double* energy, double* hessian,
+                                                  void* stream);
+
 /* A model without a structure, usable only as the workspace (device buffers
  * and stream) of d3hip_get_dispersion_batch. */
 D3HIP_API dftd3_model d3hip_new_workspace(dftd3_error error);
@@ -193,7 +212,7 @@ D3HIP_API void d3hip_set_kernel_timing(dftd3_error error, dftd3_model model, int
 
 /* Total milliseconds and launches of one kernel group since timing was
  * enabled; name is one of pack, order, cn, weights, c6, disp2, atm_list, atm,
- * cn_chain, reduce. */
+ * cn_chain, reduce, hessian. */
 D3HIP_API void d3hip_get_kernel_time(dftd3_error error, dftd3_model model, const char* name, double* total_ms,
                                      long long* calls);
 

@@ -1066,6 +1066,659 @@ int d3o_get_pairwise_dispersion(int nat, const int *num, const double *xyz, cons
     return 0;
 }
 
+/* --------------------------------------------------------- Hessian */
+/* cutoff.f90:104-137 smooth_cutoff_r2: switching function and its first and
+ * second derivative with respect to r^2. */
+static void smooth_cutoff_r2(double r2, double cutoff, double width, double *sw, double *dsw, double *d2sw)
+{
+    double r1 = sqrt(r2), dswdr = 0.0, d2swdr2 = 0.0;
+    if (width <= 0.0 || width >= cutoff) {
+        *sw = 1.0;
+    } else {
+        double inner = cutoff - width;
+        if (r1 <= inner) {
+            *sw = 1.0;
+        } else if (r1 >= cutoff) {
+            *sw = 0.0;
+        } else {
+            double x = (cutoff - r1) / width;
+            *sw = x * x * x * (10.0 + x * (-15.0 + 6.0 * x));
+            dswdr = -30.0 * x * x * (1.0 - x) * (1.0 - x) / width;
+            d2swdr2 = 60.0 * x * (1.0 - x) * (1.0 - 2.0 * x) / (width * width);
+        }
+    }
+    *dsw = 0.5 * dswdr / r1;
+    *d2sw = 0.25 * (d2swdr2 - dswdr / r1) / r2;
+}
+
+/*
+ * get_damping_kernel of every family (rational.f90, zero.f90, mzero.f90,
+ * optimizedpower.f90, cso.f90, z.f90): the pair energy e(u, C6) with u = r^2
+ * and its derivatives e_u, e_c, e_uu, e_uc, e_cc.
+ */
+static void damping_kernel(const d3o_param *p, int zi, int zj, int idi, int idj, double u, double c6,
+                           double *e, double *er, double *ec, double *err, double *erc, double *ecc)
+{
+    *e = *er = *ec = *err = *erc = *ecc = 0.0;
+    if (p->kind != K_CSO && fabs(p->s6) < EPS && fabs(p->s8) < EPS) return;
+    double rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
+    double pv = 0.0, pd = 0.0, pdd = 0.0;
+    switch (p->kind) {
+    case K_RATIONAL: {
+        double r0ij = p->a1 * sqrt(rrij) + p->a2;
+        double r0ij2 = r0ij * r0ij, r0ij6 = r0ij2 * r0ij2 * r0ij2, r0ij8 = r0ij6 * r0ij2;
+        double t6 = 1.0 / (u * u * u + r0ij6);
+        double t8 = 1.0 / (u * u * u * u + r0ij8);
+        double t6d = -3.0 * u * u * t6 * t6;
+        double t8d = -4.0 * u * u * u * t8 * t8;
+        double t6dd = -6.0 * u * t6 * t6 + 18.0 * pow(u, 4) * pow(t6, 3);
+        double t8dd = -12.0 * u * u * t8 * t8 + 32.0 * pow(u, 6) * pow(t8, 3);
+        pv = p->s6 * t6 + p->s8 * rrij * t8;
+        pd = p->s6 * t6d + p->s8 * rrij * t8d;
+        pdd = p->s6 * t6dd + p->s8 * rrij * t8dd;
+        break;
+    }
+    case K_ZERO: case K_MZERO: {
+        double r0ij = d3o_rvdw(zj, zi);
+        double alp6 = p->alp, alp8 = p->alp + 2.0;
+        double t6, t8, t6d, t8d, t6dd, t8dd;
+        if (p->kind == K_ZERO) {
+            double a6 = 0.5 * alp6, a8 = 0.5 * alp8;
+            t6 = pow(p->a1 * r0ij / sqrt(u), alp6);
+            t8 = pow(p->a2 * r0ij / sqrt(u), alp8);
+            t6d = -a6 * t6 / u;
+            t8d = -a8 * t8 / u;
+            t6dd = a6 * (a6 + 1.0) * t6 / (u * u);
+            t8dd = a8 * (a8 + 1.0) * t8 / (u * u);
+        } else {
+            double r1 = sqrt(u), betr0 = p->bet * r0ij;
+            double w6 = r1 / (p->a1 * r0ij) + betr0, w8 = r1 / (p->a2 * r0ij) + betr0;
+            double w6d = 1.0 / (p->a1 * r0ij * 2.0 * r1), w8d = 1.0 / (p->a2 * r0ij * 2.0 * r1);
+            double w6dd = -1.0 / (p->a1 * r0ij * 4.0 * u * r1), w8dd = -1.0 / (p->a2 * r0ij * 4.0 * u * r1);
+            t6 = pow(w6, -alp6);
+            t8 = pow(w8, -alp8);
+            t6d = -alp6 * pow(w6, -alp6 - 1.0) * w6d;
+            t8d = -alp8 * pow(w8, -alp8 - 1.0) * w8d;
+            t6dd = alp6 * (alp6 + 1.0) * pow(w6, -alp6 - 2.0) * w6d * w6d - alp6 * pow(w6, -alp6 - 1.0) * w6dd;
+            t8dd = alp8 * (alp8 + 1.0) * pow(w8, -alp8 - 2.0) * w8d * w8d - alp8 * pow(w8, -alp8 - 1.0) * w8dd;
+        }
+        double f6 = 1.0 / (1.0 + 6.0 * t6), f8 = 1.0 / (1.0 + 6.0 * t8);
+        double f6d = -6.0 * t6d * f6 * f6, f8d = -6.0 * t8d * f8 * f8;
+        double f6dd = -6.0 * t6dd * f6 * f6 + 72.0 * t6d * t6d * pow(f6, 3);
+        double f8dd = -6.0 * t8dd * f8 * f8 + 72.0 * t8d * t8d * pow(f8, 3);
+        double c6t = f6 / pow(u, 3);
+        double c6td = f6d / pow(u, 3) - 3.0 * f6 / pow(u, 4);
+        double c6tdd = f6dd / pow(u, 3) - 6.0 * f6d / pow(u, 4) + 12.0 * f6 / pow(u, 5);
+        double c8t = f8 / pow(u, 4);
+        double c8td = f8d / pow(u, 4) - 4.0 * f8 / pow(u, 5);
+        double c8tdd = f8dd / pow(u, 4) - 8.0 * f8d / pow(u, 5) + 20.0 * f8 / pow(u, 6);
+        pv = p->s6 * c6t + p->s8 * rrij * c8t;
+        pd = p->s6 * c6td + p->s8 * rrij * c8td;
+        pdd = p->s6 * c6tdd + p->s8 * rrij * c8tdd;
+        break;
+    }
+    case K_OP: {
+        double r0ij = p->a1 * sqrt(rrij) + p->a2;
+        double r0ij2 = r0ij * r0ij, r0ij6 = r0ij2 * r0ij2 * r0ij2, r0ij8 = r0ij6 * r0ij2;
+        double ab = pow(r0ij, p->bet), abr0ij6 = ab * r0ij6, abr0ij8 = ab * r0ij8;
+        double q = 0.5 * p->bet;
+        double d6 = pow(u, 3) + abr0ij6 * pow(u, -q);
+        double d8 = pow(u, 4) + abr0ij8 * pow(u, -q);
+        double d6d = 3.0 * u * u - q * abr0ij6 * pow(u, -q - 1.0);
+        double d8d = 4.0 * pow(u, 3) - q * abr0ij8 * pow(u, -q - 1.0);
+        double d6dd = 6.0 * u + q * (q + 1.0) * abr0ij6 * pow(u, -q - 2.0);
+        double d8dd = 12.0 * u * u + q * (q + 1.0) * abr0ij8 * pow(u, -q - 2.0);
+        double t6 = 1.0 / d6, t8 = 1.0 / d8;
+        double t6d = -d6d * t6 * t6, t8d = -d8d * t8 * t8;
+        double t6dd = -d6dd * t6 * t6 + 2.0 * d6d * d6d * pow(t6, 3);
+        double t8dd = -d8dd * t8 * t8 + 2.0 * d8d * d8d * pow(t8, 3);
+        pv = p->s6 * t6 + p->s8 * rrij * t8;
+        pd = p->s6 * t6d + p->s8 * rrij * t8d;
+        pdd = p->s6 * t6dd + p->s8 * rrij * t8dd;
+        break;
+    }
+    case K_CSO: {
+        double r1 = sqrt(u), r0ij = sqrt(rrij);
+        double dd6 = p->a3 * r0ij + p->a4, dd6_6 = pow(dd6, 6);
+        double ef = exp(r1 - p->a2 * r0ij);
+        double sf = 1.0 / (1.0 + ef);
+        double sfd = -sf * (1.0 - sf), sfdd = sf * (1.0 - sf) * (1.0 - 2.0 * sf);
+        double sig = p->s6 + p->a1 * sf, sigd = p->a1 * sfd, sigdd = p->a1 * sfdd;
+        double sigu = 0.5 * sigd / r1;
+        double siguu = 0.25 * sigdd / u - 0.25 * sigd / (u * r1);
+        double t6 = 1.0 / (u * u * u + dd6_6);
+        double t6d = -3.0 * u * u * t6 * t6;
+        double t6dd = -6.0 * u * t6 * t6 + 18.0 * pow(u, 4) * pow(t6, 3);
+        pv = sig * t6;
+        pd = sigu * t6 + sig * t6d;
+        pdd = siguu * t6 + 2.0 * sigu * t6d + sig * t6dd;
+        break;
+    }
+    case K_Z: {
+        double r0ij = p->a1 / (double)(idi + idj), b8 = r0ij * rrij;
+        double t6 = 1.0 / (u * u * u + r0ij * c6);
+        double t8 = 1.0 / (u * u * u * u + b8 * c6);
+        double t6u = -3.0 * u * u * t6 * t6, t8u = -4.0 * pow(u, 3) * t8 * t8;
+        double t6uu = -6.0 * u * t6 * t6 + 18.0 * pow(u, 4) * pow(t6, 3);
+        double t8uu = -12.0 * u * u * t8 * t8 + 32.0 * pow(u, 6) * pow(t8, 3);
+        double t6c = -r0ij * t6 * t6, t8c = -b8 * t8 * t8;
+        double t6uc = 6.0 * u * u * r0ij * pow(t6, 3), t8uc = 8.0 * pow(u, 3) * b8 * pow(t8, 3);
+        double t6cc = 2.0 * r0ij * r0ij * pow(t6, 3), t8cc = 2.0 * b8 * b8 * pow(t8, 3);
+        double pz = p->s6 * t6 + p->s8 * rrij * t8;
+        double pu = p->s6 * t6u + p->s8 * rrij * t8u;
+        double puu = p->s6 * t6uu + p->s8 * rrij * t8uu;
+        double pc = p->s6 * t6c + p->s8 * rrij * t8c;
+        double puc = p->s6 * t6uc + p->s8 * rrij * t8uc;
+        double pcc = p->s6 * t6cc + p->s8 * rrij * t8cc;
+        *e = -c6 * pz;
+        *er = -c6 * pu;
+        *ec = -pz - c6 * pc;
+        *err = -c6 * puu;
+        *erc = -pu - c6 * puc;
+        *ecc = -2.0 * pc - c6 * pcc;
+        return;
+    }
+    }
+    *e = -c6 * pv;
+    *er = -c6 * pd;
+    *ec = -pv;
+    *err = -c6 * pdd;
+    *erc = -pd;
+}
+
+/* model.f90:201-287 weight_references with the second derivative (gwd2cn). */
+static void weights2(int nat, const int *num, const double *cn, int mref, double *gw, double *gwd, double *gwd2)
+{
+    const double wf2 = 8.0;
+    memset(gw, 0, sizeof(double) * nat * mref);
+    memset(gwd, 0, sizeof(double) * nat * mref);
+    memset(gwd2, 0, sizeof(double) * nat * mref);
+    for (int iat = 0; iat < nat; ++iat) {
+        int z = num[iat], nr = g_nref[z - 1];
+        double gwt[MAX_REF], norm = 0.0, dnorm = 0.0, d2norm = 0.0, maxcn = -1.0e300;
+        for (int r = 0; r < nr; ++r) {
+            double d = cn[iat] - g_refcn[z - 1][r];
+            double w = exp(-4.0 * (d * d));
+            gwt[r] = w;
+            double dcn = g_refcn[z - 1][r] - cn[iat];
+            norm += w;
+            dnorm += wf2 * dcn * w;
+            d2norm += (wf2 * wf2 * dcn * dcn - wf2) * w;
+            if (g_refcn[z - 1][r] > maxcn) maxcn = g_refcn[z - 1][r];
+        }
+        norm = 1.0 / norm;
+        for (int r = 0; r < nr; ++r) {
+            double expw = gwt[r], dcn = g_refcn[z - 1][r] - cn[iat];
+            double expd = wf2 * dcn * expw, expdd = (wf2 * wf2 * dcn * dcn - wf2) * expw;
+            double gwk = expw * norm;
+            if (is_exceptional(gwk)) gwk = (maxcn == g_refcn[z - 1][r]) ? 1.0 : 0.0;
+            gw[iat * mref + r] = gwk;
+            double dgwk = expd * norm - expw * dnorm * norm * norm;
+            if (is_exceptional(dgwk)) dgwk = 0.0;
+            gwd[iat * mref + r] = dgwk;
+            double d2gwk = expdd * norm - 2.0 * expd * dnorm * norm * norm - expw * d2norm * norm * norm
+                         + 2.0 * expw * dnorm * dnorm * norm * norm * norm;
+            if (is_exceptional(d2gwk)) d2gwk = 0.0;
+            gwd2[iat * mref + r] = d2gwk;
+        }
+    }
+}
+
+/* model.f90:384-428 get_atomic_c6, second-derivative branch (Fortran order). */
+static void c6_second(int nat, const int *num, int mref, const double *gw, const double *gwd, const double *gwd2,
+                      const int *ghost, double *c6, double *dc6, double *d2c6, double *d2c6ij)
+{
+    size_t nn = (size_t)nat * nat;
+    memset(c6, 0, sizeof(double) * nn); memset(dc6, 0, sizeof(double) * nn);
+    memset(d2c6, 0, sizeof(double) * nn); memset(d2c6ij, 0, sizeof(double) * nn);
+    for (int iat = 0; iat < nat; ++iat) {
+        if (ghost && ghost[iat]) continue;
+        int zi = num[iat];
+        for (int jat = 0; jat <= iat; ++jat) {
+            if (ghost && ghost[jat]) continue;
+            int zj = num[jat];
+            double a = 0, di = 0, dj = 0, ddi = 0, ddj = 0, mix = 0;
+            for (int ir = 0; ir < g_nref[zi - 1]; ++ir)
+                for (int jr = 0; jr < g_nref[zj - 1]; ++jr) {
+                    double ref = d3o_get_c6(ir + 1, jr + 1, zi, zj);
+                    double wi = gw[iat * mref + ir], wj = gw[jat * mref + jr];
+                    double dwi = gwd[iat * mref + ir], dwj = gwd[jat * mref + jr];
+                    a += wi * wj * ref;
+                    di += dwi * wj * ref;
+                    dj += wi * dwj * ref;
+                    ddi += gwd2[iat * mref + ir] * wj * ref;
+                    ddj += wi * gwd2[jat * mref + jr] * ref;
+                    mix += dwi * dwj * ref;
+                }
+            c6[FIDX(iat, jat)] = a; c6[FIDX(jat, iat)] = a;
+            dc6[FIDX(iat, jat)] = di; dc6[FIDX(jat, iat)] = dj;
+            d2c6[FIDX(iat, jat)] = ddi; d2c6[FIDX(jat, iat)] = ddj;
+            d2c6ij[FIDX(iat, jat)] = mix; d2c6ij[FIDX(jat, iat)] = mix;
+        }
+    }
+}
+
+/*
+ * Derivatives of the coordination numbers (the mctc-lib exponential count of
+ * the same pairs as ncoord.f90:86-144): dr[(3 i + c) nat + k] = dCN_k / dR_ic.
+ * A pair term (i, j != i, T) with v = R_i - R_j - T adds dcf/dr v/r to the
+ * rows of i and subtracts it from the rows of j, in the columns of i and j;
+ * self images (j = i) do not depend on the positions.
+ */
+static void cn_dcndr(int nat, const int *num, const double *xyz, int ntr, const double *trans, double cutoff,
+                     double *dr)
+{
+    double cutoff2 = cutoff * cutoff;
+    memset(dr, 0, sizeof(double) * 3 * (size_t)nat * nat);
+    for (int iat = 0; iat < nat; ++iat)
+        for (int jat = 0; jat < iat; ++jat) {
+            double rc = d3o_rcov(num[iat]) + d3o_rcov(num[jat]);
+            for (int itr = 0; itr < ntr; ++itr) {
+                const double *t = trans + 3 * itr;
+                double v[3];
+                for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + t[c]);
+                double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+                double r1 = sqrt(r2);
+                double cf = 1.0 / (1.0 + exp(-KCN * (rc / r1 - 1.0)));
+                double dcf = -cf * (1.0 - cf) * KCN * rc / r2;
+                for (int c = 0; c < 3; ++c) {
+                    double g = dcf * v[c] / r1;
+                    dr[(size_t)(3 * iat + c) * nat + iat] += g;
+                    dr[(size_t)(3 * iat + c) * nat + jat] += g;
+                    dr[(size_t)(3 * jat + c) * nat + iat] -= g;
+                    dr[(size_t)(3 * jat + c) * nat + jat] -= g;
+                }
+            }
+        }
+}
+
+/* Hessian element (row, col) of the 3 nat x 3 nat matrix, Fortran order. */
+#define HIDX(r, c) ((size_t)(r) + (size_t)(c) * (size_t)ndim)
+
+/*
+ * damping.f90:393-539 get_dispersion2_hessian: the two-body second
+ * derivatives at fixed CN, and dE/dCN, d2E/(dR dCN), d2E/dCN2 through C6.
+ * dEdcndr is [3 nat][nat] and dEdcndcn [nat][nat], both row-major.
+ */
+static void disp2_hessian(const d3o_param *p, int nat, const int *num, const double *xyz, int ntr,
+                          const double *trans, double cutoff, double width, const double *c6, const double *dc6,
+                          const double *d2c6, const double *d2c6ij, double *hess, double *dEdcn,
+                          double *dEdcndr, double *dEdcndcn)
+{
+    int ndim = 3 * nat;
+    double cutoff2 = cutoff * cutoff;
+    int *id = (int *)malloc(sizeof(int) * (nat > 0 ? nat : 1));
+    species_ids(nat, num, id);
+    for (int iat = 0; iat < nat; ++iat) {
+        for (int jat = 0; jat <= iat; ++jat) {
+            double c6ij = c6[FIDX(jat, iat)];
+            double fac, cnd[2], cnd2[2][2];
+            int nc, cnat[2];
+            if (iat != jat) {
+                fac = 1.0; nc = 2; cnat[0] = iat; cnat[1] = jat;
+                cnd[0] = dc6[FIDX(iat, jat)]; cnd[1] = dc6[FIDX(jat, iat)];
+                cnd2[0][0] = d2c6[FIDX(iat, jat)]; cnd2[1][1] = d2c6[FIDX(jat, iat)];
+                cnd2[0][1] = d2c6ij[FIDX(iat, jat)]; cnd2[1][0] = d2c6ij[FIDX(iat, jat)];
+            } else {
+                fac = 0.5; nc = 1; cnat[0] = iat;
+                cnd[0] = 2.0 * dc6[FIDX(iat, iat)];
+                cnd2[0][0] = 2.0 * d2c6[FIDX(iat, iat)] + 2.0 * d2c6ij[FIDX(iat, iat)];
+            }
+            for (int jtr = 0; jtr < ntr; ++jtr) {
+                const double *t = trans + 3 * jtr;
+                double v[3];
+                for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + t[c]);
+                double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (r2 > cutoff2 || r2 < EPS) continue;
+                double e0, e0r, e0c, e0rr, e0rc, e0cc, sw, swr, swrr;
+                damping_kernel(p, num[iat], num[jat], id[iat], id[jat], r2, c6ij, &e0, &e0r, &e0c, &e0rr, &e0rc,
+                               &e0cc);
+                smooth_cutoff_r2(r2, cutoff, width, &sw, &swr, &swrr);
+                double fr = fac * (swr * e0 + sw * e0r);
+                double frr = fac * (swrr * e0 + 2.0 * swr * e0r + sw * e0rr);
+                double fc = fac * sw * e0c;
+                double frc = fac * (swr * e0c + sw * e0rc);
+                double fcc = fac * sw * e0cc;
+                double dr2[3][2] = {{0}};
+                if (iat != jat) {
+                    double block[3][3];
+                    for (int a = 0; a < 3; ++a) {
+                        for (int b = 0; b < 3; ++b) block[a][b] = 4.0 * frr * v[a] * v[b];
+                        block[a][a] += 2.0 * fr;
+                    }
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            hess[HIDX(3 * iat + a, 3 * iat + b)] += block[a][b];
+                            hess[HIDX(3 * jat + a, 3 * jat + b)] += block[a][b];
+                            hess[HIDX(3 * iat + a, 3 * jat + b)] -= block[a][b];
+                            hess[HIDX(3 * jat + a, 3 * iat + b)] -= block[a][b];
+                        }
+                    for (int c = 0; c < 3; ++c) { dr2[c][0] = 2.0 * v[c]; dr2[c][1] = -2.0 * v[c]; }
+                }
+                for (int ia = 0; ia < nc; ++ia) {
+                    dEdcn[cnat[ia]] += fc * cnd[ia];
+                    for (int ib = 0; ib < nc; ++ib)
+                        dEdcndcn[(size_t)cnat[ia] * nat + cnat[ib]] += fcc * cnd[ia] * cnd[ib] + fc * cnd2[ia][ib];
+                }
+                if (iat != jat)
+                    for (int ia = 0; ia < nc; ++ia)
+                        for (int c = 0; c < 3; ++c) {
+                            dEdcndr[(size_t)(3 * iat + c) * nat + cnat[ia]] += frc * dr2[c][0] * cnd[ia];
+                            dEdcndr[(size_t)(3 * jat + c) * nat + cnat[ia]] += frc * dr2[c][1] * cnd[ia];
+                        }
+            }
+        }
+    }
+    free(id);
+}
+
+/*
+ * atm.f90:38-429 get_atm_dispersion_hessian: every triple term as a function
+ * K(u1, u2, u3) of the squared distances u1 = |R_j - R_i|^2, u2 = |R_k - R_i|^2,
+ * u3 = |R_k - R_j|^2 (switching function, angular factor, zero damping), its
+ * gradient and Hessian in u, contracted with du/dR; plus the C6 derivatives.
+ */
+static void atm_hessian(int nat, const int *num, const double *xyz, int ntr, const double *trans, double cutoff,
+                        double width, double s9, double rs9, double alp, const double *c6, const double *dc6,
+                        const double *d2c6, const double *d2c6ij, double *hess, double *dEdcn, double *dEdcndr,
+                        double *dEdcndcn)
+{
+    if (fabs(s9) < EPS) return;
+    static const int sigp[3][3] = {{-1, -1, 0}, {1, 0, -1}, {0, 1, 1}};  /* sigp(atom, pair) */
+    static const double cl[3][3] = {{1, -1, 1}, {1, 1, -1}, {-1, 1, 1}};  /* cl(l, pair) */
+    int ndim = 3 * nat;
+    double cutoff2 = cutoff * cutoff, alp3 = alp / 3.0, aexp = 0.5 * alp3;
+    for (int iat = 0; iat < nat; ++iat) {
+        int zi = num[iat];
+        for (int jat = 0; jat <= iat; ++jat) {
+            int zj = num[jat];
+            for (int jtr = 0; jtr < ntr; ++jtr) {
+                double vec[3][3], u[3];  /* vec[pair][xyz] */
+                for (int c = 0; c < 3; ++c) vec[0][c] = xyz[3 * jat + c] + trans[3 * jtr + c] - xyz[3 * iat + c];
+                u[0] = vec[0][0] * vec[0][0] + vec[0][1] * vec[0][1] + vec[0][2] * vec[0][2];
+                if (u[0] > cutoff2 || u[0] < EPS) continue;
+                for (int kat = 0; kat <= jat; ++kat) {
+                    int zk = num[kat];
+                    double triple = triple_scale(iat, jat, kat);
+                    double r0 = rs9 * d3o_rvdw(zj, zi) * rs9 * d3o_rvdw(zk, zi) * rs9 * d3o_rvdw(zk, zj);
+                    for (int ktr = 0; ktr < ntr; ++ktr) {
+                        for (int c = 0; c < 3; ++c)
+                            vec[1][c] = xyz[3 * kat + c] + trans[3 * ktr + c] - xyz[3 * iat + c];
+                        u[1] = vec[1][0] * vec[1][0] + vec[1][1] * vec[1][1] + vec[1][2] * vec[1][2];
+                        if (u[1] > cutoff2 || u[1] < EPS) continue;
+                        for (int c = 0; c < 3; ++c) vec[2][c] = vec[1][c] - vec[0][c];
+                        u[2] = vec[2][0] * vec[2][0] + vec[2][1] * vec[2][1] + vec[2][2] * vec[2][2];
+                        if (u[2] > cutoff2 || u[2] < EPS) continue;
+                        double c6t[3] = {c6[FIDX(jat, iat)], c6[FIDX(kat, iat)], c6[FIDX(kat, jat)]};
+                        if (fabs(c6t[0]) < EPS || fabs(c6t[1]) < EPS || fabs(c6t[2]) < EPS) continue;
+                        double swp[3], dswp[3], d2swp[3];
+                        for (int ip = 0; ip < 3; ++ip) smooth_cutoff_r2(u[ip], cutoff, width, &swp[ip], &dswp[ip], &d2swp[ip]);
+                        double sval = swp[0] * swp[1] * swp[2], sp[3], spq[3][3];
+                        sp[0] = dswp[0] * swp[1] * swp[2];
+                        sp[1] = swp[0] * dswp[1] * swp[2];
+                        sp[2] = swp[0] * swp[1] * dswp[2];
+                        spq[0][0] = d2swp[0] * swp[1] * swp[2];
+                        spq[1][1] = swp[0] * d2swp[1] * swp[2];
+                        spq[2][2] = swp[0] * swp[1] * d2swp[2];
+                        spq[0][1] = spq[1][0] = dswp[0] * dswp[1] * swp[2];
+                        spq[0][2] = spq[2][0] = dswp[0] * swp[1] * dswp[2];
+                        spq[1][2] = spq[2][1] = swp[0] * dswp[1] * dswp[2];
+                        double ww = u[0] * u[1] * u[2];
+                        double wp_[3] = {u[1] * u[2], u[0] * u[2], u[0] * u[1]};
+                        double wpq[3][3] = {{0, u[2], u[1]}, {u[2], 0, u[0]}, {u[1], u[0], 0}};
+                        double lval[3] = {u[0] + u[2] - u[1], u[0] - u[2] + u[1], -u[0] + u[2] + u[1]};
+                        double nval = lval[0] * lval[1] * lval[2], np_[3], npq[3][3];
+                        for (int ip = 0; ip < 3; ++ip)
+                            np_[ip] = cl[0][ip] * lval[1] * lval[2] + cl[1][ip] * lval[0] * lval[2]
+                                    + cl[2][ip] * lval[0] * lval[1];
+                        for (int ip = 0; ip < 3; ++ip)
+                            for (int iq = 0; iq < 3; ++iq) {
+                                double tmp = 0.0;
+                                for (int il = 0; il < 3; ++il)
+                                    for (int im = 0; im < 3; ++im) {
+                                        if (il == im) continue;
+                                        tmp += cl[il][ip] * cl[im][iq] * lval[3 - il - im];
+                                    }
+                                npq[ip][iq] = tmp;
+                            }
+                        double gv = pow(ww, -2.5), gd = -2.5 * pow(ww, -3.5), gdd = 8.75 * pow(ww, -4.5);
+                        double hv = pow(ww, -1.5), hd = -1.5 * pow(ww, -2.5), hdd = 3.75 * pow(ww, -3.5);
+                        double ang = 0.375 * nval * gv + hv, angp[3], angpq[3][3];
+                        for (int ip = 0; ip < 3; ++ip)
+                            angp[ip] = 0.375 * (np_[ip] * gv + nval * gd * wp_[ip]) + hd * wp_[ip];
+                        for (int ip = 0; ip < 3; ++ip)
+                            for (int iq = 0; iq < 3; ++iq)
+                                angpq[ip][iq] = 0.375 * (npq[ip][iq] * gv + np_[ip] * gd * wp_[iq] + np_[iq] * gd * wp_[ip]
+                                                         + nval * (gdd * wp_[ip] * wp_[iq] + gd * wpq[ip][iq]))
+                                              + hdd * wp_[ip] * wp_[iq] + hd * wpq[ip][iq];
+                        double cval = 6.0 * pow(r0, alp3), tval = cval * pow(ww, -aexp), tp[3], tpq[3][3];
+                        for (int ip = 0; ip < 3; ++ip) tp[ip] = -aexp * tval * wp_[ip] / ww;
+                        for (int ip = 0; ip < 3; ++ip)
+                            for (int iq = 0; iq < 3; ++iq)
+                                tpq[ip][iq] = aexp * (aexp + 1.0) * tval * wp_[ip] * wp_[iq] / (ww * ww)
+                                            - aexp * tval * wpq[ip][iq] / ww;
+                        double fd = 1.0 / (1.0 + tval), fdp[3], fdpq[3][3];
+                        for (int ip = 0; ip < 3; ++ip) fdp[ip] = -tp[ip] * fd * fd;
+                        for (int ip = 0; ip < 3; ++ip)
+                            for (int iq = 0; iq < 3; ++iq)
+                                fdpq[ip][iq] = -tpq[ip][iq] * fd * fd + 2.0 * tp[ip] * tp[iq] * fd * fd * fd;
+                        double av = ang * fd, ap[3], apq[3][3];
+                        for (int ip = 0; ip < 3; ++ip) ap[ip] = angp[ip] * fd + ang * fdp[ip];
+                        for (int ip = 0; ip < 3; ++ip)
+                            for (int iq = 0; iq < 3; ++iq)
+                                apq[ip][iq] = angpq[ip][iq] * fd + angp[ip] * fdp[iq] + angp[iq] * fdp[ip]
+                                            + ang * fdpq[ip][iq];
+                        double kv = sval * av, kp[3], kpq[3][3];
+                        for (int ip = 0; ip < 3; ++ip) kp[ip] = sp[ip] * av + sval * ap[ip];
+                        for (int ip = 0; ip < 3; ++ip)
+                            for (int iq = 0; iq < 3; ++iq)
+                                kpq[ip][iq] = spq[ip][iq] * av + sp[ip] * ap[iq] + sp[iq] * ap[ip] + sval * apq[ip][iq];
+                        double qv = sqrt(fabs(c6t[0] * c6t[1] * c6t[2])), qm[3], qmn[3][3];
+                        for (int ip = 0; ip < 3; ++ip) qm[ip] = 0.5 * qv / c6t[ip];
+                        for (int ip = 0; ip < 3; ++ip)
+                            for (int iq = 0; iq < 3; ++iq)
+                                qmn[ip][iq] = (ip == iq) ? -0.25 * qv / (c6t[ip] * c6t[ip])
+                                                         : 0.25 * qv / (c6t[ip] * c6t[iq]);
+                        double pref = s9 * triple;
+                        int at[3] = {iat, jat, kat};
+                        double grad[3][3][3];  /* grad[atom][pair][xyz] = du_pair / dR_atom */
+                        for (int ia = 0; ia < 3; ++ia)
+                            for (int ip = 0; ip < 3; ++ip)
+                                for (int c = 0; c < 3; ++c) grad[ia][ip][c] = 2.0 * sigp[ia][ip] * vec[ip][c];
+                        double pq = pref * qv, gk[3][3][3], dg[3][3];
+                        for (int ia = 0; ia < 3; ++ia)
+                            for (int c = 0; c < 3; ++c)
+                                for (int iq = 0; iq < 3; ++iq) {
+                                    double tmp = 0.0;
+                                    for (int ip = 0; ip < 3; ++ip) tmp += kpq[ip][iq] * grad[ia][ip][c];
+                                    gk[ia][iq][c] = pq * tmp;
+                                }
+                        for (int ia = 0; ia < 3; ++ia)
+                            for (int ib = 0; ib < 3; ++ib) {
+                                double tmp = 0.0;
+                                for (int ip = 0; ip < 3; ++ip) tmp += kp[ip] * sigp[ia][ip] * sigp[ib][ip];
+                                dg[ia][ib] = 2.0 * pq * tmp;
+                            }
+                        for (int ia = 0; ia < 3; ++ia)
+                            for (int ib = 0; ib < 3; ++ib)
+                                for (int ic = 0; ic < 3; ++ic)
+                                    for (int jc = 0; jc < 3; ++jc) {
+                                        double tmp = 0.0;
+                                        for (int iq = 0; iq < 3; ++iq) tmp += gk[ia][iq][ic] * grad[ib][iq][jc];
+                                        if (ic == jc) tmp += dg[ia][ib];
+                                        hess[HIDX(3 * at[ia] + ic, 3 * at[ib] + jc)] += tmp;
+                                    }
+                        double ec[3], ecc[3][3], euc[3][3], egr[3][3][3];
+                        for (int ip = 0; ip < 3; ++ip) {
+                            ec[ip] = pref * qm[ip] * kv;
+                            for (int iq = 0; iq < 3; ++iq) {
+                                ecc[ip][iq] = pref * qmn[ip][iq] * kv;
+                                euc[iq][ip] = pref * qm[ip] * kp[iq];
+                            }
+                        }
+                        for (int ia = 0; ia < 3; ++ia)
+                            for (int c = 0; c < 3; ++c)
+                                for (int im = 0; im < 3; ++im) {
+                                    double tmp = 0.0;
+                                    for (int ip = 0; ip < 3; ++ip) tmp += euc[ip][im] * grad[ia][ip][c];
+                                    egr[ia][im][c] = tmp;
+                                }
+                        int pat[3][2] = {{iat, jat}, {iat, kat}, {jat, kat}};
+                        int nent = 0, ent_atom[6], ent_pair[6];
+                        double ent_coef[6];
+                        for (int ip = 0; ip < 3; ++ip) {
+                            int a0 = pat[ip][0], a1 = pat[ip][1];
+                            if (a0 != a1) {
+                                ent_atom[nent] = a0; ent_pair[nent] = ip; ent_coef[nent++] = dc6[FIDX(a0, a1)];
+                                ent_atom[nent] = a1; ent_pair[nent] = ip; ent_coef[nent++] = dc6[FIDX(a1, a0)];
+                            } else {
+                                ent_atom[nent] = a0; ent_pair[nent] = ip; ent_coef[nent++] = 2.0 * dc6[FIDX(a0, a0)];
+                            }
+                        }
+                        for (int ie = 0; ie < nent; ++ie) {
+                            dEdcn[ent_atom[ie]] += ec[ent_pair[ie]] * ent_coef[ie];
+                            for (int jf = 0; jf < nent; ++jf)
+                                dEdcndcn[(size_t)ent_atom[ie] * nat + ent_atom[jf]] +=
+                                    ecc[ent_pair[ie]][ent_pair[jf]] * ent_coef[ie] * ent_coef[jf];
+                            for (int ia = 0; ia < 3; ++ia)
+                                for (int c = 0; c < 3; ++c)
+                                    dEdcndr[(size_t)(3 * at[ia] + c) * nat + ent_atom[ie]] +=
+                                        egr[ia][ent_pair[ie]][c] * ent_coef[ie];
+                        }
+                        for (int ip = 0; ip < 3; ++ip) {
+                            int a0 = pat[ip][0], a1 = pat[ip][1];
+                            if (a0 != a1) {
+                                dEdcndcn[(size_t)a0 * nat + a0] += ec[ip] * d2c6[FIDX(a0, a1)];
+                                dEdcndcn[(size_t)a1 * nat + a1] += ec[ip] * d2c6[FIDX(a1, a0)];
+                                dEdcndcn[(size_t)a0 * nat + a1] += ec[ip] * d2c6ij[FIDX(a0, a1)];
+                                dEdcndcn[(size_t)a1 * nat + a0] += ec[ip] * d2c6ij[FIDX(a0, a1)];
+                            } else {
+                                dEdcndcn[(size_t)a0 * nat + a0] +=
+                                    ec[ip] * (2.0 * d2c6[FIDX(a0, a0)] + 2.0 * d2c6ij[FIDX(a0, a0)]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ncoord.f90:236-306 add_coordination_number_hessian: dE/dCN times the
+ * second derivatives of the coordination numbers. */
+static void cn_hessian(int nat, const int *num, const double *xyz, int ntr, const double *trans, double cutoff,
+                       const double *dEdcn, double *hess)
+{
+    int ndim = 3 * nat;
+    double cutoff2 = cutoff * cutoff;
+    for (int iat = 0; iat < nat; ++iat)
+        for (int jat = 0; jat < iat; ++jat) {
+            double rc = d3o_rcov(num[iat]) + d3o_rcov(num[jat]);
+            double dEdcnij = dEdcn[iat] + dEdcn[jat];
+            for (int itr = 0; itr < ntr; ++itr) {
+                double v[3];
+                for (int c = 0; c < 3; ++c) v[c] = xyz[3 * iat + c] - (xyz[3 * jat + c] + trans[3 * itr + c]);
+                double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                if (r2 > cutoff2 || r2 < 1.0e-12) continue;
+                double r1 = sqrt(r2);
+                double expterm = exp(-KCN * (rc / r1 - 1.0));
+                double cf = 1.0 / (1.0 + expterm), tmp = cf * (1.0 - cf);
+                double dcf = -tmp * KCN * rc / r2;
+                double d2cf = tmp * (1.0 - 2.0 * cf) * (KCN * rc) * (KCN * rc) / (r2 * r2)
+                            + 2.0 * tmp * KCN * rc / (r2 * r1);
+                double block[3][3];
+                for (int a = 0; a < 3; ++a) {
+                    for (int b = 0; b < 3; ++b)
+                        block[a][b] = dEdcnij * (d2cf * v[a] * v[b] / r2 - dcf * v[a] * v[b] / (r2 * r1));
+                    block[a][a] += dEdcnij * dcf / r1;
+                }
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) {
+                        hess[HIDX(3 * iat + a, 3 * iat + b)] += block[a][b];
+                        hess[HIDX(3 * jat + a, 3 * jat + b)] += block[a][b];
+                        hess[HIDX(3 * iat + a, 3 * jat + b)] -= block[a][b];
+                        hess[HIDX(3 * jat + a, 3 * iat + b)] -= block[a][b];
+                    }
+            }
+        }
+}
+
+/*
+ * disp.f90:236-308 get_dispersion_hessian (the energy as in
+ * d3o_get_dispersion): hessian is 3 nat x 3 nat in Fortran order, the array
+ * the C API of the reference returns.  Same arguments as d3o_get_dispersion.
+ */
+int d3o_get_dispersion_hessian(int nat, const int *num, const double *xyz, const double *lattice,
+                               const int *periodic, int kind, const double *par, const double *cut,
+                               const int *ghost, double *energy, double *hessian)
+{
+    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5], par[6], par[7], par[8]};
+    int ndim = 3 * nat, mref = 0;
+    for (int i = 0; i < nat; ++i)
+        if (g_nref[num[i] - 1] > mref) mref = g_nref[num[i] - 1];
+    double zero_lat[9] = {0};
+    int zero_pbc[3] = {0, 0, 0};
+    if (!lattice) lattice = zero_lat;
+    if (!periodic) periodic = zero_pbc;
+    if (energy)
+        d3o_get_dispersion(nat, num, xyz, lattice, periodic, kind, par, cut, ghost, energy, NULL, NULL, NULL,
+                           NULL, NULL);
+    size_t nn = (size_t)nat * nat;
+    double *cn = (double *)calloc(nat, sizeof(double));
+    double *dr = (double *)calloc(3 * nn, sizeof(double));
+    double *trans;
+    int ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+    d3o_cn(nat, num, xyz, ntr, trans, cut[0], cn);
+    cn_dcndr(nat, num, xyz, ntr, trans, cut[0], dr);
+    free(trans);
+    double *gw = (double *)calloc((size_t)nat * mref, sizeof(double));
+    double *gwd = (double *)calloc((size_t)nat * mref, sizeof(double));
+    double *gwd2 = (double *)calloc((size_t)nat * mref, sizeof(double));
+    weights2(nat, num, cn, mref, gw, gwd, gwd2);
+    double *c6 = (double *)calloc(nn, sizeof(double)), *dc6 = (double *)calloc(nn, sizeof(double));
+    double *d2c6 = (double *)calloc(nn, sizeof(double)), *d2c6ij = (double *)calloc(nn, sizeof(double));
+    c6_second(nat, num, mref, gw, gwd, gwd2, ghost, c6, dc6, d2c6, d2c6ij);
+    memset(hessian, 0, sizeof(double) * (size_t)ndim * ndim);
+    double *dEdcn = (double *)calloc(nat, sizeof(double));
+    double *dEdcndr = (double *)calloc(3 * nn, sizeof(double));
+    double *dEdcndcn = (double *)calloc(nn, sizeof(double));
+    if (two_body_active(&p)) {
+        ntr = lattice_alloc(periodic, lattice, cut[1], &trans);
+        disp2_hessian(&p, nat, num, xyz, ntr, trans, cut[1], cut[3], c6, dc6, d2c6, d2c6ij, hessian, dEdcn,
+                      dEdcndr, dEdcndcn);
+        free(trans);
+    }
+    ntr = lattice_alloc(periodic, lattice, cut[2], &trans);
+    atm_hessian(nat, num, xyz, ntr, trans, cut[2], cut[4], p.s9, 4.0 / 3.0, p.alp + 2.0, c6, dc6, d2c6, d2c6ij,
+                hessian, dEdcn, dEdcndr, dEdcndcn);
+    free(trans);
+    ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+    cn_hessian(nat, num, xyz, ntr, trans, cut[0], dEdcn, hessian);
+    free(trans);
+    /* hessian += dEdcndr dr^T + dr dEdcndr^T + dr dEdcndcn dr^T (disp.f90:303-306) */
+    double *tmp = (double *)calloc(3 * nn, sizeof(double));  /* dEdcndcn dr^T as [nat][3 nat] */
+    for (int k = 0; k < nat; ++k)
+        for (int y = 0; y < ndim; ++y) {
+            double s = 0.0;
+            for (int l = 0; l < nat; ++l) s += dEdcndcn[(size_t)k * nat + l] * dr[(size_t)y * nat + l];
+            tmp[(size_t)k * ndim + y] = s;
+        }
+    for (int y = 0; y < ndim; ++y)
+        for (int x = 0; x < ndim; ++x) {
+            double s = 0.0;
+            for (int k = 0; k < nat; ++k)
+                s += dEdcndr[(size_t)x * nat + k] * dr[(size_t)y * nat + k]
+                   + dr[(size_t)x * nat + k] * dEdcndr[(size_t)y * nat + k]
+                   + dr[(size_t)x * nat + k] * tmp[(size_t)k * ndim + y];
+            hessian[HIDX(x, y)] += s;
+        }
+    free(tmp);
+    free(cn); free(dr); free(gw); free(gwd); free(gwd2); free(c6); free(dc6); free(d2c6); free(d2c6ij);
+    free(dEdcn); free(dEdcndr); free(dEdcndcn);
+    return 0;
+}
+
 /* Count of the terms the reference loops visit (the work units). */
 void d3o_count_terms(int nat, const double *xyz, const double *lattice, const int *periodic,
                      const double *cut, long long *n_cn, long long *n_pair, long long *n_triple)

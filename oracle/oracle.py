@@ -221,6 +221,23 @@ def pairwise(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, par
     return e2.T.copy(), e3.T.copy()
 
 
+def hessian(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, param=None, cutoff=DEFAULT_CUTOFF,
+            ghost=None):
+    """FP64 oracle of get_dispersion_hessian (disp.f90:236-308): energy and
+    the (3n, 3n) analytic Hessian, [3 i + a, 3 j + b] = d2E / dR_ia dR_jb."""
+    L = lib()
+    num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
+    par = param_vector(kind, param)
+    cut = np.array(cutoff, dtype=np.float64)
+    n = len(num)
+    gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
+    energy = np.zeros(1)
+    h = np.zeros((3 * n, 3 * n))
+    L.d3o_get_dispersion_hessian(n, _ptr(num, _ip), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip), kind, _ptr(par),
+                                 _ptr(cut), _ptr(gh, _ip), _ptr(energy), _ptr(h))
+    return dict(energy=float(energy[0]), hessian=h.T.copy())
+
+
 def count_terms(positions, lattice=None, periodic=None, cutoff=DEFAULT_CUTOFF, triples=True):
     """Exact counts of CN, two-body and ATM terms the reference loops visit."""
     L = lib()

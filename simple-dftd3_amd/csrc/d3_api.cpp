@@ -48,8 +48,8 @@ struct _dftd3_model {
   double* d_xyz = nullptr;
   double* d_out = nullptr;  // energy | sigma[9] | gradient[nat][3]
   size_t cap_atoms = 0;
-  double* d_pairs = nullptr;  // pairwise energies: pair2[nat][nat] | pair3[nat][nat]
-  size_t cap_pairs = 0;
+  double* d_pairs = nullptr;  // pairwise energies pair2 | pair3, or the Hessian
+  size_t cap_pairs = 0;       // doubles allocated at d_pairs
   ~_dftd3_model() {
     if (d_xyz) (void)hipFree(d_xyz);
     if (d_out) (void)hipFree(d_out);
@@ -179,6 +179,16 @@ void ensure_model_buffers(_dftd3_model* m, int nat) {
     d3::hip_check(hipMalloc(&m->d_out, sizeof(double) * (10 + 3 * std::max(nat, 1))), "hipMalloc");
     m->cap_atoms = nat;
   }
+}
+
+// Device buffer of n x n results (pair energies, Hessian) of the host entry points.
+void ensure_result_buffer(_dftd3_model* m, size_t words) {
+  if (words <= m->cap_pairs && m->d_pairs) return;
+  if (m->d_pairs) d3::hip_check(hipFree(m->d_pairs), "hipFree");
+  m->d_pairs = nullptr;
+  m->cap_pairs = 0;
+  d3::hip_check(hipMalloc(&m->d_pairs, sizeof(double) * std::max<size_t>(words, 1)), "hipMalloc");
+  m->cap_pairs = words;
 }
 
 d3::SysParam make_sys_param(const _dftd3_param* p, const double* cutoff) {
@@ -528,12 +538,7 @@ void dftd3_get_pairwise_dispersion(dftd3_error error, dftd3_structure mol, dftd3
     const int nat = mol->nat;
     ensure_model_buffers(disp, nat);
     const size_t nn = static_cast<size_t>(nat) * static_cast<size_t>(nat);
-    if (nn > disp->cap_pairs || !disp->d_pairs) {
-      if (disp->d_pairs) d3::hip_check(hipFree(disp->d_pairs), "hipFree");
-      disp->d_pairs = nullptr;
-      d3::hip_check(hipMalloc(&disp->d_pairs, sizeof(double) * 2 * nn), "hipMalloc");
-      disp->cap_pairs = nn;
-    }
+    ensure_result_buffer(disp, 2 * nn);
     std::vector<d3::SystemDesc> sys(1);
     sys[0].natoms = nat;
     std::memcpy(sys[0].lattice, mol->lattice, sizeof mol->lattice);
@@ -553,6 +558,43 @@ void dftd3_get_pairwise_dispersion(dftd3_error error, dftd3_structure mol, dftd3
                                  disp->stream),
                   "download results");
     d3::hip_check(hipStreamSynchronize(disp->stream), "hipStreamSynchronize");
+  });
+}
+
+void dftd3_get_dispersion_hessian(dftd3_error error, dftd3_structure mol, dftd3_model disp, dftd3_param param,
+                                  double* energy, double* hessian) {
+  // api.f90:1115-1176
+  if (!error) return;
+  if (!mol) return fatal(error, "Molecular structure data is missing");
+  if (!disp) return fatal(error, "Dispersion model is missing");
+  if (!param) return fatal(error, "Damping parameters are missing");
+  if (!disp->has_structure || disp->nat != mol->nat)
+    return fatal(error, "Dispersion model does not match the molecular structure");
+  if (!hessian) return fatal(error, "Hessian array is missing");
+  guarded(error, [&] {
+    const int nat = mol->nat;
+    ensure_model_buffers(disp, nat);
+    const size_t nh = static_cast<size_t>(3 * nat) * static_cast<size_t>(3 * nat);
+    ensure_result_buffer(disp, nh);
+    std::vector<d3::SystemDesc> sys(1);
+    sys[0].natoms = nat;
+    std::memcpy(sys[0].lattice, mol->lattice, sizeof mol->lattice);
+    std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
+    sys[0].param = make_sys_param(param, disp->cutoff);
+    d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
+                                 disp->stream),
+                  "upload positions");
+    bool any_ghost = false;
+    for (int g : disp->ghost) any_ghost |= g != 0;
+    disp->engine->run_hessian(sys, mol->num.data(), disp->d_xyz, disp->d_out, disp->d_pairs, disp->stream,
+                              any_ghost ? disp->ghost.data() : nullptr);
+    double e = 0.0;
+    d3::hip_check(hipMemcpyAsync(&e, disp->d_out, sizeof(double), hipMemcpyDeviceToHost, disp->stream),
+                  "download results");
+    d3::hip_check(hipMemcpyAsync(hessian, disp->d_pairs, sizeof(double) * nh, hipMemcpyDeviceToHost, disp->stream),
+                  "download results");
+    d3::hip_check(hipStreamSynchronize(disp->stream), "hipStreamSynchronize");
+    if (energy) *energy = e;
   });
 }
 
@@ -621,6 +663,20 @@ void d3hip_get_pairwise_dispersion_batch(dftd3_error error, dftd3_model workspac
     std::vector<d3::SystemDesc> sys = batch_systems(nsys, natoms, lattices, periodic, params, cutoffs);
     workspace->engine->run_pairwise(sys, numbers, positions, precision, pair_energy2, pair_energy3,
                                     static_cast<hipStream_t>(stream));
+  });
+}
+
+void d3hip_get_dispersion_hessian_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
+                                        const int* numbers, const double* lattices, const bool* periodic,
+                                        const double* params, const double* cutoffs, const double* positions,
+                                        double* energy, double* hessian, void* stream) {
+  if (!error) return;
+  if (!workspace) return fatal(error, "Dispersion model is missing");
+  if (nsys <= 0 || !natoms || !numbers || !params || !positions || !hessian)
+    return fatal(error, "Invalid batch description");
+  guarded(error, [&] {
+    std::vector<d3::SystemDesc> sys = batch_systems(nsys, natoms, lattices, periodic, params, cutoffs);
+    workspace->engine->run_hessian(sys, numbers, positions, energy, hessian, static_cast<hipStream_t>(stream));
   });
 }
 

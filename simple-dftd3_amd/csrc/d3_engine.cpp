@@ -220,7 +220,7 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
-      item_off, slots, counts, counter, flag, pair3s;
+      item_off, slots, counts, counter, flag, pair3s, hbuf, e_out;
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
@@ -515,6 +515,52 @@ void Engine::run_pairwise(const std::vector<SystemDesc>& systems, const int* num
       launch_pair3_finalize(db, S, pair3, stream);
     });
   }
+  hip_check(hipGetLastError(), "kernel launch");
+  if (m.timer.enabled) m.timer.collect();
+}
+
+void Engine::run_hessian(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                         double* energy, double* hessian, hipStream_t stream, const int* ghost_host) {
+  // disp.f90:236-308: the FP64 energy evaluation leaves the coordination
+  // numbers, the pair records and the three-body lists in the workspace; the
+  // Hessian kernels reuse them.
+  if (systems.empty()) return;
+  Workspace& m = *ws_;
+  const Prepared pb = m.upload(systems, numbers, stream, ghost_host);
+  const DevBatch& db = pb.db;
+  const int natoms = db.natoms;
+  double* e = energy ? energy : m.e_out.as<double>(static_cast<size_t>(db.nsys));
+  BatchOutput out;
+  out.energy = e;
+  RunStats stats;
+  m.evaluate(pb, positions_dev, true, false, false, pb.smooth, out, nullptr, &stats, -1, 0, AtmShape{}, stream);
+  const size_t nn = static_cast<size_t>(pb.pair_total), nm = static_cast<size_t>(natoms) * db.mref;
+  const size_t words = 3 * nm + 4 * nn + 3 * nn + natoms + 3 * nn + nn + 3 * nn;
+  double* base = m.hbuf.as<double>(words);
+  HessWork h{};
+  h.cn = m.cn.as<double>(natoms);
+  h.gw = base;
+  h.gwd = h.gw + nm;
+  h.gwd2 = h.gwd + nm;
+  h.c6 = h.gwd2 + nm;
+  h.dc6 = h.c6 + nn;
+  h.d2c6 = h.dc6 + nn;
+  h.d2c6ij = h.d2c6 + nn;
+  h.dr = h.d2c6ij + nn;
+  h.dedcn = h.dr + 3 * nn;
+  h.dEdcndr = h.dedcn + natoms;
+  h.dEdcndcn = h.dEdcndr + 3 * nn;
+  h.tmp = h.dEdcndcn + nn;
+  h.hess = hessian;
+  hip_check(hipMemsetAsync(hessian, 0, sizeof(double) * 9 * nn, stream), "hipMemsetAsync");
+  hip_check(hipMemsetAsync(h.dedcn, 0, sizeof(double) * (natoms + 4 * nn), stream), "hipMemsetAsync");
+  int max_natoms = 0;
+  for (const auto& sd : systems) max_natoms = std::max(max_natoms, sd.natoms);
+  const long long total = pb.any_atm ? stats.atm_list_entries : 0;
+  m.timed("hessian", stream, [&] {
+    launch_hessian(db, h, m.off.as<long long>(natoms + 1), total > 0 ? m.slots.get(0) : nullptr, total, max_natoms,
+                   stream);
+  });
   hip_check(hipGetLastError(), "kernel launch");
   if (m.timer.enabled) m.timer.collect();
 }

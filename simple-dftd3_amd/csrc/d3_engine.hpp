@@ -77,6 +77,13 @@ class Engine {
                     int precision, double* pair2, double* pair3, hipStream_t stream,
                     const int* ghost_host = nullptr);
 
+  // Energy and analytic Hessian (disp.f90:236-308) of a batch, FP64:
+  // energy [nsys] (device, may be null) and hessian (device) holding per
+  // system a 3n x 3n block, [3i+a][3j+b] = d2E / dR_ia dR_jb, the blocks one
+  // after another (offset 9 times the sum of n^2 of the systems before).
+  void run_hessian(const std::vector<SystemDesc>& systems, const int* numbers_host, const double* positions_dev,
+                   double* energy, double* hessian, hipStream_t stream, const int* ghost_host = nullptr);
+
   // Work units of the reference's loops per system: counts_host[3*s + 0..2]
   // = CN terms, two-body pair terms, ATM triple terms (host array).
   void count_terms(const std::vector<SystemDesc>& systems, const int* numbers_host,

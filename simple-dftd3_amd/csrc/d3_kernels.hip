@@ -350,17 +350,17 @@ struct DampPair {
   R rrij, k0, k1, k2, k3, k4;
 };
 
-template <typename R>
+template <typename R, int kKind>
 __device__ __forceinline__ DampPair<R> damp_setup(const SysParam& p, R rrij, double rvdw, R c6ij, int idsum) {
   DampPair<R> d{rrij, R(0), R(0), R(0), R(0), R(0)};
-  switch (p.kind) {
+  switch (kKind) {
     case kRational:
     case kOptimizedPower: {
       const R r0 = R(p.a1) * d_sqrt(rrij) + R(p.a2);
       const R r02 = r0 * r0;
       d.k0 = r02 * r02 * r02;
       d.k1 = d.k0 * r02;
-      if (p.kind == kOptimizedPower) {
+      if (kKind == kOptimizedPower) {
         const R ab = d_pow(r0, R(p.bet));
         d.k0 *= ab;
         d.k1 *= ab;
@@ -401,13 +401,13 @@ __device__ __forceinline__ DampPair<R> damp_setup(const SysParam& p, R rrij, dou
   return d;
 }
 
-template <typename R, bool kGrad>
+template <typename R, bool kGrad, int kKind>
 __device__ __forceinline__ R damp_term(const SysParam& p, const DampPair<R>& d, R r2, R r1, R& gdisp0, R& dedc6) {
   const R s6 = R(p.s6), s8 = R(p.s8);
   R e0 = R(0);
   gdisp0 = R(0);
   dedc6 = R(0);
-  switch (p.kind) {
+  switch (kKind) {
     case kRational: {
       const R r4 = r2 * r2;
       const R t6 = d_rcp(r4 * r2 + d.k0);
@@ -523,17 +523,13 @@ __device__ __forceinline__ bool two_body_active(const SysParam& p) {
 // (j, i) of the system's n x n block (atom order, offset c6_begin), so the
 // matrix is exactly symmetric like the reference's; e_atom is then only
 // scratch.
-template <typename R, bool kGrad, bool kPairs>
-__global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* __restrict__ pair,
-                                                       double* __restrict__ e_atom, double* __restrict__ dedcn,
-                                                       double* __restrict__ grad, double* __restrict__ sig_atom,
-                                                       double* __restrict__ pair2) {
+template <typename R, bool kGrad, bool kPairs, int kKind>
+__device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __restrict__ pair,
+                                           double* __restrict__ e_atom, double* __restrict__ dedcn,
+                                           double* __restrict__ grad, double* __restrict__ sig_atom,
+                                           double* __restrict__ pair2, int i, const SysGeom& g,
+                                           const SysParam& p) {
   const int lane = wave_lane();
-  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (i >= b.natoms) return;
-  const int s = b.sys[i];
-  const SysGeom g = b.geom[s];
-  const SysParam p = b.par[s];
   const int n = g.natoms;
   const double cut2 = p.cut2 * p.cut2;
   const double4 xi = b.pos[i];
@@ -556,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
       const R4<R> q = row[jr];
       const R c6ij = q.x * q.x;
       const R dc6i = q.y * c6ij;  // dC6_ij/dCN_i
-      const DampPair<R> dp = damp_setup<R>(p, rrij, b.rvdw[spj * b.nsp + spi], c6ij, idi + b.sid[j]);
+      const DampPair<R> dp = damp_setup<R, kKind>(p, rrij, b.rvdw[spj * b.nsp + spi], c6ij, idi + b.sid[j]);
       const bool swap = j > i;
       for (int t = 0; t < g.img2_count; ++t) {
         double u0, u1, u2;
@@ -567,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
         R sw, dswdr;
         smooth_cutoff(r1, cutr, width, sw, dswdr);
         R gdisp0, dedc6;
-        const R edisp0 = damp_term<R, kGrad>(p, dp, r2, r1, gdisp0, dedc6);
+        const R edisp0 = damp_term<R, kGrad, kKind>(p, dp, r2, r1, gdisp0, dedc6);
         const R edisp = sw * edisp0;
         const double de = double(-c6ij * edisp * R(0.5));
         acc_e += de;
@@ -575,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
         if (kGrad) {
           const R gdisp = sw * gdisp0 + dswdr * edisp0 / r1;
           const R f = -c6ij * gdisp;  // dG = f * vec
-          if (p.kind == kZDamping) acc_d -= double(dc6i * sw * dedc6);
+          if (kKind == kZDamping) acc_d -= double(dc6i * sw * dedc6);
           else acc_d -= double(dc6i * edisp);
           const R v0 = R(u0), v1 = R(u1), v2 = R(u2);
           if (j != i) {
@@ -608,6 +604,33 @@ __global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* 
       double* sg = sig_atom + 6 * i;
       sg[0] = sxx; sg[1] = syy; sg[2] = szz; sg[3] = sxy; sg[4] = sxz; sg[5] = syz;
     }
+  }
+}
+
+
+// The image loop is specialised per damping family (the family is a
+// per-system parameter, uniform within a wavefront).
+template <typename R, bool kGrad, bool kPairs>
+__global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* __restrict__ pair,
+                                                       double* __restrict__ e_atom, double* __restrict__ dedcn,
+                                                       double* __restrict__ grad, double* __restrict__ sig_atom,
+                                                       double* __restrict__ pair2) {
+  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (i >= b.natoms) return;
+  const int s = b.sys[i];
+  const SysGeom g = b.geom[s];
+  const SysParam p = b.par[s];
+  switch (p.kind) {
+    case kRational: disp2_atom<R, kGrad, kPairs, kRational>(b, pair, e_atom, dedcn, grad, sig_atom, pair2, i, g, p); break;
+    case kZero: disp2_atom<R, kGrad, kPairs, kZero>(b, pair, e_atom, dedcn, grad, sig_atom, pair2, i, g, p); break;
+    case kModifiedZero:
+      disp2_atom<R, kGrad, kPairs, kModifiedZero>(b, pair, e_atom, dedcn, grad, sig_atom, pair2, i, g, p);
+      break;
+    case kOptimizedPower:
+      disp2_atom<R, kGrad, kPairs, kOptimizedPower>(b, pair, e_atom, dedcn, grad, sig_atom, pair2, i, g, p);
+      break;
+    case kCSO: disp2_atom<R, kGrad, kPairs, kCSO>(b, pair, e_atom, dedcn, grad, sig_atom, pair2, i, g, p); break;
+    default: disp2_atom<R, kGrad, kPairs, kZDamping>(b, pair, e_atom, dedcn, grad, sig_atom, pair2, i, g, p); break;
   }
 }
 
@@ -1292,6 +1315,658 @@ __global__ __launch_bounds__(kBlock) void count_triples_kernel(DevBatch b, const
   }
 }
 
+// ----------------------------------------------------------------- Hessian
+// disp.f90:236-308 get_dispersion_hessian, FP64.  Per system (atom order
+// within the system, offsets from c6_begin = sum of n^2 of the systems
+// before):
+//   c6, dc6, d2c6, d2c6ij  n x n at c6_begin       [i][j]: C6_ij and its derivatives
+//                                                  d/dCN_i, d2/dCN_i^2, d2/dCN_i dCN_j
+//   dr, dEdcndr            3n x n at 3 c6_begin    [3i+c][k]: dCN_k/dR_ic, d2E/dR_ic dCN_k
+//   dEdcndcn               n x n at c6_begin       d2E/dCN_k dCN_l
+//   tmp                    n x 3n at 3 c6_begin    dEdcndcn dr^T
+//   hess                   3n x 3n at 9 c6_begin   [3i+a][3j+b]
+// The two-body, CN and dCN/dR kernels own the rows of their atom (ordered
+// pairs (i, j, T) with u = R_i - R_j - T; every pair term is even in u or
+// changes sign with it), so they need no atomics; the three-body kernel adds
+// with atomics.
+
+// model.f90:201-287 weight_references with first and second derivatives.
+__global__ void hess_weights_kernel(DevBatch b, const double* __restrict__ cn, double* __restrict__ gw,
+                                    double* __restrict__ gwd, double* __restrict__ gwd2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.natoms) return;
+  const double wf2 = 8.0;
+  const int s = b.sp[i];
+  const int nr = b.nref[s];
+  const double* rcn = b.refcn + s * kMaxRef;
+  const double c = cn[i];
+  double gwt[kMaxRef];
+  double norm = 0.0, dnorm = 0.0, d2norm = 0.0, maxcn = -1.0e300;
+  for (int r = 0; r < nr; ++r) {
+    const double d = c - rcn[r];
+    const double w = exp(-4.0 * (d * d));
+    const double dcn = rcn[r] - c;
+    gwt[r] = w;
+    norm += w;
+    dnorm += wf2 * dcn * w;
+    d2norm += (wf2 * wf2 * dcn * dcn - wf2) * w;
+    maxcn = fmax(maxcn, rcn[r]);
+  }
+  norm = 1.0 / norm;
+  for (int r = 0; r < b.mref; ++r) {
+    double gwk = 0.0, dgwk = 0.0, d2gwk = 0.0;
+    if (r < nr) {
+      const double expw = gwt[r], dcn = rcn[r] - c;
+      const double expd = wf2 * dcn * expw, expdd = (wf2 * wf2 * dcn * dcn - wf2) * expw;
+      gwk = expw * norm;
+      if (isnan(gwk) || isinf(gwk)) gwk = (maxcn == rcn[r]) ? 1.0 : 0.0;
+      dgwk = expd * norm - expw * dnorm * (norm * norm);
+      if (isnan(dgwk) || isinf(dgwk)) dgwk = 0.0;
+      d2gwk = expdd * norm - 2.0 * expd * dnorm * (norm * norm) - expw * d2norm * (norm * norm) +
+              2.0 * expw * dnorm * dnorm * (norm * norm * norm);
+      if (isnan(d2gwk) || isinf(d2gwk)) d2gwk = 0.0;
+    }
+    gw[i * b.mref + r] = gwk;
+    gwd[i * b.mref + r] = dgwk;
+    gwd2[i * b.mref + r] = d2gwk;
+  }
+}
+
+// model.f90:384-428 get_atomic_c6, second-derivative branch: block per atom i,
+// threads over the atoms j of its system.
+__global__ __launch_bounds__(kBlock) void hess_c6_kernel(DevBatch b, const double* __restrict__ gw,
+                                                         const double* __restrict__ gwd,
+                                                         const double* __restrict__ gwd2, double* __restrict__ c6,
+                                                         double* __restrict__ dc6, double* __restrict__ d2c6,
+                                                         double* __restrict__ d2c6ij) {
+  const int i = blockIdx.x;
+  const SysGeom g = b.geom[b.sys[i]];
+  const int n = g.natoms, il = i - g.atom_begin;
+  const int spi = b.sp[i], nri = b.nref[spi], mref = b.mref;
+  const bool gi = b.ghost && b.ghost[i];
+  for (int jl = threadIdx.x; jl < n; jl += blockDim.x) {
+    const int j = g.atom_begin + jl;
+    const int spj = b.sp[j], nrj = b.nref[spj];
+    double a = 0.0, di = 0.0, ddi = 0.0, mix = 0.0;
+    if (!(gi || (b.ghost && b.ghost[j]))) {
+      const double* ref = b.c6ref + (spi * b.nsp + spj) * (kMaxRef * kMaxRef);
+      for (int ri = 0; ri < nri; ++ri)
+        for (int rj = 0; rj < nrj; ++rj) {
+          const double r = ref[ri * kMaxRef + rj];
+          const double wj = gw[j * mref + rj];
+          a += gw[i * mref + ri] * wj * r;
+          di += gwd[i * mref + ri] * wj * r;
+          ddi += gwd2[i * mref + ri] * wj * r;
+          mix += gwd[i * mref + ri] * gwd[j * mref + rj] * r;
+        }
+    }
+    const long long o = g.c6_begin + static_cast<long long>(il) * n + jl;
+    c6[o] = a;
+    dc6[o] = di;
+    d2c6[o] = ddi;
+    d2c6ij[o] = mix;
+  }
+}
+
+// dCN_k/dR_ic of the exponential count of ncoord.f90:118-133 (wavefront per
+// atom i, lanes over j != i): the pair (i, j, T) adds g = dcf/dr u/r to the
+// columns i and j of the rows of i.
+__global__ __launch_bounds__(kBlock) void hess_dcndr_kernel(DevBatch b, double* __restrict__ dr) {
+  const int lane = wave_lane();
+  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (i >= b.natoms) return;
+  const SysGeom g = b.geom[b.sys[i]];
+  const int n = g.natoms, il = i - g.atom_begin;
+  const double cut = b.par[b.sys[i]].cut_cn, cut2 = cut * cut;
+  const double4 xi = b.pos[i];
+  const double rci = b.rcov[b.sp[i]];
+  double* __restrict__ rows = dr + 3 * g.c6_begin + static_cast<long long>(3 * il) * n;
+  double si0 = 0.0, si1 = 0.0, si2 = 0.0;
+  for (int jl = lane; jl < n; jl += kWave) {
+    const int j = g.atom_begin + jl;
+    if (j == i) continue;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const double4 xj = b.pos[j];
+    const double rc = rci + b.rcov[b.sp[j]];
+    for (int t = 0; t < g.img_cn_count; ++t) {
+      double u0, u1, u2;
+      const double r2 = pair_vec(xi, xj, b.img[g.img_cn_begin + t], j > i, u0, u1, u2);
+      if (r2 > cut2 || r2 < kCnFloor) continue;
+      const double r1 = sqrt(r2);
+      const double cf = 1.0 / (1.0 + exp(-kKcn * (rc / r1 - 1.0)));
+      const double f = -cf * (1.0 - cf) * kKcn * rc / r2 / r1;
+      s0 += f * u0; s1 += f * u1; s2 += f * u2;
+    }
+    rows[jl] = s0;
+    rows[n + jl] = s1;
+    rows[2 * n + jl] = s2;
+    si0 += s0; si1 += s1; si2 += s2;
+  }
+  si0 = wave_sum(si0); si1 = wave_sum(si1); si2 = wave_sum(si2);
+  if (lane == 0) {
+    rows[il] = si0;
+    rows[n + il] = si1;
+    rows[2 * n + il] = si2;
+  }
+}
+
+// cutoff.f90:104-137 smooth_cutoff_r2.
+__device__ __forceinline__ void switch_r2(double r2, double cutoff, double width, double& sw, double& dsw,
+                                          double& d2sw) {
+  const double r1 = sqrt(r2);
+  double dswdr = 0.0, d2swdr2 = 0.0;
+  sw = 1.0;
+  if (width > 0.0 && width < cutoff && r1 > cutoff - width) {
+    if (r1 >= cutoff) {
+      sw = 0.0;
+    } else {
+      const double x = (cutoff - r1) / width;
+      sw = x * x * x * (10.0 + x * (-15.0 + 6.0 * x));
+      dswdr = -30.0 * x * x * (1.0 - x) * (1.0 - x) / width;
+      d2swdr2 = 60.0 * x * (1.0 - x) * (1.0 - 2.0 * x) / (width * width);
+    }
+  }
+  dsw = 0.5 * dswdr / r1;
+  d2sw = 0.25 * (d2swdr2 - dswdr / r1) / r2;
+}
+
+// get_damping_kernel of every family (rational.f90, zero.f90, mzero.f90,
+// optimizedpower.f90, cso.f90, z.f90): e(u, C6) for u = r^2 and e_u, e_c,
+// e_uu, e_uc, e_cc.
+__device__ void damping_kernel_u(const SysParam& p, double rrij, double rvdw, int idsum, double u, double c6,
+                                 double& e, double& er, double& ec, double& err, double& erc, double& ecc) {
+  e = er = ec = err = erc = ecc = 0.0;
+  if (p.kind != kCSO && fabs(p.s6) < kEps && fabs(p.s8) < kEps) return;
+  double pv = 0.0, pd = 0.0, pdd = 0.0;
+  switch (p.kind) {
+    case kRational: {
+      const double r0 = p.a1 * sqrt(rrij) + p.a2, r02 = r0 * r0, r06 = r02 * r02 * r02, r08 = r06 * r02;
+      const double t6 = 1.0 / (u * u * u + r06), t8 = 1.0 / (u * u * u * u + r08);
+      const double t6d = -3.0 * u * u * t6 * t6, t8d = -4.0 * u * u * u * t8 * t8;
+      const double t6dd = -6.0 * u * t6 * t6 + 18.0 * pow(u, 4) * pow(t6, 3);
+      const double t8dd = -12.0 * u * u * t8 * t8 + 32.0 * pow(u, 6) * pow(t8, 3);
+      pv = p.s6 * t6 + p.s8 * rrij * t8;
+      pd = p.s6 * t6d + p.s8 * rrij * t8d;
+      pdd = p.s6 * t6dd + p.s8 * rrij * t8dd;
+      break;
+    }
+    case kZero:
+    case kModifiedZero: {
+      const double alp6 = p.alp, alp8 = p.alp + 2.0;
+      double t6, t8, t6d, t8d, t6dd, t8dd;
+      if (p.kind == kZero) {
+        const double a6 = 0.5 * alp6, a8 = 0.5 * alp8;
+        t6 = pow(p.a1 * rvdw / sqrt(u), alp6);
+        t8 = pow(p.a2 * rvdw / sqrt(u), alp8);
+        t6d = -a6 * t6 / u;
+        t8d = -a8 * t8 / u;
+        t6dd = a6 * (a6 + 1.0) * t6 / (u * u);
+        t8dd = a8 * (a8 + 1.0) * t8 / (u * u);
+      } else {
+        const double r1 = sqrt(u), betr0 = p.bet * rvdw;
+        const double w6 = r1 / (p.a1 * rvdw) + betr0, w8 = r1 / (p.a2 * rvdw) + betr0;
+        const double w6d = 1.0 / (p.a1 * rvdw * 2.0 * r1), w8d = 1.0 / (p.a2 * rvdw * 2.0 * r1);
+        const double w6dd = -1.0 / (p.a1 * rvdw * 4.0 * u * r1), w8dd = -1.0 / (p.a2 * rvdw * 4.0 * u * r1);
+        t6 = pow(w6, -alp6);
+        t8 = pow(w8, -alp8);
+        t6d = -alp6 * pow(w6, -alp6 - 1.0) * w6d;
+        t8d = -alp8 * pow(w8, -alp8 - 1.0) * w8d;
+        t6dd = alp6 * (alp6 + 1.0) * pow(w6, -alp6 - 2.0) * w6d * w6d - alp6 * pow(w6, -alp6 - 1.0) * w6dd;
+        t8dd = alp8 * (alp8 + 1.0) * pow(w8, -alp8 - 2.0) * w8d * w8d - alp8 * pow(w8, -alp8 - 1.0) * w8dd;
+      }
+      const double f6 = 1.0 / (1.0 + 6.0 * t6), f8 = 1.0 / (1.0 + 6.0 * t8);
+      const double f6d = -6.0 * t6d * f6 * f6, f8d = -6.0 * t8d * f8 * f8;
+      const double f6dd = -6.0 * t6dd * f6 * f6 + 72.0 * t6d * t6d * pow(f6, 3);
+      const double f8dd = -6.0 * t8dd * f8 * f8 + 72.0 * t8d * t8d * pow(f8, 3);
+      const double c6t = f6 / pow(u, 3), c6td = f6d / pow(u, 3) - 3.0 * f6 / pow(u, 4);
+      const double c6tdd = f6dd / pow(u, 3) - 6.0 * f6d / pow(u, 4) + 12.0 * f6 / pow(u, 5);
+      const double c8t = f8 / pow(u, 4), c8td = f8d / pow(u, 4) - 4.0 * f8 / pow(u, 5);
+      const double c8tdd = f8dd / pow(u, 4) - 8.0 * f8d / pow(u, 5) + 20.0 * f8 / pow(u, 6);
+      pv = p.s6 * c6t + p.s8 * rrij * c8t;
+      pd = p.s6 * c6td + p.s8 * rrij * c8td;
+      pdd = p.s6 * c6tdd + p.s8 * rrij * c8tdd;
+      break;
+    }
+    case kOptimizedPower: {
+      const double r0 = p.a1 * sqrt(rrij) + p.a2, r02 = r0 * r0, r06 = r02 * r02 * r02, r08 = r06 * r02;
+      const double ab = pow(r0, p.bet), abr06 = ab * r06, abr08 = ab * r08, q = 0.5 * p.bet;
+      const double d6 = pow(u, 3) + abr06 * pow(u, -q), d8 = pow(u, 4) + abr08 * pow(u, -q);
+      const double d6d = 3.0 * u * u - q * abr06 * pow(u, -q - 1.0);
+      const double d8d = 4.0 * pow(u, 3) - q * abr08 * pow(u, -q - 1.0);
+      const double d6dd = 6.0 * u + q * (q + 1.0) * abr06 * pow(u, -q - 2.0);
+      const double d8dd = 12.0 * u * u + q * (q + 1.0) * abr08 * pow(u, -q - 2.0);
+      const double t6 = 1.0 / d6, t8 = 1.0 / d8;
+      const double t6d = -d6d * t6 * t6, t8d = -d8d * t8 * t8;
+      const double t6dd = -d6dd * t6 * t6 + 2.0 * d6d * d6d * pow(t6, 3);
+      const double t8dd = -d8dd * t8 * t8 + 2.0 * d8d * d8d * pow(t8, 3);
+      pv = p.s6 * t6 + p.s8 * rrij * t8;
+      pd = p.s6 * t6d + p.s8 * rrij * t8d;
+      pdd = p.s6 * t6dd + p.s8 * rrij * t8dd;
+      break;
+    }
+    case kCSO: {
+      const double r1 = sqrt(u), r0 = sqrt(rrij);
+      const double dd6 = p.a3 * r0 + p.a4, dd6_6 = pow(dd6, 6);
+      const double sf = 1.0 / (1.0 + exp(r1 - p.a2 * r0));
+      const double sfd = -sf * (1.0 - sf), sfdd = sf * (1.0 - sf) * (1.0 - 2.0 * sf);
+      const double sig = p.s6 + p.a1 * sf, sigd = p.a1 * sfd, sigdd = p.a1 * sfdd;
+      const double sigu = 0.5 * sigd / r1, siguu = 0.25 * sigdd / u - 0.25 * sigd / (u * r1);
+      const double t6 = 1.0 / (u * u * u + dd6_6);
+      const double t6d = -3.0 * u * u * t6 * t6;
+      const double t6dd = -6.0 * u * t6 * t6 + 18.0 * pow(u, 4) * pow(t6, 3);
+      pv = sig * t6;
+      pd = sigu * t6 + sig * t6d;
+      pdd = siguu * t6 + 2.0 * sigu * t6d + sig * t6dd;
+      break;
+    }
+    default: {  // Z damping
+      const double r0 = p.a1 / static_cast<double>(idsum), b8 = r0 * rrij;
+      const double t6 = 1.0 / (u * u * u + r0 * c6), t8 = 1.0 / (u * u * u * u + b8 * c6);
+      const double t6u = -3.0 * u * u * t6 * t6, t8u = -4.0 * pow(u, 3) * t8 * t8;
+      const double t6uu = -6.0 * u * t6 * t6 + 18.0 * pow(u, 4) * pow(t6, 3);
+      const double t8uu = -12.0 * u * u * t8 * t8 + 32.0 * pow(u, 6) * pow(t8, 3);
+      const double t6c = -r0 * t6 * t6, t8c = -b8 * t8 * t8;
+      const double t6uc = 6.0 * u * u * r0 * pow(t6, 3), t8uc = 8.0 * pow(u, 3) * b8 * pow(t8, 3);
+      const double t6cc = 2.0 * r0 * r0 * pow(t6, 3), t8cc = 2.0 * b8 * b8 * pow(t8, 3);
+      const double pz = p.s6 * t6 + p.s8 * rrij * t8, pu = p.s6 * t6u + p.s8 * rrij * t8u;
+      const double puu = p.s6 * t6uu + p.s8 * rrij * t8uu, pc = p.s6 * t6c + p.s8 * rrij * t8c;
+      const double puc = p.s6 * t6uc + p.s8 * rrij * t8uc, pcc = p.s6 * t6cc + p.s8 * rrij * t8cc;
+      e = -c6 * pz;
+      er = -c6 * pu;
+      ec = -pz - c6 * pc;
+      err = -c6 * puu;
+      erc = -pu - c6 * puc;
+      ecc = -2.0 * pc - c6 * pcc;
+      return;
+    }
+  }
+  e = -c6 * pv;
+  er = -c6 * pd;
+  ec = -pv;
+  err = -c6 * pdd;
+  erc = -pd;
+}
+
+// damping.f90:393-539 get_dispersion2_hessian (wavefront per atom i, lanes
+// over the atoms j of its system): the rows of atom i of the Hessian at
+// fixed CN, of dEdcndr, dEdcndcn and dEdcn.  Adds to zeroed arrays.
+__global__ __launch_bounds__(kBlock) void hess2_kernel(DevBatch b, const double* __restrict__ c6,
+                                                       const double* __restrict__ dc6,
+                                                       const double* __restrict__ d2c6,
+                                                       const double* __restrict__ d2c6ij, double* __restrict__ hess,
+                                                       double* __restrict__ dedcn, double* __restrict__ dEdcndr,
+                                                       double* __restrict__ dEdcndcn) {
+  const int lane = wave_lane();
+  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (i >= b.natoms) return;
+  const int s = b.sys[i];
+  const SysGeom g = b.geom[s];
+  const SysParam p = b.par[s];
+  const int n = g.natoms, il = i - g.atom_begin, ndim = 3 * n;
+  const double cut2 = p.cut2 * p.cut2;
+  const double4 xi = b.pos[i];
+  const int spi = b.sp[i];
+  const long long c0 = g.c6_begin, ri = c0 + static_cast<long long>(il) * n;
+  double* __restrict__ hrow = hess + 9 * c0 + static_cast<long long>(3 * il) * ndim;
+  double* __restrict__ drow = dEdcndr + 3 * c0 + static_cast<long long>(3 * il) * n;
+  double hii[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double dii[3] = {0.0, 0.0, 0.0}, e_cn = 0.0, e_cc = 0.0;
+  for (int jl = lane; jl < n; jl += kWave) {
+    const int j = g.atom_begin + jl;
+    const int spj = b.sp[j];
+    const double rrij = 3.0 * b.r4r2[spi] * b.r4r2[spj];
+    const double c6ij = c6[ri + jl];
+    const bool self = j == i;
+    // the reference's cnd / cnd2 of the pair, i-side entries
+    const double cni = self ? 2.0 * dc6[ri + il] : dc6[ri + jl];
+    const double cnj = self ? 0.0 : dc6[c0 + static_cast<long long>(jl) * n + il];
+    const double cnii = self ? 2.0 * d2c6[ri + il] + 2.0 * d2c6ij[ri + il] : d2c6[ri + jl];
+    const double cnij = self ? 0.0 : d2c6ij[ri + jl];
+    const double fac = self ? 0.5 : 1.0;
+    double hb[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double dij[3] = {0.0, 0.0, 0.0}, ccij = 0.0;
+    for (int t = 0; t < g.img2_count; ++t) {
+      double v0, v1, v2;
+      const double r2 = pair_vec(xi, b.pos[j], b.img[g.img2_begin + t], j > i, v0, v1, v2);
+      if (r2 > cut2 || r2 < kEps) continue;
+      double e0, e0r, e0c, e0rr, e0rc, e0cc, sw, swr, swrr;
+      damping_kernel_u(p, rrij, b.rvdw[spj * b.nsp + spi], b.sid[i] + b.sid[j], r2, c6ij, e0, e0r, e0c, e0rr,
+                       e0rc, e0cc);
+      switch_r2(r2, p.cut2, p.width2, sw, swr, swrr);
+      const double fr = fac * (swr * e0 + sw * e0r);
+      const double frr = fac * (swrr * e0 + 2.0 * swr * e0r + sw * e0rr);
+      const double fc = fac * sw * e0c;
+      const double frc = fac * (swr * e0c + sw * e0rc);
+      const double fcc = fac * sw * e0cc;
+      e_cn += fc * cni;
+      e_cc += fcc * cni * cni + fc * cnii;
+      if (!self) {
+        ccij += fcc * cni * cnj + fc * cnij;
+        const double v[3] = {v0, v1, v2};
+        for (int a = 0; a < 3; ++a) {
+          for (int c = 0; c < 3; ++c) hb[3 * a + c] += 4.0 * frr * v[a] * v[c];
+          hb[4 * a] += 2.0 * fr;
+          dii[a] += frc * 2.0 * v[a] * cni;
+          dij[a] += frc * 2.0 * v[a] * cnj;
+        }
+      }
+    }
+    if (!self) {
+      for (int a = 0; a < 3; ++a) {
+        for (int c = 0; c < 3; ++c) {
+          hrow[static_cast<long long>(a) * ndim + 3 * jl + c] -= hb[3 * a + c];
+          hii[3 * a + c] += hb[3 * a + c];
+        }
+        drow[static_cast<long long>(a) * n + jl] += dij[a];
+      }
+      dEdcndcn[ri + jl] += ccij;
+    }
+  }
+  for (int k = 0; k < 9; ++k) hii[k] = wave_sum(hii[k]);
+  for (int a = 0; a < 3; ++a) dii[a] = wave_sum(dii[a]);
+  e_cn = wave_sum(e_cn);
+  e_cc = wave_sum(e_cc);
+  if (lane == 0) {
+    for (int a = 0; a < 3; ++a) {
+      for (int c = 0; c < 3; ++c) hrow[static_cast<long long>(a) * ndim + 3 * il + c] += hii[3 * a + c];
+      drow[static_cast<long long>(a) * n + il] += dii[a];
+    }
+    dedcn[i] += e_cn;
+    dEdcndcn[ri + il] += e_cc;
+  }
+}
+
+// atm.f90:38-429 get_atm_dispersion_hessian: thread per (atom i, slot a of
+// L(i)), looping over the slots b after a, i.e. over the triple terms of the
+// three-body lists (weights as in atm_kernel: 1/2 per slot that is an image
+// of atom i, 4/3 more when both are).  Each term is K(u1, u2, u3) with
+// u1 = |v_a|^2, u2 = |v_b|^2, u3 = |v_b - v_a|^2; its gradient and Hessian in
+// u are contracted with du/dR and added with atomics.
+__global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long long* __restrict__ offset,
+                                                       const Slot<double>* __restrict__ slots,
+                                                       const double* __restrict__ c6,
+                                                       const double* __restrict__ dc6,
+                                                       const double* __restrict__ d2c6,
+                                                       const double* __restrict__ d2c6ij, double* __restrict__ hess,
+                                                       double* __restrict__ dedcn, double* __restrict__ dEdcndr,
+                                                       double* __restrict__ dEdcndcn) {
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long total = offset[b.natoms];
+  if (t >= total) return;
+  int lo = 0, hi = b.natoms;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offset[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const int i = lo;
+  const int s = b.sys[i];
+  const SysGeom g = b.geom[s];
+  const SysParam p = b.par[s];
+  const int n = g.natoms, ndim = 3 * n;
+  const long long c0 = g.c6_begin;
+  const double cut2 = p.cut3 * p.cut3, alp3 = (p.alp + 2.0) / 3.0, aexp = 0.5 * alp3;
+  const int sigp[3][3] = {{-1, -1, 0}, {1, 0, -1}, {0, 1, 1}};   // du_pair/dR_atom = 2 sigp[atom][pair] v_pair
+  const double cl[3][3] = {{1, -1, 1}, {1, 1, -1}, {-1, 1, 1}};  // coefficient of u_pair in l_l
+  const Slot<double> sa = slots[t];
+  const long long end = offset[i + 1];
+  const int at0 = i - g.atom_begin, at1 = sa.atom - g.atom_begin;
+  for (long long bi = t + 1; bi < end; ++bi) {
+    const Slot<double> sb = slots[bi];
+    double vec[3][3];
+    vec[0][0] = sa.vx; vec[0][1] = sa.vy; vec[0][2] = sa.vz;
+    vec[1][0] = sb.vx; vec[1][1] = sb.vy; vec[1][2] = sb.vz;
+    double u[3];
+    u[2] = diff_norm2(sb.vx, sb.vy, sb.vz, sa.vx, sa.vy, sa.vz, vec[2][0], vec[2][1], vec[2][2]);
+    if (u[2] > cut2 || u[2] < kEps) continue;
+    u[0] = sa.r2;
+    u[1] = sb.r2;
+    const int at2 = sb.atom - g.atom_begin;
+    const int at[3] = {at0, at1, at2};
+    const double c6t[3] = {c6[c0 + static_cast<long long>(at1) * n + at0],
+                           c6[c0 + static_cast<long long>(at2) * n + at0],
+                           c6[c0 + static_cast<long long>(at2) * n + at1]};
+    if (fabs(c6t[0]) < kEps || fabs(c6t[1]) < kEps || fabs(c6t[2]) < kEps) continue;
+    const bool selfa = sa.atom == i, selfb = sb.atom == i;
+    const double w = (selfa ? 0.5 : 1.0) * (selfb ? 0.5 : 1.0) * (selfa && selfb ? 4.0 / 3.0 : 1.0);
+    double swp[3], dswp[3], d2swp[3];
+    for (int q = 0; q < 3; ++q) switch_r2(u[q], p.cut3, p.width3, swp[q], dswp[q], d2swp[q]);
+    const double sval = swp[0] * swp[1] * swp[2];
+    const double sp[3] = {dswp[0] * swp[1] * swp[2], swp[0] * dswp[1] * swp[2], swp[0] * swp[1] * dswp[2]};
+    double spq[3][3];
+    spq[0][0] = d2swp[0] * swp[1] * swp[2];
+    spq[1][1] = swp[0] * d2swp[1] * swp[2];
+    spq[2][2] = swp[0] * swp[1] * d2swp[2];
+    spq[0][1] = spq[1][0] = dswp[0] * dswp[1] * swp[2];
+    spq[0][2] = spq[2][0] = dswp[0] * swp[1] * dswp[2];
+    spq[1][2] = spq[2][1] = swp[0] * dswp[1] * dswp[2];
+    const double ww = u[0] * u[1] * u[2];
+    const double wp[3] = {u[1] * u[2], u[0] * u[2], u[0] * u[1]};
+    const double wpq[3][3] = {{0.0, u[2], u[1]}, {u[2], 0.0, u[0]}, {u[1], u[0], 0.0}};
+    const double lv[3] = {u[0] + u[2] - u[1], u[0] - u[2] + u[1], -u[0] + u[2] + u[1]};
+    const double nval = lv[0] * lv[1] * lv[2];
+    double np_[3], npq[3][3];
+    for (int q = 0; q < 3; ++q) np_[q] = cl[0][q] * lv[1] * lv[2] + cl[1][q] * lv[0] * lv[2] + cl[2][q] * lv[0] * lv[1];
+    for (int q = 0; q < 3; ++q)
+      for (int r = 0; r < 3; ++r) {
+        double tmp = 0.0;
+        for (int l = 0; l < 3; ++l)
+          for (int m = 0; m < 3; ++m)
+            if (l != m) tmp += cl[l][q] * cl[m][r] * lv[3 - l - m];
+        npq[q][r] = tmp;
+      }
+    const double gv = pow(ww, -2.5), gd = -2.5 * pow(ww, -3.5), gdd = 8.75 * pow(ww, -4.5);
+    const double hv = pow(ww, -1.5), hd = -1.5 * pow(ww, -2.5), hdd = 3.75 * pow(ww, -3.5);
+    const double ang = 0.375 * nval * gv + hv;
+    const int spi = b.sp[i], spa = b.sp[sa.atom], spb = b.sp[sb.atom];
+    const double r0 = kRs9 * b.rvdw[spa * b.nsp + spi] * kRs9 * b.rvdw[spb * b.nsp + spi] * kRs9 *
+                      b.rvdw[spb * b.nsp + spa];
+    const double tval = 6.0 * pow(r0, alp3) * pow(ww, -aexp);
+    const double fd = 1.0 / (1.0 + tval);
+    double angp[3], tp[3], fdp[3], ap[3], kp[3];
+    for (int q = 0; q < 3; ++q) {
+      angp[q] = 0.375 * (np_[q] * gv + nval * gd * wp[q]) + hd * wp[q];
+      tp[q] = -aexp * tval * wp[q] / ww;
+      fdp[q] = -tp[q] * fd * fd;
+    }
+    const double av = ang * fd;
+    for (int q = 0; q < 3; ++q) ap[q] = angp[q] * fd + ang * fdp[q];
+    const double kv = sval * av;
+    for (int q = 0; q < 3; ++q) kp[q] = sp[q] * av + sval * ap[q];
+    double kpq[3][3];
+    for (int q = 0; q < 3; ++q)
+      for (int r = 0; r < 3; ++r) {
+        const double angpq = 0.375 * (npq[q][r] * gv + np_[q] * gd * wp[r] + np_[r] * gd * wp[q] +
+                                      nval * (gdd * wp[q] * wp[r] + gd * wpq[q][r])) +
+                             hdd * wp[q] * wp[r] + hd * wpq[q][r];
+        const double tpq = aexp * (aexp + 1.0) * tval * wp[q] * wp[r] / (ww * ww) - aexp * tval * wpq[q][r] / ww;
+        const double fdpq = -tpq * fd * fd + 2.0 * tp[q] * tp[r] * fd * fd * fd;
+        const double apq = angpq * fd + angp[q] * fdp[r] + angp[r] * fdp[q] + ang * fdpq;
+        kpq[q][r] = spq[q][r] * av + sp[q] * ap[r] + sp[r] * ap[q] + sval * apq;
+      }
+    const double qv = sqrt(fabs(c6t[0] * c6t[1] * c6t[2]));
+    const double pref = p.s9 * w, pq = pref * qv;
+    // Cartesian second derivatives at fixed CN
+    double gk[3][3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double tmp = 0.0;
+          for (int q = 0; q < 3; ++q) tmp += kpq[q][r] * 2.0 * sigp[a][q] * vec[q][c];
+          gk[a][r][c] = pq * tmp;
+        }
+    for (int a = 0; a < 3; ++a)
+      for (int bb = 0; bb < 3; ++bb) {
+        double dg = 0.0;
+        for (int q = 0; q < 3; ++q) dg += kp[q] * sigp[a][q] * sigp[bb][q];
+        dg *= 2.0 * pq;
+        double* __restrict__ blk = hess + 9 * c0 + static_cast<long long>(3 * at[a]) * ndim + 3 * at[bb];
+        for (int c = 0; c < 3; ++c)
+          for (int d = 0; d < 3; ++d) {
+            double tmp = c == d ? dg : 0.0;
+            for (int r = 0; r < 3; ++r) tmp += gk[a][r][c] * 2.0 * sigp[bb][r] * vec[r][d];
+            atomicAdd(blk + static_cast<long long>(c) * ndim + d, tmp);
+          }
+      }
+    // derivatives through C6: pairs (i, a), (i, b), (a, b)
+    double ec[3], euc[3][3], ecc[3][3];
+    for (int q = 0; q < 3; ++q) {
+      const double qm = 0.5 * qv / c6t[q];
+      ec[q] = pref * qm * kv;
+      for (int r = 0; r < 3; ++r) {
+        const double qmn = q == r ? -0.25 * qv / (c6t[q] * c6t[q]) : 0.25 * qv / (c6t[q] * c6t[r]);
+        ecc[q][r] = pref * qmn * kv;
+        euc[r][q] = pref * qm * kp[r];
+      }
+    }
+    const int pat[3][2] = {{at0, at1}, {at0, at2}, {at1, at2}};
+    int nent = 0, ent_atom[6], ent_pair[6];
+    double ent_coef[6];
+    for (int q = 0; q < 3; ++q) {
+      const int a0 = pat[q][0], a1 = pat[q][1];
+      if (a0 != a1) {
+        ent_atom[nent] = a0; ent_pair[nent] = q; ent_coef[nent++] = dc6[c0 + static_cast<long long>(a0) * n + a1];
+        ent_atom[nent] = a1; ent_pair[nent] = q; ent_coef[nent++] = dc6[c0 + static_cast<long long>(a1) * n + a0];
+      } else {
+        ent_atom[nent] = a0; ent_pair[nent] = q;
+        ent_coef[nent++] = 2.0 * dc6[c0 + static_cast<long long>(a0) * n + a0];
+      }
+    }
+    for (int ie = 0; ie < nent; ++ie) {
+      const int ka = ent_atom[ie], kq = ent_pair[ie];
+      atomicAdd(dedcn + g.atom_begin + ka, ec[kq] * ent_coef[ie]);
+      for (int jf = 0; jf < nent; ++jf)
+        atomicAdd(dEdcndcn + c0 + static_cast<long long>(ka) * n + ent_atom[jf],
+                  ecc[kq][ent_pair[jf]] * ent_coef[ie] * ent_coef[jf]);
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) {
+          double egr = 0.0;
+          for (int q = 0; q < 3; ++q) egr += euc[q][kq] * 2.0 * sigp[a][q] * vec[q][c];
+          atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * at[a] + c) * n + ka, egr * ent_coef[ie]);
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+      const int a0 = pat[q][0], a1 = pat[q][1];
+      const long long o00 = c0 + static_cast<long long>(a0) * n + a0, o01 = c0 + static_cast<long long>(a0) * n + a1;
+      if (a0 != a1) {
+        const long long o11 = c0 + static_cast<long long>(a1) * n + a1, o10 = c0 + static_cast<long long>(a1) * n + a0;
+        atomicAdd(dEdcndcn + o00, ec[q] * d2c6[o01]);
+        atomicAdd(dEdcndcn + o11, ec[q] * d2c6[o10]);
+        atomicAdd(dEdcndcn + o01, ec[q] * d2c6ij[o01]);
+        atomicAdd(dEdcndcn + o10, ec[q] * d2c6ij[o01]);
+      } else {
+        atomicAdd(dEdcndcn + o00, ec[q] * (2.0 * d2c6[o00] + 2.0 * d2c6ij[o00]));
+      }
+    }
+  }
+}
+
+// ncoord.f90:236-306 add_coordination_number_hessian (wavefront per atom i,
+// lanes over j != i): dE/dCN times the second derivatives of the CN.
+__global__ __launch_bounds__(kBlock) void hess_cn_kernel(DevBatch b, const double* __restrict__ dedcn,
+                                                         double* __restrict__ hess) {
+  const int lane = wave_lane();
+  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (i >= b.natoms) return;
+  const SysGeom g = b.geom[b.sys[i]];
+  const int n = g.natoms, il = i - g.atom_begin, ndim = 3 * n;
+  const double cut = b.par[b.sys[i]].cut_cn, cut2 = cut * cut;
+  const double4 xi = b.pos[i];
+  const double rci = b.rcov[b.sp[i]];
+  double* __restrict__ hrow = hess + 9 * g.c6_begin + static_cast<long long>(3 * il) * ndim;
+  double hii[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int jl = lane; jl < n; jl += kWave) {
+    const int j = g.atom_begin + jl;
+    if (j == i) continue;
+    const double4 xj = b.pos[j];
+    const double rc = rci + b.rcov[b.sp[j]];
+    const double dEij = dedcn[i] + dedcn[j];
+    double hb[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < g.img_cn_count; ++t) {
+      double v0, v1, v2;
+      const double r2 = pair_vec(xi, xj, b.img[g.img_cn_begin + t], j > i, v0, v1, v2);
+      if (r2 > cut2 || r2 < kCnFloor) continue;
+      const double r1 = sqrt(r2);
+      const double cf = 1.0 / (1.0 + exp(-kKcn * (rc / r1 - 1.0)));
+      const double tmp = cf * (1.0 - cf);
+      const double dcf = -tmp * kKcn * rc / r2;
+      const double d2cf = tmp * (1.0 - 2.0 * cf) * (kKcn * rc) * (kKcn * rc) / (r2 * r2) +
+                          2.0 * tmp * kKcn * rc / (r2 * r1);
+      const double v[3] = {v0, v1, v2};
+      for (int a = 0; a < 3; ++a) {
+        for (int c = 0; c < 3; ++c) hb[3 * a + c] += dEij * (d2cf * v[a] * v[c] / r2 - dcf * v[a] * v[c] / (r2 * r1));
+        hb[4 * a] += dEij * dcf / r1;
+      }
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) {
+        hrow[static_cast<long long>(a) * ndim + 3 * jl + c] -= hb[3 * a + c];
+        hii[3 * a + c] += hb[3 * a + c];
+      }
+  }
+  for (int k = 0; k < 9; ++k) hii[k] = wave_sum(hii[k]);
+  if (lane == 0)
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) hrow[static_cast<long long>(a) * ndim + 3 * il + c] += hii[3 * a + c];
+}
+
+// tmp[k][y] = sum_l dEdcndcn[k][l] dr[y][l] (n x 3n per system; grid.z = system).
+constexpr int kHT = 16;  // tile edge of the dense products
+__global__ __launch_bounds__(kHT * kHT) void hess_tmp_kernel(DevBatch b, const double* __restrict__ dEdcndcn,
+                                                             const double* __restrict__ dr, double* __restrict__ tmp) {
+  const SysGeom g = b.geom[blockIdx.z];
+  const int n = g.natoms, ndim = 3 * n;
+  const int k = blockIdx.y * kHT + threadIdx.y, y = blockIdx.x * kHT + threadIdx.x;
+  __shared__ double A[kHT][kHT + 1], B[kHT][kHT + 1];
+  const double* C = dEdcndcn + g.c6_begin;
+  const double* D = dr + 3 * g.c6_begin;
+  double acc = 0.0;
+  for (int l0 = 0; l0 < n; l0 += kHT) {
+    const int la = l0 + threadIdx.x;
+    A[threadIdx.y][threadIdx.x] = (k < n && la < n) ? C[static_cast<long long>(k) * n + la] : 0.0;
+    const int yb = blockIdx.x * kHT + threadIdx.y;
+    B[threadIdx.y][threadIdx.x] = (yb < ndim && la < n) ? D[static_cast<long long>(yb) * n + la] : 0.0;
+    __syncthreads();
+    for (int q = 0; q < kHT; ++q) acc += A[threadIdx.y][q] * B[threadIdx.x][q];
+    __syncthreads();
+  }
+  if (k < n && y < ndim) tmp[3 * g.c6_begin + static_cast<long long>(k) * ndim + y] = acc;
+}
+
+// hess[x][y] += sum_k dEdcndr[x][k] dr[y][k] + dr[x][k] dEdcndr[y][k] + dr[x][k] tmp[k][y]
+// (disp.f90:303-306; grid.z = system).
+__global__ __launch_bounds__(kHT * kHT) void hess_assemble_kernel(DevBatch b, const double* __restrict__ dEdcndr,
+                                                                  const double* __restrict__ dr,
+                                                                  const double* __restrict__ tmp,
+                                                                  double* __restrict__ hess) {
+  const SysGeom g = b.geom[blockIdx.z];
+  const int n = g.natoms, ndim = 3 * n;
+  const int x = blockIdx.y * kHT + threadIdx.y, y = blockIdx.x * kHT + threadIdx.x;
+  __shared__ double Ax[kHT][kHT + 1], Dx[kHT][kHT + 1], Ay[kHT][kHT + 1], Dy[kHT][kHT + 1], Tk[kHT][kHT + 1];
+  const double* A = dEdcndr + 3 * g.c6_begin;
+  const double* D = dr + 3 * g.c6_begin;
+  const double* T = tmp + 3 * g.c6_begin;
+  double acc = 0.0;
+  for (int k0 = 0; k0 < n; k0 += kHT) {
+    const int kk = k0 + threadIdx.x;
+    const int xr = blockIdx.y * kHT + threadIdx.y, yr = blockIdx.x * kHT + threadIdx.y;
+    const bool kok = kk < n;
+    Ax[threadIdx.y][threadIdx.x] = (xr < ndim && kok) ? A[static_cast<long long>(xr) * n + kk] : 0.0;
+    Dx[threadIdx.y][threadIdx.x] = (xr < ndim && kok) ? D[static_cast<long long>(xr) * n + kk] : 0.0;
+    Ay[threadIdx.y][threadIdx.x] = (yr < ndim && kok) ? A[static_cast<long long>(yr) * n + kk] : 0.0;
+    Dy[threadIdx.y][threadIdx.x] = (yr < ndim && kok) ? D[static_cast<long long>(yr) * n + kk] : 0.0;
+    const int kt = k0 + threadIdx.y, yt = blockIdx.x * kHT + threadIdx.x;
+    Tk[threadIdx.y][threadIdx.x] = (kt < n && yt < ndim) ? T[static_cast<long long>(kt) * ndim + yt] : 0.0;
+    __syncthreads();
+    for (int q = 0; q < kHT; ++q)
+      acc += Ax[threadIdx.y][q] * Dy[threadIdx.x][q] + Dx[threadIdx.y][q] * Ay[threadIdx.x][q] +
+             Dx[threadIdx.y][q] * Tk[q][threadIdx.x];
+    __syncthreads();
+  }
+  if (x < ndim && y < ndim) hess[9 * g.c6_begin + static_cast<long long>(x) * ndim + y] += acc;
+}
+
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -1516,6 +2191,28 @@ void launch_count_triples(const DevBatch& b, bool mixed, const long long* offset
   else
     hipLaunchKernelGGL(count_triples_kernel<double>, dim3(b.natoms), dim3(kBlock), 0, st, b, offset,
                        static_cast<const Slot<double>*>(slots), counts);
+}
+
+void launch_hessian(const DevBatch& b, const HessWork& h, const long long* offset, const void* slots,
+                    long long total_slots, int max_natoms, hipStream_t st) {
+  const dim3 wgrid(blocks_for(b.natoms, kWavesPerBlock)), blk(kBlock);
+  hipLaunchKernelGGL(hess_weights_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b, h.cn, h.gw,
+                     h.gwd, h.gwd2);
+  hipLaunchKernelGGL(hess_c6_kernel, dim3(b.natoms), dim3(kBlock), 0, st, b, h.gw, h.gwd, h.gwd2, h.c6, h.dc6,
+                     h.d2c6, h.d2c6ij);
+  hipLaunchKernelGGL(hess_dcndr_kernel, wgrid, blk, 0, st, b, h.dr);
+  hipLaunchKernelGGL(hess2_kernel, wgrid, blk, 0, st, b, h.c6, h.dc6, h.d2c6, h.d2c6ij, h.hess, h.dedcn, h.dEdcndr,
+                     h.dEdcndcn);
+  if (total_slots > 0)
+    hipLaunchKernelGGL(hess3_kernel, dim3(blocks_for(total_slots, kBlock)), dim3(kBlock), 0, st, b, offset,
+                       static_cast<const Slot<double>*>(slots), h.c6, h.dc6, h.d2c6, h.d2c6ij, h.hess, h.dedcn,
+                       h.dEdcndr, h.dEdcndcn);
+  hipLaunchKernelGGL(hess_cn_kernel, wgrid, blk, 0, st, b, h.dedcn, h.hess);
+  const int ndim = 3 * max_natoms;
+  hipLaunchKernelGGL(hess_tmp_kernel, dim3(blocks_for(ndim, kHT), blocks_for(max_natoms, kHT), b.nsys),
+                     dim3(kHT, kHT), 0, st, b, h.dEdcndcn, h.dr, h.tmp);
+  hipLaunchKernelGGL(hess_assemble_kernel, dim3(blocks_for(ndim, kHT), blocks_for(ndim, kHT), b.nsys),
+                     dim3(kHT, kHT), 0, st, b, h.dEdcndr, h.dr, h.tmp, h.hess);
 }
 
 }  // namespace d3

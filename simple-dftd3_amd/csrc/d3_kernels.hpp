@@ -103,6 +103,20 @@ void launch_atm_pairs(const DevBatch& b, bool mixed, bool smooth, const long lon
                       const long long* offset, const void* slots, const DevWork& w, double* pair3s,
                       hipStream_t stream);
 void launch_pair3_finalize(const DevBatch& b, const double* pair3s, double* out, hipStream_t stream);
+// Analytic Hessian (FP64) of a batch; per system, atom order within the
+// system, offsets from c6_begin: c6, dc6, d2c6, d2c6ij and dEdcndcn n x n at
+// c6_begin; dr and dEdcndr 3n x n and tmp n x 3n at 3 c6_begin; hess 3n x 3n
+// at 9 c6_begin.  hess, dedcn, dEdcndr and dEdcndcn must be zeroed; the
+// FP64 three-body lists (offset, slots; total_slots entries) must be filled
+// and cn computed.  max_natoms: largest system of the batch.
+struct HessWork {
+  const double* cn;
+  double *gw, *gwd, *gwd2;
+  double *c6, *dc6, *d2c6, *d2c6ij;
+  double *dr, *dedcn, *dEdcndr, *dEdcndcn, *tmp, *hess;
+};
+void launch_hessian(const DevBatch& b, const HessWork& h, const long long* offset, const void* slots,
+                    long long total_slots, int max_natoms, hipStream_t stream);
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t stream);
 void launch_reduce(const DevBatch& b, const DevWork& w, double* energy, double* sigma, hipStream_t stream);
 void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream_t stream);

@@ -17,12 +17,13 @@ this library implements:
   ``gradient`` (n, 3) and ``virial`` (3, 3) in Hartree and bohr
   (interface.py:505-616), and ``get_pairwise_dispersion(param)`` returning
   the ``additive pairwise energy`` and ``non-additive pairwise energy``
-  (n, n) matrices (interface.py:618-636).
+  (n, n) matrices (interface.py:618-636), and ``get_hessian(param)`` returning
+  the ``energy`` and the analytic (3n, 3n) ``hessian`` (interface.py:637-658).
 
 All evaluation happens on the GPU through libd3hip.so.  Additions beyond the
-reference: ``DispersionModel.set_precision("fp64" | "mixed")``.  The Hessian,
-Ewald summation and counterpoise correction of the reference are not part of
-this library; their classes
+reference: ``DispersionModel.set_precision("fp64" | "mixed")``.  Ewald
+summation, work partitions and counterpoise correction of the reference are
+not part of this library; their classes
 exist so that imports keep working, and raise ``NotImplementedError`` when
 used.
 """
@@ -303,7 +304,14 @@ class DispersionModel(Structure):
         }
 
     def get_hessian(self, param: DampingParam) -> dict:
-        raise NotImplementedError("The dispersion Hessian is not available in this library")
+        """Analytical second derivatives of the dispersion energy with respect
+        to the nuclear coordinates (interface.py:637-658): ``energy`` and the
+        symmetric ``3*n`` by ``3*n`` ``hessian`` in Hartree per Bohr squared,
+        index ``3*i + c`` for component ``c`` of atom ``i`` (always FP64)."""
+        _energy = np.array(0.0)
+        _hessian = np.zeros((3 * len(self), 3 * len(self)))
+        library.get_dispersion_hessian(self._mol, self._disp, param._param, _energy, _hessian)
+        return {"energy": _energy, "hessian": _hessian}
 
 
 def _rename_kwargs(kwargs, old_name, new_name):

@@ -62,6 +62,9 @@ _SIGNATURES = {
     "dftd3_delete_param": (None, [ctypes.POINTER(_vp)]),
     "dftd3_get_dispersion": (None, [_vp, _vp, _vp, _vp, _dp, _dp, _dp]),
     "dftd3_get_pairwise_dispersion": (None, [_vp, _vp, _vp, _vp, _dp, _dp]),
+    "dftd3_get_dispersion_hessian": (None, [_vp, _vp, _vp, _vp, _dp, _dp]),
+    "d3hip_get_dispersion_hessian_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, _vp, _vp,
+                                                  _vp, _vp]),
     "d3hip_set_model_precision": (None, [_vp, _vp, ctypes.c_int]),
     "d3hip_new_workspace": (_vp, [_vp]),
     "d3hip_get_dispersion_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, ctypes.c_int,
@@ -300,6 +303,23 @@ def get_pairwise_dispersion(mol: StructureHandle, disp: ModelHandle, param: Para
     """Evaluate the pairwise representation of the dispersion energy"""
     return error_check(lib.dftd3_get_pairwise_dispersion)(mol.handle, disp.handle, param.handle,
                                                           _cast(_dp, energy2), _cast(_dp, energy3))
+
+
+def get_dispersion_hessian(mol: StructureHandle, disp: ModelHandle, param: ParamHandle, energy: np.ndarray,
+                           hessian: np.ndarray) -> None:
+    """Evaluate the energy and the analytic Hessian of the dispersion energy"""
+    return error_check(lib.dftd3_get_dispersion_hessian)(mol.handle, disp.handle, param.handle,
+                                                         _cast(_dp, energy), _cast(_dp, hessian))
+
+
+def get_dispersion_hessian_batch(workspace: ModelHandle, natoms, numbers, lattices, periodic, params, cutoffs,
+                                 positions_ptr: int, energy_ptr: Optional[int], hessian_ptr: int,
+                                 stream: Optional[int]) -> None:
+    """Batched energies and Hessians on device pointers (include/d3hip.h, Part 2)"""
+    return error_check(lib.d3hip_get_dispersion_hessian_batch)(
+        workspace.handle, len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices),
+        _cast(_bp, periodic), _cast(_dp, params), _cast(_dp, cutoffs), positions_ptr, energy_ptr, hessian_ptr,
+        stream)
 
 
 def get_pairwise_dispersion_batch(workspace: ModelHandle, natoms, numbers, lattices, periodic, params, cutoffs,

@@ -169,6 +169,34 @@ def blocks(out: dict, key: str):
     return views
 
 
+def run_hessian(spec: BatchSpec, positions: torch.Tensor, ws: Optional[library.ModelHandle] = None) -> dict:
+    """Energies and analytic Hessians of a batch (disp.f90:236-308), FP64, on
+    the current stream: ``energy`` (nsys,) and ``hessian``, a flat float64
+    tensor holding per system a (3n, 3n) block, the blocks one after another;
+    ``hessian_blocks(out)`` splits it into views."""
+    _check_positions(positions)
+    dev = positions.device
+    n = spec.natoms.astype(np.int64)
+    out = dict(energy=torch.empty(len(spec), dtype=torch.float64, device=dev),
+               hessian=torch.empty(int(np.sum(9 * n * n)), dtype=torch.float64, device=dev), natoms=spec.natoms)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    with torch.cuda.device(dev):
+        library.get_dispersion_hessian_batch(
+            ws or workspace(dev), spec.natoms, spec.numbers, spec.lattices, spec.periodic, spec.params, spec.cutoffs,
+            positions.data_ptr(), out["energy"].data_ptr(), out["hessian"].data_ptr(), stream)
+    return out
+
+
+def hessian_blocks(out: dict):
+    """(3n, 3n) views of the per-system blocks of ``run_hessian``."""
+    views, off = [], 0
+    for n in out["natoms"]:
+        m = 3 * int(n)
+        views.append(out["hessian"][off:off + m * m].view(m, m))
+        off += m * m
+    return views
+
+
 def count_terms(spec: BatchSpec, positions: torch.Tensor) -> np.ndarray:
     """(nsys, 3) int64 array of CN terms, two-body pair terms and ATM triple
     terms as the reference's loops visit them (counted on the GPU)."""

@@ -492,3 +492,66 @@ def test_z_damping_species_ids():
             perm[[k, last]] = perm[[last, k]]
             break
     assert e(num[perm], pos[perm]) == pytest.approx(base, rel=1e-13)
+
+
+# ------------------------------------------------------------------ Hessian
+WATER_DIMER = (np.array([8, 1, 1, 8, 1, 1]),
+               np.array([[-2.85, -0.05, +0.00], [-1.05, +0.00, +0.00], [-3.40, +1.65, +0.00],
+                         [+2.65, +0.00, +0.00], [+3.30, +1.00, +1.45], [+3.30, +1.00, -1.45]]))
+PBE0_ATM = dict(s6=1.0, s8=1.2177, a1=0.4145, a2=4.8593, s9=1.0, alp=14.0)  # param.f90:364-366, atm
+
+
+def _fd_hessian(num, pos, lat, pbc, kind, par, cut, step, ghost=None):
+    n = len(num)
+    h = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for c in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, c] += step
+            pm[i, c] -= step
+            gp = O.dispersion(num, pp, lat, pbc, kind, par, cut, ghost=ghost)["gradient"]
+            gm = O.dispersion(num, pm, lat, pbc, kind, par, cut, ghost=ghost)["gradient"]
+            h[3 * i + c] = (gp - gm).ravel() / (2 * step)
+    return h
+
+
+@pytest.mark.parametrize("ghost", [None, [2, 3]])
+def test_hessian_python_interface_case(ghost):
+    """python/dftd3/test_interface.py:359-442: the water dimer with PBE0-D3(BJ)
+    + ATM (and with ghost atoms 2, 3): symmetric to 1e-12, equal to central
+    differences of the analytic gradient (step 1e-6) within 1e-7, no NaN."""
+    num, pos = WATER_DIMER
+    mask = None
+    if ghost is not None:
+        mask = np.zeros(len(num), dtype=np.int32)
+        mask[ghost] = 1
+    r = O.hessian(num, pos, None, None, O.RATIONAL, PBE0_ATM, O.DEFAULT_CUTOFF, ghost=mask)
+    h = r["hessian"]
+    assert not np.isnan(h).any()
+    np.testing.assert_allclose(h, h.T, rtol=0, atol=1e-12)
+    fd = _fd_hessian(num, pos, None, None, O.RATIONAL, PBE0_ATM, O.DEFAULT_CUTOFF, 1e-6, ghost=mask)
+    np.testing.assert_allclose(h, fd, rtol=0, atol=1e-7)
+    e = O.dispersion(num, pos, None, None, O.RATIONAL, PBE0_ATM, O.DEFAULT_CUTOFF, grad=False, ghost=mask)["energy"]
+    assert r["energy"] == pytest.approx(e, rel=1e-14)
+
+
+@pytest.mark.parametrize("kind", ALL_KINDS)
+def test_hessian_finite_difference_of_gradient(kind):
+    """Analytic Hessian of every damping family (two-body, ATM, CN and the
+    dCN/dR products) against central differences of the analytic gradient
+    (h = 1e-4 bohr, truncation ~1e-10): a molecule, and for rational and zero
+    damping a small periodic cell with self images and smooth cutoff widths."""
+    from dftd3 import synthetic
+    num, pos = synthetic.molecule_c0(seed=4, n=8)
+    p = PARAMS[kind]
+    h = O.hessian(num, pos, None, None, _kind(kind), p)["hessian"]
+    np.testing.assert_allclose(h, h.T, rtol=0, atol=1e-15)
+    fd = _fd_hessian(num, pos, None, None, _kind(kind), p, O.DEFAULT_CUTOFF, 1e-4)
+    np.testing.assert_allclose(h, fd, rtol=0, atol=1e-6 * np.abs(h).max())
+    if kind in ("rational", "zero"):
+        from helpers import cell
+        cnum, cpos, lat = cell(7, 3, 8.0)
+        cut = (20.0, 25.0, 12.0, 2.0, 3.0)
+        h = O.hessian(cnum, cpos, lat, [True] * 3, _kind(kind), p, cut)["hessian"]
+        fd = _fd_hessian(cnum, cpos, lat, [True] * 3, _kind(kind), p, cut, 1e-4)
+        np.testing.assert_allclose(h, fd, rtol=0, atol=1e-6 * np.abs(h).max())
