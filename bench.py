@@ -311,6 +311,55 @@ def bench_pairwise(args, systems, config, spec, positions, counts):
     return 0
 
 
+def bench_hessian(args, systems, config, spec, positions, counts):
+    """Analytic Hessian (get_dispersion_hessian, disp.f90:236-308, FP64) of the
+    workload, timed per kernel with HIP events, against the oracle's
+    restatement on the host (single-threaded loops, one evaluation)."""
+    import torch
+    from dftd3 import library, torch_api
+    ws = torch_api.workspace(positions.device)
+    run = lambda: torch_api.run_hessian(spec, positions, ws=ws)  # noqa: E731
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    library.set_kernel_timing(ws, True)
+    t_ev = timed_steps(run, args.steps, torch.cuda.synchronize)
+    km = {k: library.get_kernel_time(ws, k) for k in KERNEL_GROUPS + ("hessian",)}
+    library.set_kernel_timing(ws, False)
+    t = timed_steps(run, args.steps, torch.cuda.synchronize)
+    out = run()
+    torch.cuda.synchronize()
+    km = {k: v[0] / max(v[1], 1) for k, v in km.items() if v[1]}
+    log(f"fp64: energy + Hessian {t * 1e3:.2f} ms/step (Hessian kernels {km.get('hessian', 0.0):.2f} ms)")
+    n = config["natoms"]
+    line = dict(
+        metric=METRIC, value=counts[1] / t, unit="pair-terms/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=t * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+        data="synthetic",
+        config=dict(config, pair_terms=int(counts[1]), triple_terms=int(counts[2]), cn_terms=int(counts[0]),
+                    parallelism="single-gpu", hessian_dim=3 * n,
+                    step="energy + analytic (3n x 3n) Hessian, FP64"),
+        molecules_per_s=config["cells"] / t, triple_terms_per_s=counts[2] / t,
+        ms_per_step_with_kernel_events=t_ev * 1e3, kernel_ms=km, roofline=None,
+    )
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        s = systems[0]
+        log("cpu baseline: oracle Hessian, one evaluation")
+        t0 = time.perf_counter()
+        ref = oracle.hessian(s["numbers"], s["positions"], s.get("lattice"), s.get("periodic"), oracle.RATIONAL,
+                             s["param"])
+        t_cpu = time.perf_counter() - t0
+        h = torch_api.hessian_blocks(out)[0].cpu().numpy()
+        line["max_abs_diff_vs_oracle"] = float(np.abs(h - ref["hessian"]).max())
+        line["cpu_baseline"] = dict(value=float(counts[1] / t_cpu), unit="pair-terms/s", cores=1, kind="port",
+                                    sample=f"oracle Hessian, one full evaluation ({t_cpu:.2f} s)",
+                                    seconds_per_eval=t_cpu)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +373,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pairwise", action="store_true",
                     help="time the pairwise-resolved energies (pair energy matrices) of the workload instead")
+    ap.add_argument("--hessian", action="store_true",
+                    help="time the energy and analytic Hessian of the workload instead (FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has a single MI355X; "
@@ -353,6 +404,8 @@ def main():
         return bench_autograd(args, systems, config, spec, positions, counts)
     if args.pairwise:
         return bench_pairwise(args, systems, config, spec, positions, counts)
+    if args.hessian:
+        return bench_hessian(args, systems, config, spec, positions, counts)
     ws = torch_api.workspace(dev)
     outs = {}
     res = {}

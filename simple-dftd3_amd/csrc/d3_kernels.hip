@@ -1676,12 +1676,15 @@ __global__ __launch_bounds__(kBlock) void hess2_kernel(DevBatch b, const double*
   }
 }
 
-// atm.f90:38-429 get_atm_dispersion_hessian: thread per (atom i, slot a of
-// L(i)), looping over the slots b after a, i.e. over the triple terms of the
+// atm.f90:38-429 get_atm_dispersion_hessian: wavefront per (atom i, slot a
+// of L(i)), lanes over the slots b after a, i.e. over the triple terms of the
 // three-body lists (weights as in atm_kernel: 1/2 per slot that is an image
 // of atom i, 4/3 more when both are).  Each term is K(u1, u2, u3) with
 // u1 = |v_a|^2, u2 = |v_b|^2, u3 = |v_b - v_a|^2; its gradient and Hessian in
-// u are contracted with du/dR and added with atomics.
+// u are contracted with du/dR.  Entries between the atoms of i and a (the
+// Hessian blocks, dEdcndr, dEdcndcn and dEdcn entries every term of the
+// wavefront shares) are summed in registers and added once per wavefront;
+// the entries of the atom of b are added with atomics.
 __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long long* __restrict__ offset,
                                                        const Slot<double>* __restrict__ slots,
                                                        const double* __restrict__ c6,
@@ -1690,7 +1693,8 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
                                                        const double* __restrict__ d2c6ij, double* __restrict__ hess,
                                                        double* __restrict__ dedcn, double* __restrict__ dEdcndr,
                                                        double* __restrict__ dEdcndcn) {
-  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int lane = wave_lane();
+  const long long t = static_cast<long long>(blockIdx.x) * kWavesPerBlock + (threadIdx.x >> 6);
   const long long total = offset[b.natoms];
   if (t >= total) return;
   int lo = 0, hi = b.natoms;
@@ -1711,7 +1715,17 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
   const Slot<double> sa = slots[t];
   const long long end = offset[i + 1];
   const int at0 = i - g.atom_begin, at1 = sa.atom - g.atom_begin;
-  for (long long bi = t + 1; bi < end; ++bi) {
+  // position of an atom among the fixed atoms (i, atom of a), -1 otherwise
+  auto fixed = [&](int atom) { return atom == at0 ? 0 : (atom == at1 ? 1 : -1); };
+  double hf[4][9], df[2][3][2], cf[2][2], ef[2];
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 9; ++c) hf[k][c] = 0.0;
+  for (int a = 0; a < 2; ++a) {
+    for (int c = 0; c < 3; ++c) df[a][c][0] = df[a][c][1] = 0.0;
+    cf[a][0] = cf[a][1] = 0.0;
+    ef[a] = 0.0;
+  }
+  for (long long bi = t + 1 + lane; bi < end; bi += kWave) {
     const Slot<double> sb = slots[bi];
     double vec[3][3];
     vec[0][0] = sa.vx; vec[0][1] = sa.vy; vec[0][2] = sa.vz;
@@ -1800,12 +1814,14 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
         double dg = 0.0;
         for (int q = 0; q < 3; ++q) dg += kp[q] * sigp[a][q] * sigp[bb][q];
         dg *= 2.0 * pq;
+        const int fa = fixed(at[a]), fb = fixed(at[bb]);
         double* __restrict__ blk = hess + 9 * c0 + static_cast<long long>(3 * at[a]) * ndim + 3 * at[bb];
         for (int c = 0; c < 3; ++c)
           for (int d = 0; d < 3; ++d) {
             double tmp = c == d ? dg : 0.0;
             for (int r = 0; r < 3; ++r) tmp += gk[a][r][c] * 2.0 * sigp[bb][r] * vec[r][d];
-            atomicAdd(blk + static_cast<long long>(c) * ndim + d, tmp);
+            if (fa >= 0 && fb >= 0) hf[2 * fa + fb][3 * c + d] += tmp;
+            else atomicAdd(blk + static_cast<long long>(c) * ndim + d, tmp);
           }
       }
     // derivatives through C6: pairs (i, a), (i, b), (a, b)
@@ -1833,31 +1849,70 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
       }
     }
     for (int ie = 0; ie < nent; ++ie) {
-      const int ka = ent_atom[ie], kq = ent_pair[ie];
-      atomicAdd(dedcn + g.atom_begin + ka, ec[kq] * ent_coef[ie]);
-      for (int jf = 0; jf < nent; ++jf)
-        atomicAdd(dEdcndcn + c0 + static_cast<long long>(ka) * n + ent_atom[jf],
-                  ecc[kq][ent_pair[jf]] * ent_coef[ie] * ent_coef[jf]);
-      for (int a = 0; a < 3; ++a)
+      const int ka = ent_atom[ie], kq = ent_pair[ie], fk = fixed(ka);
+      if (fk >= 0) ef[fk] += ec[kq] * ent_coef[ie];
+      else atomicAdd(dedcn + g.atom_begin + ka, ec[kq] * ent_coef[ie]);
+      for (int jf = 0; jf < nent; ++jf) {
+        const double v = ecc[kq][ent_pair[jf]] * ent_coef[ie] * ent_coef[jf];
+        const int fj = fixed(ent_atom[jf]);
+        if (fk >= 0 && fj >= 0) cf[fk][fj] += v;
+        else atomicAdd(dEdcndcn + c0 + static_cast<long long>(ka) * n + ent_atom[jf], v);
+      }
+      for (int a = 0; a < 3; ++a) {
+        const int fa = fixed(at[a]);
         for (int c = 0; c < 3; ++c) {
           double egr = 0.0;
           for (int q = 0; q < 3; ++q) egr += euc[q][kq] * 2.0 * sigp[a][q] * vec[q][c];
-          atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * at[a] + c) * n + ka, egr * ent_coef[ie]);
+          if (fa >= 0 && fk >= 0) df[fa][c][fk] += egr * ent_coef[ie];
+          else atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * at[a] + c) * n + ka, egr * ent_coef[ie]);
         }
+      }
     }
     for (int q = 0; q < 3; ++q) {
       const int a0 = pat[q][0], a1 = pat[q][1];
       const long long o00 = c0 + static_cast<long long>(a0) * n + a0, o01 = c0 + static_cast<long long>(a0) * n + a1;
+      const int f0 = fixed(a0), f1 = fixed(a1);
       if (a0 != a1) {
         const long long o11 = c0 + static_cast<long long>(a1) * n + a1, o10 = c0 + static_cast<long long>(a1) * n + a0;
-        atomicAdd(dEdcndcn + o00, ec[q] * d2c6[o01]);
-        atomicAdd(dEdcndcn + o11, ec[q] * d2c6[o10]);
-        atomicAdd(dEdcndcn + o01, ec[q] * d2c6ij[o01]);
-        atomicAdd(dEdcndcn + o10, ec[q] * d2c6ij[o01]);
+        const double v00 = ec[q] * d2c6[o01], v11 = ec[q] * d2c6[o10], vx = ec[q] * d2c6ij[o01];
+        if (f0 >= 0 && f1 >= 0) {
+          cf[f0][f0] += v00;
+          cf[f1][f1] += v11;
+          cf[f0][f1] += vx;
+          cf[f1][f0] += vx;
+        } else {
+          atomicAdd(dEdcndcn + o00, v00);
+          atomicAdd(dEdcndcn + o11, v11);
+          atomicAdd(dEdcndcn + o01, vx);
+          atomicAdd(dEdcndcn + o10, vx);
+        }
       } else {
-        atomicAdd(dEdcndcn + o00, ec[q] * (2.0 * d2c6[o00] + 2.0 * d2c6ij[o00]));
+        const double v = ec[q] * (2.0 * d2c6[o00] + 2.0 * d2c6ij[o00]);
+        if (f0 >= 0) cf[f0][f0] += v;
+        else atomicAdd(dEdcndcn + o00, v);
       }
     }
+  }
+  // the entries between the fixed atoms, once per wavefront
+  const int fat[2] = {at0, at1};
+  for (int fa = 0; fa < 2; ++fa) {
+    if (fa == 1 && at1 == at0) break;  // a is an image of i: everything is in the first row
+    for (int fb = 0; fb < 2; ++fb) {
+      if (fb == 1 && at1 == at0) break;
+      for (int c = 0; c < 9; ++c) {
+        const double v = wave_sum(hf[2 * fa + fb][c]);
+        if (lane == 0 && v != 0.0)
+          atomicAdd(hess + 9 * c0 + static_cast<long long>(3 * fat[fa] + c / 3) * ndim + 3 * fat[fb] + c % 3, v);
+      }
+      const double v = wave_sum(cf[fa][fb]);
+      if (lane == 0 && v != 0.0) atomicAdd(dEdcndcn + c0 + static_cast<long long>(fat[fa]) * n + fat[fb], v);
+      for (int c = 0; c < 3; ++c) {
+        const double w = wave_sum(df[fa][c][fb]);
+        if (lane == 0 && w != 0.0) atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * fat[fa] + c) * n + fat[fb], w);
+      }
+    }
+    const double v = wave_sum(ef[fa]);
+    if (lane == 0 && v != 0.0) atomicAdd(dedcn + g.atom_begin + fat[fa], v);
   }
 }
 
@@ -2204,7 +2259,7 @@ void launch_hessian(const DevBatch& b, const HessWork& h, const long long* offse
   hipLaunchKernelGGL(hess2_kernel, wgrid, blk, 0, st, b, h.c6, h.dc6, h.d2c6, h.d2c6ij, h.hess, h.dedcn, h.dEdcndr,
                      h.dEdcndcn);
   if (total_slots > 0)
-    hipLaunchKernelGGL(hess3_kernel, dim3(blocks_for(total_slots, kBlock)), dim3(kBlock), 0, st, b, offset,
+    hipLaunchKernelGGL(hess3_kernel, dim3(blocks_for(total_slots, kWavesPerBlock)), dim3(kBlock), 0, st, b, offset,
                        static_cast<const Slot<double>*>(slots), h.c6, h.dc6, h.d2c6, h.d2c6ij, h.hess, h.dedcn,
                        h.dEdcndr, h.dEdcndcn);
   hipLaunchKernelGGL(hess_cn_kernel, wgrid, blk, 0, st, b, h.dedcn, h.hess);
