@@ -25,6 +25,7 @@
  * gradient[n][3], sigma[3][3] stored in Fortran order sigma(a,b) at
  * sigma[(b)*3 + (a)] -- i.e. the array the C API of the reference returns.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1717,6 +1718,489 @@ int d3o_get_dispersion_hessian(int nat, const int *num, const double *xyz, const
     free(cn); free(dr); free(gw); free(gwd); free(gwd2); free(c6); free(dc6); free(d2c6); free(d2c6ij);
     free(dEdcn); free(dEdcndr); free(dEdcndcn);
     return 0;
+}
+
+/* ------------------------------------------------------ reciprocal space */
+/* fourier/jacobi.f90:40-122 symmetric_eigendecomposition: cyclic Jacobi
+ * rotations (at most 100 sweeps), eigenvalues sorted by decreasing magnitude.
+ * amat, evec: ndim x ndim column-major. */
+static void jacobi_eig(int ndim, const double *amat, double *eval, double *evec)
+{
+#define A_(r, c) work[(size_t)(r) + (size_t)(c) * ndim]
+#define V_(r, c) evec[(size_t)(r) + (size_t)(c) * ndim]
+    memset(evec, 0, sizeof(double) * ndim * ndim);
+    for (int p = 0; p < ndim; ++p) V_(p, p) = 1.0;
+    memset(eval, 0, sizeof(double) * ndim);
+    double anorm = 0.0;
+    for (int k = 0; k < ndim * ndim; ++k) anorm += amat[k] * amat[k];
+    anorm = sqrt(anorm);
+    if (anorm <= 0.0) return;
+    double skip = 0.01 * EPS * anorm;
+    double *work = (double *)malloc(sizeof(double) * ndim * ndim);
+    memcpy(work, amat, sizeof(double) * ndim * ndim);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < ndim - 1; ++p)
+            for (int q = p + 1; q < ndim; ++q) off += A_(p, q) * A_(p, q);
+        if (sqrt(2.0 * off) <= EPS * anorm) break;
+        for (int p = 0; p < ndim - 1; ++p)
+            for (int q = p + 1; q < ndim; ++q) {
+                if (fabs(A_(p, q)) <= skip) continue;
+                double theta = 0.5 * (A_(q, q) - A_(p, p)) / A_(p, q);
+                double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+                double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+                for (int j = 0; j < ndim; ++j) {
+                    double ajp = A_(j, p), ajq = A_(j, q);
+                    A_(j, p) = c * ajp - sn * ajq;
+                    A_(j, q) = sn * ajp + c * ajq;
+                }
+                for (int j = 0; j < ndim; ++j) {
+                    double apj = A_(p, j), aqj = A_(q, j);
+                    A_(p, j) = c * apj - sn * aqj;
+                    A_(q, j) = sn * apj + c * aqj;
+                }
+                A_(p, q) = 0.0;
+                A_(q, p) = 0.0;
+                for (int j = 0; j < ndim; ++j) {
+                    double vjp = V_(j, p), vjq = V_(j, q);
+                    V_(j, p) = c * vjp - sn * vjq;
+                    V_(j, q) = sn * vjp + c * vjq;
+                }
+            }
+    }
+    for (int p = 0; p < ndim; ++p) eval[p] = A_(p, p);
+    for (int p = 0; p < ndim - 1; ++p) {
+        int imax = p;
+        for (int k = p + 1; k < ndim; ++k)
+            if (fabs(eval[k]) > fabs(eval[imax])) imax = k;
+        if (imax == p) continue;
+        double t = eval[p]; eval[p] = eval[imax]; eval[imax] = t;
+        for (int j = 0; j < ndim; ++j) { t = V_(j, p); V_(j, p) = V_(j, imax); V_(j, imax) = t; }
+    }
+    free(work);
+#undef A_
+#undef V_
+}
+
+/* fourier/decomposition.f90:104-175 new_lowrank_c6 over the species of the
+ * structure (first appearance order): the reference C6 of all (species,
+ * reference) pairs as a symmetric matrix, its eigenpairs by decreasing
+ * magnitude, kept until the largest relative error (floor 1e-6 of the largest
+ * entry) is at most the tolerance, or the requested rank.  vec is
+ * [nid][MAX_REF][rank]. */
+typedef struct { int rank, nid, ndim; double error, *lambda, *vec; } d3o_lowrank;
+
+static void lowrank_new(int nat, const int *num, int want_rank, double tol, d3o_lowrank *lr, int *id)
+{
+    species_ids(nat, num, id);
+    int nid = 0;
+    for (int i = 0; i < nat; ++i) if (id[i] > nid) nid = id[i];
+    int zsp[MAX_ELEM] = {0};
+    for (int i = 0; i < nat; ++i) zsp[id[i] - 1] = num[i];
+    int ndim = 0, spmap[MAX_ELEM * MAX_REF], refmap[MAX_ELEM * MAX_REF];
+    for (int sp = 0; sp < nid; ++sp)
+        for (int r = 0; r < g_nref[zsp[sp] - 1]; ++r) { spmap[ndim] = sp; refmap[ndim] = r; ++ndim; }
+    double *amat = (double *)calloc((size_t)ndim * ndim, sizeof(double));
+    for (int ii = 0; ii < ndim; ++ii)
+        for (int jj = 0; jj < ndim; ++jj)
+            amat[jj + ii * ndim] = d3o_get_c6(refmap[jj] + 1, refmap[ii] + 1, zsp[spmap[jj]], zsp[spmap[ii]]);
+    double *eval = (double *)malloc(sizeof(double) * ndim), *evec = (double *)malloc(sizeof(double) * ndim * ndim);
+    jacobi_eig(ndim, amat, eval, evec);
+    double amax = 0.0;
+    for (int k = 0; k < ndim * ndim; ++k) if (fabs(amat[k]) > amax) amax = fabs(amat[k]);
+    double floor_ = 1.0e-6 * amax;
+    double *res = (double *)malloc(sizeof(double) * ndim * ndim);
+    memcpy(res, amat, sizeof(double) * ndim * ndim);
+    int nrank = ndim;
+    double err = 0.0;
+    for (int il = 0; il < ndim; ++il) {
+        for (int ii = 0; ii < ndim; ++ii)
+            for (int jj = 0; jj < ndim; ++jj)
+                res[jj + ii * ndim] -= eval[il] * evec[jj + il * ndim] * evec[ii + il * ndim];
+        err = 0.0;
+        for (int k = 0; k < ndim * ndim; ++k) {
+            double e = fabs(res[k]) / fmax(fabs(amat[k]), floor_);
+            if (e > err) err = e;
+        }
+        if (want_rank > 0) {
+            if (il + 1 >= (want_rank < ndim ? want_rank : ndim)) { nrank = il + 1; break; }
+        } else if (err <= tol) {
+            nrank = il + 1;
+            break;
+        }
+    }
+    lr->rank = nrank;
+    lr->nid = nid;
+    lr->ndim = ndim;
+    lr->error = err;
+    lr->lambda = (double *)malloc(sizeof(double) * nrank);
+    lr->vec = (double *)calloc((size_t)nid * MAX_REF * nrank, sizeof(double));
+    for (int il = 0; il < nrank; ++il) {
+        lr->lambda[il] = eval[il];
+        for (int ii = 0; ii < ndim; ++ii)
+            lr->vec[((size_t)spmap[ii] * MAX_REF + refmap[ii]) * nrank + il] = evec[ii + il * ndim];
+    }
+    free(amat); free(eval); free(evec); free(res);
+}
+
+/* decomposition.f90:202-242 get_weights: c6l[i][l] = vec(:, id_i, l) . w_i (and
+ * dc6l with the weight derivatives); ghost atoms have zero rows. */
+static void lowrank_weights(const d3o_lowrank *lr, int nat, const int *num, const int *id, const int *ghost,
+                            int mref, const double *gw, const double *gwd, double *c6l, double *dc6l)
+{
+    int R = lr->rank;
+    for (int i = 0; i < nat; ++i)
+        for (int l = 0; l < R; ++l) {
+            double a = 0.0, d = 0.0;
+            if (!(ghost && ghost[i]))
+                for (int r = 0; r < g_nref[num[i] - 1]; ++r) {
+                    double v = lr->vec[((size_t)(id[i] - 1) * MAX_REF + r) * R + l];
+                    a += v * gw[i * mref + r];
+                    if (gwd) d += v * gwd[i * mref + r];
+                }
+            c6l[(size_t)i * R + l] = a;
+            if (dc6l) dc6l[(size_t)i * R + l] = d;
+        }
+}
+
+/* decomposition.f90:247-296 get_atomic_c6 of the low-rank model (Fortran order). */
+static void lowrank_c6(const d3o_lowrank *lr, int nat, const double *c6l, const double *dc6l, double *c6,
+                       double *dc6)
+{
+    int R = lr->rank;
+    for (int i = 0; i < nat; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double a = 0.0, di = 0.0, dj = 0.0;
+            for (int l = 0; l < R; ++l) {
+                a += lr->lambda[l] * c6l[(size_t)i * R + l] * c6l[(size_t)j * R + l];
+                if (dc6l) {
+                    di += lr->lambda[l] * dc6l[(size_t)i * R + l] * c6l[(size_t)j * R + l];
+                    dj += lr->lambda[l] * c6l[(size_t)i * R + l] * dc6l[(size_t)j * R + l];
+                }
+            }
+            c6[FIDX(i, j)] = a; c6[FIDX(j, i)] = a;
+            if (dc6) { dc6[FIDX(i, j)] = di; dc6[FIDX(j, i)] = dj; }
+        }
+}
+
+/* fourier/kernel.f90: a term pref * r^m / (r^alpha + radius^alpha). */
+typedef struct { double pref; int m, alpha; double radius; } d3o_term;
+
+/* kernel.f90:83-129 get_fourier_transform */
+static void fourier_transform(const d3o_term *t, double k, double *phi, double *dphi)
+{
+    *phi = 0.0; *dphi = 0.0;
+    if (fabs(t->pref) <= 0.0) return;
+    int nexp = t->m - t->alpha + 2;
+    double pre = 4.0 * M_PI * M_PI / t->alpha * t->pref * pow(t->radius, nexp);
+    if (k <= 0.0) {
+        *phi = pre * t->radius / sin(M_PI * (t->m + 3) / (double)t->alpha);
+        return;
+    }
+    int npole = t->alpha / 2;
+    double krad = k * t->radius, acc = 0.0, dacc = 0.0;
+    for (int ip = 0; ip < npole; ++ip) {
+        double theta = M_PI * (2 * ip + 1) / (double)t->alpha;
+        double ct = cos(theta), st = sin(theta);
+        double arg = 0.5 * M_PI + nexp * theta + krad * ct;
+        double damp = exp(-krad * st);
+        acc += sin(arg) * damp;
+        dacc += cos(arg + theta) * damp;
+    }
+    *phi = pre * acc / k;
+    *dphi = (pre * t->radius * dacc - *phi) / k;
+}
+
+/* kernel.f90:133-147 get_potential_zero, :151-164 get_reciprocal_cutoff */
+static double potential_zero(const d3o_term *t) { return t->m > 0 ? 0.0 : t->pref / pow(t->radius, t->alpha); }
+static double reciprocal_cutoff(const d3o_term *t, double tol)
+{
+    return -log(tol) / (t->radius * sin(M_PI / (double)t->alpha));
+}
+
+/* get_fourier_terms of the rational (rational.f90) and zero (zero.f90:127-163)
+ * damping for the species of atoms zi, zj; returns the number of terms. */
+static int fourier_terms(const d3o_param *p, int zi, int zj, d3o_term *t)
+{
+    if (fabs(p->s6) < EPS && fabs(p->s8) < EPS) return 0;
+    double rrij = 3 * d3o_r4r2(zi) * d3o_r4r2(zj);
+    if (p->kind == K_RATIONAL) {
+        double r0 = p->a1 * sqrt(rrij) + p->a2;
+        t[0] = (d3o_term){p->s6, 0, 6, r0};
+        t[1] = (d3o_term){p->s8 * rrij, 0, 8, r0};
+    } else {
+        int alp6 = (int)lround(p->alp), alp8 = alp6 + 2;
+        double r0 = d3o_rvdw(zj, zi);
+        t[0] = (d3o_term){p->s6, alp6 - 6, alp6, pow(6.0, 1.0 / alp6) * p->a1 * r0};
+        t[1] = (d3o_term){p->s8 * rrij, alp8 - 8, alp8, pow(6.0, 1.0 / alp8) * p->a2 * r0};
+    }
+    return 2;
+}
+
+/*
+ * fourier/ewald.f90:47-252 get_dispersion_ewald with damping.f90:306-385 (the
+ * default reciprocal cutoff from the damping radii, kcut_tolerance 1e-8):
+ * the two-body energy of the 3D-periodic system as a sum over reciprocal
+ * lattice vectors of the structure factors of the low-rank C6 weights.
+ * Returns nonzero when the damping function has no Fourier representation.
+ */
+static int ewald2(const d3o_param *p, int nat, const int *num, const int *id, const double *xyz,
+                  const double *latflat, const d3o_lowrank *lr, const double *c6l, const double *dc6l, double kcut_req,
+                  double *energy, double *dEdcn, double *gradient, double *sigma, int grad)
+{
+    int nid = lr->nid, R = lr->rank;
+    int zsp[MAX_ELEM] = {0};
+    for (int i = 0; i < nat; ++i) zsp[id[i] - 1] = num[i];
+    d3o_term *terms = (d3o_term *)calloc((size_t)nid * nid * 2, sizeof(d3o_term));
+    int *nterm = (int *)calloc((size_t)nid * nid, sizeof(int));
+    double kcut = 0.0;
+    for (int a = 0; a < nid; ++a)
+        for (int b = 0; b < nid; ++b) {
+            nterm[a * nid + b] = fourier_terms(p, zsp[a], zsp[b], terms + 2 * (a * nid + b));
+            for (int t = 0; t < nterm[a * nid + b]; ++t) {
+                const d3o_term *tt = terms + 2 * (a * nid + b) + t;
+                /* kernel.f90:63-78 is_supported_term */
+                if (!(tt->alpha > 0 && tt->m >= 0 && tt->alpha % 2 == 0 && tt->m % 2 == 0 && tt->m + 3 < tt->alpha
+                      && tt->radius > 0.0)) {
+                    free(terms); free(nterm);
+                    return 2;
+                }
+                double kc = reciprocal_cutoff(tt, 1.0e-8);
+                if (kc > kcut) kcut = kc;
+            }
+        }
+    if (kcut <= 0.0) { free(terms); free(nterm); return 0; }
+    if (kcut_req > 0.0) kcut = kcut_req;
+    /* reciprocal vectors b_j (a_i . b_j = 2 pi delta_ij): row j of 2 pi M^-1,
+     * M the matrix whose columns are the lattice vectors (ewald.f90:118) */
+    double L[3][3], inv[3][3], rec[9];
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) L[r][c] = latflat[3 * r + c];
+    matinv3(L, inv);
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) rec[3 * r + c] = 2.0 * M_PI * inv[r][c];
+    double vol = fabs(L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) - L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0])
+                      + L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]));
+    int pbc3[3] = {1, 1, 1};
+    int nk = d3o_lattice_points(pbc3, rec, kcut, NULL, 0);
+    double *kp = (double *)malloc(sizeof(double) * 3 * nk);
+    d3o_lattice_points(pbc3, rec, kcut, kp, nk);
+    double *esum = (double *)calloc(nat, sizeof(double)), *dsum = (double *)calloc(nat, sizeof(double));
+    double *gsum = (double *)calloc(3 * (size_t)nat, sizeof(double)), ssum[9] = {0};
+    for (int i = 0; i < nat; ++i) {
+        double pval = 0.0;
+        int a = id[i] - 1;
+        for (int t = 0; t < nterm[a * nid + a]; ++t) pval += potential_zero(terms + 2 * (a * nid + a) + t);
+        for (int l = 0; l < R; ++l) {
+            esum[i] += 0.5 * lr->lambda[l] * c6l[(size_t)i * R + l] * c6l[(size_t)i * R + l] * pval;
+            if (grad) dsum[i] += lr->lambda[l] * c6l[(size_t)i * R + l] * dc6l[(size_t)i * R + l] * pval;
+        }
+    }
+    double erecip = 0.0;
+    for (int i = 0; i < nat; ++i) erecip -= esum[i];
+    double *phihat = (double *)malloc(sizeof(double) * nid * nid), *dphihat = (double *)malloc(sizeof(double) * nid * nid);
+    double *sfr = (double *)malloc(sizeof(double) * R * nid), *sfi = (double *)malloc(sizeof(double) * R * nid);
+    double *fr = (double *)malloc(sizeof(double) * R * nid), *fi = (double *)malloc(sizeof(double) * R * nid);
+    double *cr = (double *)malloc(sizeof(double) * nat), *ci = (double *)malloc(sizeof(double) * nat);
+    double *el = (double *)calloc(nat, sizeof(double)), *dl = (double *)calloc(nat, sizeof(double));
+    double *gl = (double *)calloc(3 * (size_t)nat, sizeof(double)), sl[9] = {0};
+    for (int ik = 0; ik < nk; ++ik) {
+        const double *kv = kp + 3 * ik;
+        double knorm = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+        if (knorm > kcut) continue;
+        for (int a = 0; a < nid; ++a)
+            for (int b = 0; b <= a; ++b) {
+                double ph = 0.0, dph = 0.0;
+                for (int t = 0; t < nterm[a * nid + b]; ++t) {
+                    double pv, dpv;
+                    fourier_transform(terms + 2 * (a * nid + b) + t, knorm, &pv, &dpv);
+                    ph += pv; dph += dpv;
+                }
+                phihat[b * nid + a] = phihat[a * nid + b] = ph;
+                dphihat[b * nid + a] = dphihat[a * nid + b] = dph;
+            }
+        memset(sfr, 0, sizeof(double) * R * nid);
+        memset(sfi, 0, sizeof(double) * R * nid);
+        for (int i = 0; i < nat; ++i) {
+            double kr = kv[0] * xyz[3 * i] + kv[1] * xyz[3 * i + 1] + kv[2] * xyz[3 * i + 2];
+            cr[i] = cos(kr); ci[i] = sin(kr);
+            int a = id[i] - 1;
+            for (int l = 0; l < R; ++l) {
+                sfr[l * nid + a] += c6l[(size_t)i * R + l] * cr[i];
+                sfi[l * nid + a] += c6l[(size_t)i * R + l] * ci[i];
+            }
+        }
+        memset(fr, 0, sizeof(double) * R * nid);
+        memset(fi, 0, sizeof(double) * R * nid);
+        for (int a = 0; a < nid; ++a)
+            for (int b = 0; b < nid; ++b)
+                for (int l = 0; l < R; ++l) {
+                    fr[l * nid + a] += phihat[b * nid + a] * sfr[l * nid + b];
+                    fi[l * nid + a] += phihat[b * nid + a] * sfi[l * nid + b];
+                }
+        for (int i = 0; i < nat; ++i) {
+            int a = id[i] - 1;
+            for (int l = 0; l < R; ++l) {
+                /* zval = phase * conj(fvec) */
+                double zre = cr[i] * fr[l * nid + a] + ci[i] * fi[l * nid + a];
+                double zim = ci[i] * fr[l * nid + a] - cr[i] * fi[l * nid + a];
+                el[i] -= 0.5 * lr->lambda[l] * c6l[(size_t)i * R + l] * zre / vol;
+                if (grad) {
+                    dl[i] -= lr->lambda[l] * dc6l[(size_t)i * R + l] * zre / vol;
+                    for (int c = 0; c < 3; ++c)
+                        gl[3 * i + c] += lr->lambda[l] * c6l[(size_t)i * R + l] * zim * kv[c] / vol;
+                }
+            }
+        }
+        if (grad && knorm > 0.0) {
+            double qq = 0.0;
+            for (int a = 0; a < nid; ++a)
+                for (int b = 0; b < nid; ++b)
+                    for (int l = 0; l < R; ++l)
+                        qq += lr->lambda[l] * dphihat[b * nid + a]
+                            * (sfr[l * nid + a] * sfr[l * nid + b] + sfi[l * nid + a] * sfi[l * nid + b]);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) sl[c * 3 + r] += 0.5 * qq * kv[r] * kv[c] / (knorm * vol);
+        }
+    }
+    for (int i = 0; i < nat; ++i) {
+        esum[i] += el[i];
+        dsum[i] += dl[i];
+        for (int c = 0; c < 3; ++c) gsum[3 * i + c] += gl[3 * i + c];
+    }
+    for (int k = 0; k < 9; ++k) ssum[k] += sl[k];
+    for (int i = 0; i < nat; ++i) energy[i] += esum[i];
+    if (grad) {
+        for (int i = 0; i < nat; ++i) erecip += esum[i];
+        for (int c = 0; c < 3; ++c) ssum[c * 3 + c] -= erecip;
+        for (int i = 0; i < nat; ++i) dEdcn[i] += dsum[i];
+        for (int k = 0; k < 3 * nat; ++k) gradient[k] += gsum[k];
+        for (int k = 0; k < 9; ++k) sigma[k] += ssum[k];
+    }
+    free(terms); free(nterm); free(kp); free(esum); free(dsum); free(gsum); free(phihat); free(dphihat);
+    free(sfr); free(sfi); free(fr); free(fi); free(cr); free(ci); free(el); free(dl); free(gl);
+    return 0;
+}
+
+/*
+ * disp.f90:56-185 with a low-rank model (model.f90:185-196, 379-382): all C6
+ * from the low-rank expansion; under three-dimensional periodicity the
+ * two-body term is the reciprocal lattice sum (disp.f90:104-111, 142-145),
+ * otherwise the real-space sum.  ewald = {rank, tolerance, kcut} with the
+ * defaults of api.f90:447-451 for non-positive values.  Returns 1 when the
+ * damping function does not support the Ewald summation, 2 when a pair
+ * potential has no closed-form transform (damping.f90:345-352).
+ */
+int d3o_get_dispersion_ewald(int nat, const int *num, const double *xyz, const double *lattice,
+                             const int *periodic, int kind, const double *par, const double *cut,
+                             const int *ghost, const double *ewald, double *energy, double *gradient,
+                             double *sigma, double *energies_out)
+{
+    d3o_param p = {kind, par[0], par[1], par[2], par[3], par[4], par[5], par[6], par[7], par[8]};
+    double zero_lat[9] = {0};
+    int zero_pbc[3] = {0, 0, 0};
+    if (!lattice) lattice = zero_lat;
+    if (!periodic) periodic = zero_pbc;
+    int recip = periodic[0] && periodic[1] && periodic[2];
+    if (recip && !(kind == K_RATIONAL || (kind == K_ZERO && fabs(p.alp - lround(p.alp)) <= sqrt(EPS))))
+        return 1;
+    int grad = (gradient != NULL) || (sigma != NULL);
+    int mref = 0;
+    for (int i = 0; i < nat; ++i)
+        if (g_nref[num[i] - 1] > mref) mref = g_nref[num[i] - 1];
+    int *id = (int *)malloc(sizeof(int) * nat);
+    d3o_lowrank lr;
+    lowrank_new(nat, num, ewald[0] > 0 ? (int)ewald[0] : 0, ewald[1] > 0.0 ? ewald[1] : 1.0e-4, &lr, id);
+    double *cn = (double *)calloc(nat, sizeof(double)), *trans;
+    int ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+    d3o_cn(nat, num, xyz, ntr, trans, cut[0], cn);
+    free(trans);
+    double *gw = (double *)calloc((size_t)nat * mref, sizeof(double));
+    double *gwd = grad ? (double *)calloc((size_t)nat * mref, sizeof(double)) : NULL;
+    d3o_weights(nat, num, cn, mref, gw, gwd);
+    double *c6l = (double *)calloc((size_t)nat * lr.rank, sizeof(double));
+    double *dc6l = (double *)calloc((size_t)nat * lr.rank, sizeof(double));
+    lowrank_weights(&lr, nat, num, id, ghost, mref, gw, gwd, c6l, grad ? dc6l : NULL);
+    double *c6 = (double *)calloc((size_t)nat * nat, sizeof(double));
+    double *dc6 = grad ? (double *)calloc((size_t)nat * nat, sizeof(double)) : NULL;
+    lowrank_c6(&lr, nat, c6l, grad ? dc6l : NULL, c6, dc6);
+    double *en = (double *)calloc(nat, sizeof(double));
+    double *dEdcn = grad ? (double *)calloc(nat, sizeof(double)) : NULL;
+    double *gl = grad ? (double *)calloc(3 * (size_t)nat, sizeof(double)) : NULL;
+    double sl[9] = {0};
+    int rc = 0;
+    if (recip) {
+        rc = ewald2(&p, nat, num, id, xyz, lattice, &lr, c6l, dc6l, ewald[2], en, dEdcn, gl, sl, grad);
+    } else {
+        ntr = lattice_alloc(periodic, lattice, cut[1], &trans);
+        disp2(&p, nat, num, xyz, ntr, trans, cut[1], cut[3], c6, dc6, en, dEdcn, gl, sl, grad);
+        free(trans);
+    }
+    if (rc == 0) {
+        ntr = lattice_alloc(periodic, lattice, cut[2], &trans);
+        atm(nat, num, xyz, ntr, trans, cut[2], cut[4], p.s9, 4.0 / 3.0, p.alp + 2.0, c6, dc6, en, dEdcn, gl, sl,
+            grad);
+        free(trans);
+        if (grad) {
+            ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+            d3o_cn_derivs(nat, num, xyz, ntr, trans, cut[0], dEdcn, gl, sl);
+            free(trans);
+            if (gradient) memcpy(gradient, gl, sizeof(double) * 3 * nat);
+            if (sigma) memcpy(sigma, sl, sizeof sl);
+        }
+        double e = 0.0;
+        for (int i = 0; i < nat; ++i) e += en[i];
+        *energy = e;
+        if (energies_out) memcpy(energies_out, en, sizeof(double) * nat);
+    }
+    free(id); free(lr.lambda); free(lr.vec); free(cn); free(gw); free(gwd); free(c6l); free(dc6l); free(c6);
+    free(dc6); free(en); free(dEdcn); free(gl);
+    return rc;
+}
+
+/* The low-rank expansion of a structure's species (decomposition.f90:104-160):
+ * returns the rank; error receives the largest relative error of the kept
+ * expansion, lambda (if not null, >= ndim entries) the kept eigenvalues and
+ * ndim_out the size of the reference C6 matrix. */
+int d3o_lowrank_info(int nat, const int *num, int rank, double tolerance, double *error, double *lambda,
+                     int *ndim_out)
+{
+    int *id = (int *)malloc(sizeof(int) * nat);
+    d3o_lowrank lr;
+    lowrank_new(nat, num, rank, tolerance > 0.0 ? tolerance : 1.0e-4, &lr, id);
+    if (error) *error = lr.error;
+    if (ndim_out) *ndim_out = lr.ndim;
+    if (lambda) memcpy(lambda, lr.lambda, sizeof(double) * lr.rank);
+    int r = lr.rank;
+    free(id); free(lr.lambda); free(lr.vec);
+    return r;
+}
+
+/* Pair C6 of the low-rank expansion from given reference weights gw
+ * [nat][mref] (decomposition.f90:247-296); c6 nat x nat. */
+void d3o_lowrank_atomic_c6(int nat, const int *num, int rank, double tolerance, int mref, const double *gw,
+                           const int *ghost, double *c6)
+{
+    int *id = (int *)malloc(sizeof(int) * nat);
+    d3o_lowrank lr;
+    lowrank_new(nat, num, rank, tolerance > 0.0 ? tolerance : 1.0e-4, &lr, id);
+    double *c6l = (double *)calloc((size_t)nat * lr.rank, sizeof(double));
+    lowrank_weights(&lr, nat, num, id, ghost, mref, gw, NULL, c6l, NULL);
+    lowrank_c6(&lr, nat, c6l, NULL, c6, NULL);
+    free(id); free(lr.lambda); free(lr.vec); free(c6l);
+}
+
+/* Eigenpairs of a symmetric matrix (column-major) by the cyclic Jacobi
+ * method of the reference, sorted by decreasing magnitude. */
+void d3o_jacobi(int ndim, const double *amat, double *eval, double *evec) { jacobi_eig(ndim, amat, eval, evec); }
+
+/* Fourier transform of pref r^m / (r^alpha + radius^alpha) at wave number k
+ * (kernel.f90:83-129), its value at the origin and reciprocal cutoff. */
+void d3o_fourier_term(double pref, int m, int alpha, double radius, double k, double *phi, double *dphi,
+                      double *pot0, double *kcut)
+{
+    d3o_term t = {pref, m, alpha, radius};
+    fourier_transform(&t, k, phi, dphi);
+    if (pot0) *pot0 = potential_zero(&t);
+    if (kcut) *kcut = reciprocal_cutoff(&t, 1.0e-8);
 }
 
 /* Count of the terms the reference loops visit (the work units). */

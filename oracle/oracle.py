@@ -62,6 +62,12 @@ def lib(aatoau=ORACLE_AATOAU):
         _lib.d3o_get_c6.restype = ctypes.c_double
         _lib.d3o_get_c6.argtypes = [ctypes.c_int] * 4
         _lib.d3o_num_threads.restype = ctypes.c_int
+        _lib.d3o_fourier_term.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double] + [ctypes.c_void_p] * 4
+        _lib.d3o_lowrank_info.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _lib.d3o_lowrank_atomic_c6.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         set_tables(aatoau)
     return _lib
 
@@ -236,6 +242,87 @@ def hessian(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, para
     L.d3o_get_dispersion_hessian(n, _ptr(num, _ip), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip), kind, _ptr(par),
                                  _ptr(cut), _ptr(gh, _ip), _ptr(energy), _ptr(h))
     return dict(energy=float(energy[0]), hessian=h.T.copy())
+
+
+class EwaldUnsupported(ValueError):
+    """The damping function has no reciprocal-space representation."""
+
+
+def ewald(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, param=None, cutoff=DEFAULT_CUTOFF,
+          grad=True, ghost=None, rank=0, tolerance=0.0, kcut=0.0):
+    """FP64 oracle of get_dispersion with a low-rank model (model.f90:185-196):
+    all C6 from the low-rank expansion, the two-body term summed over the
+    reciprocal lattice (fourier/ewald.f90) under 3D periodicity and in real
+    space otherwise.  rank / tolerance / kcut as set_model_ewald
+    (api.f90:447-451; non-positive values select the defaults).  Returns the
+    dict of dispersion(); raises EwaldUnsupported for damping functions
+    without a Fourier representation."""
+    L = lib()
+    num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
+    par = param_vector(kind, param)
+    cut = np.array(cutoff, dtype=np.float64)
+    n = len(num)
+    gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
+    ew = np.array([rank, tolerance, kcut], dtype=np.float64)
+    energy = np.zeros(1)
+    g = np.zeros((n, 3)) if grad else None
+    s = np.zeros((3, 3)) if grad else None
+    en = np.zeros(n)
+    rc = L.d3o_get_dispersion_ewald(n, _ptr(num, _ip), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip), kind, _ptr(par),
+                                    _ptr(cut), _ptr(gh, _ip), _ptr(ew), _ptr(energy), _ptr(g), _ptr(s), _ptr(en))
+    if rc == 1:
+        raise EwaldUnsupported("Damping function does not support Ewald summation")
+    if rc == 2:
+        raise EwaldUnsupported("Damped pair potential has no known Fourier transform")
+    out = dict(energy=float(energy[0]), energies=en)
+    if grad:
+        out.update(gradient=g, sigma=s, virial=s.copy())
+    return out
+
+
+def lowrank_info(numbers, rank=0, tolerance=0.0):
+    """(rank, max relative error, eigenvalues kept, size of the reference C6
+    matrix) of the low-rank expansion of a structure's species."""
+    L = lib()
+    num = np.ascontiguousarray(numbers, dtype=np.int32)
+    err = ctypes.c_double()
+    nd = ctypes.c_int()
+    lam = np.zeros(7 * len(set(num.tolist())))
+    r = L.d3o_lowrank_info(len(num), _ptr(num, _ip), int(rank), float(tolerance), ctypes.byref(err), _ptr(lam),
+                           ctypes.byref(nd))
+    return r, err.value, lam[:r].copy(), nd.value
+
+
+def lowrank_atomic_c6(numbers, gw, rank=0, tolerance=0.0, ghost=None):
+    """Pair C6 (n, n) of the low-rank expansion from reference weights gw."""
+    L = lib()
+    num = np.ascontiguousarray(numbers, dtype=np.int32)
+    gw = np.ascontiguousarray(gw, dtype=np.float64)
+    n = len(num)
+    gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
+    c6 = np.zeros((n, n))
+    L.d3o_lowrank_atomic_c6(n, _ptr(num, _ip), int(rank), float(tolerance), gw.shape[1], _ptr(gw), _ptr(gh, _ip),
+                            _ptr(c6))
+    return c6.T.copy()
+
+
+def jacobi(amat):
+    """Eigenvalues (decreasing magnitude) and eigenvectors (columns) by the
+    reference's cyclic Jacobi method."""
+    a = np.asfortranarray(amat, dtype=np.float64)
+    n = a.shape[0]
+    ev = np.zeros(n)
+    vec = np.zeros((n, n), order="F")
+    lib().d3o_jacobi(n, a.ctypes.data_as(_dp), _ptr(ev), vec.ctypes.data_as(_dp))
+    return ev, vec
+
+
+def fourier_term(pref, m, alpha, radius, k):
+    """(phi, dphi/dk, value at r = 0, reciprocal cutoff at 1e-8) of the term
+    pref r^m / (r^alpha + radius^alpha)."""
+    out = [ctypes.c_double() for _ in range(4)]
+    lib().d3o_fourier_term(float(pref), int(m), int(alpha), float(radius), float(k), *[ctypes.byref(o) for o in out])
+    return tuple(o.value for o in out)
 
 
 def count_terms(positions, lattice=None, periodic=None, cutoff=DEFAULT_CUTOFF, triples=True):
