@@ -72,6 +72,21 @@ D3HIP_API void dftd3_set_model_ghost_index(dftd3_error error, dftd3_model model,
 D3HIP_API void dftd3_set_model_realspace_cutoff_smooth(dftd3_error error, dftd3_model model, double disp2,
                                                        double disp3, double cn, double width2, double width3);
 
+/* s-dftd3.h:172-184 dftd3_set_model_ewald (api.f90:419-453): a low-rank
+ * expansion of the reference C6 (rank > 0 fixed, else the smallest rank
+ * within tolerance, default 1e-4) for every C6 of the model, and under 3D
+ * periodic boundary conditions the two-body term summed over the reciprocal
+ * lattice within kcut (> 0, else from the damping radii).  mesh is accepted
+ * for the reference's signature: the vectors are always summed directly
+ * (the reference's mesh < 0 branch). */
+D3HIP_API void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int rank, double tolerance, double kcut,
+                                     int mesh);
+
+/* No counterpart in the reference (its d3_lowrank_c6%rank and %error):
+ * the rank and largest relative error of the model's low-rank expansion,
+ * 0 and 0 without one. */
+D3HIP_API void d3hip_get_model_lowrank(dftd3_error error, dftd3_model model, int* rank, double* max_error);
+
 /* s-dftd3.h:208-209 (api.f90) */
 D3HIP_API void dftd3_delete_model(dftd3_model* disp);
 

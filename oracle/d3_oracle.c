@@ -1747,7 +1747,7 @@ static void jacobi_eig(int ndim, const double *amat, double *eval, double *evec)
             for (int q = p + 1; q < ndim; ++q) {
                 if (fabs(A_(p, q)) <= skip) continue;
                 double theta = 0.5 * (A_(q, q) - A_(p, p)) / A_(p, q);
-                double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+                double t = copysign(1.0, theta) / (fabs(theta) + sqrt(1.0 + theta * theta));
                 double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
                 for (int j = 0; j < ndim; ++j) {
                     double ajp = A_(j, p), ajq = A_(j, q);

@@ -41,6 +41,7 @@ struct _dftd3_model {
   std::vector<int> num;
   double cutoff[5] = {40.0, 60.0, 40.0, 0.0, 0.0};  // cn, disp2, disp3, width2, width3 (cutoff.f90:26-32)
   std::vector<int> ghost;
+  std::unique_ptr<d3::LowRank> lowrank;  // set_model_ewald (model.f90:185-196)
   int precision = d3::kFP64;
   std::unique_ptr<d3::Engine> engine;
   hipStream_t stream = nullptr;
@@ -316,6 +317,34 @@ void dftd3_set_model_realspace_cutoff_smooth(dftd3_error error, dftd3_model mode
   model->cutoff[4] = width3;
 }
 
+void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int rank, double tolerance, double kcut,
+                           int mesh) {
+  // api.f90:425-453: non-positive values select the defaults (rank from the
+  // tolerance 1e-4, cutoff from the damping radii); the expansion of the
+  // model's species is made here, once (model.f90:185-196).  The vectors are
+  // always summed directly: mesh only selects the reference's interpolation
+  // of the same sum, which is not used.
+  if (!error) return;
+  if (!model) return fatal(error, "D3 dispersion model is missing");
+  (void)mesh;
+  guarded(error, [&] {
+    std::vector<int> species;
+    for (int z : model->num)
+      if (std::find(species.begin(), species.end(), z) == species.end()) species.push_back(z);
+    auto lr = std::make_unique<d3::LowRank>(d3::make_lowrank(species, rank > 0 ? rank : 0,
+                                                             tolerance > 0.0 ? tolerance : 1.0e-4));
+    lr->kcut = kcut > 0.0 ? kcut : 0.0;
+    model->lowrank = std::move(lr);
+  });
+}
+
+void d3hip_get_model_lowrank(dftd3_error error, dftd3_model model, int* rank, double* max_error) {
+  if (!error) return;
+  if (!model) return fatal(error, "D3 dispersion model is missing");
+  if (rank) *rank = model->lowrank ? model->lowrank->rank : 0;
+  if (max_error) *max_error = model->lowrank ? model->lowrank->error : 0.0;
+}
+
 void dftd3_set_model_ghost_index(dftd3_error error, dftd3_model model, const int* ghost, int nidx) {
   if (!error) return;
   if (!model) return fatal(error, "D3 dispersion model is missing");
@@ -501,6 +530,7 @@ void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model di
     std::memcpy(sys[0].lattice, mol->lattice, sizeof mol->lattice);
     std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
     sys[0].param = make_sys_param(param, disp->cutoff);
+    sys[0].lowrank = disp->lowrank.get();
     const bool grad = gradient || sigma;
     d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
                                  disp->stream),
@@ -544,6 +574,7 @@ void dftd3_get_pairwise_dispersion(dftd3_error error, dftd3_structure mol, dftd3
     std::memcpy(sys[0].lattice, mol->lattice, sizeof mol->lattice);
     std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
     sys[0].param = make_sys_param(param, disp->cutoff);
+    sys[0].lowrank = disp->lowrank.get();
     d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
                                  disp->stream),
                   "upload positions");
@@ -581,6 +612,7 @@ void dftd3_get_dispersion_hessian(dftd3_error error, dftd3_structure mol, dftd3_
     std::memcpy(sys[0].lattice, mol->lattice, sizeof mol->lattice);
     std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
     sys[0].param = make_sys_param(param, disp->cutoff);
+    sys[0].lowrank = disp->lowrank.get();
     d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
                                  disp->stream),
                   "upload positions");

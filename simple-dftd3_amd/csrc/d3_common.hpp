@@ -28,9 +28,18 @@ enum Precision : int { kFP64 = 0, kMixed = 1 };
 struct SysParam {
   int kind;
   int grad;
+  int ewald;  // two-body term in reciprocal space (the real-space pass adds nothing)
   double s6, s8, s9, a1, a2, alp, bet, a3, a4;
   double cut_cn, cut2, cut3, width2, width3;
 };
+
+// One term pref r^m / (r^alpha + radius^alpha) of a damped pair potential
+// (fourier/kernel.f90:42-58); the rational and zero damping are sums of two.
+struct FourierTerm {
+  double pref, radius;
+  int m, alpha;
+};
+constexpr int kMaxEwaldSpecies = 8;  // distinct elements of a reciprocal-space system
 
 // Atom range and lattice-image ranges of one system of a batch.  Images are
 // stored per cutoff (CN, two-body, three-body) exactly as get_lattice_points

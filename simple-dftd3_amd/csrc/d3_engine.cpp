@@ -152,6 +152,8 @@ struct Prepared {
   bool periodic_atm = false;   // some three-body system has lattice images
   long long slot_bound = 0;    // sum over three-body systems of (il + 1) * images
   long long item_bound = 0;    // work items of those bounds
+  bool ewald = false;          // the (single) system sums its two-body term in reciprocal space
+  EwaldBatch eb{};             // its vectors, expansion and species chunks (nk = 0: no two-body term)
 };
 
 namespace {
@@ -220,7 +222,7 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
-      item_off, slots, counts, counter, flag, pair3s, hbuf, e_out;
+      item_off, slots, counts, counter, flag, pair3s, hbuf, e_out, ew_c6l, ew_dc6l, ew_part, ew_phi, ew_f;
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
@@ -320,6 +322,27 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
           c6ref[(size_t(a) * nsp + c) * kMaxRef * kMaxRef + ri * kMaxRef + rj] = reference_c6(ri, rj, z, zc);
     }
   }
+  // A low-rank model replaces every reference C6 by its expansion
+  // (model.f90:379-382), so the pair records, the real-space two-body term
+  // and the three-body term use the low-rank C6 unchanged.
+  const LowRank* lr = nullptr;
+  for (const auto& s : systems)
+    if (s.lowrank) lr = s.lowrank;
+  if (lr) {
+    if (nsys != 1) throw Error("A low-rank model evaluates one structure per call");
+    std::vector<int> lsp(nsp, -1);
+    for (int a = 0; a < nsp; ++a)
+      for (size_t q = 0; q < lr->species.size(); ++q)
+        if (lr->species[q] == zlist[a]) lsp[a] = static_cast<int>(q);
+    for (int a = 0; a < nsp; ++a)
+      if (lsp[a] < 0) throw Error("Low-rank model does not match the molecular structure");
+    for (int a = 0; a < nsp; ++a)
+      for (int c = 0; c < nsp; ++c)
+        for (int ri = 0; ri < nref[a]; ++ri)
+          for (int rj = 0; rj < nref[c]; ++rj)
+            c6ref[(size_t(a) * nsp + c) * kMaxRef * kMaxRef + ri * kMaxRef + rj] =
+                lr->reference_c6(lsp[a], ri, lsp[c], rj);
+  }
   std::vector<SysGeom> geom(nsys);
   std::vector<SysParam> par(nsys);
   std::vector<double4> img;
@@ -332,6 +355,11 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     g.natoms = sd.natoms;
     a += sd.natoms;
     par[k] = sd.param;
+    if (lr && sd.periodic[0] && sd.periodic[1] && sd.periodic[2]) {  // disp.f90:104-111
+      if (!supports_ewald(sd.param)) throw Error("Damping function does not support Ewald summation");
+      par[k].ewald = 1;
+      pb.ewald = true;
+    }
     auto add = [&](double cut, int& begin, int& count) {
       const std::vector<double> tr = lattice_points(sd.periodic, sd.lattice, cut);
       begin = static_cast<int>(img.size());
@@ -355,6 +383,71 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
       }
     }
   }
+  // Reciprocal-space set-up (damping.f90:306-385, ewald.f90:113-121).
+  std::vector<double4> kv;
+  std::vector<double> lam, vecb, pval;
+  std::vector<FourierTerm> terms;
+  std::vector<int> nterm, sorder, chunk_seg;
+  std::vector<int4> chunks;
+  double volume = 0.0;
+  if (pb.ewald) {
+    const SystemDesc& sd = systems[0];
+    if (nsp > kMaxEwaldSpecies)
+      throw Error("Ewald summation supports at most " + std::to_string(kMaxEwaldSpecies) + " distinct elements");
+    terms.assign(size_t(nsp) * nsp * 2, FourierTerm{0.0, 1.0, 0, 2});
+    nterm.assign(size_t(nsp) * nsp, 0);
+    double kcut = 0.0;
+    for (int a = 0; a < nsp; ++a)
+      for (int c = 0; c < nsp; ++c) {
+        FourierTerm* t = &terms[2 * (size_t(a) * nsp + c)];
+        const int nt = fourier_terms(par[0], zlist[a], zlist[c], t);
+        nterm[size_t(a) * nsp + c] = nt;
+        for (int q = 0; q < nt; ++q) {
+          if (!fourier_supported(t[q])) throw Error("Damped pair potential has no known Fourier transform");
+          kcut = std::max(kcut, fourier_cutoff(t[q], 1.0e-8));
+        }
+      }
+    if (kcut > 0.0) {  // else every scaling factor vanishes: no two-body term
+      if (lr->kcut > 0.0) kcut = lr->kcut;
+      const std::vector<double> kp = reciprocal_vectors(sd.lattice, kcut, volume);
+      for (size_t q = 0; q < kp.size() / 3; ++q) {
+        const double x = kp[3 * q], y = kp[3 * q + 1], z = kp[3 * q + 2];
+        if (std::sqrt(x * x + y * y + z * z) > kcut) continue;
+        // one vector of each +-k pair (the first nonzero component positive)
+        const bool zero = x == 0.0 && y == 0.0 && z == 0.0;
+        const bool upper = x > 0.0 || (x == 0.0 && (y > 0.0 || (y == 0.0 && z > 0.0)));
+        if (zero || upper) kv.push_back(make_double4(x, y, z, zero ? 1.0 : 2.0));
+      }
+      const int R = lr->rank;
+      lam = lr->lambda;
+      vecb.assign(size_t(nsp) * kMaxRef * R, 0.0);
+      pval.assign(nsp, 0.0);
+      for (int a = 0; a < nsp; ++a) {
+        int q = 0;
+        while (lr->species[q] != zlist[a]) ++q;
+        for (int r = 0; r < kMaxRef; ++r)
+          for (int l = 0; l < R; ++l) vecb[(size_t(a) * kMaxRef + r) * R + l] = lr->vec[(size_t(q) * kMaxRef + r) * R + l];
+        for (int t = 0; t < nterm[size_t(a) * nsp + a]; ++t)
+          pval[a] += fourier_potential_zero(terms[2 * (size_t(a) * nsp + a) + t]);
+      }
+      // atoms grouped by species, each group cut into chunks so that the
+      // structure-factor pass has some 8192 wavefronts
+      const int ktiles = static_cast<int>((kv.size() + kWave - 1) / kWave);
+      const int want = std::max(nsp, (8192 + ktiles - 1) / ktiles);
+      const int chunk = std::max(16, (natoms + want - 1) / want);
+      chunk_seg.assign(nsp + 1, 0);
+      for (int a = 0; a < nsp; ++a) {
+        const int begin = static_cast<int>(sorder.size());
+        for (int i = 0; i < natoms; ++i)
+          if (sp[i] == a) sorder.push_back(i);
+        const int end = static_cast<int>(sorder.size());
+        chunk_seg[a] = static_cast<int>(chunks.size());
+        for (int q = begin; q < end; q += chunk) chunks.push_back(make_int4(a, q, std::min(end, q + chunk), 0));
+      }
+      chunk_seg[nsp] = static_cast<int>(chunks.size());
+    }
+  }
+
   std::vector<char>& sb = staging;
   sb.clear();
   std::vector<int> ghost;
@@ -364,6 +457,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
                o_par = stage(sb, par), o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2),
                o_nref = stage(sb, nref), o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref),
                o_rvdw = stage(sb, rvdw);
+  const size_t o_kv = stage(sb, kv), o_lam = stage(sb, lam), o_vec = stage(sb, vecb), o_pval = stage(sb, pval),
+               o_terms = stage(sb, terms), o_nterm = stage(sb, nterm), o_sorder = stage(sb, sorder),
+               o_chunks = stage(sb, chunks), o_cseg = stage(sb, chunk_seg);
   char* dmeta = static_cast<char*>(meta.get(sb.size()));
   char* host = pinned.get(sb.size());
   std::memcpy(host, sb.data(), sb.size());
@@ -397,6 +493,25 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   pb.periodic_atm = periodic_atm;
   pb.slot_bound = slot_bound;
   pb.item_bound = item_bound;
+  if (pb.ewald) {
+    EwaldBatch& e = pb.eb;
+    e.nk = static_cast<int>(kv.size());
+    e.rank = lr->rank;
+    e.nsp = nsp;
+    e.natoms = natoms;
+    e.atom_begin = 0;
+    e.nchunks = static_cast<int>(chunks.size());
+    e.inv_vol = volume > 0.0 ? 1.0 / volume : 0.0;
+    e.kv = reinterpret_cast<const double4*>(dmeta + o_kv);
+    e.lambda = reinterpret_cast<const double*>(dmeta + o_lam);
+    e.vec = reinterpret_cast<const double*>(dmeta + o_vec);
+    e.pval = reinterpret_cast<const double*>(dmeta + o_pval);
+    e.terms = reinterpret_cast<const FourierTerm*>(dmeta + o_terms);
+    e.nterm = reinterpret_cast<const int*>(dmeta + o_nterm);
+    e.sorder = reinterpret_cast<const int*>(dmeta + o_sorder);
+    e.chunks = reinterpret_cast<const int4*>(dmeta + o_chunks);
+    e.chunk_seg = reinterpret_cast<const int*>(dmeta + o_cseg);
+  }
   return pb;
 }
 
@@ -441,6 +556,17 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
   timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
   timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
   timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, nullptr, stream); });
+  if (pb.ewald && pb.eb.nk > 0) {
+    const EwaldBatch& e = pb.eb;
+    EwaldWork ew{};
+    const size_t nk = static_cast<size_t>(e.nk), R = static_cast<size_t>(e.rank);
+    ew.c6l = ew_c6l.as<double>(size_t(natoms) * R);
+    ew.dc6l = grad ? ew_dc6l.as<double>(size_t(natoms) * R) : nullptr;
+    ew.part = ew_part.as<double2>(size_t(e.nchunks) * R * nk);
+    ew.phi = ew_phi.as<double2>(size_t(e.nsp) * (e.nsp + 1) / 2 * nk);
+    ew.F = ew_f.as<double2>(size_t(e.nsp) * R * nk);
+    timed("ewald", stream, [&] { launch_ewald(db, e, grad, w, ew, stream); });
+  }
   if (pb.any_atm) {
     list_offsets(pb, w.pair, mixed, stream);
     long long* o = off.as<long long>(natoms + 1);
@@ -485,6 +611,9 @@ void Engine::run_pairwise(const std::vector<SystemDesc>& systems, const int* num
   // disp.f90:403-457: CN, weights and C6 as for the energy (no derivatives),
   // then the pairwise two-body and three-body energies
   if (systems.empty()) return;
+  for (const auto& sd : systems)  // disp.f90:429-433
+    if (sd.lowrank && sd.periodic[0] && sd.periodic[1] && sd.periodic[2])
+      throw Error("Pairwise analysis is only available for the real space summation");
   Workspace& m = *ws_;
   const Prepared pb = m.upload(systems, numbers, stream, ghost_host);
   const DevBatch& db = pb.db;
@@ -525,6 +654,8 @@ void Engine::run_hessian(const std::vector<SystemDesc>& systems, const int* numb
   // numbers, the pair records and the three-body lists in the workspace; the
   // Hessian kernels reuse them.
   if (systems.empty()) return;
+  for (const auto& sd : systems)  // disp.f90:113-119
+    if (sd.lowrank) throw Error("Hessian is only available for the real space summation");
   Workspace& m = *ws_;
   const Prepared pb = m.upload(systems, numbers, stream, ghost_host);
   const DevBatch& db = pb.db;
@@ -631,6 +762,8 @@ Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, cons
     : st_(new State) {
   try {
     if (systems.empty()) throw Error("Invalid molecular structure data provided");
+    for (const auto& sd : systems)
+      if (sd.lowrank) throw Error("A planned batch does not take a low-rank model");
     State& s = *st_;
     s.mixed = precision == kMixed;
     s.grad = grad;

@@ -20,12 +20,46 @@ struct Error : std::runtime_error {
 
 void hip_check(hipError_t err, const char* what);
 
+// Low-rank expansion of the reference C6 coefficients of a structure's
+// species (fourier/decomposition.f90:35-97): C6ref(si ri, sj rj) ~
+// sum_l lambda_l vec(si, ri, l) vec(sj, rj, l) over the kept eigenpairs.
+struct LowRank {
+  int rank = 0;                 // kept eigenpairs
+  int ndim = 0;                 // size of the reference C6 matrix
+  double error = 0.0;           // largest relative error of the kept expansion
+  double kcut = 0.0;            // reciprocal cutoff, 0: from the damping radii
+  std::vector<int> species;     // atomic numbers in first-appearance order
+  std::vector<double> lambda;   // [rank]
+  std::vector<double> vec;      // [species][kMaxRef][rank]
+  double reference_c6(int si, int ri, int sj, int rj) const;
+};
+
+// The expansion of these species with a fixed rank (> 0) or the smallest
+// rank within the tolerance (decomposition.f90:104-175).
+LowRank make_lowrank(const std::vector<int>& species, int rank, double tolerance);
+
+bool supports_ewald(const SysParam& p);
+// Terms of the pair potential of elements zi, zj (0 or 2; rational and zero
+// damping only).
+int fourier_terms(const SysParam& p, int zi, int zj, FourierTerm terms[2]);
+bool fourier_supported(const FourierTerm& t);
+double fourier_potential_zero(const FourierTerm& t);
+double fourier_cutoff(const FourierTerm& t, double tolerance);
+// Reciprocal lattice vectors within kcut ([k][3], the zero vector
+// included) and the cell volume.
+std::vector<double> reciprocal_vectors(const double lattice[9], double kcut, double& volume);
+
 // One system of a batch, as the host describes it.
 struct SystemDesc {
   int natoms = 0;
   double lattice[9] = {0};  // rows = lattice vectors (bohr)
   int periodic[3] = {0, 0, 0};
   SysParam param{};
+  // Low-rank model (model.f90:185-196, 379-382): every C6 from the
+  // expansion; with 3D periodicity the two-body term is summed over the
+  // reciprocal lattice (disp.f90:104-111).  A batch with a low-rank system
+  // holds only that system.
+  const LowRank* lowrank = nullptr;
 };
 
 // Outputs of a batch run (device pointers; any may be null except energy).

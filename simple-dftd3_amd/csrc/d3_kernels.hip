@@ -537,7 +537,7 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
   const R r4r2i = R(b.r4r2[spi]);
   double acc_e = 0.0, acc_d = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
   double sxx = 0.0, syy = 0.0, szz = 0.0, sxy = 0.0, sxz = 0.0, syz = 0.0;
-  const bool active = two_body_active(p);
+  const bool active = two_body_active(p) && !p.ewald;  // reciprocal space: ewald kernels
   const R cutr = R(p.cut2), width = R(p.width2);
   const int idi = b.sid[i];
   const R4<R>* row = pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n;
@@ -2022,6 +2022,274 @@ __global__ __launch_bounds__(kHT * kHT) void hess_assemble_kernel(DevBatch b, co
 }
 
 
+// ---------------------------------------------------- reciprocal space
+// fourier/ewald.f90:47-252, the two-body energy of a 3D-periodic system as a
+// sum over reciprocal vectors k of the structure factors of the low-rank C6
+// factors, S_la(k) = sum_{i of species a} c6l_il exp(i k.x_i).  The
+// vectors come in +-k pairs with S(-k) = conj S(k) and identical
+// contributions, so only one of each pair is visited, with weight 2 (the
+// zero vector: weight 1); kv[k].w holds the weight.
+
+// kernel.f90:83-129 get_fourier_transform: 4 pi / k int r sin(kr) f(r) dr of
+// f = pref r^m / (r^alpha + R^alpha) by the residues at the alpha/2 poles
+// of the upper half plane, and its derivative in k.
+__device__ void fourier_transform(const FourierTerm& t, double k, double& phi, double& dphi) {
+  phi = 0.0;
+  dphi = 0.0;
+  if (fabs(t.pref) <= 0.0) return;
+  const int nexp = t.m - t.alpha + 2;
+  const double pre = 4.0 * M_PI * M_PI / t.alpha * t.pref * pow(t.radius, nexp);
+  if (k <= 0.0) {
+    phi = pre * t.radius / sin(M_PI * (t.m + 3) / double(t.alpha));
+    return;
+  }
+  const double kr = k * t.radius;
+  double acc = 0.0, dacc = 0.0;
+  for (int p = 0; p < t.alpha / 2; ++p) {
+    const double th = M_PI * (2 * p + 1) / double(t.alpha);
+    double st, ct;
+    sincos(th, &st, &ct);
+    const double arg = 0.5 * M_PI + nexp * th + kr * ct;
+    const double damp = exp(-kr * st);
+    acc += sin(arg) * damp;
+    dacc += cos(arg + th) * damp;
+  }
+  phi = pre * acc / k;
+  dphi = (pre * t.radius * dacc - phi) / k;
+}
+
+// decomposition.f90:202-242 get_weights: c6l_il = sum_r vec(sp_i, r, l) w_ir
+// (and dc6l with dw/dCN); zero rows for ghost atoms.  The self-interaction
+// of the reciprocal sum (ewald.f90:131-147), 1/2 sum_l lambda_l c6l_il^2
+// phi_aa(0) and its CN derivative, is added to the atom's energy here.  One
+// thread per atom of the system.
+__global__ __launch_bounds__(kBlock) void lowrank_kernel(DevBatch b, EwaldBatch e, const double* __restrict__ gw,
+                                                         const double* __restrict__ dgw, double* __restrict__ c6l,
+                                                         double* __restrict__ dc6l, double* __restrict__ e_atom,
+                                                         double* __restrict__ dedcn) {
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= e.natoms) return;
+  const int i = e.atom_begin + il;
+  const int sp = b.sp[i];
+  const int nr = b.nref[sp];
+  const bool ghost = b.ghost && b.ghost[i];
+  double es = 0.0, ds = 0.0;
+  for (int l = 0; l < e.rank; ++l) {
+    double a = 0.0, d = 0.0;
+    if (!ghost) {
+      for (int r = 0; r < nr; ++r) {
+        const double v = e.vec[(sp * kMaxRef + r) * e.rank + l];
+        a += v * gw[i * b.mref + r];
+        if (dgw) d += v * dgw[i * b.mref + r];
+      }
+    }
+    c6l[static_cast<long long>(il) * e.rank + l] = a;
+    if (dgw) dc6l[static_cast<long long>(il) * e.rank + l] = d;
+    es += 0.5 * e.lambda[l] * a * a * e.pval[sp];
+    ds += e.lambda[l] * a * d * e.pval[sp];
+  }
+  e_atom[i] += es;
+  if (dgw) dedcn[i] += ds;
+}
+
+// Partial structure factors: one wavefront per (64 vectors, chunk of atoms
+// of one species), lane = vector; the chunk's atoms are wave-uniform, so
+// their factors c6l_il come through scalar loads and each lane evaluates one
+// phase per atom.  Factors l0 .. l0+lcount (lcount <= kL) of the partial sum
+// over the chunk go to part[chunk][l][k] (no atomics; ewald_f sums the
+// chunks of a species).
+template <int kL>
+__global__ __launch_bounds__(kBlock) void ewald_sf_kernel(DevBatch b, EwaldBatch e, const double* __restrict__ c6l,
+                                                          int l0, int lcount, double2* __restrict__ part) {
+  const int ktiles = (e.nk + kWave - 1) / kWave;
+  const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (item >= ktiles * e.nchunks) return;
+  const int chunk = item / ktiles;
+  const int k = (item % ktiles) * kWave + wave_lane();
+  const double4 kv = e.kv[min(k, e.nk - 1)];
+  const int4 c = e.chunks[chunk];
+  double sr[kL], si[kL];
+#pragma unroll
+  for (int l = 0; l < kL; ++l) {
+    sr[l] = 0.0;
+    si[l] = 0.0;
+  }
+  for (int q = c.y; q < c.z; ++q) {
+    const int i = e.sorder[q];
+    const double4 x = b.pos[i];
+    double s, co;
+    sincos(kv.x * x.x + kv.y * x.y + kv.z * x.z, &s, &co);
+    const double* f = c6l + static_cast<long long>(i - e.atom_begin) * e.rank + l0;
+#pragma unroll
+    for (int l = 0; l < kL; ++l) {
+      if (l < lcount) {
+        const double w = f[l];
+        sr[l] += w * co;
+        si[l] += w * s;
+      }
+    }
+  }
+  if (k < e.nk) {
+#pragma unroll
+    for (int l = 0; l < kL; ++l)
+      if (l < lcount) part[(static_cast<long long>(chunk) * e.rank + l0 + l) * e.nk + k] = make_double2(sr[l], si[l]);
+  }
+}
+
+// Per vector k (one lane each): S_al from the chunk partials, the transform
+// phihat_ab(k) of the pair potential and F_al = sum_b phihat_ab S_bl
+// (written for ewald_atom); and the virial of the vector (ewald.f90:221-233):
+// 1/2 qq k k^T / (|k| V) with qq = sum_l lambda_l sum_ab dphihat_ab
+// Re(S_al conj S_bl), minus its energy E_k on the diagonal (ewald.f90:
+// 248-250).  The virial is summed over the wavefront and added to the
+// per-atom virial of the system's first atom.
+__global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const double2* __restrict__ part,
+                                                         double2* __restrict__ phi, double2* __restrict__ F,
+                                                         double* __restrict__ sig_atom, int grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = k < e.nk;
+  const int nsp = e.nsp;
+  double sg[6] = {0, 0, 0, 0, 0, 0};
+  if (valid) {
+    const double4 kv = e.kv[k];
+    const double knorm = sqrt(kv.x * kv.x + kv.y * kv.y + kv.z * kv.z);
+    for (int a = 0; a < nsp; ++a)
+      for (int c = 0; c <= a; ++c) {
+        double p = 0.0, dp = 0.0;
+        for (int t = 0; t < e.nterm[a * nsp + c]; ++t) {
+          double pv, dpv;
+          fourier_transform(e.terms[2 * (a * nsp + c) + t], knorm, pv, dpv);
+          p += pv;
+          dp += dpv;
+        }
+        phi[static_cast<long long>(a * (a + 1) / 2 + c) * e.nk + k] = make_double2(p, dp);
+      }
+    double ek = 0.0, qq = 0.0;
+    for (int l = 0; l < e.rank; ++l) {
+      double s_re[kMaxEwaldSpecies], s_im[kMaxEwaldSpecies];
+#pragma unroll
+      for (int a = 0; a < kMaxEwaldSpecies; ++a) {
+        s_re[a] = 0.0;
+        s_im[a] = 0.0;
+        if (a < nsp) {
+          for (int ch = e.chunk_seg[a]; ch < e.chunk_seg[a + 1]; ++ch) {
+            const double2 v = part[(static_cast<long long>(ch) * e.rank + l) * e.nk + k];
+            s_re[a] += v.x;
+            s_im[a] += v.y;
+          }
+        }
+      }
+      double el = 0.0, ql = 0.0;
+#pragma unroll
+      for (int a = 0; a < kMaxEwaldSpecies; ++a) {
+        if (a >= nsp) continue;
+        double f_re = 0.0, f_im = 0.0;
+#pragma unroll
+        for (int c = 0; c < kMaxEwaldSpecies; ++c) {
+          if (c >= nsp) continue;
+          const int pr = a >= c ? a * (a + 1) / 2 + c : c * (c + 1) / 2 + a;
+          const double2 pv = phi[static_cast<long long>(pr) * e.nk + k];
+          f_re += pv.x * s_re[c];
+          f_im += pv.x * s_im[c];
+          ql += pv.y * (s_re[a] * s_re[c] + s_im[a] * s_im[c]);
+        }
+        F[(static_cast<long long>(a) * e.rank + l) * e.nk + k] = make_double2(f_re, f_im);
+        el += s_re[a] * f_re + s_im[a] * f_im;
+      }
+      ek += e.lambda[l] * el;
+      qq += e.lambda[l] * ql;
+    }
+    if (grad) {
+      // E_k = -1/(2V) sum_l lambda_l sum_a Re(S_al conj F_al), weighted
+      const double w = kv.w;
+      const double erec = -0.5 * w * ek * e.inv_vol;
+      const double h = knorm > 0.0 ? 0.5 * w * qq * e.inv_vol / knorm : 0.0;
+      sg[0] = h * kv.x * kv.x - erec;
+      sg[1] = h * kv.y * kv.y - erec;
+      sg[2] = h * kv.z * kv.z - erec;
+      sg[3] = h * kv.x * kv.y;
+      sg[4] = h * kv.x * kv.z;
+      sg[5] = h * kv.y * kv.z;
+    }
+  }
+  if (grad) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sg[c] = wave_sum(sg[c]);
+    if (wave_lane() == 0)
+      for (int c = 0; c < 6; ++c) atomicAdd(&sig_atom[6 * e.atom_begin + c], sg[c]);
+  }
+}
+
+// Per atom i and a range of vectors (one wavefront, lane = vector):
+// A = sum_l lambda_l c6l_il conj F_{a_i l}(k) and B the same with dc6l, then
+// (ewald.f90:193-212) E_i -= 1/(2V) Re(e^{ik.x} A), dE/dCN_i -= 1/V
+// Re(e^{ik.x} B), grad_i += 1/V Im(e^{ik.x} A) k, each weighted; the
+// wavefront's sums are added to the atom's accumulators.
+template <bool kGrad>
+__global__ __launch_bounds__(kBlock) void ewald_atom_kernel(DevBatch b, EwaldBatch e, const double* __restrict__ c6l,
+                                                            const double* __restrict__ dc6l,
+                                                            const double2* __restrict__ F, int ksplit,
+                                                            double* __restrict__ e_atom, double* __restrict__ dedcn,
+                                                            double* __restrict__ grad) {
+  const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (item >= e.natoms * ksplit) return;
+  const int il = item / ksplit, part = item % ksplit;
+  const int i = e.atom_begin + il;
+  const int sp = b.sp[i];
+  const double4 x = b.pos[i];
+  const int ktiles = (e.nk + kWave - 1) / kWave;
+  const int t0 = static_cast<int>(static_cast<long long>(ktiles) * part / ksplit);
+  const int t1 = static_cast<int>(static_cast<long long>(ktiles) * (part + 1) / ksplit);
+  const double* f = c6l + static_cast<long long>(il) * e.rank;
+  const double* df = kGrad ? dc6l + static_cast<long long>(il) * e.rank : nullptr;
+  const double2* Fa = F + static_cast<long long>(sp) * e.rank * e.nk;
+  double acc_e = 0.0, acc_d = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  for (int t = t0; t < t1; ++t) {
+    const int k = t * kWave + wave_lane();
+    if (k >= e.nk) continue;
+    const double4 kv = e.kv[k];
+    double a_re = 0.0, a_im = 0.0, b_re = 0.0, b_im = 0.0;
+    for (int l = 0; l < e.rank; ++l) {
+      const double2 fv = Fa[static_cast<long long>(l) * e.nk + k];
+      const double cl = e.lambda[l] * f[l];
+      a_re += cl * fv.x;
+      a_im += cl * fv.y;
+      if (kGrad) {
+        const double dl = e.lambda[l] * df[l];
+        b_re += dl * fv.x;
+        b_im += dl * fv.y;
+      }
+    }
+    double s, c;
+    sincos(kv.x * x.x + kv.y * x.y + kv.z * x.z, &s, &c);
+    // e^{ik.x} conj(F) = (c + i s)(F_re - i F_im)
+    acc_e += kv.w * (c * a_re + s * a_im);
+    if (kGrad) {
+      acc_d += kv.w * (c * b_re + s * b_im);
+      const double zim = kv.w * (s * a_re - c * a_im);
+      g0 += zim * kv.x;
+      g1 += zim * kv.y;
+      g2 += zim * kv.z;
+    }
+  }
+  acc_e = wave_sum(acc_e);
+  if (kGrad) {
+    acc_d = wave_sum(acc_d);
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
+  }
+  if (wave_lane() == 0) {
+    atomicAdd(&e_atom[i], -0.5 * acc_e * e.inv_vol);
+    if (kGrad) {
+      atomicAdd(&dedcn[i], -acc_d * e.inv_vol);
+      atomicAdd(&grad[3 * i], g0 * e.inv_vol);
+      atomicAdd(&grad[3 * i + 1], g1 * e.inv_vol);
+      atomicAdd(&grad[3 * i + 2], g2 * e.inv_vol);
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -2222,6 +2490,34 @@ void launch_atm_pairs(const DevBatch& b, bool mixed, bool smooth, const long lon
 void launch_pair3_finalize(const DevBatch& b, const double* pair3s, double* out, hipStream_t st) {
   hipLaunchKernelGGL(pair3_finalize_kernel, dim3(blocks_for(b.natoms, kWavesPerBlock)), dim3(kBlock), 0, st, b,
                      pair3s, out);
+}
+
+void launch_ewald(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(lowrank_kernel, dim3(blocks_for(e.natoms, kBlock)), dim3(kBlock), 0, st, b, e, w.gw,
+                     grad ? w.dgw : nullptr, ew.c6l, ew.dc6l, w.e_atom, w.dedcn);
+  const int ktiles = blocks_for(e.nk, kWave);
+  const dim3 sf_grid(blocks_for(static_cast<long long>(ktiles) * e.nchunks, kWavesPerBlock));
+  for (int l0 = 0; l0 < e.rank; l0 += 32) {
+    const int lc = std::min(32, e.rank - l0);
+    if (lc <= 8)
+      hipLaunchKernelGGL(ewald_sf_kernel<8>, sf_grid, dim3(kBlock), 0, st, b, e, ew.c6l, l0, lc, ew.part);
+    else if (lc <= 16)
+      hipLaunchKernelGGL(ewald_sf_kernel<16>, sf_grid, dim3(kBlock), 0, st, b, e, ew.c6l, l0, lc, ew.part);
+    else
+      hipLaunchKernelGGL(ewald_sf_kernel<32>, sf_grid, dim3(kBlock), 0, st, b, e, ew.c6l, l0, lc, ew.part);
+  }
+  hipLaunchKernelGGL(ewald_f_kernel, dim3(blocks_for(e.nk, kBlock)), dim3(kBlock), 0, st, e, ew.part, ew.phi, ew.F,
+                     w.sig_atom, grad ? 1 : 0);
+  // some 16k wavefronts: each atom's vectors split into ksplit ranges
+  const int ksplit = std::max(1, std::min(ktiles, (16384 + e.natoms - 1) / e.natoms));
+  const dim3 at_grid(blocks_for(static_cast<long long>(e.natoms) * ksplit, kWavesPerBlock));
+  if (grad)
+    hipLaunchKernelGGL(ewald_atom_kernel<true>, at_grid, dim3(kBlock), 0, st, b, e, ew.c6l, ew.dc6l, ew.F, ksplit,
+                       w.e_atom, w.dedcn, w.grad);
+  else
+    hipLaunchKernelGGL(ewald_atom_kernel<false>, at_grid, dim3(kBlock), 0, st, b, e, ew.c6l, ew.dc6l, ew.F, ksplit,
+                       w.e_atom, w.dedcn, w.grad);
 }
 
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t st) {

@@ -117,6 +117,33 @@ struct HessWork {
 };
 void launch_hessian(const DevBatch& b, const HessWork& h, const long long* offset, const void* slots,
                     long long total_slots, int max_natoms, hipStream_t stream);
+// Reciprocal-space two-body term of one 3D-periodic system with a low-rank
+// model (fourier/ewald.f90:47-252): one vector of each +-k pair within the
+// cutoff, kv = (k, weight); the low-rank expansion in batch species order;
+// the atoms of the system grouped by species and cut into chunks.
+struct EwaldBatch {
+  int nk, rank, nsp, natoms, atom_begin, nchunks;
+  double inv_vol;
+  const double4* kv;         // [nk]
+  const double* lambda;      // [rank]
+  const double* vec;         // [nsp][kMaxRef][rank]
+  const double* pval;        // [nsp] pair potential of (a, a) at r = 0
+  const FourierTerm* terms;  // [nsp][nsp][2]
+  const int* nterm;          // [nsp][nsp]
+  const int* sorder;         // [natoms] atoms by species (global indices)
+  const int4* chunks;        // [nchunks] {species, begin, end, 0} into sorder
+  const int* chunk_seg;      // [nsp + 1] chunks of each species
+};
+struct EwaldWork {
+  double *c6l, *dc6l;        // [natoms][rank] (dc6l null without gradient)
+  double2 *part;             // [nchunks][rank][nk] partial structure factors
+  double2 *phi;              // [nsp (nsp + 1) / 2][nk] transform and its k-derivative
+  double2 *F;                // [nsp][rank][nk] sum_b phihat_ab S_bl
+};
+// Adds the reciprocal-space energy (and dE/dCN, gradient, virial) to the
+// accumulators of w; runs after launch_disp2.
+void launch_ewald(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
+                  hipStream_t stream);
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t stream);
 void launch_reduce(const DevBatch& b, const DevWork& w, double* energy, double* sigma, hipStream_t stream);
 void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream_t stream);

@@ -285,8 +285,22 @@ class DispersionModel(Structure):
             results.update(virial=_sigma)
         return results
 
-    def set_ewald_summation(self, *args, **kwargs) -> None:
-        raise NotImplementedError("Ewald summation is not available in this library")
+    def set_ewald_summation(self, rank: int = 0, tolerance: float = 0.0, kcut: float = 0.0, mesh: int = 0) -> None:
+        """Evaluate the two-body dispersion energy by summation over the
+        reciprocal lattice (interface.py:540-568): every C6 of the model comes
+        from a low-rank expansion of the reference C6 (rank > 0 fixed, else
+        the smallest within ``tolerance``, default 1e-4); under 3D periodic
+        boundary conditions the two-body term is summed over the reciprocal
+        lattice within ``kcut`` (default from the damping radii) and the
+        two-body real-space cutoff is ignored.  Rational and zero damping
+        only.  The vectors are always summed directly, the reference's
+        ``mesh < 0`` branch; ``mesh`` is accepted and has no effect."""
+        library.set_model_ewald(self._disp, rank, tolerance, kcut, mesh)
+
+    def get_lowrank(self) -> tuple:
+        """(rank, largest relative error of the reference C6) of the low-rank
+        expansion set by set_ewald_summation, (0, 0.0) without one."""
+        return library.get_model_lowrank(self._disp)
 
     def set_work_partition(self, part: int, nparts: int) -> None:
         raise NotImplementedError("Work partitions are not available in this library")

@@ -44,6 +44,8 @@ _SIGNATURES = {
     "dftd3_set_model_realspace_cutoff": (None, [_vp, _vp, _d, _d, _d]),
     "dftd3_set_model_realspace_cutoff_smooth": (None, [_vp, _vp, _d, _d, _d, _d, _d]),
     "dftd3_set_model_ghost_index": (None, [_vp, _vp, _ip, ctypes.c_int]),
+    "dftd3_set_model_ewald": (None, [_vp, _vp, ctypes.c_int, _d, _d, ctypes.c_int]),
+    "d3hip_get_model_lowrank": (None, [_vp, _vp, _ip, _dp]),
     "dftd3_delete_model": (None, [ctypes.POINTER(_vp)]),
     "dftd3_new_zero_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d]),
     "dftd3_load_zero_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
@@ -212,6 +214,19 @@ def set_model_ghost_index(disp: ModelHandle, ghost: np.ndarray) -> None:
     """Disable dispersion contributions from selected atoms (0-based indices)"""
     idx = np.ascontiguousarray(np.asarray(ghost, dtype="i4") + 1)
     return error_check(lib.dftd3_set_model_ghost_index)(disp.handle, _cast(_ip, idx), idx.size)
+
+
+def set_model_ewald(disp: ModelHandle, rank: int, tolerance: float, kcut: float, mesh: int) -> None:
+    """Use a low-rank C6 expansion and the reciprocal-space two-body sum"""
+    return error_check(lib.dftd3_set_model_ewald)(disp.handle, int(rank), float(tolerance), float(kcut), int(mesh))
+
+
+def get_model_lowrank(disp: ModelHandle) -> tuple:
+    """(rank, largest relative error) of the model's low-rank expansion"""
+    rank = ctypes.c_int(0)
+    err = ctypes.c_double(0.0)
+    error_check(lib.d3hip_get_model_lowrank)(disp.handle, ctypes.byref(rank), ctypes.byref(err))
+    return rank.value, err.value
 
 
 def set_model_precision(disp: ModelHandle, precision: int) -> None:

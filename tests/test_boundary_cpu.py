@@ -244,3 +244,32 @@ def test_plan_api_errors_without_a_gpu():
     h = ctypes.c_void_p(None)
     L.lib.d3hip_delete_plan(ctypes.byref(h))  # NULL is accepted
     L.lib.d3hip_delete_plan(None)
+
+
+@pytest.mark.parametrize("rank,tolerance", [(0, 0.0), (0, 1e-3), (0, 1e-6), (3, 0.0), (10000, 0.0)])
+def test_lowrank_expansion_matches_oracle(goldens, rank, tolerance):
+    """set_ewald_summation makes the low-rank expansion on the host
+    (decomposition.f90:104-175): same rank and largest relative error as the
+    oracle's restatement, for the default and explicit tolerances and ranks."""
+    import oracle as O
+    from dftd3.interface import DispersionModel
+    g = goldens["validation_01-energy-d3-bj"]
+    num, xyz = np.array(g["numbers"]), np.array(g["positions"])
+    model = DispersionModel(num, xyz)
+    assert model.get_lowrank() == (0, 0.0)
+    model.set_ewald_summation(rank=rank, tolerance=tolerance, mesh=-1)
+    r, err = model.get_lowrank()
+    ro, erro, _, ndim = O.lowrank_info(num, rank, tolerance)
+    assert r == ro and r <= ndim
+    assert err == pytest.approx(erro, rel=1e-6, abs=1e-13)
+
+
+def test_ewald_without_gpu_fails_loudly(numbers, positions):
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from dftd3.interface import DispersionModel, RationalDampingParam
+    model = DispersionModel(numbers, positions, lattice=np.eye(3) * 30.0, periodic=np.full(3, True))
+    model.set_ewald_summation()
+    with pytest.raises(RuntimeError, match="HIP"):
+        model.get_dispersion(RationalDampingParam(method="pbe0"), grad=True)
