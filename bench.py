@@ -360,6 +360,78 @@ def bench_hessian(args, systems, config, spec, positions, counts):
     return 0
 
 
+def bench_ewald(args, systems, config, spec, positions, counts):
+    """Reciprocal-space two-body term (set_ewald_summation: low-rank C6 with
+    the default tolerance, cutoff from the damping radii, direct sum over the
+    reciprocal lattice) of one 3D-periodic cell, E+grad+sigma, FP64, through
+    the model API, timed per kernel with HIP events; CPU baseline: the
+    oracle's restatement (OpenMP over the vectors) on a bounded sample."""
+    import torch
+    from dftd3 import library
+    from dftd3.interface import DispersionModel, RationalDampingParam
+    s = systems[0]
+    if not all(s.get("periodic") or [False]):
+        raise SystemExit("--ewald needs a 3D-periodic workload (c1)")
+    p = s["param"]
+    param = RationalDampingParam(s6=p["s6"], s8=p["s8"], s9=p["s9"], a1=p["a1"], a2=p["a2"], alp=p["alp"])
+    model = DispersionModel(s["numbers"], s["positions"], s["lattice"], np.asarray(s["periodic"]))
+    model.set_ewald_summation()
+    run = lambda: model.get_dispersion(param, grad=True)  # noqa: E731
+    for _ in range(args.warmup):
+        out = run()
+    groups = KERNEL_GROUPS + ("lowrank", "ewald_sf", "ewald_f", "ewald_atom")
+    library.set_kernel_timing(model._disp, True)
+    t_ev = timed_steps(run, args.steps, lambda: None)
+    km = {k: library.get_kernel_time(model._disp, k) for k in groups}
+    library.set_kernel_timing(model._disp, False)
+    t = timed_steps(run, args.steps, lambda: None)
+    km = {k: v[0] / max(v[1], 1) for k, v in km.items() if v[1]}
+    rank, lr_err, kcut, nvec = library.get_model_lowrank(model._disp, last_evaluation=True)
+    n = config["natoms"]
+    units = n * nvec  # (atom, vector) pairs of one pass, one vector of each +-k pair
+    ops_sf = 4 * rank + 12         # phase (two complex products), 2 rank FMAs per (atom, vector)
+    ops_atom = 8 * rank + 32       # 4 rank FMAs (A, B), phase, energy / dE/dCN / gradient sums
+    t_sf, t_atom = km.get("ewald_sf", 0.0) / 1e3, km.get("ewald_atom", 0.0) / 1e3
+    ach_atom = units * ops_atom / t_atom / 1e12 if t_atom > 0 else 0.0
+    ach_sf = units * ops_sf / t_sf / 1e12 if t_sf > 0 else 0.0
+    log(f"ewald: rank {rank}, kcut {kcut:.3f}, {nvec} vectors, {t * 1e3:.2f} ms/step; "
+        f"sf {t_sf * 1e3:.3f} ms ({ach_sf:.2f} TFLOP/s), atom {t_atom * 1e3:.3f} ms ({ach_atom:.2f} TFLOP/s)")
+    line = dict(
+        metric=METRIC, value=units / t, unit="atom-vector-terms/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=t * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f64", data="synthetic",
+        config=dict(config, parallelism="single-gpu", step="E+grad+sigma with the two-body term in reciprocal space",
+                    lowrank_rank=rank, lowrank_max_error=lr_err, kcut=kcut, vectors=int(nvec),
+                    atom_vector_terms=int(units), triple_terms=int(counts[2])),
+        molecules_per_s=config["cells"] / t, ms_per_step_with_kernel_events=t_ev * 1e3, kernel_ms=km,
+        host_path="positions uploaded and results downloaded every step (model API, 48 KB each way)",
+        roofline=dict(kernel="ewald_atom_kernel<true>", bound="valu", achieved=ach_atom, peak=PEAK_TFLOPS["fp64"],
+                      unit="TFLOP/s", frac=ach_atom / PEAK_TFLOPS["fp64"], traffic=None, ops_per_term=ops_atom,
+                      terms_per_launch=int(units), launch_ms=t_atom * 1e3,
+                      sf_kernel=dict(achieved=ach_sf, ops_per_term=ops_sf, launch_ms=t_sf * 1e3)),
+    )
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        os.environ["OMP_NUM_THREADS"] = str(args.cpu_threads)
+        import oracle
+        stride = args.cpu_stride * 10
+        log(f"cpu baseline: oracle on every {stride}-th vector and atom, {args.cpu_threads} threads")
+        oracle.set_sample_stride(stride)
+        try:
+            t0 = time.perf_counter()
+            oracle.ewald(s["numbers"], s["positions"], s["lattice"], s["periodic"], oracle.RATIONAL, s["param"])
+            t_cpu = (time.perf_counter() - t0) * stride
+        finally:
+            oracle.set_sample_stride(1)
+        line["cpu_baseline"] = dict(value=float(units / t_cpu), unit="atom-vector-terms/s",
+                                    cores=oracle.num_threads(), kind="port",
+                                    sample=f"oracle E+grad+sigma with every loop over vectors and outer atoms "
+                                           f"restricted to every {stride}-th entry, time scaled by {stride} "
+                                           f"({t_cpu:.1f} s per full evaluation)", seconds_per_eval_estimate=t_cpu)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -375,6 +447,8 @@ def main():
                     help="time the pairwise-resolved energies (pair energy matrices) of the workload instead")
     ap.add_argument("--hessian", action="store_true",
                     help="time the energy and analytic Hessian of the workload instead (FP64)")
+    ap.add_argument("--ewald", action="store_true",
+                    help="time the workload with the two-body term summed in reciprocal space instead (FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has a single MI355X; "
@@ -406,6 +480,8 @@ def main():
         return bench_pairwise(args, systems, config, spec, positions, counts)
     if args.hessian:
         return bench_hessian(args, systems, config, spec, positions, counts)
+    if args.ewald:
+        return bench_ewald(args, systems, config, spec, positions, counts)
     ws = torch_api.workspace(dev)
     outs = {}
     res = {}

@@ -83,9 +83,12 @@ D3HIP_API void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int r
                                      int mesh);
 
 /* No counterpart in the reference (its d3_lowrank_c6%rank and %error):
- * the rank and largest relative error of the model's low-rank expansion,
- * 0 and 0 without one. */
-D3HIP_API void d3hip_get_model_lowrank(dftd3_error error, dftd3_model model, int* rank, double* max_error);
+ * the rank and largest relative error of the model's low-rank expansion
+ * (0 and 0 without one); the reciprocal cutoff and the number of vectors
+ * (one of each +-k pair and k = 0) of the last reciprocal-space evaluation.
+ * Any output may be NULL. */
+D3HIP_API void d3hip_get_model_lowrank(dftd3_error error, dftd3_model model, int* rank, double* max_error,
+                                       double* kcut, long long* vectors);
 
 /* s-dftd3.h:208-209 (api.f90) */
 D3HIP_API void dftd3_delete_model(dftd3_model* disp);

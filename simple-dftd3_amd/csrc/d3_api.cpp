@@ -338,11 +338,15 @@ void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int rank, doubl
   });
 }
 
-void d3hip_get_model_lowrank(dftd3_error error, dftd3_model model, int* rank, double* max_error) {
+void d3hip_get_model_lowrank(dftd3_error error, dftd3_model model, int* rank, double* max_error, double* kcut,
+                             long long* vectors) {
   if (!error) return;
   if (!model) return fatal(error, "D3 dispersion model is missing");
-  if (rank) *rank = model->lowrank ? model->lowrank->rank : 0;
-  if (max_error) *max_error = model->lowrank ? model->lowrank->error : 0.0;
+  const d3::LowRank* lr = model->lowrank.get();
+  if (rank) *rank = lr ? lr->rank : 0;
+  if (max_error) *max_error = lr ? lr->error : 0.0;
+  if (kcut) *kcut = lr ? lr->last_kcut : 0.0;
+  if (vectors) *vectors = lr ? lr->last_vectors : 0;
 }
 
 void dftd3_set_model_ghost_index(dftd3_error error, dftd3_model model, const int* ghost, int nidx) {

@@ -4,6 +4,7 @@
 //
 // Order of one evaluation (disp.f90:56-185):
 //   pack -> cn -> weights -> pair (C6 and derivatives) -> disp2 ->
+//   [reciprocal space: lowrank -> ewald_sf -> ewald_f -> ewald_atom] ->
 //   atm_list (count, scan, fill) -> atm -> cn_chain -> reduce
 #include <hip/hip_runtime.h>
 
@@ -222,7 +223,8 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
-      item_off, slots, counts, counter, flag, pair3s, hbuf, e_out, ew_c6l, ew_dc6l, ew_part, ew_phi, ew_f;
+      item_off, slots, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
+      ew_phi, ew_f;
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
@@ -384,12 +386,11 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     }
   }
   // Reciprocal-space set-up (damping.f90:306-385, ewald.f90:113-121).
-  std::vector<double4> kv;
+  ReciprocalTiles rt;
   std::vector<double> lam, vecb, pval;
   std::vector<FourierTerm> terms;
-  std::vector<int> nterm, sorder, chunk_seg;
-  std::vector<int4> chunks;
-  double volume = 0.0;
+  std::vector<int> nterm, sorder, spos, chunk_seg;
+  std::vector<int4> chunks, groups;
   if (pb.ewald) {
     const SystemDesc& sd = systems[0];
     if (nsp > kMaxEwaldSpecies)
@@ -409,15 +410,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
       }
     if (kcut > 0.0) {  // else every scaling factor vanishes: no two-body term
       if (lr->kcut > 0.0) kcut = lr->kcut;
-      const std::vector<double> kp = reciprocal_vectors(sd.lattice, kcut, volume);
-      for (size_t q = 0; q < kp.size() / 3; ++q) {
-        const double x = kp[3 * q], y = kp[3 * q + 1], z = kp[3 * q + 2];
-        if (std::sqrt(x * x + y * y + z * z) > kcut) continue;
-        // one vector of each +-k pair (the first nonzero component positive)
-        const bool zero = x == 0.0 && y == 0.0 && z == 0.0;
-        const bool upper = x > 0.0 || (x == 0.0 && (y > 0.0 || (y == 0.0 && z > 0.0)));
-        if (zero || upper) kv.push_back(make_double4(x, y, z, zero ? 1.0 : 2.0));
-      }
+      rt = reciprocal_tiles(sd.lattice, kcut);
       const int R = lr->rank;
       lam = lr->lambda;
       vecb.assign(size_t(nsp) * kMaxRef * R, 0.0);
@@ -430,10 +423,11 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
         for (int t = 0; t < nterm[size_t(a) * nsp + a]; ++t)
           pval[a] += fourier_potential_zero(terms[2 * (size_t(a) * nsp + a) + t]);
       }
-      // atoms grouped by species, each group cut into chunks so that the
-      // structure-factor pass has some 8192 wavefronts
-      const int ktiles = static_cast<int>((kv.size() + kWave - 1) / kWave);
-      const int want = std::max(nsp, (8192 + ktiles - 1) / ktiles);
+      // atoms grouped by species; each species cut into chunks so that the
+      // structure-factor pass has some 8192 wavefronts, and into groups of
+      // kEwaldGroup for the per-atom pass
+      const int ntiles = static_cast<int>(rt.tiles.size());
+      const int want = std::max(nsp, (8192 + ntiles - 1) / std::max(ntiles, 1));
       const int chunk = std::max(16, (natoms + want - 1) / want);
       chunk_seg.assign(nsp + 1, 0);
       for (int a = 0; a < nsp; ++a) {
@@ -443,9 +437,15 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
         const int end = static_cast<int>(sorder.size());
         chunk_seg[a] = static_cast<int>(chunks.size());
         for (int q = begin; q < end; q += chunk) chunks.push_back(make_int4(a, q, std::min(end, q + chunk), 0));
+        for (int q = begin; q < end; q += kEwaldGroup)
+          groups.push_back(make_int4(a, q, std::min(end, q + kEwaldGroup), 0));
       }
       chunk_seg[nsp] = static_cast<int>(chunks.size());
+      spos.assign(natoms, 0);
+      for (int q = 0; q < natoms; ++q) spos[sorder[q]] = q;
     }
+    lr->last_kcut = kcut;
+    lr->last_vectors = rt.vectors;
   }
 
   std::vector<char>& sb = staging;
@@ -457,9 +457,10 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
                o_par = stage(sb, par), o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2),
                o_nref = stage(sb, nref), o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref),
                o_rvdw = stage(sb, rvdw);
-  const size_t o_kv = stage(sb, kv), o_lam = stage(sb, lam), o_vec = stage(sb, vecb), o_pval = stage(sb, pval),
-               o_terms = stage(sb, terms), o_nterm = stage(sb, nterm), o_sorder = stage(sb, sorder),
-               o_chunks = stage(sb, chunks), o_cseg = stage(sb, chunk_seg);
+  const size_t o_tiles = stage(sb, rt.tiles), o_lam = stage(sb, lam), o_vec = stage(sb, vecb),
+               o_pval = stage(sb, pval), o_terms = stage(sb, terms), o_nterm = stage(sb, nterm),
+               o_sorder = stage(sb, sorder), o_chunks = stage(sb, chunks), o_cseg = stage(sb, chunk_seg),
+               o_groups = stage(sb, groups), o_spos = stage(sb, spos);
   char* dmeta = static_cast<char*>(meta.get(sb.size()));
   char* host = pinned.get(sb.size());
   std::memcpy(host, sb.data(), sb.size());
@@ -495,14 +496,24 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   pb.item_bound = item_bound;
   if (pb.ewald) {
     EwaldBatch& e = pb.eb;
-    e.nk = static_cast<int>(kv.size());
+    e.ntiles = static_cast<int>(rt.tiles.size());
     e.rank = lr->rank;
+    e.rank_pad = (lr->rank + 7) / 8 * 8;
     e.nsp = nsp;
     e.natoms = natoms;
     e.atom_begin = 0;
     e.nchunks = static_cast<int>(chunks.size());
-    e.inv_vol = volume > 0.0 ? 1.0 / volume : 0.0;
-    e.kv = reinterpret_cast<const double4*>(dmeta + o_kv);
+    e.ngroups = static_cast<int>(groups.size());
+    e.inv_vol = rt.volume > 0.0 ? 1.0 / rt.volume : 0.0;
+    std::memcpy(e.rec, rt.rec, sizeof e.rec);
+    e.rep0 = rt.rep[0];
+    e.rep1 = rt.rep[1];
+    e.rep2 = rt.rep[2];
+    e.toff0 = 0;
+    e.toff1 = 2 * rt.rep[0] + 1;
+    e.toff2 = e.toff1 + 2 * rt.rep[1] + 1;
+    e.tstride = e.toff2 + 2 * rt.rep[2] + 1;
+    e.tiles = reinterpret_cast<const int4*>(dmeta + o_tiles);
     e.lambda = reinterpret_cast<const double*>(dmeta + o_lam);
     e.vec = reinterpret_cast<const double*>(dmeta + o_vec);
     e.pval = reinterpret_cast<const double*>(dmeta + o_pval);
@@ -511,6 +522,8 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     e.sorder = reinterpret_cast<const int*>(dmeta + o_sorder);
     e.chunks = reinterpret_cast<const int4*>(dmeta + o_chunks);
     e.chunk_seg = reinterpret_cast<const int*>(dmeta + o_cseg);
+    e.groups = reinterpret_cast<const int4*>(dmeta + o_groups);
+    e.spos = reinterpret_cast<const int*>(dmeta + o_spos);
   }
   return pb;
 }
@@ -556,16 +569,21 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
   timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
   timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
   timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, nullptr, stream); });
-  if (pb.ewald && pb.eb.nk > 0) {
+  if (pb.ewald && pb.eb.ntiles > 0) {
     const EwaldBatch& e = pb.eb;
     EwaldWork ew{};
-    const size_t nk = static_cast<size_t>(e.nk), R = static_cast<size_t>(e.rank);
-    ew.c6l = ew_c6l.as<double>(size_t(natoms) * R);
-    ew.dc6l = grad ? ew_dc6l.as<double>(size_t(natoms) * R) : nullptr;
+    const size_t nk = static_cast<size_t>(e.ntiles) * kWave, R = static_cast<size_t>(e.rank);
+    ew.etab = ew_tab.as<double2>(size_t(natoms) * e.tstride);
+    ew.c6l = ew_c6l.as<double>(size_t(natoms) * e.rank_pad);
+    ew.lc6l = ew_lc6l.as<double>((size_t(natoms) + kEwaldGroup) * R);
+    ew.ldc6l = grad ? ew_ldc6l.as<double>((size_t(natoms) + kEwaldGroup) * R) : nullptr;
     ew.part = ew_part.as<double2>(size_t(e.nchunks) * R * nk);
     ew.phi = ew_phi.as<double2>(size_t(e.nsp) * (e.nsp + 1) / 2 * nk);
     ew.F = ew_f.as<double2>(size_t(e.nsp) * R * nk);
-    timed("ewald", stream, [&] { launch_ewald(db, e, grad, w, ew, stream); });
+    timed("lowrank", stream, [&] { launch_lowrank(db, e, grad, w, ew, stream); });
+    timed("ewald_sf", stream, [&] { launch_ewald_sf(e, ew, stream); });
+    timed("ewald_f", stream, [&] { launch_ewald_f(e, grad, w, ew, stream); });
+    timed("ewald_atom", stream, [&] { launch_ewald_atom(e, grad, w, ew, stream); });
   }
   if (pb.any_atm) {
     list_offsets(pb, w.pair, mixed, stream);

@@ -31,6 +31,10 @@ struct LowRank {
   std::vector<int> species;     // atomic numbers in first-appearance order
   std::vector<double> lambda;   // [rank]
   std::vector<double> vec;      // [species][kMaxRef][rank]
+  // the last reciprocal-space evaluation: its cutoff and number of vectors
+  // (one of each +-k pair and the zero vector)
+  mutable double last_kcut = 0.0;
+  mutable long long last_vectors = 0;
   double reference_c6(int si, int ri, int sj, int rj) const;
 };
 
@@ -45,9 +49,18 @@ int fourier_terms(const SysParam& p, int zi, int zj, FourierTerm terms[2]);
 bool fourier_supported(const FourierTerm& t);
 double fourier_potential_zero(const FourierTerm& t);
 double fourier_cutoff(const FourierTerm& t, double tolerance);
-// Reciprocal lattice vectors within kcut ([k][3], the zero vector
-// included) and the cell volume.
-std::vector<double> reciprocal_vectors(const double lattice[9], double kcut, double& volume);
+// Reciprocal lattice vectors within kcut, one of each +-k pair (and k = 0),
+// as tiles {h, k, l0, count} of up to kWave vectors h b1 + k b2 + l b3 with
+// consecutive l; the reciprocal vectors b (rows), the index range |n_d| <=
+// rep[d] and the cell volume.
+struct ReciprocalTiles {
+  double rec[9] = {0};
+  int rep[3] = {0, 0, 0};
+  double volume = 0.0;
+  long long vectors = 0;
+  std::vector<int4> tiles;
+};
+ReciprocalTiles reciprocal_tiles(const double lattice[9], double kcut);
 
 // One system of a batch, as the host describes it.
 struct SystemDesc {
@@ -126,8 +139,8 @@ class Engine {
   // Per-kernel timing with HIP events on the launch stream (resets totals).
   void set_kernel_timing(bool enable);
   // Total milliseconds and number of launches of a kernel group since the
-  // timing was enabled: pack, order, cn, weights, c6, disp2, atm_list, atm,
-  // cn_chain, reduce.
+  // timing was enabled: pack, order, cn, weights, c6, disp2, lowrank,
+  // ewald_sf, ewald_f, ewald_atom, atm_list, atm, cn_chain, reduce, hessian.
   std::pair<double, long long> kernel_time(const std::string& name) const;
 
  private:

@@ -2025,10 +2025,14 @@ __global__ __launch_bounds__(kHT * kHT) void hess_assemble_kernel(DevBatch b, co
 // ---------------------------------------------------- reciprocal space
 // fourier/ewald.f90:47-252, the two-body energy of a 3D-periodic system as a
 // sum over reciprocal vectors k of the structure factors of the low-rank C6
-// factors, S_la(k) = sum_{i of species a} c6l_il exp(i k.x_i).  The
-// vectors come in +-k pairs with S(-k) = conj S(k) and identical
-// contributions, so only one of each pair is visited, with weight 2 (the
-// zero vector: weight 1); kv[k].w holds the weight.
+// factors, S_la(k) = sum_{i of species a} c6l_il exp(i k.x_i).  Of each +-k
+// pair one vector is visited (S(-k) = conj S(k), identical contributions)
+// with weight 2, k = 0 with weight 1.  Vectors k = h b1 + k b2 + l b3 come
+// in tiles of kWave consecutive l on one line (h, k); lane = vector, and
+// the flat vector index is tile * kWave + lane (lanes past a tile's count
+// are padding with weight 0).  The phase exp(i k.x) is the product of
+// per-atom tables exp(i n b_d.x), so the sums over vectors and atoms
+// evaluate no sine or cosine.
 
 // kernel.f90:83-129 get_fourier_transform: 4 pi / k int r sin(kr) f(r) dr of
 // f = pref r^m / (r^alpha + R^alpha) by the residues at the alpha/2 poles
@@ -2058,33 +2062,86 @@ __device__ void fourier_transform(const FourierTerm& t, double k, double& phi, d
   dphi = (pre * t.radius * dacc - phi) / k;
 }
 
+// The vector of lane `lane` of a tile, its weight (0 for padding lanes) and
+// its entries of the phase tables (h, k wave-uniform; l clamped to the tile
+// for padding lanes).
+struct TileVec {
+  double kx, ky, kz, w;
+  int e1, e2, e3;
+};
+__device__ __forceinline__ TileVec tile_vector(const EwaldBatch& e, const int4 tile, int lane) {
+  TileVec v;
+  const bool valid = lane < tile.w;
+  const int l = tile.z + (valid ? lane : 0);
+  v.kx = e.rec[0] * tile.x + e.rec[3] * tile.y + e.rec[6] * l;
+  v.ky = e.rec[1] * tile.x + e.rec[4] * tile.y + e.rec[7] * l;
+  v.kz = e.rec[2] * tile.x + e.rec[5] * tile.y + e.rec[8] * l;
+  v.w = !valid ? 0.0 : (tile.x == 0 && tile.y == 0 && l == 0) ? 1.0 : 2.0;
+  v.e1 = e.toff0 + e.rep0 + tile.x;
+  v.e2 = e.toff1 + e.rep1 + tile.y;
+  v.e3 = e.toff2 + e.rep2 + l;
+  return v;
+}
+
+// exp(i k.x) of atom row E for the vector v: E[h] E[k] (uniform) E[l].
+__device__ __forceinline__ void tile_phase(const double2* __restrict__ E, const TileVec& v, double& c, double& s) {
+  const double2 p1 = E[v.e1], p2 = E[v.e2], p3 = E[v.e3];
+  const double pr = p1.x * p2.x - p1.y * p2.y, pi = p1.x * p2.y + p1.y * p2.x;
+  c = pr * p3.x - pi * p3.y;
+  s = pr * p3.y + pi * p3.x;
+}
+
+// Phase tables, one row per atom in species order (row r for atom
+// sorder[r]): etab[r][toff_d + rep_d + n] = exp(i n b_d.x), |n| <= rep_d.
+__global__ __launch_bounds__(kBlock) void ewald_table_kernel(DevBatch b, EwaldBatch e, double2* __restrict__ etab) {
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<long long>(e.natoms) * e.tstride) return;
+  const int row = static_cast<int>(t / e.tstride), q = static_cast<int>(t % e.tstride);
+  const int d = q < e.toff1 ? 0 : q < e.toff2 ? 1 : 2;
+  const int n = q - (d == 0 ? e.toff0 + e.rep0 : d == 1 ? e.toff1 + e.rep1 : e.toff2 + e.rep2);
+  const double bx = d == 0 ? e.rec[0] : d == 1 ? e.rec[3] : e.rec[6];
+  const double by = d == 0 ? e.rec[1] : d == 1 ? e.rec[4] : e.rec[7];
+  const double bz = d == 0 ? e.rec[2] : d == 1 ? e.rec[5] : e.rec[8];
+  const double4 x = b.pos[e.sorder[row]];
+  double s, c;
+  sincos(n * (bx * x.x + by * x.y + bz * x.z), &s, &c);
+  etab[t] = make_double2(c, s);
+}
+
 // decomposition.f90:202-242 get_weights: c6l_il = sum_r vec(sp_i, r, l) w_ir
-// (and dc6l with dw/dCN); zero rows for ghost atoms.  The self-interaction
-// of the reciprocal sum (ewald.f90:131-147), 1/2 sum_l lambda_l c6l_il^2
-// phi_aa(0) and its CN derivative, is added to the atom's energy here.  One
-// thread per atom of the system.
+// (and dc6l with dw/dCN); zero rows for ghost atoms.  Stored by the atom's
+// position r in species order: c6l[r][l] (rows of rank_pad, zero padded)
+// for the structure factors, lambda_l c6l and lambda_l dc6l as lc6l[l][r]
+// and ldc6l[l][r] (rows of natoms + kEwaldGroup) for the per-atom pass.  The
+// self-interaction of the reciprocal sum (ewald.f90:131-147), 1/2 sum_l
+// lambda_l c6l_il^2 phi_aa(0) and its CN derivative, is added to the atom's
+// energy here.  One thread per atom of the system.
 __global__ __launch_bounds__(kBlock) void lowrank_kernel(DevBatch b, EwaldBatch e, const double* __restrict__ gw,
                                                          const double* __restrict__ dgw, double* __restrict__ c6l,
-                                                         double* __restrict__ dc6l, double* __restrict__ e_atom,
-                                                         double* __restrict__ dedcn) {
+                                                         double* __restrict__ lc6l, double* __restrict__ ldc6l,
+                                                         double* __restrict__ e_atom, double* __restrict__ dedcn) {
   const int il = blockIdx.x * blockDim.x + threadIdx.x;
   if (il >= e.natoms) return;
   const int i = e.atom_begin + il;
+  const int row = e.spos[il];
+  const long long ld = e.natoms + kEwaldGroup;
   const int sp = b.sp[i];
   const int nr = b.nref[sp];
   const bool ghost = b.ghost && b.ghost[i];
   double es = 0.0, ds = 0.0;
-  for (int l = 0; l < e.rank; ++l) {
+  for (int l = 0; l < e.rank_pad; ++l) {
     double a = 0.0, d = 0.0;
-    if (!ghost) {
+    if (!ghost && l < e.rank) {
       for (int r = 0; r < nr; ++r) {
         const double v = e.vec[(sp * kMaxRef + r) * e.rank + l];
         a += v * gw[i * b.mref + r];
         if (dgw) d += v * dgw[i * b.mref + r];
       }
     }
-    c6l[static_cast<long long>(il) * e.rank + l] = a;
-    if (dgw) dc6l[static_cast<long long>(il) * e.rank + l] = d;
+    c6l[static_cast<long long>(row) * e.rank_pad + l] = a;
+    if (l >= e.rank) continue;
+    lc6l[l * ld + row] = e.lambda[l] * a;
+    if (dgw) ldc6l[l * ld + row] = e.lambda[l] * d;
     es += 0.5 * e.lambda[l] * a * a * e.pval[sp];
     ds += e.lambda[l] * a * d * e.pval[sp];
   }
@@ -2092,21 +2149,22 @@ __global__ __launch_bounds__(kBlock) void lowrank_kernel(DevBatch b, EwaldBatch 
   if (dgw) dedcn[i] += ds;
 }
 
-// Partial structure factors: one wavefront per (64 vectors, chunk of atoms
-// of one species), lane = vector; the chunk's atoms are wave-uniform, so
-// their factors c6l_il come through scalar loads and each lane evaluates one
-// phase per atom.  Factors l0 .. l0+lcount (lcount <= kL) of the partial sum
-// over the chunk go to part[chunk][l][k] (no atomics; ewald_f sums the
-// chunks of a species).
+// Partial structure factors: one wavefront per (tile, chunk of atoms of one
+// species), lane = vector; the chunk's atoms are wave-uniform, so their
+// factors and the h, k table entries come through scalar loads.  Factors
+// l0 .. l0+lcount of the partial sum over the chunk go to
+// part[chunk][l][vector] (no atomics; ewald_f sums the chunks of a species);
+// kL = lcount rounded up to 8, the factor rows are zero padded to it.
 template <int kL>
-__global__ __launch_bounds__(kBlock) void ewald_sf_kernel(DevBatch b, EwaldBatch e, const double* __restrict__ c6l,
-                                                          int l0, int lcount, double2* __restrict__ part) {
-  const int ktiles = (e.nk + kWave - 1) / kWave;
-  const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (item >= ktiles * e.nchunks) return;
-  const int chunk = item / ktiles;
-  const int k = (item % ktiles) * kWave + wave_lane();
-  const double4 kv = e.kv[min(k, e.nk - 1)];
+__global__ __launch_bounds__(kBlock) void ewald_sf_kernel(EwaldBatch e, const double* __restrict__ c6l,
+                                                          const double2* __restrict__ etab, int l0, int lcount,
+                                                          double2* __restrict__ part) {
+  // wave-uniform: the tile, the chunk and its atoms' data come through scalar loads
+  const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (item >= e.ntiles * e.nchunks) return;
+  const int chunk = item / e.ntiles, t = item % e.ntiles;
+  const int lane = wave_lane();
+  const TileVec v = tile_vector(e, e.tiles[t], lane);
   const int4 c = e.chunks[chunk];
   double sr[kL], si[kL];
 #pragma unroll
@@ -2115,25 +2173,24 @@ __global__ __launch_bounds__(kBlock) void ewald_sf_kernel(DevBatch b, EwaldBatch
     si[l] = 0.0;
   }
   for (int q = c.y; q < c.z; ++q) {
-    const int i = e.sorder[q];
-    const double4 x = b.pos[i];
-    double s, co;
-    sincos(kv.x * x.x + kv.y * x.y + kv.z * x.z, &s, &co);
-    const double* f = c6l + static_cast<long long>(i - e.atom_begin) * e.rank + l0;
+    double co, s;
+    tile_phase(etab + static_cast<long long>(q) * e.tstride, v, co, s);
+    const double* f = c6l + static_cast<long long>(q) * e.rank_pad + l0;  // l0 + kL <= rank_pad
 #pragma unroll
     for (int l = 0; l < kL; ++l) {
-      if (l < lcount) {
-        const double w = f[l];
-        sr[l] += w * co;
-        si[l] += w * s;
-      }
+      const double w = f[l];
+      sr[l] += w * co;
+      si[l] += w * s;
     }
   }
-  if (k < e.nk) {
+  const long long k = static_cast<long long>(t) * kWave + lane;
+  const long long nk = static_cast<long long>(e.ntiles) * kWave;
+  const bool valid = v.w > 0.0;
 #pragma unroll
-    for (int l = 0; l < kL; ++l)
-      if (l < lcount) part[(static_cast<long long>(chunk) * e.rank + l0 + l) * e.nk + k] = make_double2(sr[l], si[l]);
-  }
+  for (int l = 0; l < kL; ++l)
+    if (l < lcount)
+      part[(static_cast<long long>(chunk) * e.rank + l0 + l) * nk + k] =
+          valid ? make_double2(sr[l], si[l]) : make_double2(0.0, 0.0);
 }
 
 // Per vector k (one lane each): S_al from the chunk partials, the transform
@@ -2146,23 +2203,24 @@ __global__ __launch_bounds__(kBlock) void ewald_sf_kernel(DevBatch b, EwaldBatch
 __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const double2* __restrict__ part,
                                                          double2* __restrict__ phi, double2* __restrict__ F,
                                                          double* __restrict__ sig_atom, int grad) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = k < e.nk;
+  const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long nk = static_cast<long long>(e.ntiles) * kWave;
+  const int t = static_cast<int>(k / kWave);
   const int nsp = e.nsp;
   double sg[6] = {0, 0, 0, 0, 0, 0};
-  if (valid) {
-    const double4 kv = e.kv[k];
-    const double knorm = sqrt(kv.x * kv.x + kv.y * kv.y + kv.z * kv.z);
+  if (t < e.ntiles) {
+    const TileVec v = tile_vector(e, e.tiles[t], static_cast<int>(k % kWave));
+    const double knorm = sqrt(v.kx * v.kx + v.ky * v.ky + v.kz * v.kz);
     for (int a = 0; a < nsp; ++a)
       for (int c = 0; c <= a; ++c) {
         double p = 0.0, dp = 0.0;
-        for (int t = 0; t < e.nterm[a * nsp + c]; ++t) {
+        for (int q = 0; q < e.nterm[a * nsp + c]; ++q) {
           double pv, dpv;
-          fourier_transform(e.terms[2 * (a * nsp + c) + t], knorm, pv, dpv);
+          fourier_transform(e.terms[2 * (a * nsp + c) + q], knorm, pv, dpv);
           p += pv;
           dp += dpv;
         }
-        phi[static_cast<long long>(a * (a + 1) / 2 + c) * e.nk + k] = make_double2(p, dp);
+        phi[static_cast<long long>(a * (a + 1) / 2 + c) * nk + k] = make_double2(p, dp);
       }
     double ek = 0.0, qq = 0.0;
     for (int l = 0; l < e.rank; ++l) {
@@ -2173,9 +2231,9 @@ __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const dou
         s_im[a] = 0.0;
         if (a < nsp) {
           for (int ch = e.chunk_seg[a]; ch < e.chunk_seg[a + 1]; ++ch) {
-            const double2 v = part[(static_cast<long long>(ch) * e.rank + l) * e.nk + k];
-            s_re[a] += v.x;
-            s_im[a] += v.y;
+            const double2 pv = part[(static_cast<long long>(ch) * e.rank + l) * nk + k];
+            s_re[a] += pv.x;
+            s_im[a] += pv.y;
           }
         }
       }
@@ -2188,28 +2246,27 @@ __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const dou
         for (int c = 0; c < kMaxEwaldSpecies; ++c) {
           if (c >= nsp) continue;
           const int pr = a >= c ? a * (a + 1) / 2 + c : c * (c + 1) / 2 + a;
-          const double2 pv = phi[static_cast<long long>(pr) * e.nk + k];
+          const double2 pv = phi[static_cast<long long>(pr) * nk + k];
           f_re += pv.x * s_re[c];
           f_im += pv.x * s_im[c];
           ql += pv.y * (s_re[a] * s_re[c] + s_im[a] * s_im[c]);
         }
-        F[(static_cast<long long>(a) * e.rank + l) * e.nk + k] = make_double2(f_re, f_im);
+        F[(static_cast<long long>(a) * e.rank + l) * nk + k] = make_double2(f_re, f_im);
         el += s_re[a] * f_re + s_im[a] * f_im;
       }
       ek += e.lambda[l] * el;
       qq += e.lambda[l] * ql;
     }
-    if (grad) {
+    if (grad && v.w > 0.0) {
       // E_k = -1/(2V) sum_l lambda_l sum_a Re(S_al conj F_al), weighted
-      const double w = kv.w;
-      const double erec = -0.5 * w * ek * e.inv_vol;
-      const double h = knorm > 0.0 ? 0.5 * w * qq * e.inv_vol / knorm : 0.0;
-      sg[0] = h * kv.x * kv.x - erec;
-      sg[1] = h * kv.y * kv.y - erec;
-      sg[2] = h * kv.z * kv.z - erec;
-      sg[3] = h * kv.x * kv.y;
-      sg[4] = h * kv.x * kv.z;
-      sg[5] = h * kv.y * kv.z;
+      const double erec = -0.5 * v.w * ek * e.inv_vol;
+      const double h = knorm > 0.0 ? 0.5 * v.w * qq * e.inv_vol / knorm : 0.0;
+      sg[0] = h * v.kx * v.kx - erec;
+      sg[1] = h * v.ky * v.ky - erec;
+      sg[2] = h * v.kz * v.kz - erec;
+      sg[3] = h * v.kx * v.ky;
+      sg[4] = h * v.kx * v.kz;
+      sg[5] = h * v.ky * v.kz;
     }
   }
   if (grad) {
@@ -2220,72 +2277,107 @@ __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const dou
   }
 }
 
-// Per atom i and a range of vectors (one wavefront, lane = vector):
-// A = sum_l lambda_l c6l_il conj F_{a_i l}(k) and B the same with dc6l, then
-// (ewald.f90:193-212) E_i -= 1/(2V) Re(e^{ik.x} A), dE/dCN_i -= 1/V
-// Re(e^{ik.x} B), grad_i += 1/V Im(e^{ik.x} A) k, each weighted; the
-// wavefront's sums are added to the atom's accumulators.
+// Per group of up to kEwaldGroup atoms of one species and a range of tiles
+// (one wavefront, lane = vector): A_i = sum_l lambda_l c6l_il conj F_al(k)
+// and B_i the same with dc6l, each F load shared by the group; then
+// (ewald.f90:193-212) E_i -= 1/(2V) Re(e^{ik.x} A_i), dE/dCN_i -= 1/V
+// Re(e^{ik.x} B_i), grad_i += 1/V Im(e^{ik.x} A_i) k, weighted; the
+// wavefront's sums are added to the atoms' accumulators.
 template <bool kGrad>
-__global__ __launch_bounds__(kBlock) void ewald_atom_kernel(DevBatch b, EwaldBatch e, const double* __restrict__ c6l,
-                                                            const double* __restrict__ dc6l,
+__global__ __launch_bounds__(kBlock) void ewald_atom_kernel(EwaldBatch e, const double* __restrict__ lc6l,
+                                                            const double* __restrict__ ldc6l,
+                                                            const double2* __restrict__ etab,
                                                             const double2* __restrict__ F, int ksplit,
                                                             double* __restrict__ e_atom, double* __restrict__ dedcn,
                                                             double* __restrict__ grad) {
-  const int item = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (item >= e.natoms * ksplit) return;
-  const int il = item / ksplit, part = item % ksplit;
-  const int i = e.atom_begin + il;
-  const int sp = b.sp[i];
-  const double4 x = b.pos[i];
-  const int ktiles = (e.nk + kWave - 1) / kWave;
-  const int t0 = static_cast<int>(static_cast<long long>(ktiles) * part / ksplit);
-  const int t1 = static_cast<int>(static_cast<long long>(ktiles) * (part + 1) / ksplit);
-  const double* f = c6l + static_cast<long long>(il) * e.rank;
-  const double* df = kGrad ? dc6l + static_cast<long long>(il) * e.rank : nullptr;
-  const double2* Fa = F + static_cast<long long>(sp) * e.rank * e.nk;
-  double acc_e = 0.0, acc_d = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+  // wave-uniform: the group, its atoms' factors and the tiles come through scalar loads
+  const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (item >= e.ngroups * ksplit) return;
+  // neighbouring wavefronts: neighbouring groups (mostly one species) on the
+  // same tile range, so their F loads meet in the caches
+  const int g = item % e.ngroups, part = item / e.ngroups;
+  const int4 grp = e.groups[g];
+  const int cnt = grp.z - grp.y;
+  const int lane = wave_lane();
+  const long long ld = e.natoms + kEwaldGroup;
+  const double* gl = lc6l + grp.y;   // gl[l * ld + j]: lambda_l c6l of the group's atom j
+  const double* gd = kGrad ? ldc6l + grp.y : nullptr;
+  const long long nk = static_cast<long long>(e.ntiles) * kWave;
+  const double2* Fa = F + static_cast<long long>(grp.x) * e.rank * nk;
+  const int t0 = static_cast<int>(static_cast<long long>(e.ntiles) * part / ksplit);
+  const int t1 = static_cast<int>(static_cast<long long>(e.ntiles) * (part + 1) / ksplit);
+  double ae[kEwaldGroup], ad[kEwaldGroup], g0[kEwaldGroup], g1[kEwaldGroup], g2[kEwaldGroup];
+#pragma unroll
+  for (int j = 0; j < kEwaldGroup; ++j) ae[j] = ad[j] = g0[j] = g1[j] = g2[j] = 0.0;
   for (int t = t0; t < t1; ++t) {
-    const int k = t * kWave + wave_lane();
-    if (k >= e.nk) continue;
-    const double4 kv = e.kv[k];
-    double a_re = 0.0, a_im = 0.0, b_re = 0.0, b_im = 0.0;
-    for (int l = 0; l < e.rank; ++l) {
-      const double2 fv = Fa[static_cast<long long>(l) * e.nk + k];
-      const double cl = e.lambda[l] * f[l];
-      a_re += cl * fv.x;
-      a_im += cl * fv.y;
-      if (kGrad) {
-        const double dl = e.lambda[l] * df[l];
-        b_re += dl * fv.x;
-        b_im += dl * fv.y;
+    const TileVec v = tile_vector(e, e.tiles[t], lane);
+    const long long k = static_cast<long long>(t) * kWave + lane;
+    double a_re[kEwaldGroup], a_im[kEwaldGroup], b_re[kEwaldGroup], b_im[kEwaldGroup];
+#pragma unroll
+    for (int j = 0; j < kEwaldGroup; ++j) a_re[j] = a_im[j] = b_re[j] = b_im[j] = 0.0;
+    // four factors per step, their F loads issued together
+    for (int l0 = 0; l0 < e.rank; l0 += 4) {
+      double2 fv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        fv[u] = l0 + u < e.rank ? Fa[static_cast<long long>(l0 + u) * nk + k] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (l0 + u >= e.rank) break;
+        const int l = l0 + u;
+        // all kEwaldGroup columns (the padding past the group is discarded):
+        // adjacent loads the compiler merges
+#pragma unroll
+        for (int j = 0; j < kEwaldGroup; ++j) {
+          const double cl = gl[l * ld + j];
+          a_re[j] += cl * fv[u].x;
+          a_im[j] += cl * fv[u].y;
+          if (kGrad) {
+            const double dl = gd[l * ld + j];
+            b_re[j] += dl * fv[u].x;
+            b_im[j] += dl * fv[u].y;
+          }
+        }
       }
     }
-    double s, c;
-    sincos(kv.x * x.x + kv.y * x.y + kv.z * x.z, &s, &c);
-    // e^{ik.x} conj(F) = (c + i s)(F_re - i F_im)
-    acc_e += kv.w * (c * a_re + s * a_im);
-    if (kGrad) {
-      acc_d += kv.w * (c * b_re + s * b_im);
-      const double zim = kv.w * (s * a_re - c * a_im);
-      g0 += zim * kv.x;
-      g1 += zim * kv.y;
-      g2 += zim * kv.z;
+#pragma unroll
+    for (int j = 0; j < kEwaldGroup; ++j) {
+      if (j < cnt) {
+        double c, s;
+        tile_phase(etab + static_cast<long long>(grp.y + j) * e.tstride, v, c, s);
+        // e^{ik.x} conj(F) = (c + i s)(F_re - i F_im)
+        ae[j] += v.w * (c * a_re[j] + s * a_im[j]);
+        if (kGrad) {
+          ad[j] += v.w * (c * b_re[j] + s * b_im[j]);
+          const double zim = v.w * (s * a_re[j] - c * a_im[j]);
+          g0[j] += zim * v.kx;
+          g1[j] += zim * v.ky;
+          g2[j] += zim * v.kz;
+        }
+      }
     }
   }
-  acc_e = wave_sum(acc_e);
-  if (kGrad) {
-    acc_d = wave_sum(acc_d);
-    g0 = wave_sum(g0);
-    g1 = wave_sum(g1);
-    g2 = wave_sum(g2);
-  }
-  if (wave_lane() == 0) {
-    atomicAdd(&e_atom[i], -0.5 * acc_e * e.inv_vol);
-    if (kGrad) {
-      atomicAdd(&dedcn[i], -acc_d * e.inv_vol);
-      atomicAdd(&grad[3 * i], g0 * e.inv_vol);
-      atomicAdd(&grad[3 * i + 1], g1 * e.inv_vol);
-      atomicAdd(&grad[3 * i + 2], g2 * e.inv_vol);
+#pragma unroll
+  for (int j = 0; j < kEwaldGroup; ++j) {
+    if (j < cnt) {
+      const double se = wave_sum(ae[j]);
+      double sd = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      if (kGrad) {
+        sd = wave_sum(ad[j]);
+        s0 = wave_sum(g0[j]);
+        s1 = wave_sum(g1[j]);
+        s2 = wave_sum(g2[j]);
+      }
+      if (lane == 0) {
+        const int i = e.sorder[grp.y + j];
+        atomicAdd(&e_atom[i], -0.5 * se * e.inv_vol);
+        if (kGrad) {
+          atomicAdd(&dedcn[i], -sd * e.inv_vol);
+          atomicAdd(&grad[3 * i], s0 * e.inv_vol);
+          atomicAdd(&grad[3 * i + 1], s1 * e.inv_vol);
+          atomicAdd(&grad[3 * i + 2], s2 * e.inv_vol);
+        }
+      }
     }
   }
 }
@@ -2492,32 +2584,48 @@ void launch_pair3_finalize(const DevBatch& b, const double* pair3s, double* out,
                      pair3s, out);
 }
 
-void launch_ewald(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
-                  hipStream_t st) {
+void launch_lowrank(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(ewald_table_kernel, dim3(blocks_for(static_cast<long long>(e.natoms) * e.tstride, kBlock)),
+                     dim3(kBlock), 0, st, b, e, ew.etab);
   hipLaunchKernelGGL(lowrank_kernel, dim3(blocks_for(e.natoms, kBlock)), dim3(kBlock), 0, st, b, e, w.gw,
-                     grad ? w.dgw : nullptr, ew.c6l, ew.dc6l, w.e_atom, w.dedcn);
-  const int ktiles = blocks_for(e.nk, kWave);
-  const dim3 sf_grid(blocks_for(static_cast<long long>(ktiles) * e.nchunks, kWavesPerBlock));
+                     grad ? w.dgw : nullptr, ew.c6l, ew.lc6l, ew.ldc6l, w.e_atom, w.dedcn);
+}
+
+void launch_ewald_sf(const EwaldBatch& e, const EwaldWork& ew, hipStream_t st) {
+  const dim3 grid(blocks_for(static_cast<long long>(e.ntiles) * e.nchunks, kWavesPerBlock));
   for (int l0 = 0; l0 < e.rank; l0 += 32) {
     const int lc = std::min(32, e.rank - l0);
     if (lc <= 8)
-      hipLaunchKernelGGL(ewald_sf_kernel<8>, sf_grid, dim3(kBlock), 0, st, b, e, ew.c6l, l0, lc, ew.part);
+      hipLaunchKernelGGL(ewald_sf_kernel<8>, grid, dim3(kBlock), 0, st, e, ew.c6l, ew.etab, l0, lc, ew.part);
     else if (lc <= 16)
-      hipLaunchKernelGGL(ewald_sf_kernel<16>, sf_grid, dim3(kBlock), 0, st, b, e, ew.c6l, l0, lc, ew.part);
+      hipLaunchKernelGGL(ewald_sf_kernel<16>, grid, dim3(kBlock), 0, st, e, ew.c6l, ew.etab, l0, lc, ew.part);
+    else if (lc <= 24)
+      hipLaunchKernelGGL(ewald_sf_kernel<24>, grid, dim3(kBlock), 0, st, e, ew.c6l, ew.etab, l0, lc, ew.part);
     else
-      hipLaunchKernelGGL(ewald_sf_kernel<32>, sf_grid, dim3(kBlock), 0, st, b, e, ew.c6l, l0, lc, ew.part);
+      hipLaunchKernelGGL(ewald_sf_kernel<32>, grid, dim3(kBlock), 0, st, e, ew.c6l, ew.etab, l0, lc, ew.part);
   }
-  hipLaunchKernelGGL(ewald_f_kernel, dim3(blocks_for(e.nk, kBlock)), dim3(kBlock), 0, st, e, ew.part, ew.phi, ew.F,
-                     w.sig_atom, grad ? 1 : 0);
-  // some 16k wavefronts: each atom's vectors split into ksplit ranges
-  const int ksplit = std::max(1, std::min(ktiles, (16384 + e.natoms - 1) / e.natoms));
-  const dim3 at_grid(blocks_for(static_cast<long long>(e.natoms) * ksplit, kWavesPerBlock));
+}
+
+void launch_ewald_f(const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew, hipStream_t st) {
+  hipLaunchKernelGGL(ewald_f_kernel, dim3(blocks_for(static_cast<long long>(e.ntiles) * kWave, kBlock)), dim3(kBlock),
+                     0, st, e, ew.part, ew.phi, ew.F, w.sig_atom, grad ? 1 : 0);
+}
+
+int ewald_atom_split(const EwaldBatch& e) {
+  // some 16k wavefronts: each group's tiles split into ksplit ranges
+  return std::max(1, std::min(e.ntiles, (16384 + e.ngroups - 1) / std::max(e.ngroups, 1)));
+}
+
+void launch_ewald_atom(const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew, hipStream_t st) {
+  const int ksplit = ewald_atom_split(e);
+  const dim3 grid(blocks_for(static_cast<long long>(e.ngroups) * ksplit, kWavesPerBlock));
   if (grad)
-    hipLaunchKernelGGL(ewald_atom_kernel<true>, at_grid, dim3(kBlock), 0, st, b, e, ew.c6l, ew.dc6l, ew.F, ksplit,
-                       w.e_atom, w.dedcn, w.grad);
+    hipLaunchKernelGGL(ewald_atom_kernel<true>, grid, dim3(kBlock), 0, st, e, ew.lc6l, ew.ldc6l, ew.etab, ew.F,
+                       ksplit, w.e_atom, w.dedcn, w.grad);
   else
-    hipLaunchKernelGGL(ewald_atom_kernel<false>, at_grid, dim3(kBlock), 0, st, b, e, ew.c6l, ew.dc6l, ew.F, ksplit,
-                       w.e_atom, w.dedcn, w.grad);
+    hipLaunchKernelGGL(ewald_atom_kernel<false>, grid, dim3(kBlock), 0, st, e, ew.lc6l, ew.ldc6l, ew.etab, ew.F,
+                       ksplit, w.e_atom, w.dedcn, w.grad);
 }
 
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t st) {

@@ -119,31 +119,47 @@ void launch_hessian(const DevBatch& b, const HessWork& h, const long long* offse
                     long long total_slots, int max_natoms, hipStream_t stream);
 // Reciprocal-space two-body term of one 3D-periodic system with a low-rank
 // model (fourier/ewald.f90:47-252): one vector of each +-k pair within the
-// cutoff, kv = (k, weight); the low-rank expansion in batch species order;
-// the atoms of the system grouped by species and cut into chunks.
+// cutoff, in tiles {h, k, l0, count} of up to kWave vectors h b1 + k b2 +
+// l b3 (flat vector index tile * kWave + lane); the low-rank expansion in
+// batch species order; the atoms of the system grouped by species, cut into
+// chunks (structure factors) and groups of up to kEwaldGroup (per-atom pass).
+constexpr int kEwaldGroup = 4;
 struct EwaldBatch {
-  int nk, rank, nsp, natoms, atom_begin, nchunks;
+  int ntiles, rank, rank_pad, nsp, natoms, atom_begin, nchunks, ngroups;  // rank_pad: rank rounded up to 8
   double inv_vol;
-  const double4* kv;         // [nk]
+  double rec[9];             // b_1, b_2, b_3 (rows)
+  int rep0, rep1, rep2;      // |h| <= rep0, |k| <= rep1, |l| <= rep2
+  int tstride, toff0, toff1, toff2;  // phase table row: direction d at toff_d + rep_d + n
+  const int4* tiles;         // [ntiles]
   const double* lambda;      // [rank]
   const double* vec;         // [nsp][kMaxRef][rank]
   const double* pval;        // [nsp] pair potential of (a, a) at r = 0
   const FourierTerm* terms;  // [nsp][nsp][2]
   const int* nterm;          // [nsp][nsp]
   const int* sorder;         // [natoms] atoms by species (global indices)
+  const int* spos;           // [natoms] position of each atom of the system in sorder
   const int4* chunks;        // [nchunks] {species, begin, end, 0} into sorder
   const int* chunk_seg;      // [nsp + 1] chunks of each species
+  const int4* groups;        // [ngroups] {species, begin, end, 0} into sorder, end - begin <= kEwaldGroup
 };
 struct EwaldWork {
-  double *c6l, *dc6l;        // [natoms][rank] (dc6l null without gradient)
-  double2 *part;             // [nchunks][rank][nk] partial structure factors
+  // rows and columns by position in species order (sorder)
+  double2 *etab;             // [natoms][tstride] exp(i n b_d.x)
+  double *c6l;               // [natoms][rank_pad], zero padded
+  double *lc6l, *ldc6l;      // [rank][natoms + kEwaldGroup]: lambda c6l, lambda dc6l (null without gradient)
+  double2 *part;             // [nchunks][rank][nk] partial structure factors (nk = ntiles * kWave)
   double2 *phi;              // [nsp (nsp + 1) / 2][nk] transform and its k-derivative
   double2 *F;                // [nsp][rank][nk] sum_b phihat_ab S_bl
 };
-// Adds the reciprocal-space energy (and dE/dCN, gradient, virial) to the
-// accumulators of w; runs after launch_disp2.
-void launch_ewald(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
-                  hipStream_t stream);
+// The reciprocal-space energy (and dE/dCN, gradient, virial), added to the
+// accumulators of w after launch_disp2, in this order: the phase tables and
+// low-rank factors (and the self-interaction), the partial structure
+// factors, the per-vector transforms F and virial, the per-atom sums.
+void launch_lowrank(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
+                    hipStream_t stream);
+void launch_ewald_sf(const EwaldBatch& e, const EwaldWork& ew, hipStream_t stream);
+void launch_ewald_f(const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew, hipStream_t stream);
+void launch_ewald_atom(const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew, hipStream_t stream);
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t stream);
 void launch_reduce(const DevBatch& b, const DevWork& w, double* energy, double* sigma, hipStream_t stream);
 void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream_t stream);

@@ -186,25 +186,70 @@ double fourier_cutoff(const FourierTerm& t, double tolerance) {
   return -std::log(tolerance) / (t.radius * std::sin(M_PI / t.alpha));
 }
 
-std::vector<double> reciprocal_vectors(const double lattice[9], double kcut, double& volume) {
+ReciprocalTiles reciprocal_tiles(const double lattice[9], double kcut) {
   // ewald.f90:118-121: b_j with a_i . b_j = 2 pi delta_ij (rows of lattice
-  // are the a_i), the points of that lattice within kcut
+  // are the a_i) and the points h b1 + k b2 + l b3 within kcut, in the box
+  // that get_lattice_points spans (cutoff.f90:190-227).  Of each +-k pair
+  // the one with (h, k, l) lexicographically positive is kept.  A line (h, k)
+  // meets the sphere in one interval of l, found from the quadratic in l and
+  // trimmed with the exact test |k| <= kcut on the vector as the reference
+  // forms it; the interval is cut into tiles of kWave consecutive l.
+  ReciprocalTiles out;
   const double* a = lattice;
   const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
                      a[2] * (a[3] * a[7] - a[4] * a[6]);
-  volume = std::fabs(det);
-  if (volume <= 0.0) throw Error("Lattice vectors are linearly dependent");
-  // b_j = 2 pi (a_k x a_l) / det for (j, k, l) cyclic
-  double rec[9];
-  for (int j = 0; j < 3; ++j) {
+  out.volume = std::fabs(det);
+  if (out.volume <= 0.0) throw Error("Lattice vectors are linearly dependent");
+  double* b = out.rec;
+  for (int j = 0; j < 3; ++j) {  // b_j = 2 pi (a_k x a_l) / det, (j, k, l) cyclic
     const double* u = a + 3 * ((j + 1) % 3);
     const double* v = a + 3 * ((j + 2) % 3);
-    rec[3 * j + 0] = 2.0 * M_PI * (u[1] * v[2] - u[2] * v[1]) / det;
-    rec[3 * j + 1] = 2.0 * M_PI * (u[2] * v[0] - u[0] * v[2]) / det;
-    rec[3 * j + 2] = 2.0 * M_PI * (u[0] * v[1] - u[1] * v[0]) / det;
+    b[3 * j + 0] = 2.0 * M_PI * (u[1] * v[2] - u[2] * v[1]) / det;
+    b[3 * j + 1] = 2.0 * M_PI * (u[2] * v[0] - u[0] * v[2]) / det;
+    b[3 * j + 2] = 2.0 * M_PI * (u[0] * v[1] - u[1] * v[0]) / det;
   }
-  const int all[3] = {1, 1, 1};
-  return lattice_points(all, rec, kcut);
+  // repetitions per direction as get_translations (cutoff.f90:190-227)
+  for (int d = 0; d < 3; ++d) {
+    const double* u = b + 3 * ((d + 1) % 3);
+    const double* v = b + 3 * ((d + 2) % 3);
+    double n[3] = {u[1] * v[2] - v[1] * u[2], u[2] * v[0] - v[2] * u[0], u[0] * v[1] - v[0] * u[1]};
+    const double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double cs = (n[0] * b[3 * d] + n[1] * b[3 * d + 1] + n[2] * b[3 * d + 2]) / len;
+    out.rep[d] = static_cast<int>(std::ceil(std::fabs(kcut / cs)));
+  }
+  auto norm = [&](int h, int k, int l) {
+    double s = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const double x = b[c] * h + b[3 + c] * k + b[6 + c] * l;
+      s += x * x;
+    }
+    return std::sqrt(s);
+  };
+  const double* b3 = b + 6;
+  const double bb = b3[0] * b3[0] + b3[1] * b3[1] + b3[2] * b3[2];
+  for (int h = 0; h <= out.rep[0]; ++h) {
+    for (int k = h == 0 ? 0 : -out.rep[1]; k <= out.rep[1]; ++k) {
+      double c[3];
+      for (int q = 0; q < 3; ++q) c[q] = b[q] * h + b[3 + q] * k;
+      const double cb = c[0] * b3[0] + c[1] * b3[1] + c[2] * b3[2];
+      const double cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+      const double disc = cb * cb - bb * (cc - kcut * kcut);
+      if (disc < 0.0) continue;
+      const double sq = std::sqrt(disc);
+      int lo = static_cast<int>(std::floor((-cb - sq) / bb)) - 1;
+      int hi = static_cast<int>(std::ceil((-cb + sq) / bb)) + 1;
+      lo = std::max(lo, h == 0 && k == 0 ? 0 : -out.rep[2]);
+      hi = std::min(hi, out.rep[2]);
+      while (lo <= hi && norm(h, k, lo) > kcut) ++lo;
+      while (hi >= lo && norm(h, k, hi) > kcut) --hi;
+      for (int l = lo; l <= hi; l += kWave) {
+        const int cnt = std::min(kWave, hi - l + 1);
+        out.tiles.push_back(make_int4(h, k, l, cnt));
+        out.vectors += cnt;
+      }
+    }
+  }
+  return out;
 }
 
 }  // namespace d3

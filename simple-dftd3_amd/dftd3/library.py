@@ -45,7 +45,7 @@ _SIGNATURES = {
     "dftd3_set_model_realspace_cutoff_smooth": (None, [_vp, _vp, _d, _d, _d, _d, _d]),
     "dftd3_set_model_ghost_index": (None, [_vp, _vp, _ip, ctypes.c_int]),
     "dftd3_set_model_ewald": (None, [_vp, _vp, ctypes.c_int, _d, _d, ctypes.c_int]),
-    "d3hip_get_model_lowrank": (None, [_vp, _vp, _ip, _dp]),
+    "d3hip_get_model_lowrank": (None, [_vp, _vp, _ip, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "dftd3_delete_model": (None, [ctypes.POINTER(_vp)]),
     "dftd3_new_zero_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d]),
     "dftd3_load_zero_damping": (_vp, [_vp, ctypes.c_char_p, ctypes.c_bool]),
@@ -221,11 +221,18 @@ def set_model_ewald(disp: ModelHandle, rank: int, tolerance: float, kcut: float,
     return error_check(lib.dftd3_set_model_ewald)(disp.handle, int(rank), float(tolerance), float(kcut), int(mesh))
 
 
-def get_model_lowrank(disp: ModelHandle) -> tuple:
-    """(rank, largest relative error) of the model's low-rank expansion"""
+def get_model_lowrank(disp: ModelHandle, last_evaluation: bool = False) -> tuple:
+    """(rank, largest relative error) of the model's low-rank expansion; with
+    last_evaluation also the reciprocal cutoff and number of vectors of the
+    last reciprocal-space evaluation"""
     rank = ctypes.c_int(0)
     err = ctypes.c_double(0.0)
-    error_check(lib.d3hip_get_model_lowrank)(disp.handle, ctypes.byref(rank), ctypes.byref(err))
+    kcut = ctypes.c_double(0.0)
+    nvec = ctypes.c_longlong(0)
+    error_check(lib.d3hip_get_model_lowrank)(disp.handle, ctypes.byref(rank), ctypes.byref(err), ctypes.byref(kcut),
+                                             ctypes.byref(nvec))
+    if last_evaluation:
+        return rank.value, err.value, kcut.value, nvec.value
     return rank.value, err.value
 
 
