@@ -2090,12 +2090,317 @@ static int ewald2(const d3o_param *p, int nat, const int *num, const int *id, co
     return 0;
 }
 
+/* ------------------------------------------------------ particle mesh */
+#define SPME_ORDER 6 /* spme.f90:47 spline_order */
+
+/* fft.f90:48-60 valid_mesh_size: the next power of two */
+static int valid_mesh(int npoint)
+{
+    int n = 1;
+    while (n < npoint) n *= 2;
+    return n;
+}
+
+/* spme.f90:457-499 fill_bspline: cardinal B-spline weights of order 6 at the
+ * offset w in [0, 1) and their derivatives (from the order-5 weights). */
+static void fill_bspline(double w, double *theta, double *dtheta)
+{
+    for (int k = 0; k < SPME_ORDER; ++k) theta[k] = 0.0;
+    theta[0] = 1.0 - w;
+    theta[1] = w;
+    for (int korder = 3; korder <= SPME_ORDER - 1; ++korder) {
+        double div = 1.0 / (korder - 1);
+        theta[korder - 1] = div * w * theta[korder - 2];
+        for (int j = 1; j <= korder - 2; ++j)
+            theta[korder - j - 1] = div * ((w + j) * theta[korder - j - 2] + (korder - j - w) * theta[korder - j - 1]);
+        theta[0] = div * (1.0 - w) * theta[0];
+    }
+    dtheta[0] = -theta[0];
+    for (int j = 2; j <= SPME_ORDER; ++j) dtheta[j - 1] = theta[j - 2] - theta[j - 1];
+    double div = 1.0 / (SPME_ORDER - 1);
+    theta[SPME_ORDER - 1] = div * w * theta[SPME_ORDER - 2];
+    for (int j = 1; j <= SPME_ORDER - 2; ++j)
+        theta[SPME_ORDER - j - 1] =
+            div * ((w + j) * theta[SPME_ORDER - j - 2] + (SPME_ORDER - j - w) * theta[SPME_ORDER - j - 1]);
+    theta[0] = div * (1.0 - w) * theta[0];
+}
+
+/* Radix-2 transform of a strided complex sequence in place (re/im arrays),
+ * exp(sign 2 pi i j k / n), unnormalised; fft.f90:136-188 (isign = sign). */
+static void fft_1d(double *re, double *im, int n, size_t stride, int sign)
+{
+    if (n <= 1) return;
+    for (int i = 1, j = 0; i < n; ++i) { /* bit reversal */
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t = re[i * stride]; re[i * stride] = re[j * stride]; re[j * stride] = t;
+            t = im[i * stride]; im[i * stride] = im[j * stride]; im[j * stride] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len / 2;
+        for (int k = 0; k < half; ++k) {
+            double ang = sign * 2.0 * M_PI * k / len;
+            double wr = cos(ang), wi = sin(ang);
+            for (int i = k; i < n; i += len) {
+                size_t a = (size_t)i * stride, b = (size_t)(i + half) * stride;
+                double tr = wr * re[b] - wi * im[b], ti = wr * im[b] + wi * re[b];
+                re[b] = re[a] - tr; im[b] = im[a] - ti;
+                re[a] += tr; im[a] += ti;
+            }
+        }
+    }
+}
+
+/* fft.f90:86-131 fft_3d on a grid [m3][m2][m1] (index g1 fastest). */
+static void fft_3d(double *re, double *im, const int *m, int sign)
+{
+    for (int i3 = 0; i3 < m[2]; ++i3)
+        for (int i2 = 0; i2 < m[1]; ++i2) {
+            size_t o = ((size_t)i3 * m[1] + i2) * m[0];
+            fft_1d(re + o, im + o, m[0], 1, sign);
+        }
+    for (int i3 = 0; i3 < m[2]; ++i3)
+        for (int i1 = 0; i1 < m[0]; ++i1) {
+            size_t o = (size_t)i3 * m[1] * m[0] + i1;
+            fft_1d(re + o, im + o, m[1], m[0], sign);
+        }
+    for (int i2 = 0; i2 < m[1]; ++i2)
+        for (int i1 = 0; i1 < m[0]; ++i1) {
+            size_t o = (size_t)i2 * m[0] + i1;
+            fft_1d(re + o, im + o, m[2], (size_t)m[0] * m[1], sign);
+        }
+}
+
+/* fourier/spme.f90:78-328 get_dispersion_spme with spme.f90:55-74 (mesh from
+ * the cutoff) and :346-529 (mesh kernel, spline weights, Euler factors): the
+ * low-rank factors spread onto one mesh per species, transformed, contracted
+ * with the kernel, transformed back and gathered onto the atoms, term by
+ * term of the expansion.  mesh_req > 0 fixes the mesh points per direction. */
+static int spme2(const d3o_param *p, int nat, const int *num, const int *id, const double *xyz,
+                 const double *latflat, const d3o_lowrank *lr, const double *c6l, const double *dc6l,
+                 double kcut_req, int mesh_req, const int *ghost, double *energy, double *dEdcn, double *gradient,
+                 double *sigma, int grad)
+{
+    int nid = lr->nid, R = lr->rank;
+    int zsp[MAX_ELEM] = {0};
+    for (int i = 0; i < nat; ++i) zsp[id[i] - 1] = num[i];
+    d3o_term *terms = (d3o_term *)calloc((size_t)nid * nid * 2, sizeof(d3o_term));
+    int *nterm = (int *)calloc((size_t)nid * nid, sizeof(int));
+    double kcut = 0.0;
+    for (int a = 0; a < nid; ++a)
+        for (int b = 0; b < nid; ++b) {
+            nterm[a * nid + b] = fourier_terms(p, zsp[a], zsp[b], terms + 2 * (a * nid + b));
+            for (int t = 0; t < nterm[a * nid + b]; ++t) {
+                const d3o_term *tt = terms + 2 * (a * nid + b) + t;
+                if (!(tt->alpha > 0 && tt->m >= 0 && tt->alpha % 2 == 0 && tt->m % 2 == 0 && tt->m + 3 < tt->alpha
+                      && tt->radius > 0.0)) {
+                    free(terms); free(nterm);
+                    return 2;
+                }
+                double kc = reciprocal_cutoff(tt, 1.0e-8);
+                if (kc > kcut) kcut = kc;
+            }
+        }
+    if (kcut <= 0.0) { free(terms); free(nterm); return 0; }
+    if (kcut_req > 0.0) kcut = kcut_req;
+    double L[3][3], inv[3][3];
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) L[r][c] = latflat[3 * r + c];
+    matinv3(L, inv); /* inv = M^-1, M columns = lattice vectors */
+    double rec[9]; /* rows: b_j = 2 pi row j of M^-1 */
+    for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) rec[3 * r + c] = 2.0 * M_PI * inv[r][c];
+    double vol = fabs(L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) - L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0])
+                      + L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]));
+    int m[3];
+    for (int d = 0; d < 3; ++d) {
+        double len = sqrt(L[d][0] * L[d][0] + L[d][1] * L[d][1] + L[d][2] * L[d][2]);
+        m[d] = mesh_req > 0 ? valid_mesh(mesh_req) : valid_mesh((int)ceil(kcut * len / M_PI));
+    }
+    size_t ng = (size_t)m[0] * m[1] * m[2];
+    /* spline weights per atom and direction */
+    int *base = (int *)malloc(sizeof(int) * 3 * nat);
+    double *th = (double *)malloc(sizeof(double) * 3 * SPME_ORDER * nat);
+    double *dth = (double *)malloc(sizeof(double) * 3 * SPME_ORDER * nat);
+    for (int i = 0; i < nat; ++i)
+        for (int d = 0; d < 3; ++d) {
+            double f = inv[d][0] * xyz[3 * i] + inv[d][1] * xyz[3 * i + 1] + inv[d][2] * xyz[3 * i + 2];
+            double u = m[d] * (f - floor(f));
+            base[3 * i + d] = (int)floor(u) - SPME_ORDER;
+            fill_bspline(u - floor(u), th + (3 * i + d) * SPME_ORDER, dth + (3 * i + d) * SPME_ORDER);
+        }
+    /* Euler factors */
+    double *bfac = (double *)malloc(sizeof(double) * 3 * (m[0] > m[1] ? (m[0] > m[2] ? m[0] : m[2]) : (m[1] > m[2] ? m[1] : m[2])));
+    int mmax = m[0] > m[1] ? (m[0] > m[2] ? m[0] : m[2]) : (m[1] > m[2] ? m[1] : m[2]);
+    {
+        double spl[SPME_ORDER], dspl[SPME_ORDER];
+        fill_bspline(0.0, spl, dspl);
+        for (int d = 0; d < 3; ++d)
+            for (int im = 0; im < m[d]; ++im) {
+                double dr = 0.0, di = 0.0;
+                for (int jm = 0; jm < SPME_ORDER - 1; ++jm) {
+                    double arg = 2.0 * M_PI * im * jm / m[d];
+                    dr += spl[jm] * cos(arg);
+                    di += spl[jm] * sin(arg);
+                }
+                bfac[d * mmax + im] = 1.0 / (dr * dr + di * di);
+            }
+    }
+    /* mesh kernel */
+    int npair = nid * (nid + 1) / 2;
+    double *phik = (double *)malloc(sizeof(double) * npair * ng);
+    double *dphik = (double *)malloc(sizeof(double) * npair * ng);
+    double *kvecs = (double *)malloc(sizeof(double) * 3 * ng);
+    for (int i3 = 0; i3 < m[2]; ++i3)
+        for (int i2 = 0; i2 < m[1]; ++i2)
+            for (int i1 = 0; i1 < m[0]; ++i1) {
+                size_t g = ((size_t)i3 * m[1] + i2) * m[0] + i1;
+                int j1 = i1 - m[0] * ((2 * i1) / m[0]), j2 = i2 - m[1] * ((2 * i2) / m[1]);
+                int j3 = i3 - m[2] * ((2 * i3) / m[2]);
+                double kv[3];
+                for (int c = 0; c < 3; ++c) kv[c] = rec[c] * j1 + rec[3 + c] * j2 + rec[6 + c] * j3;
+                memcpy(kvecs + 3 * g, kv, sizeof kv);
+                double kn = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+                for (int a = 0; a < nid; ++a)
+                    for (int b = 0; b <= a; ++b) {
+                        double ph = 0.0, dph = 0.0;
+                        for (int t = 0; t < nterm[a * nid + b]; ++t) {
+                            double pv, dpv;
+                            fourier_transform(terms + 2 * (a * nid + b) + t, kn, &pv, &dpv);
+                            ph += pv; dph += dpv;
+                        }
+                        int pr = a * (a + 1) / 2 + b;
+                        phik[(size_t)pr * ng + g] = ph;
+                        dphik[(size_t)pr * ng + g] = dph;
+                    }
+            }
+    double dudr[3][3]; /* du_d / dx_c = m_d rec_d[c] / 2 pi */
+    for (int d = 0; d < 3; ++d) for (int c = 0; c < 3; ++c) dudr[d][c] = m[d] * rec[3 * d + c] / (2.0 * M_PI);
+    double *esum = (double *)calloc(nat, sizeof(double)), *dsum = (double *)calloc(nat, sizeof(double));
+    double *gsum = (double *)calloc(3 * (size_t)nat, sizeof(double)), ssum[9] = {0};
+    for (int i = 0; i < nat; ++i) { /* spme.f90:157-171 self-interaction */
+        int a = id[i] - 1;
+        double pval = 0.0;
+        for (int t = 0; t < nterm[a * nid + a]; ++t) pval += potential_zero(terms + 2 * (a * nid + a) + t);
+        for (int l = 0; l < R; ++l) {
+            esum[i] += 0.5 * lr->lambda[l] * c6l[(size_t)i * R + l] * c6l[(size_t)i * R + l] * pval;
+            if (grad) dsum[i] += lr->lambda[l] * c6l[(size_t)i * R + l] * dc6l[(size_t)i * R + l] * pval;
+        }
+    }
+    double *qr = (double *)malloc(sizeof(double) * nid * ng), *qi = (double *)malloc(sizeof(double) * nid * ng);
+    double *tr = (double *)malloc(sizeof(double) * ng), *ti = (double *)malloc(sizeof(double) * ng);
+    double erecip = 0.0, escale = -0.5 / vol;
+    for (int l = 0; l < R; ++l) {
+        memset(qr, 0, sizeof(double) * nid * ng);
+        memset(qi, 0, sizeof(double) * nid * ng);
+        for (int i = 0; i < nat; ++i) {
+            if (ghost && ghost[i]) continue;
+            int a = id[i] - 1;
+            for (int j3 = 1; j3 <= SPME_ORDER; ++j3) {
+                int g3 = ((base[3 * i + 2] + j3) % m[2] + m[2]) % m[2];
+                for (int j2 = 1; j2 <= SPME_ORDER; ++j2) {
+                    int g2 = ((base[3 * i + 1] + j2) % m[1] + m[1]) % m[1];
+                    for (int j1 = 1; j1 <= SPME_ORDER; ++j1) {
+                        int g1 = ((base[3 * i] + j1) % m[0] + m[0]) % m[0];
+                        double w = th[(3 * i) * SPME_ORDER + j1 - 1] * th[(3 * i + 1) * SPME_ORDER + j2 - 1]
+                                 * th[(3 * i + 2) * SPME_ORDER + j3 - 1];
+                        qr[(size_t)a * ng + ((size_t)g3 * m[1] + g2) * m[0] + g1] += c6l[(size_t)i * R + l] * w;
+                    }
+                }
+            }
+        }
+        for (int a = 0; a < nid; ++a) fft_3d(qr + (size_t)a * ng, qi + (size_t)a * ng, m, +1);
+        for (int a = 0; a < nid; ++a) {
+            for (size_t g = 0; g < ng; ++g) {
+                int i1 = (int)(g % m[0]), i2 = (int)((g / m[0]) % m[1]), i3 = (int)(g / ((size_t)m[0] * m[1]));
+                double sfac = bfac[i1] * bfac[mmax + i2] * bfac[2 * mmax + i3];
+                double xr = 0.0, xi = 0.0;
+                for (int b = 0; b < nid; ++b) {
+                    int pr = a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a;
+                    double kk = sfac * phik[(size_t)pr * ng + g];
+                    xr += kk * qr[(size_t)b * ng + g];
+                    xi += kk * qi[(size_t)b * ng + g];
+                }
+                tr[g] = xr; ti[g] = xi;
+            }
+            fft_3d(tr, ti, m, -1);
+            for (int i = 0; i < nat; ++i) {
+                if ((ghost && ghost[i]) || id[i] - 1 != a) continue;
+                double w = 0.0, dw[3] = {0, 0, 0};
+                for (int j3 = 1; j3 <= SPME_ORDER; ++j3) {
+                    int g3 = ((base[3 * i + 2] + j3) % m[2] + m[2]) % m[2];
+                    for (int j2 = 1; j2 <= SPME_ORDER; ++j2) {
+                        int g2 = ((base[3 * i + 1] + j2) % m[1] + m[1]) % m[1];
+                        for (int j1 = 1; j1 <= SPME_ORDER; ++j1) {
+                            int g1 = ((base[3 * i] + j1) % m[0] + m[0]) % m[0];
+                            double sf = tr[((size_t)g3 * m[1] + g2) * m[0] + g1];
+                            double t1 = th[(3 * i) * SPME_ORDER + j1 - 1], t2 = th[(3 * i + 1) * SPME_ORDER + j2 - 1];
+                            double t3 = th[(3 * i + 2) * SPME_ORDER + j3 - 1];
+                            w += t1 * t2 * t3 * sf;
+                            if (grad) {
+                                double d1 = dth[(3 * i) * SPME_ORDER + j1 - 1] * t2 * t3;
+                                double d2 = t1 * dth[(3 * i + 1) * SPME_ORDER + j2 - 1] * t3;
+                                double d3 = t1 * t2 * dth[(3 * i + 2) * SPME_ORDER + j3 - 1];
+                                for (int c = 0; c < 3; ++c)
+                                    dw[c] += (dudr[0][c] * d1 + dudr[1][c] * d2 + dudr[2][c] * d3) * sf;
+                            }
+                        }
+                    }
+                }
+                esum[i] += escale * lr->lambda[l] * c6l[(size_t)i * R + l] * w;
+                erecip += escale * lr->lambda[l] * c6l[(size_t)i * R + l] * w;
+                if (grad) {
+                    dsum[i] += 2.0 * escale * lr->lambda[l] * dc6l[(size_t)i * R + l] * w;
+                    for (int c = 0; c < 3; ++c)
+                        gsum[3 * i + c] += 2.0 * escale * lr->lambda[l] * c6l[(size_t)i * R + l] * dw[c];
+                }
+            }
+        }
+        if (grad) { /* spme.f90:284-314 */
+            for (size_t g = 0; g < ng; ++g) {
+                const double *kv = kvecs + 3 * g;
+                double kn = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+                if (kn <= 0.0) continue;
+                int i1 = (int)(g % m[0]), i2 = (int)((g / m[0]) % m[1]), i3 = (int)(g / ((size_t)m[0] * m[1]));
+                double sfac = bfac[i1] * bfac[mmax + i2] * bfac[2 * mmax + i3];
+                double qq = 0.0;
+                for (int a = 0; a < nid; ++a)
+                    for (int b = 0; b < nid; ++b) {
+                        int pr = a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a;
+                        qq += lr->lambda[l] * dphik[(size_t)pr * ng + g] * sfac
+                            * (qr[(size_t)a * ng + g] * qr[(size_t)b * ng + g] + qi[(size_t)a * ng + g] * qi[(size_t)b * ng + g]);
+                    }
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) ssum[c * 3 + r] += 0.5 * qq * kv[r] * kv[c] / (kn * vol);
+            }
+        }
+    }
+    for (int i = 0; i < nat; ++i) energy[i] += esum[i];
+    if (grad) {
+        for (int c = 0; c < 3; ++c) ssum[c * 3 + c] -= erecip;
+        for (int i = 0; i < nat; ++i) dEdcn[i] += dsum[i];
+        for (int k = 0; k < 3 * nat; ++k) gradient[k] += gsum[k];
+        for (int k = 0; k < 9; ++k) sigma[k] += ssum[k];
+    }
+    free(terms); free(nterm); free(base); free(th); free(dth); free(bfac); free(phik); free(dphik); free(kvecs);
+    free(esum); free(dsum); free(gsum); free(qr); free(qi); free(tr); free(ti);
+    return 0;
+}
+
+/* The oracle's transform (for its check against an independent FFT): grid
+ * [m3][m2][m1] complex, sign of the exponent. */
+void d3o_fft3d(double *re, double *im, const int *m, int sign) { fft_3d(re, im, m, sign); }
+void d3o_bspline(double w, double *theta, double *dtheta) { fill_bspline(w, theta, dtheta); }
+
 /*
  * disp.f90:56-185 with a low-rank model (model.f90:185-196, 379-382): all C6
  * from the low-rank expansion; under three-dimensional periodicity the
  * two-body term is the reciprocal lattice sum (disp.f90:104-111, 142-145),
- * otherwise the real-space sum.  ewald = {rank, tolerance, kcut} with the
- * defaults of api.f90:447-451 for non-positive values.  Returns 1 when the
+ * otherwise the real-space sum.  ewald = {rank, tolerance, kcut, mesh} with
+ * the defaults of api.f90:447-451 for non-positive values; mesh < 0 sums the
+ * reciprocal vectors directly, mesh >= 0 on the particle mesh (0: mesh from
+ * the cutoff, damping.f90:365-380).  Returns 1 when the
  * damping function does not support the Ewald summation, 2 when a pair
  * potential has no closed-form transform (damping.f90:345-352).
  */
@@ -2137,7 +2442,10 @@ int d3o_get_dispersion_ewald(int nat, const int *num, const double *xyz, const d
     double *gl = grad ? (double *)calloc(3 * (size_t)nat, sizeof(double)) : NULL;
     double sl[9] = {0};
     int rc = 0;
-    if (recip) {
+    if (recip && ewald[3] >= 0.0) {
+        rc = spme2(&p, nat, num, id, xyz, lattice, &lr, c6l, dc6l, ewald[2], (int)ewald[3], ghost, en, dEdcn, gl, sl,
+                   grad);
+    } else if (recip) {
         rc = ewald2(&p, nat, num, id, xyz, lattice, &lr, c6l, dc6l, ewald[2], en, dEdcn, gl, sl, grad);
     } else {
         ntr = lattice_alloc(periodic, lattice, cut[1], &trans);

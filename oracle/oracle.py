@@ -249,21 +249,23 @@ class EwaldUnsupported(ValueError):
 
 
 def ewald(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, param=None, cutoff=DEFAULT_CUTOFF,
-          grad=True, ghost=None, rank=0, tolerance=0.0, kcut=0.0):
+          grad=True, ghost=None, rank=0, tolerance=0.0, kcut=0.0, mesh=-1):
     """FP64 oracle of get_dispersion with a low-rank model (model.f90:185-196):
     all C6 from the low-rank expansion, the two-body term summed over the
     reciprocal lattice (fourier/ewald.f90) under 3D periodicity and in real
     space otherwise.  rank / tolerance / kcut as set_model_ewald
-    (api.f90:447-451; non-positive values select the defaults).  Returns the
-    dict of dispersion(); raises EwaldUnsupported for damping functions
-    without a Fourier representation."""
+    (api.f90:447-451; non-positive values select the defaults); mesh < 0 (the
+    default here) sums the reciprocal vectors directly (fourier/ewald.f90),
+    mesh >= 0 on the particle mesh (fourier/spme.f90; 0 derives the mesh from
+    the cutoff).  Returns the dict of dispersion(); raises EwaldUnsupported
+    for damping functions without a Fourier representation."""
     L = lib()
     num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
     par = param_vector(kind, param)
     cut = np.array(cutoff, dtype=np.float64)
     n = len(num)
     gh = None if ghost is None else np.ascontiguousarray(ghost, dtype=np.int32)
-    ew = np.array([rank, tolerance, kcut], dtype=np.float64)
+    ew = np.array([rank, tolerance, kcut, mesh], dtype=np.float64)
     energy = np.zeros(1)
     g = np.zeros((n, 3)) if grad else None
     s = np.zeros((3, 3)) if grad else None
@@ -315,6 +317,27 @@ def jacobi(amat):
     vec = np.zeros((n, n), order="F")
     lib().d3o_jacobi(n, a.ctypes.data_as(_dp), _ptr(ev), vec.ctypes.data_as(_dp))
     return ev, vec
+
+
+def fft3d(grid, sign):
+    """The oracle's radix-2 transform of a complex grid [m3][m2][m1] with
+    exponent sign `sign` (unnormalised)."""
+    g = np.ascontiguousarray(grid, dtype=np.complex128)
+    re = np.ascontiguousarray(g.real)
+    im = np.ascontiguousarray(g.imag)
+    m = np.array(g.shape[::-1], dtype=np.int32)
+    lib().d3o_fft3d(_ptr(re), _ptr(im), _ptr(m, _ip), int(sign))
+    return re + 1j * im
+
+
+def bspline(w):
+    """(theta[6], dtheta[6]) of the order-6 cardinal B-spline at offset w."""
+    L = lib()
+    L.d3o_bspline.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+    t = np.zeros(6)
+    d = np.zeros(6)
+    L.d3o_bspline(float(w), _ptr(t), _ptr(d))
+    return t, d
 
 
 def fourier_term(pref, m, alpha, radius, k):

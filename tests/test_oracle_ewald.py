@@ -275,3 +275,81 @@ def test_ewald_unsupported():
         O.ewald(num, xyz, lat, PBC, O.ZERO, dict(PBE_ZERO, alp=13.5), CUTOFF, grad=False)
     # not 3D periodic: the real-space path serves every family
     O.ewald(num, xyz, lat, [True, True, False], O.MZERO, PBE_MZERO, CUTOFF, grad=False)
+
+
+# ------------------------------------------------------------- particle mesh
+def test_fft_against_numpy():
+    """fft.f90: unnormalised radix-2 transform, exp(sign 2 pi i ...)."""
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=(4, 8, 16)) + 1j * rng.normal(size=(4, 8, 16))
+    np.testing.assert_allclose(O.fft3d(g, -1), np.fft.fftn(g), atol=1e-12)
+    np.testing.assert_allclose(O.fft3d(g, +1), np.fft.ifftn(g) * g.size, atol=1e-12)
+
+
+@pytest.mark.parametrize("w", [0.0, 0.13, 0.5, 0.77, 0.999])
+def test_bspline_weights(w):
+    """spme.f90:457-499: the order-6 weights are the cardinal B-spline M6 at
+    w + 5 - j (partition of unity, closed form by the recursion on M_n) and
+    the derivative weights are dM6/dw."""
+    t, d = O.bspline(w)
+
+    def m_n(x, n):
+        if n == 2:
+            return max(0.0, 1.0 - abs(x - 1.0))
+        return (x * m_n(x, n - 1) + (n - x) * m_n(x - 1, n - 1)) / (n - 1)
+    ref = [m_n(w + 5 - j, 6) for j in range(6)]
+    np.testing.assert_allclose(t, ref, atol=1e-15)
+    assert t.sum() == pytest.approx(1.0, abs=1e-15) and abs(d.sum()) <= 1e-15
+    h = 1e-6
+    if 0.0 < w - h and w + h < 1.0:
+        np.testing.assert_allclose(d, (O.bspline(w + h)[0] - O.bspline(w - h)[0]) / (2 * h), atol=1e-8)
+
+
+def test_spme_matches_direct_sum():
+    """test_fourier.f90 spme-energy / spme-gradient: on the 2x2x2 supercell
+    the particle mesh (64 points, and the mesh derived from the cutoff)
+    reproduces the direct reciprocal sum: energy and gradient within 1e-8,
+    virial within 1e-7."""
+    num, xyz, lat = cubic(8.0, 2)
+    ew = O.ewald(num, xyz, lat, PBC, O.RATIONAL, PBE_BJ, CUTOFF, rank=10000, kcut=10.0)
+    for mesh in (64, 0):
+        pm = O.ewald(num, xyz, lat, PBC, O.RATIONAL, PBE_BJ, CUTOFF, rank=10000, kcut=10.0, mesh=mesh)
+        assert abs(pm["energy"] - ew["energy"]) <= 1e-8
+        assert np.abs(pm["gradient"] - ew["gradient"]).max() <= 1e-8
+        assert np.abs(pm["sigma"] - ew["sigma"]).max() <= 1e-7
+
+
+def test_spme_default_mesh_python_case():
+    """test_interface.py:283-296: the default (mesh from the cutoff) equals the
+    direct sum within 1e-8 on the one-cell C/O case, kcut 10."""
+    num, xyz, lat = cubic(8.0, 1)
+    ew = O.ewald(num, xyz, lat, PBC, O.RATIONAL, PBE_BJ, CUTOFF, grad=False, kcut=10.0)["energy"]
+    pm = O.ewald(num, xyz, lat, PBC, O.RATIONAL, PBE_BJ, CUTOFF, grad=False, kcut=10.0, mesh=0)["energy"]
+    assert abs(pm - ew) <= 1e-8
+
+
+@pytest.mark.parametrize("kind,par", [(O.RATIONAL, dict(PBE_BJ, s9=1.0)), (O.ZERO, PBE_ZERO)])
+def test_spme_gradient_and_virial_fd(kind, par):
+    """The mesh energy is a smooth function of the positions and the cell:
+    its analytic gradient and virial against central differences (a
+    strained 10-atom cell, mesh 16)."""
+    num, xyz, lat = cell(7, 10, 11.0)
+    kw = dict(rank=4, kcut=6.0, mesh=16)
+    res = O.ewald(num, xyz, lat, PBC, kind, par, CUTOFF, **kw)
+    e = lambda x, l: O.ewald(num, x, l, PBC, kind, par, CUTOFF, grad=False, **kw)["energy"]  # noqa: E731
+    h = 1e-4
+    g = np.zeros_like(xyz)
+    for i in range(len(num)):
+        for c in range(3):
+            xp = xyz.copy(); xp[i, c] += h
+            xm = xyz.copy(); xm[i, c] -= h
+            g[i, c] = (e(xp, lat) - e(xm, lat)) / (2 * h)
+    assert np.abs(res["gradient"] - g).max() <= 1e-8
+    hs = 1e-5
+    sg = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(3):
+            ep = np.eye(3); ep[b, a] += hs
+            em = np.eye(3); em[b, a] -= hs
+            sg[a, b] = (e(xyz @ ep.T, lat @ ep.T) - e(xyz @ em.T, lat @ em.T)) / (2 * hs)
+    assert np.abs(res["sigma"].T - sg).max() <= 1e-7
