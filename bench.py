@@ -375,11 +375,13 @@ def bench_ewald(args, systems, config, spec, positions, counts):
     p = s["param"]
     param = RationalDampingParam(s6=p["s6"], s8=p["s8"], s9=p["s9"], a1=p["a1"], a2=p["a2"], alp=p["alp"])
     model = DispersionModel(s["numbers"], s["positions"], s["lattice"], np.asarray(s["periodic"]))
-    model.set_ewald_summation()
+    model.set_ewald_summation(mesh=args.ewald_mesh)
+    spme = args.ewald_mesh >= 0
     run = lambda: model.get_dispersion(param, grad=True)  # noqa: E731
     for _ in range(args.warmup):
         out = run()
-    groups = KERNEL_GROUPS + ("lowrank", "ewald_sf", "ewald_f", "ewald_atom")
+    groups = KERNEL_GROUPS + ("lowrank", "ewald_sf", "ewald_f", "ewald_atom", "spme_setup", "spme_spread",
+                              "spme_fft", "spme_conv", "spme_gather")
     library.set_kernel_timing(model._disp, True)
     t_ev = timed_steps(run, args.steps, lambda: None)
     km = {k: library.get_kernel_time(model._disp, k) for k in groups}
@@ -388,6 +390,8 @@ def bench_ewald(args, systems, config, spec, positions, counts):
     km = {k: v[0] / max(v[1], 1) for k, v in km.items() if v[1]}
     rank, lr_err, kcut, nvec = library.get_model_lowrank(model._disp, last_evaluation=True)
     n = config["natoms"]
+    if spme:
+        return report_spme(args, config, km, t, t_ev, rank, lr_err, kcut, nvec, counts)
     units = n * nvec  # (atom, vector) pairs of one pass, one vector of each +-k pair
     ops_sf = 4 * rank + 12         # phase (two complex products), 2 rank FMAs per (atom, vector)
     ops_atom = 8 * rank + 32       # 4 rank FMAs (A, B), phase, energy / dE/dCN / gradient sums
@@ -432,6 +436,24 @@ def bench_ewald(args, systems, config, spec, positions, counts):
     return 0
 
 
+def report_spme(args, config, km, t, t_ev, rank, lr_err, kcut, npoints, counts):
+    """The particle-mesh variant of the --ewald line: per-kernel times, no
+    roofline (the transforms are hipFFT's) and no CPU baseline."""
+    line = dict(
+        metric=METRIC, value=config["cells"] / t, unit="cells/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=t * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+        data="synthetic",
+        config=dict(config, parallelism="single-gpu",
+                    step="E+grad+sigma with the two-body term on a particle mesh (order-6 B-splines)",
+                    lowrank_rank=rank, lowrank_max_error=lr_err, kcut=kcut, mesh_points=int(npoints),
+                    triple_terms=int(counts[2])),
+        ms_per_step_with_kernel_events=t_ev * 1e3, kernel_ms=km, roofline=None,
+        host_path="positions uploaded and results downloaded every step (model API, 48 KB each way)")
+    log(f"spme: rank {rank}, kcut {kcut:.3f}, {npoints} mesh points, {t * 1e3:.2f} ms/step")
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -449,6 +471,9 @@ def main():
                     help="time the energy and analytic Hessian of the workload instead (FP64)")
     ap.add_argument("--ewald", action="store_true",
                     help="time the workload with the two-body term summed in reciprocal space instead (FP64)")
+    ap.add_argument("--ewald-mesh", type=int, default=-1,
+                    help="with --ewald: < 0 sums the reciprocal vectors directly, >= 0 uses a particle mesh "
+                         "(0: derived from the cutoff)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has a single MI355X; "

@@ -320,13 +320,13 @@ void dftd3_set_model_realspace_cutoff_smooth(dftd3_error error, dftd3_model mode
 void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int rank, double tolerance, double kcut,
                            int mesh) {
   // api.f90:425-453: non-positive values select the defaults (rank from the
-  // tolerance 1e-4, cutoff from the damping radii); the expansion of the
-  // model's species is made here, once (model.f90:185-196).  The vectors are
-  // always summed directly: mesh only selects the reference's interpolation
-  // of the same sum, which is not used.
+  // tolerance 1e-4, cutoff from the damping radii); mesh < 0 sums the
+  // vectors directly, 0 uses a particle mesh derived from the cutoff, > 0 a
+  // mesh of that many points per direction (rounded up to a power of two).
+  // The expansion of the model's species is made here, once
+  // (model.f90:185-196).
   if (!error) return;
   if (!model) return fatal(error, "D3 dispersion model is missing");
-  (void)mesh;
   guarded(error, [&] {
     std::vector<int> species;
     for (int z : model->num)
@@ -334,6 +334,7 @@ void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int rank, doubl
     auto lr = std::make_unique<d3::LowRank>(d3::make_lowrank(species, rank > 0 ? rank : 0,
                                                              tolerance > 0.0 ? tolerance : 1.0e-4));
     lr->kcut = kcut > 0.0 ? kcut : 0.0;
+    lr->mesh = mesh;
     model->lowrank = std::move(lr);
   });
 }

@@ -4,9 +4,12 @@
 //
 // Order of one evaluation (disp.f90:56-185):
 //   pack -> cn -> weights -> pair (C6 and derivatives) -> disp2 ->
-//   [reciprocal space: lowrank -> ewald_sf -> ewald_f -> ewald_atom] ->
+//   [reciprocal space: lowrank -> ewald_sf -> ewald_f -> ewald_atom, or on a
+//    particle mesh: lowrank -> spme_setup -> per term (spread -> FFT -> conv ->
+//    FFT -> gather)] ->
 //   atm_list (count, scan, fill) -> atm -> cn_chain -> reduce
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -155,6 +158,9 @@ struct Prepared {
   long long item_bound = 0;    // work items of those bounds
   bool ewald = false;          // the (single) system sums its two-body term in reciprocal space
   EwaldBatch eb{};             // its vectors, expansion and species chunks (nk = 0: no two-body term)
+  bool spme = false;           // ... on a particle mesh instead of the vectors
+  SpmeBatch sb{};
+  int spme_nsp = 0;
 };
 
 namespace {
@@ -224,7 +230,32 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
       item_off, slots, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
-      ew_phi, ew_f;
+      ew_phi, ew_f, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
+  // batched real <-> half-complex 3D transforms of the particle mesh (one per
+  // species), kept while the mesh and the species count stay the same
+  hipfftHandle fft_fwd = 0, fft_inv = 0;
+  int fft_dims[4] = {0, 0, 0, 0};
+  ~Workspace() {
+    if (fft_fwd) (void)hipfftDestroy(fft_fwd);
+    if (fft_inv) (void)hipfftDestroy(fft_inv);
+  }
+  static void fft_check(hipfftResult r, const char* what) {
+    if (r != HIPFFT_SUCCESS) throw Error(std::string("hipFFT error in ") + what + ": " + std::to_string(int(r)));
+  }
+  void fft_plans(const SpmeBatch& q, int nsp, hipStream_t stream) {
+    const int want[4] = {q.m0, q.m1, q.m2, nsp};
+    if (!fft_fwd || std::memcmp(want, fft_dims, sizeof want) != 0) {
+      if (fft_fwd) fft_check(hipfftDestroy(fft_fwd), "hipfftDestroy");
+      if (fft_inv) fft_check(hipfftDestroy(fft_inv), "hipfftDestroy");
+      fft_fwd = fft_inv = 0;
+      int n[3] = {q.m2, q.m1, q.m0};  // row-major, g1 fastest
+      fft_check(hipfftPlanMany(&fft_fwd, 3, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_D2Z, nsp), "hipfftPlanMany");
+      fft_check(hipfftPlanMany(&fft_inv, 3, n, nullptr, 1, 0, nullptr, 1, 0, HIPFFT_Z2D, nsp), "hipfftPlanMany");
+      std::memcpy(fft_dims, want, sizeof want);
+    }
+    fft_check(hipfftSetStream(fft_fwd, stream), "hipfftSetStream");
+    fft_check(hipfftSetStream(fft_inv, stream), "hipfftSetStream");
+  }
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
@@ -387,6 +418,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   }
   // Reciprocal-space set-up (damping.f90:306-385, ewald.f90:113-121).
   ReciprocalTiles rt;
+  SpmeMesh sm;
   std::vector<double> lam, vecb, pval;
   std::vector<FourierTerm> terms;
   std::vector<int> nterm, sorder, spos, chunk_seg;
@@ -410,7 +442,12 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
       }
     if (kcut > 0.0) {  // else every scaling factor vanishes: no two-body term
       if (lr->kcut > 0.0) kcut = lr->kcut;
-      rt = reciprocal_tiles(sd.lattice, kcut);
+      if (lr->mesh >= 0) {
+        pb.spme = true;
+        sm = spme_mesh(sd.lattice, kcut, lr->mesh);
+      } else {
+        rt = reciprocal_tiles(sd.lattice, kcut);
+      }
       const int R = lr->rank;
       lam = lr->lambda;
       vecb.assign(size_t(nsp) * kMaxRef * R, 0.0);
@@ -445,7 +482,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
       for (int q = 0; q < natoms; ++q) spos[sorder[q]] = q;
     }
     lr->last_kcut = kcut;
-    lr->last_vectors = rt.vectors;
+    lr->last_vectors = pb.spme ? static_cast<long long>(sm.m[0]) * sm.m[1] * sm.m[2] : rt.vectors;
   }
 
   std::vector<char>& sb = staging;
@@ -460,7 +497,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   const size_t o_tiles = stage(sb, rt.tiles), o_lam = stage(sb, lam), o_vec = stage(sb, vecb),
                o_pval = stage(sb, pval), o_terms = stage(sb, terms), o_nterm = stage(sb, nterm),
                o_sorder = stage(sb, sorder), o_chunks = stage(sb, chunks), o_cseg = stage(sb, chunk_seg),
-               o_groups = stage(sb, groups), o_spos = stage(sb, spos);
+               o_groups = stage(sb, groups), o_spos = stage(sb, spos), o_bfac = stage(sb, sm.bfac);
   char* dmeta = static_cast<char*>(meta.get(sb.size()));
   char* host = pinned.get(sb.size());
   std::memcpy(host, sb.data(), sb.size());
@@ -524,6 +561,26 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     e.chunk_seg = reinterpret_cast<const int*>(dmeta + o_cseg);
     e.groups = reinterpret_cast<const int4*>(dmeta + o_groups);
     e.spos = reinterpret_cast<const int*>(dmeta + o_spos);
+    if (pb.spme) {
+      SpmeBatch& q = pb.sb;
+      q.m0 = sm.m[0];
+      q.m1 = sm.m[1];
+      q.m2 = sm.m[2];
+      q.mh = sm.m[0] / 2 + 1;
+      q.mmax = sm.mmax;
+      q.ng = static_cast<long long>(q.m0) * q.m1 * q.m2;
+      q.nch = static_cast<long long>(q.mh) * q.m1 * q.m2;
+      std::memcpy(q.rec, sm.rec, sizeof q.rec);
+      const int md[3] = {q.m0, q.m1, q.m2};
+      for (int k = 0; k < 9; ++k) {
+        q.inv[k] = sm.rec[k] / (2.0 * M_PI);
+        q.dudr[k] = md[k / 3] * sm.rec[k] / (2.0 * M_PI);
+      }
+      q.inv_vol = 1.0 / sm.volume;
+      q.bfac = reinterpret_cast<const double*>(dmeta + o_bfac);
+      pb.spme_nsp = nsp;
+      e.ntiles = 0;
+    }
   }
   return pb;
 }
@@ -569,7 +626,41 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
   timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
   timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
   timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, nullptr, stream); });
-  if (pb.ewald && pb.eb.ntiles > 0) {
+  if (pb.ewald && pb.spme) {
+    // fourier/spme.f90:78-328, term by term of the expansion
+    const EwaldBatch& e = pb.eb;
+    const SpmeBatch& q = pb.sb;
+    const size_t R = static_cast<size_t>(e.rank), npair = size_t(e.nsp) * (e.nsp + 1) / 2;
+    EwaldWork ew{};
+    ew.c6l = ew_c6l.as<double>(size_t(natoms) * e.rank_pad);
+    ew.lc6l = ew_lc6l.as<double>((size_t(natoms) + kEwaldGroup) * R);
+    ew.ldc6l = grad ? ew_ldc6l.as<double>((size_t(natoms) + kEwaldGroup) * R) : nullptr;
+    SpmeWork sw{};
+    sw.base = sp_base.as<int>(size_t(natoms) * 3);
+    sw.theta = sp_theta.as<double>(size_t(natoms) * 3 * kSplineOrder);
+    sw.dtheta = sp_dtheta.as<double>(size_t(natoms) * 3 * kSplineOrder);
+    sw.Q = sp_q.as<double>(size_t(e.nsp) * q.ng);
+    sw.T = sw.Q;  // the spread mesh is dead once transformed
+    sw.Qh = sp_qh.as<double2>(size_t(e.nsp) * q.nch);
+    sw.Th = sp_th.as<double2>(size_t(e.nsp) * q.nch);
+    sw.K = sp_k.as<double>(npair * q.nch);
+    sw.Dm = grad ? sp_dm.as<double>(npair * q.nch) : nullptr;
+    sw.Dmir = grad ? sp_dmir.as<double>(npair * q.nch) : nullptr;
+    fft_plans(q, e.nsp, stream);
+    timed("lowrank", stream, [&] { launch_lowrank(db, e, grad, w, ew, stream); });
+    timed("spme_setup", stream, [&] { launch_spme_setup(db, e, q, grad, sw, stream); });
+    for (int l = 0; l < e.rank; ++l) {
+      timed("spme_spread", stream, [&] {
+        hip_check(hipMemsetAsync(sw.Q, 0, sizeof(double) * size_t(e.nsp) * q.ng, stream), "hipMemsetAsync");
+        launch_spme_spread(db, e, q, ew, sw, l, stream);
+      });
+      timed("spme_fft", stream, [&] { fft_check(hipfftExecD2Z(fft_fwd, sw.Q, reinterpret_cast<hipfftDoubleComplex*>(sw.Qh)), "hipfftExecD2Z"); });
+      timed("spme_conv", stream, [&] { launch_spme_conv(e, q, grad, w, sw, l, stream); });
+      timed("spme_fft", stream, [&] { fft_check(hipfftExecZ2D(fft_inv, reinterpret_cast<hipfftDoubleComplex*>(sw.Th), sw.T), "hipfftExecZ2D"); });
+      timed("spme_gather", stream, [&] { launch_spme_gather(db, e, q, grad, w, ew, sw, l, stream); });
+    }
+  }
+  if (pb.ewald && !pb.spme && pb.eb.ntiles > 0) {
     const EwaldBatch& e = pb.eb;
     EwaldWork ew{};
     const size_t nk = static_cast<size_t>(e.ntiles) * kWave, R = static_cast<size_t>(e.rank);

@@ -28,6 +28,7 @@ struct LowRank {
   int ndim = 0;                 // size of the reference C6 matrix
   double error = 0.0;           // largest relative error of the kept expansion
   double kcut = 0.0;            // reciprocal cutoff, 0: from the damping radii
+  int mesh = 0;                 // < 0: direct sum over the vectors; 0: particle mesh from the cutoff; > 0: mesh points
   std::vector<int> species;     // atomic numbers in first-appearance order
   std::vector<double> lambda;   // [rank]
   std::vector<double> vec;      // [species][kMaxRef][rank]
@@ -61,6 +62,17 @@ struct ReciprocalTiles {
   std::vector<int4> tiles;
 };
 ReciprocalTiles reciprocal_tiles(const double lattice[9], double kcut);
+// Particle mesh of a cell: points per lattice direction, reciprocal basis,
+// volume and Euler factors [3][mmax].
+struct SpmeMesh {
+  int m[3] = {1, 1, 1};
+  int mmax = 1;
+  double rec[9] = {0};
+  double volume = 0.0;
+  std::vector<double> bfac;
+};
+SpmeMesh spme_mesh(const double lattice[9], double kcut, int mesh_req);
+void bspline6_host(double w, double* theta, double* dtheta);
 
 // One system of a batch, as the host describes it.
 struct SystemDesc {

@@ -2382,6 +2382,260 @@ __global__ __launch_bounds__(kBlock) void ewald_atom_kernel(EwaldBatch e, const 
   }
 }
 
+// ------------------------------------------------------- particle mesh
+// fourier/spme.f90:78-328: the low-rank factors spread onto one mesh per
+// species with order-6 cardinal B-splines, transformed (real to half-complex,
+// hipFFT), contracted with the tabulated kernel, transformed back and
+// gathered onto the atoms, term by term of the expansion.  Mesh points are
+// stored [g3][g2][g1] (g1 fastest), half-complex ones [g3][g2][h1], h1 <= m0/2.
+
+__device__ __forceinline__ int mesh_wrap(int i, int m) { return i - m * ((2 * i) / m); }  // spme.f90:393-395
+
+// spme.f90:457-499 fill_bspline (order 6) at offset w in [0, 1).
+__device__ void bspline6(double w, double* th, double* dth) {
+  for (int k = 0; k < kSplineOrder; ++k) th[k] = 0.0;
+  th[0] = 1.0 - w;
+  th[1] = w;
+  for (int korder = 3; korder <= kSplineOrder - 1; ++korder) {
+    const double div = 1.0 / (korder - 1);
+    th[korder - 1] = div * w * th[korder - 2];
+    for (int j = 1; j <= korder - 2; ++j)
+      th[korder - j - 1] = div * ((w + j) * th[korder - j - 2] + (korder - j - w) * th[korder - j - 1]);
+    th[0] = div * (1.0 - w) * th[0];
+  }
+  dth[0] = -th[0];
+  for (int j = 2; j <= kSplineOrder; ++j) dth[j - 1] = th[j - 2] - th[j - 1];
+  const double div = 1.0 / (kSplineOrder - 1);
+  th[kSplineOrder - 1] = div * w * th[kSplineOrder - 2];
+  for (int j = 1; j <= kSplineOrder - 2; ++j)
+    th[kSplineOrder - j - 1] =
+        div * ((w + j) * th[kSplineOrder - j - 2] + (kSplineOrder - j - w) * th[kSplineOrder - j - 1]);
+  th[0] = div * (1.0 - w) * th[0];
+}
+
+// spme.f90:417-453 get_spline_weights: per atom of the system and direction
+// the first mesh point (base + 1) and the weights and their derivatives.
+__global__ __launch_bounds__(kBlock) void spme_spline_kernel(DevBatch b, EwaldBatch e, SpmeBatch s,
+                                                             int* __restrict__ base, double* __restrict__ theta,
+                                                             double* __restrict__ dtheta) {
+  const int il = blockIdx.x * blockDim.x + threadIdx.x;
+  if (il >= e.natoms) return;
+  const double4 x = b.pos[e.atom_begin + il];
+  const int md[3] = {s.m0, s.m1, s.m2};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double f = s.inv[3 * d] * x.x + s.inv[3 * d + 1] * x.y + s.inv[3 * d + 2] * x.z;
+    const double u = md[d] * (f - floor(f));
+    base[3 * il + d] = static_cast<int>(floor(u)) - kSplineOrder;
+    bspline6(u - floor(u), theta + (3 * il + d) * kSplineOrder, dtheta + (3 * il + d) * kSplineOrder);
+  }
+}
+
+// spme.f90:346-413 get_mesh_kernel with spme.f90:503-529 (Euler factors),
+// one thread per half-complex point m: the kernel of every species pair,
+// K = sfac (phihat(k_m) + phihat(k_-m)) / 2 (the real part of the
+// reference's back transform is the transform of the kernel symmetrised over
+// +-m; k_-m differs from -k_m on the Nyquist planes); with the gradient, the
+// virial weights sfac dphihat at m and, when -m is not stored, at -m.
+__global__ __launch_bounds__(kBlock) void spme_table_kernel(EwaldBatch e, SpmeBatch s, double* __restrict__ K,
+                                                            double* __restrict__ Dm, double* __restrict__ Dmir) {
+  const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= s.nch) return;
+  const int h1 = static_cast<int>(p % s.mh), g2 = static_cast<int>((p / s.mh) % s.m1);
+  const int g3 = static_cast<int>(p / (static_cast<long long>(s.mh) * s.m1));
+  const int r1 = (s.m0 - h1) % s.m0, r2 = (s.m1 - g2) % s.m1, r3 = (s.m2 - g3) % s.m2;
+  const bool stored_mirror = r1 == h1 || (2 * h1 == s.m0);  // -m is a stored point (or m itself)
+  double km[3], kr[3];
+  const int j1 = mesh_wrap(h1, s.m0), j2 = mesh_wrap(g2, s.m1), j3 = mesh_wrap(g3, s.m2);
+  const int q1 = mesh_wrap(r1, s.m0), q2 = mesh_wrap(r2, s.m1), q3 = mesh_wrap(r3, s.m2);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    km[c] = s.rec[c] * j1 + s.rec[3 + c] * j2 + s.rec[6 + c] * j3;
+    kr[c] = s.rec[c] * q1 + s.rec[3 + c] * q2 + s.rec[6 + c] * q3;
+  }
+  const double nm = sqrt(km[0] * km[0] + km[1] * km[1] + km[2] * km[2]);
+  const double nr = sqrt(kr[0] * kr[0] + kr[1] * kr[1] + kr[2] * kr[2]);
+  const double sfac = s.bfac[h1] * s.bfac[s.mmax + g2] * s.bfac[2 * s.mmax + g3];
+  const int nsp = e.nsp;
+  for (int a = 0; a < nsp; ++a)
+    for (int c = 0; c <= a; ++c) {
+      double pm = 0.0, dpm = 0.0, pr = 0.0, dpr = 0.0;
+      for (int t = 0; t < e.nterm[a * nsp + c]; ++t) {
+        double pv, dpv;
+        fourier_transform(e.terms[2 * (a * nsp + c) + t], nm, pv, dpv);
+        pm += pv;
+        dpm += dpv;
+        if (nr != nm) {
+          fourier_transform(e.terms[2 * (a * nsp + c) + t], nr, pv, dpv);
+          pr += pv;
+          dpr += dpv;
+        }
+      }
+      if (nr == nm) {
+        pr = pm;
+        dpr = dpm;
+      }
+      const long long o = static_cast<long long>(a * (a + 1) / 2 + c) * s.nch + p;
+      K[o] = sfac * 0.5 * (pm + pr);
+      if (Dm) {
+        Dm[o] = nm > 0.0 ? sfac * dpm : 0.0;
+        Dmir[o] = (!stored_mirror && nr > 0.0) ? sfac * dpr : 0.0;
+      }
+    }
+}
+
+// Spreading of term l (spme.f90:201-218): one block per atom, one thread per
+// each of its 6^3 mesh points.
+__global__ __launch_bounds__(kBlock) void spme_spread_kernel(DevBatch b, EwaldBatch e, SpmeBatch s,
+                                                             const double* __restrict__ c6l,
+                                                             const int* __restrict__ base,
+                                                             const double* __restrict__ theta, int l,
+                                                             double* __restrict__ Q) {
+  const int il = blockIdx.x;
+  const int j = threadIdx.x;
+  if (j >= kSplineOrder * kSplineOrder * kSplineOrder) return;
+  const int j1 = j % kSplineOrder, j2 = (j / kSplineOrder) % kSplineOrder, j3 = j / (kSplineOrder * kSplineOrder);
+  const int g1 = ((base[3 * il] + j1 + 1) % s.m0 + s.m0) % s.m0;
+  const int g2 = ((base[3 * il + 1] + j2 + 1) % s.m1 + s.m1) % s.m1;
+  const int g3 = ((base[3 * il + 2] + j3 + 1) % s.m2 + s.m2) % s.m2;
+  const double* th = theta + 3 * il * kSplineOrder;
+  const double w = th[j1] * th[kSplineOrder + j2] * th[2 * kSplineOrder + j3];
+  const double f = c6l[static_cast<long long>(e.spos[il]) * e.rank_pad + l];
+  const int a = b.sp[e.atom_begin + il];
+  atomicAdd(&Q[static_cast<long long>(a) * s.ng + (static_cast<long long>(g3) * s.m1 + g2) * s.m0 + g1], f * w);
+}
+
+// Contraction with the kernel (spme.f90:225-242) per half-complex point and
+// the virial of term l (spme.f90:284-314): qq at m (and at -m where it is not
+// stored) times k k^T / (2 |k| V), summed over the wavefront and added to the
+// per-atom virial of the system's first atom.
+__global__ __launch_bounds__(kBlock) void spme_conv_kernel(EwaldBatch e, SpmeBatch s, const double2* __restrict__ Qh,
+                                                           const double* __restrict__ K, const double* __restrict__ Dm,
+                                                           const double* __restrict__ Dmir, int l,
+                                                           double2* __restrict__ Th, double* __restrict__ sig_atom) {
+  const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int nsp = e.nsp;
+  double sg[6] = {0, 0, 0, 0, 0, 0};
+  if (p < s.nch) {
+    double qr[kMaxEwaldSpecies], qi[kMaxEwaldSpecies];
+#pragma unroll
+    for (int a = 0; a < kMaxEwaldSpecies; ++a) {
+      const double2 v = a < nsp ? Qh[static_cast<long long>(a) * s.nch + p] : make_double2(0.0, 0.0);
+      qr[a] = v.x;
+      qi[a] = v.y;
+    }
+    double qm = 0.0, qx = 0.0;
+#pragma unroll
+    for (int a = 0; a < kMaxEwaldSpecies; ++a) {
+      if (a >= nsp) continue;
+      double tr = 0.0, ti = 0.0;
+#pragma unroll
+      for (int c = 0; c < kMaxEwaldSpecies; ++c) {
+        if (c >= nsp) continue;
+        const long long o = static_cast<long long>(a >= c ? a * (a + 1) / 2 + c : c * (c + 1) / 2 + a) * s.nch + p;
+        const double kk = K[o];
+        tr += kk * qr[c];
+        ti += kk * qi[c];
+        if (Dm) {
+          const double re = qr[a] * qr[c] + qi[a] * qi[c];
+          qm += Dm[o] * re;
+          qx += Dmir[o] * re;
+        }
+      }
+      Th[static_cast<long long>(a) * s.nch + p] = make_double2(tr, ti);
+    }
+    if (Dm) {
+      const int h1 = static_cast<int>(p % s.mh), g2 = static_cast<int>((p / s.mh) % s.m1);
+      const int g3 = static_cast<int>(p / (static_cast<long long>(s.mh) * s.m1));
+      const int r1 = (s.m0 - h1) % s.m0, r2 = (s.m1 - g2) % s.m1, r3 = (s.m2 - g3) % s.m2;
+      const int jj[2][3] = {{mesh_wrap(h1, s.m0), mesh_wrap(g2, s.m1), mesh_wrap(g3, s.m2)},
+                            {mesh_wrap(r1, s.m0), mesh_wrap(r2, s.m1), mesh_wrap(r3, s.m2)}};
+      const double qq[2] = {e.lambda[l] * qm, e.lambda[l] * qx};
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        double k[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) k[c] = s.rec[c] * jj[q][0] + s.rec[3 + c] * jj[q][1] + s.rec[6 + c] * jj[q][2];
+        const double kn = sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+        if (kn <= 0.0) continue;
+        const double h = 0.5 * qq[q] * s.inv_vol / kn;
+        sg[0] += h * k[0] * k[0];
+        sg[1] += h * k[1] * k[1];
+        sg[2] += h * k[2] * k[2];
+        sg[3] += h * k[0] * k[1];
+        sg[4] += h * k[0] * k[2];
+        sg[5] += h * k[1] * k[2];
+      }
+    }
+  }
+  if (Dm) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sg[c] = wave_sum(sg[c]);
+    if (wave_lane() == 0)
+      for (int c = 0; c < 6; ++c) atomicAdd(&sig_atom[6 * e.atom_begin + c], sg[c]);
+  }
+}
+
+// Gathering of term l (spme.f90:246-282): one wavefront per atom, lanes over
+// its 6^3 mesh points of its species' back-transformed mesh; energy, dE/dCN,
+// gradient, and the energy on the virial diagonal (spme.f90:319-322).
+template <bool kGrad>
+__global__ __launch_bounds__(kBlock) void spme_gather_kernel(DevBatch b, EwaldBatch e, SpmeBatch s,
+                                                             const double* __restrict__ T,
+                                                             const int* __restrict__ base,
+                                                             const double* __restrict__ theta,
+                                                             const double* __restrict__ dtheta,
+                                                             const double* __restrict__ lc6l,
+                                                             const double* __restrict__ ldc6l, int l,
+                                                             double* __restrict__ e_atom, double* __restrict__ dedcn,
+                                                             double* __restrict__ grad, double* __restrict__ sig_atom) {
+  const int il = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (il >= e.natoms) return;
+  const int i = e.atom_begin + il;
+  const int a = b.sp[i];
+  const double* Ta = T + static_cast<long long>(a) * s.ng;
+  const double* th = theta + 3 * il * kSplineOrder;
+  const double* dth = dtheta + 3 * il * kSplineOrder;
+  double w = 0.0, d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int j = wave_lane(); j < kSplineOrder * kSplineOrder * kSplineOrder; j += kWave) {
+    const int j1 = j % kSplineOrder, j2 = (j / kSplineOrder) % kSplineOrder, j3 = j / (kSplineOrder * kSplineOrder);
+    const int g1 = ((base[3 * il] + j1 + 1) % s.m0 + s.m0) % s.m0;
+    const int g2 = ((base[3 * il + 1] + j2 + 1) % s.m1 + s.m1) % s.m1;
+    const int g3 = ((base[3 * il + 2] + j3 + 1) % s.m2 + s.m2) % s.m2;
+    const double sf = Ta[(static_cast<long long>(g3) * s.m1 + g2) * s.m0 + g1];
+    const double t1 = th[j1], t2 = th[kSplineOrder + j2], t3 = th[2 * kSplineOrder + j3];
+    w += t1 * t2 * t3 * sf;
+    if (kGrad) {
+      const double u1 = dth[j1] * t2 * t3, u2 = t1 * dth[kSplineOrder + j2] * t3;
+      const double u3 = t1 * t2 * dth[2 * kSplineOrder + j3];
+      d0 += (s.dudr[0] * u1 + s.dudr[3] * u2 + s.dudr[6] * u3) * sf;
+      d1 += (s.dudr[1] * u1 + s.dudr[4] * u2 + s.dudr[7] * u3) * sf;
+      d2 += (s.dudr[2] * u1 + s.dudr[5] * u2 + s.dudr[8] * u3) * sf;
+    }
+  }
+  w = wave_sum(w);
+  if (kGrad) {
+    d0 = wave_sum(d0);
+    d1 = wave_sum(d1);
+    d2 = wave_sum(d2);
+  }
+  if (wave_lane() == 0) {
+    const long long ld = e.natoms + kEwaldGroup;
+    const int row = e.spos[il];
+    const double escale = -0.5 * s.inv_vol;
+    const double lc = lc6l[l * ld + row];
+    const double en = escale * lc * w;
+    e_atom[i] += en;
+    if (kGrad) {
+      dedcn[i] += 2.0 * escale * ldc6l[l * ld + row] * w;
+      grad[3 * i] += 2.0 * escale * lc * d0;
+      grad[3 * i + 1] += 2.0 * escale * lc * d1;
+      grad[3 * i + 2] += 2.0 * escale * lc * d2;
+      for (int c = 0; c < 3; ++c) atomicAdd(&sig_atom[6 * e.atom_begin + c], -en);
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -2586,10 +2840,42 @@ void launch_pair3_finalize(const DevBatch& b, const double* pair3s, double* out,
 
 void launch_lowrank(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
                     hipStream_t st) {
-  hipLaunchKernelGGL(ewald_table_kernel, dim3(blocks_for(static_cast<long long>(e.natoms) * e.tstride, kBlock)),
-                     dim3(kBlock), 0, st, b, e, ew.etab);
+  if (ew.etab)
+    hipLaunchKernelGGL(ewald_table_kernel, dim3(blocks_for(static_cast<long long>(e.natoms) * e.tstride, kBlock)),
+                       dim3(kBlock), 0, st, b, e, ew.etab);
   hipLaunchKernelGGL(lowrank_kernel, dim3(blocks_for(e.natoms, kBlock)), dim3(kBlock), 0, st, b, e, w.gw,
                      grad ? w.dgw : nullptr, ew.c6l, ew.lc6l, ew.ldc6l, w.e_atom, w.dedcn);
+}
+
+void launch_spme_setup(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, const SpmeWork& sw,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(spme_spline_kernel, dim3(blocks_for(e.natoms, kBlock)), dim3(kBlock), 0, st, b, e, s, sw.base,
+                     sw.theta, sw.dtheta);
+  hipLaunchKernelGGL(spme_table_kernel, dim3(blocks_for(s.nch, kBlock)), dim3(kBlock), 0, st, e, s, sw.K,
+                     grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr);
+}
+
+void launch_spme_spread(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, const EwaldWork& ew,
+                        const SpmeWork& sw, int l, hipStream_t st) {
+  hipLaunchKernelGGL(spme_spread_kernel, dim3(e.natoms), dim3(kBlock), 0, st, b, e, s, ew.c6l, sw.base, sw.theta, l,
+                     sw.Q);
+}
+
+void launch_spme_conv(const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w, const SpmeWork& sw, int l,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(spme_conv_kernel, dim3(blocks_for(s.nch, kBlock)), dim3(kBlock), 0, st, e, s, sw.Qh, sw.K,
+                     grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr, l, sw.Th, w.sig_atom);
+}
+
+void launch_spme_gather(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w,
+                        const EwaldWork& ew, const SpmeWork& sw, int l, hipStream_t st) {
+  const dim3 grid(blocks_for(e.natoms, kWavesPerBlock));
+  if (grad)
+    hipLaunchKernelGGL(spme_gather_kernel<true>, grid, dim3(kBlock), 0, st, b, e, s, sw.T, sw.base, sw.theta,
+                       sw.dtheta, ew.lc6l, ew.ldc6l, l, w.e_atom, w.dedcn, w.grad, w.sig_atom);
+  else
+    hipLaunchKernelGGL(spme_gather_kernel<false>, grid, dim3(kBlock), 0, st, b, e, s, sw.T, sw.base, sw.theta,
+                       sw.dtheta, ew.lc6l, ew.ldc6l, l, w.e_atom, w.dedcn, w.grad, w.sig_atom);
 }
 
 void launch_ewald_sf(const EwaldBatch& e, const EwaldWork& ew, hipStream_t st) {

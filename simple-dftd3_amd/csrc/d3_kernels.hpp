@@ -152,14 +152,47 @@ struct EwaldWork {
   double2 *F;                // [nsp][rank][nk] sum_b phihat_ab S_bl
 };
 // The reciprocal-space energy (and dE/dCN, gradient, virial), added to the
-// accumulators of w after launch_disp2, in this order: the phase tables and
-// low-rank factors (and the self-interaction), the partial structure
+// accumulators of w after launch_disp2, in this order: the phase tables (if
+// ew.etab) and low-rank factors (and the self-interaction), the partial structure
 // factors, the per-vector transforms F and virial, the per-atom sums.
 void launch_lowrank(const DevBatch& b, const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew,
                     hipStream_t stream);
 void launch_ewald_sf(const EwaldBatch& e, const EwaldWork& ew, hipStream_t stream);
 void launch_ewald_f(const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew, hipStream_t stream);
 void launch_ewald_atom(const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew, hipStream_t stream);
+// Particle-mesh evaluation of the reciprocal sum (fourier/spme.f90): the
+// mesh [m2][m1][m0] (g1 fastest), its half-complex transform [m2][m1][mh]
+// (mh = m0 / 2 + 1), order-6 B-splines; the expansion, species and atom order
+// of the EwaldBatch (no tiles).
+constexpr int kSplineOrder = 6;  // spme.f90:47
+struct SpmeBatch {
+  int m0, m1, m2, mh, mmax;
+  long long ng, nch;         // mesh points, half-complex points
+  double inv[9];             // fractional coordinates f_d = inv[3 d ..] . x (rows b_d / 2 pi)
+  double rec[9];             // b_1, b_2, b_3 (rows)
+  double dudr[9];            // du_d / dx_c = m_d b_d[c] / 2 pi at [3 d + c]
+  double inv_vol;
+  const double* bfac;        // [3][mmax] Euler factors (spme.f90:503-529)
+};
+struct SpmeWork {
+  int* base;                 // [natoms][3]
+  double *theta, *dtheta;    // [natoms][3][kSplineOrder]
+  double* Q;                 // [nsp][ng] spread factors of one term (also the back transform)
+  double2 *Qh, *Th;          // [nsp][nch] transforms, kernel products
+  double* T;                 // [nsp][ng] back-transformed meshes (may alias Q)
+  double *K, *Dm, *Dmir;     // [nsp (nsp + 1) / 2][nch] kernel and virial weights (Dm, Dmir with gradient)
+};
+// Spline weights and kernel table of an evaluation; then per term l: spread
+// into Q (zeroed by the caller), [Q -> Qh transform], contract Qh -> Th
+// (and the term's virial), [Th -> T back transform], gather.
+void launch_spme_setup(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, const SpmeWork& sw,
+                       hipStream_t stream);
+void launch_spme_spread(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, const EwaldWork& ew,
+                        const SpmeWork& sw, int l, hipStream_t stream);
+void launch_spme_conv(const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w, const SpmeWork& sw, int l,
+                      hipStream_t stream);
+void launch_spme_gather(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w,
+                        const EwaldWork& ew, const SpmeWork& sw, int l, hipStream_t stream);
 void launch_cn_chain(const DevBatch& b, bool mixed, const DevWork& w, hipStream_t stream);
 void launch_reduce(const DevBatch& b, const DevWork& w, double* energy, double* sigma, hipStream_t stream);
 void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream_t stream);

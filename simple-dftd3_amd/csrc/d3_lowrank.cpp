@@ -6,6 +6,7 @@
 // and vectors run on the device (d3_kernels.hip).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <string>
 #include <vector>
@@ -250,6 +251,62 @@ ReciprocalTiles reciprocal_tiles(const double lattice[9], double kcut) {
     }
   }
   return out;
+}
+
+SpmeMesh spme_mesh(const double lattice[9], double kcut, int mesh_req) {
+  // spme.f90:55-74 get_spme_mesh / damping.f90:368-375: mesh_req > 0 rounded
+  // up to a power of two (fft.f90:48-60), else enough points to resolve
+  // 2 pi / kcut along each lattice vector; spme.f90:503-529 Euler factors.
+  SpmeMesh out;
+  auto valid = [](long long n) {
+    int m = 1;
+    while (m < n) m *= 2;
+    return m;
+  };
+  ReciprocalTiles rt = reciprocal_tiles(lattice, 0.0);  // basis and volume only
+  out.volume = rt.volume;
+  std::memcpy(out.rec, rt.rec, sizeof out.rec);
+  for (int d = 0; d < 3; ++d) {
+    const double* a = lattice + 3 * d;
+    const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    out.m[d] = mesh_req > 0 ? valid(mesh_req) : valid(static_cast<long long>(std::ceil(kcut * len / M_PI)));
+  }
+  double spl[6] = {0}, dspl[6];
+  bspline6_host(0.0, spl, dspl);
+  out.mmax = std::max(out.m[0], std::max(out.m[1], out.m[2]));
+  out.bfac.assign(3 * size_t(out.mmax), 0.0);
+  for (int d = 0; d < 3; ++d)
+    for (int im = 0; im < out.m[d]; ++im) {
+      double re = 0.0, im_ = 0.0;
+      for (int jm = 0; jm < 5; ++jm) {
+        const double arg = 2.0 * M_PI * im * jm / out.m[d];
+        re += spl[jm] * std::cos(arg);
+        im_ += spl[jm] * std::sin(arg);
+      }
+      out.bfac[size_t(d) * out.mmax + im] = 1.0 / (re * re + im_ * im_);
+    }
+  return out;
+}
+
+void bspline6_host(double w, double* th, double* dth) {
+  // spme.f90:457-499 fill_bspline, order 6
+  const int n = 6;
+  for (int k = 0; k < n; ++k) th[k] = 0.0;
+  th[0] = 1.0 - w;
+  th[1] = w;
+  for (int korder = 3; korder <= n - 1; ++korder) {
+    const double div = 1.0 / (korder - 1);
+    th[korder - 1] = div * w * th[korder - 2];
+    for (int j = 1; j <= korder - 2; ++j)
+      th[korder - j - 1] = div * ((w + j) * th[korder - j - 2] + (korder - j - w) * th[korder - j - 1]);
+    th[0] = div * (1.0 - w) * th[0];
+  }
+  dth[0] = -th[0];
+  for (int j = 2; j <= n; ++j) dth[j - 1] = th[j - 2] - th[j - 1];
+  const double div = 1.0 / (n - 1);
+  th[n - 1] = div * w * th[n - 2];
+  for (int j = 1; j <= n - 2; ++j) th[n - j - 1] = div * ((w + j) * th[n - j - 2] + (n - j - w) * th[n - j - 1]);
+  th[0] = div * (1.0 - w) * th[0];
 }
 
 }  // namespace d3

@@ -1,7 +1,8 @@
 """Reciprocal-space two-body term on the GPU (set_ewald_summation /
-dftd3_set_model_ewald) against the oracle's restatement of
-fourier/ewald.f90 and decomposition.f90, which tests/test_oracle_ewald.py
-pins.
+dftd3_set_model_ewald), the direct sum over the vectors (mesh < 0) and the
+particle mesh (mesh >= 0), against the oracle's restatement of
+fourier/ewald.f90, spme.f90 and decomposition.f90, which
+tests/test_oracle_ewald.py pins (the same mesh on both sides).
 
 Tolerances as for the real-space path: FP64 |dE| <= 1e-10 |E|, max |dg| <=
 1e-9 Eh/bohr, max |dsigma| <= 1e-9 Eh; mixed precision (the reciprocal sum
@@ -75,13 +76,20 @@ def _check(res, ref, n, precision="fp64", grad=True):
 
 
 CASES = [
-    ("cubic-bj-full", lambda: cubic(8.0, 1), O.RATIONAL, PBE_BJ, dict(rank=10000, kcut=10.0)),
-    ("cubic-bj-default", lambda: cubic(8.0, 1), O.RATIONAL, PBE_BJ, dict()),
-    ("cubic-zero", lambda: cubic(8.0, 1), O.ZERO, PBE_ZERO, dict(kcut=12.0)),
-    ("supercell-bj-atm", lambda: cubic(8.0, 2), O.RATIONAL, dict(PBE_BJ, s9=1.0), dict(tolerance=1e-3)),
-    ("triclinic-bj-atm", lambda: cell(7, 10, 11.0), O.RATIONAL, dict(PBE_BJ, s9=1.0), dict(rank=10000)),
-    ("triclinic-zero-atm", lambda: cell(11, 24, 13.0), O.ZERO, dict(PBE_ZERO, s9=1.0), dict(rank=6, kcut=14.0)),
-    ("triclinic-bj-96", lambda: cell(3, 96, 19.0), O.RATIONAL, dict(PBE_BJ, s9=1.0), dict()),
+    ("cubic-bj-full", lambda: cubic(8.0, 1), O.RATIONAL, PBE_BJ, dict(rank=10000, kcut=10.0, mesh=-1)),
+    ("cubic-bj-default", lambda: cubic(8.0, 1), O.RATIONAL, PBE_BJ, dict(mesh=-1)),
+    ("cubic-zero", lambda: cubic(8.0, 1), O.ZERO, PBE_ZERO, dict(kcut=12.0, mesh=-1)),
+    ("supercell-bj-atm", lambda: cubic(8.0, 2), O.RATIONAL, dict(PBE_BJ, s9=1.0), dict(tolerance=1e-3, mesh=-1)),
+    ("triclinic-bj-atm", lambda: cell(7, 10, 11.0), O.RATIONAL, dict(PBE_BJ, s9=1.0), dict(rank=10000, mesh=-1)),
+    ("triclinic-zero-atm", lambda: cell(11, 24, 13.0), O.ZERO, dict(PBE_ZERO, s9=1.0),
+     dict(rank=6, kcut=14.0, mesh=-1)),
+    ("triclinic-bj-96", lambda: cell(3, 96, 19.0), O.RATIONAL, dict(PBE_BJ, s9=1.0), dict(mesh=-1)),
+    # particle mesh
+    ("mesh-cubic-bj-default", lambda: cubic(8.0, 1), O.RATIONAL, PBE_BJ, dict()),
+    ("mesh-supercell-bj-64", lambda: cubic(8.0, 2), O.RATIONAL, PBE_BJ, dict(rank=10000, kcut=10.0, mesh=64)),
+    ("mesh-triclinic-bj-atm-24", lambda: cell(7, 10, 11.0), O.RATIONAL, dict(PBE_BJ, s9=1.0),
+     dict(rank=4, kcut=6.0, mesh=24)),
+    ("mesh-triclinic-zero-atm", lambda: cell(11, 24, 13.0), O.ZERO, dict(PBE_ZERO, s9=1.0), dict(kcut=8.0, mesh=0)),
 ]
 
 
@@ -89,8 +97,8 @@ CASES = [
 def test_ewald_matches_oracle(name, make, kind, par, ew):
     num, xyz, lat = make()
     _step(f"{name}: n={len(num)}")
-    ref = O.ewald(num, xyz, lat, PBC, kind, par, CUTOFF, **ew)
-    m = _model(num, xyz, lat, PBC, mesh=-1, **ew)
+    ref = O.ewald(num, xyz, lat, PBC, kind, par, CUTOFF, **dict(dict(mesh=0), **ew))
+    m = _model(num, xyz, lat, PBC, **ew)
     res = m.get_dispersion(_param(kind, par), grad=True)
     _check(res, ref, len(num))
     e_only = m.get_dispersion(_param(kind, par), grad=False)
@@ -103,9 +111,11 @@ def test_ewald_ghost_atoms():
     ghost[[1, 4, 9]] = 1
     par = dict(PBE_BJ, s9=1.0)
     _step("ghost atoms")
-    ref = O.ewald(num, xyz, lat, PBC, O.RATIONAL, par, CUTOFF, ghost=ghost, rank=10000)
-    res = _model(num, xyz, lat, PBC, ghost=ghost, rank=10000).get_dispersion(_param(O.RATIONAL, par), grad=True)
-    _check(res, ref, len(num))
+    for mesh in (-1, 32):
+        ref = O.ewald(num, xyz, lat, PBC, O.RATIONAL, par, CUTOFF, ghost=ghost, rank=10000, kcut=8.0, mesh=mesh)
+        res = _model(num, xyz, lat, PBC, ghost=ghost, rank=10000, kcut=8.0, mesh=mesh).get_dispersion(
+            _param(O.RATIONAL, par), grad=True)
+        _check(res, ref, len(num))
 
 
 def test_ewald_mixed_precision():
@@ -113,7 +123,7 @@ def test_ewald_mixed_precision():
     par = dict(PBE_BJ, s9=1.0)
     _step("mixed precision")
     ref = O.ewald(num, xyz, lat, PBC, O.RATIONAL, par, CUTOFF)
-    res = _model(num, xyz, lat, PBC, precision="mixed").get_dispersion(_param(O.RATIONAL, par), grad=True)
+    res = _model(num, xyz, lat, PBC, precision="mixed", mesh=-1).get_dispersion(_param(O.RATIONAL, par), grad=True)
     _check(res, ref, len(num), precision="mixed")
 
 
@@ -145,7 +155,11 @@ def test_ewald_update_structure_and_converged_sum():
     disp = DispersionModel(num, xyz, lattice=lat, periodic=np.full(3, True))
     disp.set_ewald_summation(kcut=10.0, mesh=-1)
     ewald = disp.get_dispersion(param, grad=False)["energy"]
+    mesh_model = DispersionModel(num, xyz, lattice=lat, periodic=np.full(3, True))
+    mesh_model.set_ewald_summation(kcut=10.0)
+    mesh = mesh_model.get_dispersion(param, grad=False)["energy"]
     assert ewald == pytest.approx(converged, abs=1e-7)
+    assert mesh == pytest.approx(ewald, abs=1e-8)
     assert truncated != pytest.approx(converged, abs=1e-6)
     moved = xyz + np.array([[0.3, -0.2, 0.1], [0.0, 0.4, -0.3]])
     disp.update(moved)
