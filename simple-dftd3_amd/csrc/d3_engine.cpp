@@ -161,6 +161,7 @@ struct Prepared {
   bool spme = false;           // ... on a particle mesh instead of the vectors
   SpmeBatch sb{};
   int spme_nsp = 0;
+  std::vector<char> spme_key;  // what the kernel table depends on: mesh, basis, pair terms
 };
 
 namespace {
@@ -235,6 +236,8 @@ struct Workspace {
   // species), kept while the mesh and the species count stay the same
   hipfftHandle fft_fwd = 0, fft_inv = 0;
   int fft_dims[4] = {0, 0, 0, 0};
+  std::vector<char> spme_key;  // the kernel table in sp_k / sp_dm / sp_dmir
+  bool spme_key_grad = false;  // ... with its virial weights
   ~Workspace() {
     if (fft_fwd) (void)hipfftDestroy(fft_fwd);
     if (fft_inv) (void)hipfftDestroy(fft_inv);
@@ -580,6 +583,15 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
       q.bfac = reinterpret_cast<const double*>(dmeta + o_bfac);
       pb.spme_nsp = nsp;
       e.ntiles = 0;
+      auto put = [&](const void* d, size_t n) {
+        const char* c = static_cast<const char*>(d);
+        pb.spme_key.insert(pb.spme_key.end(), c, c + n);
+      };
+      put(sm.m, sizeof sm.m);
+      put(sm.rec, sizeof sm.rec);
+      put(&nsp, sizeof nsp);
+      put(terms.data(), terms.size() * sizeof(FourierTerm));
+      put(nterm.data(), nterm.size() * sizeof(int));
     }
   }
   return pb;
@@ -647,8 +659,14 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
     sw.Dm = grad ? sp_dm.as<double>(npair * q.nch) : nullptr;
     sw.Dmir = grad ? sp_dmir.as<double>(npair * q.nch) : nullptr;
     fft_plans(q, e.nsp, stream);
+    // the kernel table is kept while the cell, parameters and mesh stay
+    const bool table = pb.spme_key != spme_key || (grad && !spme_key_grad);
     timed("lowrank", stream, [&] { launch_lowrank(db, e, grad, w, ew, stream); });
-    timed("spme_setup", stream, [&] { launch_spme_setup(db, e, q, grad, sw, stream); });
+    timed("spme_setup", stream, [&] { launch_spme_setup(db, e, q, grad, table, sw, stream); });
+    if (table) {
+      spme_key = pb.spme_key;
+      spme_key_grad = grad;
+    }
     for (int l = 0; l < e.rank; ++l) {
       timed("spme_spread", stream, [&] {
         hip_check(hipMemsetAsync(sw.Q, 0, sizeof(double) * size_t(e.nsp) * q.ng, stream), "hipMemsetAsync");

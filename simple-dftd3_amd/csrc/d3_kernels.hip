@@ -2051,12 +2051,13 @@ __device__ void fourier_transform(const FourierTerm& t, double k, double& phi, d
   double acc = 0.0, dacc = 0.0;
   for (int p = 0; p < t.alpha / 2; ++p) {
     const double th = M_PI * (2 * p + 1) / double(t.alpha);
-    double st, ct;
+    double st, ct, sa, ca;
     sincos(th, &st, &ct);
     const double arg = 0.5 * M_PI + nexp * th + kr * ct;
     const double damp = exp(-kr * st);
-    acc += sin(arg) * damp;
-    dacc += cos(arg + th) * damp;
+    sincos(arg, &sa, &ca);
+    acc += sa * damp;
+    dacc += (ca * ct - sa * st) * damp;  // cos(arg + th)
   }
   phi = pre * acc / k;
   dphi = (pre * t.radius * dacc - phi) / k;
@@ -2507,46 +2508,55 @@ __global__ __launch_bounds__(kBlock) void spme_spread_kernel(DevBatch b, EwaldBa
 
 // Contraction with the kernel (spme.f90:225-242) per half-complex point and
 // the virial of term l (spme.f90:284-314): qq at m (and at -m where it is not
-// stored) times k k^T / (2 |k| V), summed over the wavefront and added to the
-// per-atom virial of the system's first atom.
+// stored) times k k^T / (2 |k| V).  Grid-stride over the points; the virial
+// is summed over the block and added to the per-atom virial of the system's
+// first atom once per block.
 __global__ __launch_bounds__(kBlock) void spme_conv_kernel(EwaldBatch e, SpmeBatch s, const double2* __restrict__ Qh,
                                                            const double* __restrict__ K, const double* __restrict__ Dm,
                                                            const double* __restrict__ Dmir, int l,
                                                            double2* __restrict__ Th, double* __restrict__ sig_atom) {
-  const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int nsp = e.nsp;
   double sg[6] = {0, 0, 0, 0, 0, 0};
-  if (p < s.nch) {
-    double qr[kMaxEwaldSpecies], qi[kMaxEwaldSpecies];
+  for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < s.nch;
+       p += static_cast<long long>(gridDim.x) * blockDim.x) {
+    double qr[kMaxEwaldSpecies], qi[kMaxEwaldSpecies], tr[kMaxEwaldSpecies], ti[kMaxEwaldSpecies];
 #pragma unroll
     for (int a = 0; a < kMaxEwaldSpecies; ++a) {
       const double2 v = a < nsp ? Qh[static_cast<long long>(a) * s.nch + p] : make_double2(0.0, 0.0);
       qr[a] = v.x;
       qi[a] = v.y;
+      tr[a] = 0.0;
+      ti[a] = 0.0;
     }
     double qm = 0.0, qx = 0.0;
+    // each pair once: T_a += K_ac Q_c, T_c += K_ac Q_a
 #pragma unroll
     for (int a = 0; a < kMaxEwaldSpecies; ++a) {
-      if (a >= nsp) continue;
-      double tr = 0.0, ti = 0.0;
 #pragma unroll
-      for (int c = 0; c < kMaxEwaldSpecies; ++c) {
-        if (c >= nsp) continue;
-        const long long o = static_cast<long long>(a >= c ? a * (a + 1) / 2 + c : c * (c + 1) / 2 + a) * s.nch + p;
+      for (int c = 0; c <= a; ++c) {
+        if (a >= nsp) continue;
+        const long long o = static_cast<long long>(a * (a + 1) / 2 + c) * s.nch + p;
         const double kk = K[o];
-        tr += kk * qr[c];
-        ti += kk * qi[c];
+        tr[a] += kk * qr[c];
+        ti[a] += kk * qi[c];
+        if (c != a) {
+          tr[c] += kk * qr[a];
+          ti[c] += kk * qi[a];
+        }
         if (Dm) {
-          const double re = qr[a] * qr[c] + qi[a] * qi[c];
+          const double re = (c == a ? 1.0 : 2.0) * (qr[a] * qr[c] + qi[a] * qi[c]);
           qm += Dm[o] * re;
           qx += Dmir[o] * re;
         }
       }
-      Th[static_cast<long long>(a) * s.nch + p] = make_double2(tr, ti);
     }
+#pragma unroll
+    for (int a = 0; a < kMaxEwaldSpecies; ++a)
+      if (a < nsp) Th[static_cast<long long>(a) * s.nch + p] = make_double2(tr[a], ti[a]);
     if (Dm) {
-      const int h1 = static_cast<int>(p % s.mh), g2 = static_cast<int>((p / s.mh) % s.m1);
-      const int g3 = static_cast<int>(p / (static_cast<long long>(s.mh) * s.m1));
+      const int h1 = static_cast<int>(p % s.mh);
+      const int rest = static_cast<int>(p / s.mh);
+      const int g2 = rest % s.m1, g3 = rest / s.m1;
       const int r1 = (s.m0 - h1) % s.m0, r2 = (s.m1 - g2) % s.m1, r3 = (s.m2 - g3) % s.m2;
       const int jj[2][3] = {{mesh_wrap(h1, s.m0), mesh_wrap(g2, s.m1), mesh_wrap(g3, s.m2)},
                             {mesh_wrap(r1, s.m0), mesh_wrap(r2, s.m1), mesh_wrap(r3, s.m2)}};
@@ -2569,10 +2579,18 @@ __global__ __launch_bounds__(kBlock) void spme_conv_kernel(EwaldBatch e, SpmeBat
     }
   }
   if (Dm) {
+    __shared__ double red[kWavesPerBlock][6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) sg[c] = wave_sum(sg[c]);
+    const int wv = threadIdx.x >> 6;
     if (wave_lane() == 0)
-      for (int c = 0; c < 6; ++c) atomicAdd(&sig_atom[6 * e.atom_begin + c], sg[c]);
+      for (int c = 0; c < 6; ++c) red[wv][c] = sg[c];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      double v = 0.0;
+      for (int q = 0; q < kWavesPerBlock; ++q) v += red[q][threadIdx.x];
+      atomicAdd(&sig_atom[6 * e.atom_begin + threadIdx.x], v);
+    }
   }
 }
 
@@ -2847,12 +2865,13 @@ void launch_lowrank(const DevBatch& b, const EwaldBatch& e, bool grad, const Dev
                      grad ? w.dgw : nullptr, ew.c6l, ew.lc6l, ew.ldc6l, w.e_atom, w.dedcn);
 }
 
-void launch_spme_setup(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, const SpmeWork& sw,
-                       hipStream_t st) {
+void launch_spme_setup(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, bool table,
+                       const SpmeWork& sw, hipStream_t st) {
   hipLaunchKernelGGL(spme_spline_kernel, dim3(blocks_for(e.natoms, kBlock)), dim3(kBlock), 0, st, b, e, s, sw.base,
                      sw.theta, sw.dtheta);
-  hipLaunchKernelGGL(spme_table_kernel, dim3(blocks_for(s.nch, kBlock)), dim3(kBlock), 0, st, e, s, sw.K,
-                     grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr);
+  if (table)
+    hipLaunchKernelGGL(spme_table_kernel, dim3(blocks_for(s.nch, kBlock)), dim3(kBlock), 0, st, e, s, sw.K,
+                       grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr);
 }
 
 void launch_spme_spread(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, const EwaldWork& ew,
@@ -2863,7 +2882,8 @@ void launch_spme_spread(const DevBatch& b, const EwaldBatch& e, const SpmeBatch&
 
 void launch_spme_conv(const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w, const SpmeWork& sw, int l,
                       hipStream_t st) {
-  hipLaunchKernelGGL(spme_conv_kernel, dim3(blocks_for(s.nch, kBlock)), dim3(kBlock), 0, st, e, s, sw.Qh, sw.K,
+  const int blocks = static_cast<int>(std::min<long long>(blocks_for(s.nch, kBlock), 4096));
+  hipLaunchKernelGGL(spme_conv_kernel, dim3(blocks), dim3(kBlock), 0, st, e, s, sw.Qh, sw.K,
                      grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr, l, sw.Th, w.sig_atom);
 }
 

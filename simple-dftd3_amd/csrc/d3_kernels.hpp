@@ -182,11 +182,12 @@ struct SpmeWork {
   double* T;                 // [nsp][ng] back-transformed meshes (may alias Q)
   double *K, *Dm, *Dmir;     // [nsp (nsp + 1) / 2][nch] kernel and virial weights (Dm, Dmir with gradient)
 };
-// Spline weights and kernel table of an evaluation; then per term l: spread
-// into Q (zeroed by the caller), [Q -> Qh transform], contract Qh -> Th
-// (and the term's virial), [Th -> T back transform], gather.
-void launch_spme_setup(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, const SpmeWork& sw,
-                       hipStream_t stream);
+// Spline weights of an evaluation and (with table) the kernel table, which
+// depends on the cell, the parameters and the mesh only; then per term l:
+// spread into Q (zeroed by the caller), [Q -> Qh transform], contract
+// Qh -> Th (and the term's virial), [Th -> T back transform], gather.
+void launch_spme_setup(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, bool table,
+                       const SpmeWork& sw, hipStream_t stream);
 void launch_spme_spread(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, const EwaldWork& ew,
                         const SpmeWork& sw, int l, hipStream_t stream);
 void launch_spme_conv(const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w, const SpmeWork& sw, int l,

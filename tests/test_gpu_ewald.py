@@ -166,6 +166,15 @@ def test_ewald_update_structure_and_converged_sum():
     pdict = dict(s6=1.0, s8=0.7875, a1=0.4289, a2=4.4407, s9=0.0, alp=14.0)
     ref = O.ewald(num, moved, lat, PBC, O.RATIONAL, pdict, (40.0, 60.0, 40.0, 0.0, 0.0), grad=False, kcut=10.0)
     assert disp.get_dispersion(param, grad=False)["energy"] == pytest.approx(ref["energy"], rel=1e-10)
+    # the particle mesh keeps its kernel table while the cell stays; new
+    # coordinates, then a new cell
+    mesh_model.update(moved)
+    ref = O.ewald(num, moved, lat, PBC, O.RATIONAL, pdict, (40.0, 60.0, 40.0, 0.0, 0.0), kcut=10.0, mesh=0)
+    _check(mesh_model.get_dispersion(param, grad=True), ref, len(num))
+    lat2 = lat * 1.05
+    mesh_model.update(moved, lat2)
+    ref = O.ewald(num, moved, lat2, PBC, O.RATIONAL, pdict, (40.0, 60.0, 40.0, 0.0, 0.0), kcut=10.0, mesh=0)
+    _check(mesh_model.get_dispersion(param, grad=True), ref, len(num))
 
 
 def test_ewald_errors():
