@@ -19,6 +19,10 @@
  * python/dftd3/test_interface.py energies) plus finite differences of the
  * energy for gradient and virial, closed forms (dimer CN, isolated atom C6,
  * weight normalisation) and brute-force enumeration of the lattice images.
+ * The reciprocal-space part (low-rank C6, direct Ewald sum, particle mesh;
+ * tests/test_oracle_ewald.py): closed-form and quadrature transforms,
+ * LAPACK eigenpairs, the full-rank limit, converged real-space lattice sums,
+ * numpy's FFT, the B-spline recursion, invariances and finite differences.
  *
  * Conventions: positions xyz[n][3] in bohr (Fortran xyz(3,n)), lattice
  * lat[3][3] with rows = lattice vectors (Fortran lattice(3,3) columns),
