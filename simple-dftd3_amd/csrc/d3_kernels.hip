@@ -2239,9 +2239,16 @@ __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const dou
         }
       }
       double el = 0.0, ql = 0.0;
+      // a runtime loop, as in spme_conv_kernel (register pressure)
+#pragma unroll 1
+      for (int a = 0; a < nsp; ++a) {
+        double sa_re = 0.0, sa_im = 0.0;
 #pragma unroll
-      for (int a = 0; a < kMaxEwaldSpecies; ++a) {
-        if (a >= nsp) continue;
+        for (int c = 0; c < kMaxEwaldSpecies; ++c)
+          if (c == a) {
+            sa_re = s_re[c];
+            sa_im = s_im[c];
+          }
         double f_re = 0.0, f_im = 0.0;
 #pragma unroll
         for (int c = 0; c < kMaxEwaldSpecies; ++c) {
@@ -2250,10 +2257,10 @@ __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const dou
           const double2 pv = phi[static_cast<long long>(pr) * nk + k];
           f_re += pv.x * s_re[c];
           f_im += pv.x * s_im[c];
-          ql += pv.y * (s_re[a] * s_re[c] + s_im[a] * s_im[c]);
+          ql += pv.y * (sa_re * s_re[c] + sa_im * s_im[c]);
         }
         F[(static_cast<long long>(a) * e.rank + l) * nk + k] = make_double2(f_re, f_im);
-        el += s_re[a] * f_re + s_im[a] * f_im;
+        el += sa_re * f_re + sa_im * f_im;
       }
       ek += e.lambda[l] * el;
       qq += e.lambda[l] * ql;
@@ -2519,40 +2526,41 @@ __global__ __launch_bounds__(kBlock) void spme_conv_kernel(EwaldBatch e, SpmeBat
   double sg[6] = {0, 0, 0, 0, 0, 0};
   for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < s.nch;
        p += static_cast<long long>(gridDim.x) * blockDim.x) {
-    double qr[kMaxEwaldSpecies], qi[kMaxEwaldSpecies], tr[kMaxEwaldSpecies], ti[kMaxEwaldSpecies];
+    double qr[kMaxEwaldSpecies], qi[kMaxEwaldSpecies];
 #pragma unroll
     for (int a = 0; a < kMaxEwaldSpecies; ++a) {
       const double2 v = a < nsp ? Qh[static_cast<long long>(a) * s.nch + p] : make_double2(0.0, 0.0);
       qr[a] = v.x;
       qi[a] = v.y;
-      tr[a] = 0.0;
-      ti[a] = 0.0;
     }
     double qm = 0.0, qx = 0.0;
-    // each pair once: T_a += K_ac Q_c, T_c += K_ac Q_a
+    // a runtime loop (a fully unrolled pair loop hoists every table load
+    // into registers and runs at one wavefront per SIMD)
+#pragma unroll 1
+    for (int a = 0; a < nsp; ++a) {
+      double ar = 0.0, ai = 0.0;  // Q_a, selected without indexing the register array
 #pragma unroll
-    for (int a = 0; a < kMaxEwaldSpecies; ++a) {
-#pragma unroll
-      for (int c = 0; c <= a; ++c) {
-        if (a >= nsp) continue;
-        const long long o = static_cast<long long>(a * (a + 1) / 2 + c) * s.nch + p;
-        const double kk = K[o];
-        tr[a] += kk * qr[c];
-        ti[a] += kk * qi[c];
-        if (c != a) {
-          tr[c] += kk * qr[a];
-          ti[c] += kk * qi[a];
+      for (int c = 0; c < kMaxEwaldSpecies; ++c)
+        if (c == a) {
+          ar = qr[c];
+          ai = qi[c];
         }
-        if (Dm) {
-          const double re = (c == a ? 1.0 : 2.0) * (qr[a] * qr[c] + qi[a] * qi[c]);
+      double tr = 0.0, ti = 0.0;
+#pragma unroll
+      for (int c = 0; c < kMaxEwaldSpecies; ++c) {
+        if (c >= nsp) continue;
+        const long long o = static_cast<long long>(a >= c ? a * (a + 1) / 2 + c : c * (c + 1) / 2 + a) * s.nch + p;
+        const double kk = K[o];
+        tr += kk * qr[c];
+        ti += kk * qi[c];
+        if (Dm && c <= a) {  // each pair once
+          const double re = (c == a ? 1.0 : 2.0) * (ar * qr[c] + ai * qi[c]);
           qm += Dm[o] * re;
           qx += Dmir[o] * re;
         }
       }
+      Th[static_cast<long long>(a) * s.nch + p] = make_double2(tr, ti);
     }
-#pragma unroll
-    for (int a = 0; a < kMaxEwaldSpecies; ++a)
-      if (a < nsp) Th[static_cast<long long>(a) * s.nch + p] = make_double2(tr[a], ti[a]);
     if (Dm) {
       const int h1 = static_cast<int>(p % s.mh);
       const int rest = static_cast<int>(p / s.mh);
