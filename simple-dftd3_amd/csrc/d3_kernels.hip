@@ -1114,8 +1114,11 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   }
 }
 
+// At least three wavefronts per SIMD: the FP64 E+grad instantiation otherwise
+// takes 184 VGPRs (two wavefronts) and runs 14 % slower; with the hint it
+// takes 168 and spills 20 bytes.
 template <typename R, bool kGrad, bool kSmooth, int kW>
-__global__ __launch_bounds__(kW * kWave) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
+__global__ __launch_bounds__(kW * kWave) __attribute__((amdgpu_waves_per_eu(3))) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
                                                          const long long* __restrict__ offset,
                                                          const Slot<R>* __restrict__ slots,
                                                          const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
