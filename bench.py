@@ -465,6 +465,8 @@ def main():
                     help="run one device copy of this size first (reference kernel for FETCH/WRITE_SIZE counters)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--oracle-check", action="store_true",
+                    help="also evaluate the workload once with the FP64 oracle and report both paths' errors")
     ap.add_argument("--pairwise", action="store_true",
                     help="time the pairwise-resolved energies (pair energy matrices) of the workload instead")
     ap.add_argument("--hessian", action="store_true",
@@ -561,8 +563,34 @@ def main():
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on every {args.cpu_stride}-th atom, {args.cpu_threads} threads")
         line["cpu_baseline"] = cpu_baseline(systems, counts, args.cpu_stride, args.cpu_threads)
+    if args.oracle_check:
+        line["error_vs_oracle"] = oracle_errors(systems, outs, args.cpu_threads)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def oracle_errors(systems, outs, threads):
+    """Energy (per atom), gradient and virial of both paths against one full
+    FP64 oracle evaluation of the (single-system) workload."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    s = systems[0]
+    log(f"oracle: one full evaluation on {threads} threads")
+    t0 = time.perf_counter()
+    ref = oracle.dispersion(s["numbers"], s["positions"], s.get("lattice"), s.get("periodic"), oracle.RATIONAL,
+                            s["param"])
+    t_ref = time.perf_counter() - t0
+    n = len(s["numbers"])
+    res = dict(oracle_seconds=t_ref)
+    for prec, out in outs.items():
+        e = float(out["energy"].reshape(-1)[0].item())
+        g = out["gradient"].reshape(-1, 3).cpu().numpy()
+        sg = out["sigma"].reshape(3, 3).cpu().numpy()
+        res[prec] = dict(energy_per_atom=abs(e - ref["energy"]) / n, energy_rel=abs(e - ref["energy"]) / abs(ref["energy"]),
+                         max_gradient=float(np.abs(g - ref["gradient"]).max()),
+                         max_sigma=float(np.abs(sg - ref["sigma"]).max()))
+    return res
 
 
 if __name__ == "__main__":
