@@ -2596,3 +2596,13 @@ int d3o_num_threads(void)
     return 1;
 #endif
 }
+
+/* Threads of the following parallel loops (timing of the baseline). */
+void d3o_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

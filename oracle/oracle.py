@@ -364,6 +364,11 @@ def num_threads():
     return lib().d3o_num_threads()
 
 
+def set_num_threads(n: int):
+    """Threads of the oracle's parallel loops from now on."""
+    lib().d3o_set_num_threads(int(n))
+
+
 def set_sample_stride(stride: int):
     """Visit only outer atoms iat % stride == 0 (bounded timing samples)."""
     lib().d3o_set_sample_stride(int(stride))
