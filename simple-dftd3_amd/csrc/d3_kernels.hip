@@ -1718,15 +1718,21 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
   const Slot<double> sa = slots[t];
   const long long end = offset[i + 1];
   const int at0 = i - g.atom_begin, at1 = sa.atom - g.atom_begin;
-  // position of an atom among the fixed atoms (i, atom of a), -1 otherwise
-  auto fixed = [&](int atom) { return atom == at0 ? 0 : (atom == at1 ? 1 : -1); };
-  double hf[4][9], df[2][3][2], cf[2][2], ef[2];
-  for (int k = 0; k < 4; ++k)
-    for (int c = 0; c < 9; ++c) hf[k][c] = 0.0;
-  for (int a = 0; a < 2; ++a) {
-    for (int c = 0; c < 3; ++c) df[a][c][0] = df[a][c][1] = 0.0;
-    cf[a][0] = cf[a][1] = 0.0;
-    ef[a] = 0.0;
+  // Entries between the triple positions 0 (atom i) and 1 (atom of slot a),
+  // the same for every lane, are summed in registers (static indices) and
+  // added once per wavefront; entries with position 2 (atom of slot b) are
+  // global atomics.
+  double hf[2][2][9], df[2][3][2], cf[2][2], ef[2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int c = 0; c < 9; ++c) hf[x][y][c] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) df[x][c][0] = df[x][c][1] = 0.0;
+    cf[x][0] = cf[x][1] = 0.0;
+    ef[x] = 0.0;
   }
   for (long long bi = t + 1 + lane; bi < end; bi += kWave) {
     const Slot<double> sb = slots[bi];
@@ -1812,18 +1818,21 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
           for (int q = 0; q < 3; ++q) tmp += kpq[q][r] * 2.0 * sigp[a][q] * vec[q][c];
           gk[a][r][c] = pq * tmp;
         }
+#pragma unroll
     for (int a = 0; a < 3; ++a)
+#pragma unroll
       for (int bb = 0; bb < 3; ++bb) {
         double dg = 0.0;
         for (int q = 0; q < 3; ++q) dg += kp[q] * sigp[a][q] * sigp[bb][q];
         dg *= 2.0 * pq;
-        const int fa = fixed(at[a]), fb = fixed(at[bb]);
         double* __restrict__ blk = hess + 9 * c0 + static_cast<long long>(3 * at[a]) * ndim + 3 * at[bb];
+#pragma unroll
         for (int c = 0; c < 3; ++c)
+#pragma unroll
           for (int d = 0; d < 3; ++d) {
             double tmp = c == d ? dg : 0.0;
             for (int r = 0; r < 3; ++r) tmp += gk[a][r][c] * 2.0 * sigp[bb][r] * vec[r][d];
-            if (fa >= 0 && fb >= 0) hf[2 * fa + fb][3 * c + d] += tmp;
+            if (a < 2 && bb < 2) hf[a < 2 ? a : 0][bb < 2 ? bb : 0][3 * c + d] += tmp;
             else atomicAdd(blk + static_cast<long long>(c) * ndim + d, tmp);
           }
       }
@@ -1838,51 +1847,60 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
         euc[r][q] = pref * qm * kp[r];
       }
     }
-    const int pat[3][2] = {{at0, at1}, {at0, at2}, {at1, at2}};
-    int nent = 0, ent_atom[6], ent_pair[6];
+    // the (position, pair) entries of dC6/dCN: both atoms of each pair, or
+    // one entry of twice the derivative when the pair's two atoms coincide
+    // (the second then has coefficient 0)
+    constexpr int kEntPos[6] = {0, 1, 0, 2, 1, 2}, kEntPair[6] = {0, 0, 1, 1, 2, 2};
     double ent_coef[6];
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const int a0 = pat[q][0], a1 = pat[q][1];
+      const int a0 = at[kEntPos[2 * q]], a1 = at[kEntPos[2 * q + 1]];
       if (a0 != a1) {
-        ent_atom[nent] = a0; ent_pair[nent] = q; ent_coef[nent++] = dc6[c0 + static_cast<long long>(a0) * n + a1];
-        ent_atom[nent] = a1; ent_pair[nent] = q; ent_coef[nent++] = dc6[c0 + static_cast<long long>(a1) * n + a0];
+        ent_coef[2 * q] = dc6[c0 + static_cast<long long>(a0) * n + a1];
+        ent_coef[2 * q + 1] = dc6[c0 + static_cast<long long>(a1) * n + a0];
       } else {
-        ent_atom[nent] = a0; ent_pair[nent] = q;
-        ent_coef[nent++] = 2.0 * dc6[c0 + static_cast<long long>(a0) * n + a0];
+        ent_coef[2 * q] = 2.0 * dc6[c0 + static_cast<long long>(a0) * n + a0];
+        ent_coef[2 * q + 1] = 0.0;
       }
     }
-    for (int ie = 0; ie < nent; ++ie) {
-      const int ka = ent_atom[ie], kq = ent_pair[ie], fk = fixed(ka);
-      if (fk >= 0) ef[fk] += ec[kq] * ent_coef[ie];
-      else atomicAdd(dedcn + g.atom_begin + ka, ec[kq] * ent_coef[ie]);
-      for (int jf = 0; jf < nent; ++jf) {
-        const double v = ecc[kq][ent_pair[jf]] * ent_coef[ie] * ent_coef[jf];
-        const int fj = fixed(ent_atom[jf]);
-        if (fk >= 0 && fj >= 0) cf[fk][fj] += v;
-        else atomicAdd(dEdcndcn + c0 + static_cast<long long>(ka) * n + ent_atom[jf], v);
+#pragma unroll
+    for (int ie = 0; ie < 6; ++ie) {
+      const int kpos = kEntPos[ie], kq = kEntPair[ie], ka = at[kpos];
+      const double ce = ent_coef[ie];
+      if (ce == 0.0) continue;
+      if (kpos < 2) ef[kpos < 2 ? kpos : 0] += ec[kq] * ce;
+      else atomicAdd(dedcn + g.atom_begin + ka, ec[kq] * ce);
+#pragma unroll
+      for (int jf = 0; jf < 6; ++jf) {
+        const int jpos = kEntPos[jf];
+        const double v = ecc[kq][kEntPair[jf]] * ce * ent_coef[jf];
+        if (kpos < 2 && jpos < 2) cf[kpos < 2 ? kpos : 0][jpos < 2 ? jpos : 0] += v;
+        else if (ent_coef[jf] != 0.0) atomicAdd(dEdcndcn + c0 + static_cast<long long>(ka) * n + at[jpos], v);
       }
+#pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const int fa = fixed(at[a]);
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
           double egr = 0.0;
           for (int q = 0; q < 3; ++q) egr += euc[q][kq] * 2.0 * sigp[a][q] * vec[q][c];
-          if (fa >= 0 && fk >= 0) df[fa][c][fk] += egr * ent_coef[ie];
-          else atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * at[a] + c) * n + ka, egr * ent_coef[ie]);
+          if (a < 2 && kpos < 2) df[a < 2 ? a : 0][c][kpos < 2 ? kpos : 0] += egr * ce;
+          else atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * at[a] + c) * n + ka, egr * ce);
         }
       }
     }
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const int a0 = pat[q][0], a1 = pat[q][1];
+      const int p0 = kEntPos[2 * q], p1 = kEntPos[2 * q + 1];  // positions of the pair's atoms
+      const int a0 = at[p0], a1 = at[p1];
       const long long o00 = c0 + static_cast<long long>(a0) * n + a0, o01 = c0 + static_cast<long long>(a0) * n + a1;
-      const int f0 = fixed(a0), f1 = fixed(a1);
       if (a0 != a1) {
         const long long o11 = c0 + static_cast<long long>(a1) * n + a1, o10 = c0 + static_cast<long long>(a1) * n + a0;
         const double v00 = ec[q] * d2c6[o01], v11 = ec[q] * d2c6[o10], vx = ec[q] * d2c6ij[o01];
-        if (f0 >= 0 && f1 >= 0) {
-          cf[f0][f0] += v00;
-          cf[f1][f1] += v11;
-          cf[f0][f1] += vx;
-          cf[f1][f0] += vx;
+        if (p1 < 2) {  // pair (i, a): positions 0 and 1
+          cf[0][0] += v00;
+          cf[1][1] += v11;
+          cf[0][1] += vx;
+          cf[1][0] += vx;
         } else {
           atomicAdd(dEdcndcn + o00, v00);
           atomicAdd(dEdcndcn + o11, v11);
@@ -1891,31 +1909,34 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
         }
       } else {
         const double v = ec[q] * (2.0 * d2c6[o00] + 2.0 * d2c6ij[o00]);
-        if (f0 >= 0) cf[f0][f0] += v;
+        if (p1 < 2) cf[0][0] += v;
         else atomicAdd(dEdcndcn + o00, v);
       }
     }
   }
-  // the entries between the fixed atoms, once per wavefront
+  // the entries between positions 0 and 1, once per wavefront (both refer
+  // to atom i when slot a is an image of it)
   const int fat[2] = {at0, at1};
-  for (int fa = 0; fa < 2; ++fa) {
-    if (fa == 1 && at1 == at0) break;  // a is an image of i: everything is in the first row
-    for (int fb = 0; fb < 2; ++fb) {
-      if (fb == 1 && at1 == at0) break;
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+#pragma unroll
       for (int c = 0; c < 9; ++c) {
-        const double v = wave_sum(hf[2 * fa + fb][c]);
+        const double v = wave_sum(hf[x][y][c]);
         if (lane == 0 && v != 0.0)
-          atomicAdd(hess + 9 * c0 + static_cast<long long>(3 * fat[fa] + c / 3) * ndim + 3 * fat[fb] + c % 3, v);
+          atomicAdd(hess + 9 * c0 + static_cast<long long>(3 * fat[x] + c / 3) * ndim + 3 * fat[y] + c % 3, v);
       }
-      const double v = wave_sum(cf[fa][fb]);
-      if (lane == 0 && v != 0.0) atomicAdd(dEdcndcn + c0 + static_cast<long long>(fat[fa]) * n + fat[fb], v);
+      const double v = wave_sum(cf[x][y]);
+      if (lane == 0 && v != 0.0) atomicAdd(dEdcndcn + c0 + static_cast<long long>(fat[x]) * n + fat[y], v);
+#pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double w = wave_sum(df[fa][c][fb]);
-        if (lane == 0 && w != 0.0) atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * fat[fa] + c) * n + fat[fb], w);
+        const double w = wave_sum(df[x][c][y]);
+        if (lane == 0 && w != 0.0) atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * fat[x] + c) * n + fat[y], w);
       }
     }
-    const double v = wave_sum(ef[fa]);
-    if (lane == 0 && v != 0.0) atomicAdd(dedcn + g.atom_begin + fat[fa], v);
+    const double v = wave_sum(ef[x]);
+    if (lane == 0 && v != 0.0) atomicAdd(dedcn + g.atom_begin + fat[x], v);
   }
 }
 
