@@ -51,6 +51,12 @@ struct SysGeom {
   int img2_begin, img2_count;
   int img3_begin, img3_count;
   long long c6_begin;  // offset of the n x n C6 block of this system
+  // 3D-periodic systems: the two-body images form the box |n_d| <= box2[d]
+  // in list order (n_x outer, n_z inner); frac rows b_d / 2 pi give the
+  // fractional coordinates, span2[d] = cut2 |b_d| / 2 pi.  box2[0] < 0: no box.
+  int box2[3];
+  double frac[9];
+  double span2[3];
 };
 
 // Host-side element tables (generated from data/d3_tables.npz).

@@ -228,6 +228,45 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
   return out;
 }
 
+// The index box of a 3D-periodic system's two-body images (SysGeom::box2):
+// the reciprocal rows, the repetitions read back from the images, and a check
+// that the images are that box in list order (else no box is used).
+void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count, SysGeom& g) {
+  g.box2[0] = g.box2[1] = g.box2[2] = -1;
+  if (!(sd.periodic[0] && sd.periodic[1] && sd.periodic[2])) return;
+  const double* a = sd.lattice;
+  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+  if (!(std::fabs(det) > 0.0)) return;
+  double fr[9];
+  for (int j = 0; j < 3; ++j) {  // b_j / 2 pi = (a_k x a_l) / det
+    const double* u = a + 3 * ((j + 1) % 3);
+    const double* v = a + 3 * ((j + 2) % 3);
+    fr[3 * j + 0] = (u[1] * v[2] - u[2] * v[1]) / det;
+    fr[3 * j + 1] = (u[2] * v[0] - u[0] * v[2]) / det;
+    fr[3 * j + 2] = (u[0] * v[1] - u[1] * v[0]) / det;
+  }
+  int rep[3] = {0, 0, 0};
+  std::vector<int> idx(3 * size_t(count));
+  for (int t = 0; t < count; ++t)
+    for (int d = 0; d < 3; ++d) {
+      const double x = fr[3 * d] * imgs[t].x + fr[3 * d + 1] * imgs[t].y + fr[3 * d + 2] * imgs[t].z;
+      idx[3 * t + d] = static_cast<int>(std::lround(x));
+      rep[d] = std::max(rep[d], std::abs(idx[3 * t + d]));
+    }
+  if (count != (2 * rep[0] + 1) * (2 * rep[1] + 1) * (2 * rep[2] + 1)) return;
+  for (int t = 0; t < count; ++t) {
+    const int want = ((idx[3 * t] + rep[0]) * (2 * rep[1] + 1) + (idx[3 * t + 1] + rep[1])) * (2 * rep[2] + 1) +
+                     (idx[3 * t + 2] + rep[2]);
+    if (want != t) return;
+  }
+  for (int d = 0; d < 3; ++d) {
+    g.box2[d] = rep[d];
+    g.span2[d] = cut * std::sqrt(fr[3 * d] * fr[3 * d] + fr[3 * d + 1] * fr[3 * d + 1] + fr[3 * d + 2] * fr[3 * d + 2]);
+  }
+  std::memcpy(g.frac, fr, sizeof fr);
+}
+
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
       item_off, slots, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
@@ -405,6 +444,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     add(sd.param.cut_cn, g.img_cn_begin, g.img_cn_count);
     add(sd.param.cut2, g.img2_begin, g.img2_count);
     add(sd.param.cut3, g.img3_begin, g.img3_count);
+    image_box(sd, sd.param.cut2, &img[g.img2_begin], g.img2_count, g);
     g.c6_begin = pair_total;
     pair_total += static_cast<long long>(sd.natoms) * sd.natoms;
     if (std::fabs(sd.param.s9) >= kEps) {

@@ -554,10 +554,10 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
       const R dc6i = q.y * c6ij;  // dC6_ij/dCN_i
       const DampPair<R> dp = damp_setup<R, kKind>(p, rrij, b.rvdw[spj * b.nsp + spi], c6ij, idi + b.sid[j]);
       const bool swap = j > i;
-      for (int t = 0; t < g.img2_count; ++t) {
+      auto image = [&](int t) {
         double u0, u1, u2;
         const double r2d = pair_vec(xi, xj, b.img[g.img2_begin + t], swap, u0, u1, u2);
-        if (r2d > cut2 || r2d < kEps) continue;
+        if (r2d > cut2 || r2d < kEps) return;
         const R r2 = R(r2d);
         const R r1 = d_sqrt(r2);
         R sw, dswdr;
@@ -581,6 +581,26 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
           sxx += double(h * v0 * v0); syy += double(h * v1 * v1); szz += double(h * v2 * v2);
           sxy += double(h * v0 * v1); sxz += double(h * v0 * v2); syz += double(h * v1 * v2);
         }
+      };
+      if (g.box2[0] >= 0) {
+        // only the images whose index can put the pair within the cutoff:
+        // |x_i - x_j - T| <= cut bounds n_d within the cut / plane spacing of
+        // the fractional difference f_d; the subset keeps the list's order
+        int lo[3], hi[3];
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double f = g.frac[3 * d] * dx + g.frac[3 * d + 1] * dy + g.frac[3 * d + 2] * dz;
+          lo[d] = max(-g.box2[d], static_cast<int>(ceil(f - g.span2[d] - 1e-6)));
+          hi[d] = min(g.box2[d], static_cast<int>(floor(f + g.span2[d] + 1e-6)));
+        }
+        const int w1 = 2 * g.box2[1] + 1, w2 = 2 * g.box2[2] + 1;
+        for (int ix = lo[0]; ix <= hi[0]; ++ix)
+          for (int iy = lo[1]; iy <= hi[1]; ++iy)
+            for (int iz = lo[2]; iz <= hi[2]; ++iz)
+              image(((ix + g.box2[0]) * w1 + (iy + g.box2[1])) * w2 + (iz + g.box2[2]));
+      } else {
+        for (int t = 0; t < g.img2_count; ++t) image(t);
       }
       if (kPairs) {
         const long long il = i - g.atom_begin, jl = j - g.atom_begin;
