@@ -45,18 +45,21 @@ constexpr int kMaxEwaldSpecies = 8;  // distinct elements of a reciprocal-space 
 // stored per cutoff (CN, two-body, three-body) exactly as get_lattice_points
 // enumerates them (cutoff.f90:140-186), so the pair and triple sets are the
 // reference's.
+enum ImageSet { kImgCn = 0, kImg2 = 1, kImg3 = 2 };
+
 struct SysGeom {
   int atom_begin, natoms;
   int img_cn_begin, img_cn_count;
   int img2_begin, img2_count;
   int img3_begin, img3_count;
   long long c6_begin;  // offset of the n x n C6 block of this system
-  // 3D-periodic systems: the two-body images form the box |n_d| <= box2[d]
-  // in list order (n_x outer, n_z inner); frac rows b_d / 2 pi give the
-  // fractional coordinates, span2[d] = cut2 |b_d| / 2 pi.  box2[0] < 0: no box.
-  int box2[3];
+  // 3D-periodic systems: the images of set s (kImgCn, kImg2, kImg3) form the
+  // index box |n_d| <= box[s][d] in list order (n_x outer, n_z inner); the
+  // rows of frac (b_d / 2 pi) give fractional coordinates and span[s][d] =
+  // cut_s |b_d| / 2 pi.  box[s][0] < 0: no box, every image is visited.
+  int box[3][3];
   double frac[9];
-  double span2[3];
+  double span[3][3];
 };
 
 // Host-side element tables (generated from data/d3_tables.npz).

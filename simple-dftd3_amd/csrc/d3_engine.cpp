@@ -228,11 +228,11 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
   return out;
 }
 
-// The index box of a 3D-periodic system's two-body images (SysGeom::box2):
+// The index box of one image set of a 3D-periodic system (SysGeom::box):
 // the reciprocal rows, the repetitions read back from the images, and a check
 // that the images are that box in list order (else no box is used).
-void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count, SysGeom& g) {
-  g.box2[0] = g.box2[1] = g.box2[2] = -1;
+void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count, int set, SysGeom& g) {
+  g.box[set][0] = g.box[set][1] = g.box[set][2] = -1;
   if (!(sd.periodic[0] && sd.periodic[1] && sd.periodic[2])) return;
   const double* a = sd.lattice;
   const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
@@ -261,8 +261,8 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
     if (want != t) return;
   }
   for (int d = 0; d < 3; ++d) {
-    g.box2[d] = rep[d];
-    g.span2[d] = cut * std::sqrt(fr[3 * d] * fr[3 * d] + fr[3 * d + 1] * fr[3 * d + 1] + fr[3 * d + 2] * fr[3 * d + 2]);
+    g.box[set][d] = rep[d];
+    g.span[set][d] = cut * std::sqrt(fr[3 * d] * fr[3 * d] + fr[3 * d + 1] * fr[3 * d + 1] + fr[3 * d + 2] * fr[3 * d + 2]);
   }
   std::memcpy(g.frac, fr, sizeof fr);
 }
@@ -444,7 +444,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     add(sd.param.cut_cn, g.img_cn_begin, g.img_cn_count);
     add(sd.param.cut2, g.img2_begin, g.img2_count);
     add(sd.param.cut3, g.img3_begin, g.img3_count);
-    image_box(sd, sd.param.cut2, &img[g.img2_begin], g.img2_count, g);
+    image_box(sd, sd.param.cut_cn, &img[g.img_cn_begin], g.img_cn_count, kImgCn, g);
+    image_box(sd, sd.param.cut2, &img[g.img2_begin], g.img2_count, kImg2, g);
+    image_box(sd, sd.param.cut3, &img[g.img3_begin], g.img3_count, kImg3, g);
     g.c6_begin = pair_total;
     pair_total += static_cast<long long>(sd.natoms) * sd.natoms;
     if (std::fabs(sd.param.s9) >= kEps) {

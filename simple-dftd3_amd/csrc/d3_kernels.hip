@@ -87,6 +87,37 @@ __device__ __forceinline__ double pair_vec(const double4& xi, const double4& xj,
   return v0 * v0 + v1 * v1 + v2 * v2;
 }
 
+// Calls image(t) for the images t of one set (kImgCn, kImg2, kImg3) that can
+// bring atoms i and j within the set's cutoff, in the order of the list:
+// |x_i - x_j - T_n| <= cut bounds each index n_d to within cut |b_d| / 2 pi
+// of the fractional coordinate f_d of x_i - x_j, so only that sub-box of the
+// index box is visited (3D-periodic systems); otherwise every image.  The
+// caller still applies the exact distance test, so the accepted images and
+// their order are those of the full list.
+template <typename F>
+__device__ __forceinline__ void for_images(const SysGeom& g, int set, int count, const double4& xi,
+                                           const double4& xj, F&& image) {
+  const int* box = g.box[set];
+  if (box[0] < 0) {
+    for (int t = 0; t < count; ++t) image(t);
+    return;
+  }
+  int lo[3], hi[3];
+  const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double f = g.frac[3 * d] * dx + g.frac[3 * d + 1] * dy + g.frac[3 * d + 2] * dz;
+    lo[d] = max(-box[d], static_cast<int>(ceil(f - g.span[set][d] - 1e-6)));
+    hi[d] = min(box[d], static_cast<int>(floor(f + g.span[set][d] + 1e-6)));
+  }
+  const int w1 = 2 * box[1] + 1, w2 = 2 * box[2] + 1;
+  for (int ix = lo[0]; ix <= hi[0]; ++ix)
+    for (int iy = lo[1]; iy <= hi[1]; ++iy) {
+      const int base = ((ix + box[0]) * w1 + (iy + box[1])) * w2 + box[2];
+      for (int iz = lo[2]; iz <= hi[2]; ++iz) image(base + iz);
+    }
+}
+
 // Three-body vector x_j + T - x_i (atm.f90:711,721) and its squared length,
 // evaluated as the reference does.
 __device__ __forceinline__ double atm_vec(const double4& xj, const double4& t, const double4& xi, double& v0,
@@ -223,13 +254,13 @@ __global__ __launch_bounds__(kBlock) void cn_kernel(DevBatch b, double* __restri
     const double4 xj = b.pos[j];
     const R rc = rci + R(b.rcov[b.sp[j]]);
     const bool swap = j > i;
-    for (int t = 0; t < g.img_cn_count; ++t) {
+    for_images(g, kImgCn, g.img_cn_count, xi, xj, [&](int t) {
       double u0, u1, u2;
       const double r2 = pair_vec(xi, xj, b.img[g.img_cn_begin + t], swap, u0, u1, u2);
-      if (r2 > cut2 || r2 < kCnFloor) continue;
+      if (r2 > cut2 || r2 < kCnFloor) return;
       const R r1 = d_sqrt(R(r2));
       acc += double(d_rcp(R(1) + d_exp(-R(kKcn) * (rc / r1 - R(1)))));
-    }
+    });
   }
   acc = wave_sum(acc);
   if (lane == 0) cn[i] = acc;
@@ -582,26 +613,7 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
           sxy += double(h * v0 * v1); sxz += double(h * v0 * v2); syz += double(h * v1 * v2);
         }
       };
-      if (g.box2[0] >= 0) {
-        // only the images whose index can put the pair within the cutoff:
-        // |x_i - x_j - T| <= cut bounds n_d within the cut / plane spacing of
-        // the fractional difference f_d; the subset keeps the list's order
-        int lo[3], hi[3];
-        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const double f = g.frac[3 * d] * dx + g.frac[3 * d + 1] * dy + g.frac[3 * d + 2] * dz;
-          lo[d] = max(-g.box2[d], static_cast<int>(ceil(f - g.span2[d] - 1e-6)));
-          hi[d] = min(g.box2[d], static_cast<int>(floor(f + g.span2[d] + 1e-6)));
-        }
-        const int w1 = 2 * g.box2[1] + 1, w2 = 2 * g.box2[2] + 1;
-        for (int ix = lo[0]; ix <= hi[0]; ++ix)
-          for (int iy = lo[1]; iy <= hi[1]; ++iy)
-            for (int iz = lo[2]; iz <= hi[2]; ++iz)
-              image(((ix + g.box2[0]) * w1 + (iy + g.box2[1])) * w2 + (iz + g.box2[2]));
-      } else {
-        for (int t = 0; t < g.img2_count; ++t) image(t);
-      }
+      for_images(g, kImg2, g.img2_count, xi, xj, image);
       if (kPairs) {
         const long long il = i - g.atom_begin, jl = j - g.atom_begin;
         pair2[g.c6_begin + il * n + jl] = ej;
@@ -698,12 +710,12 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
     }
     const double4 xj = valid ? b.pos[j] : xi;
     if (valid) {
-      for (int t = 0; t < g.img3_count; ++t) {
+      for_images(g, kImg3, g.img3_count, xi, xj, [&](int t) {
         double v0, v1, v2;
         const double r2 = atm_vec(xj, b.img[g.img3_begin + t], xi, v0, v1, v2);
-        if (r2 > cut2 || r2 < kEps) continue;
+        if (r2 > cut2 || r2 < kEps) return;
         ++cnt;
-      }
+      });
     }
     int incl = cnt;  // inclusive scan of the lane counts
 #pragma unroll
@@ -714,10 +726,10 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
     const int wtot = __shfl(incl, kWave - 1, kWave);
     if (kFill && valid && cnt > 0) {
       long long w = total + incl - cnt;
-      for (int t = 0; t < g.img3_count; ++t) {
+      for_images(g, kImg3, g.img3_count, xi, xj, [&](int t) {
         double v0, v1, v2;
         const double r2 = atm_vec(xj, b.img[g.img3_begin + t], xi, v0, v1, v2);
-        if (r2 > cut2 || r2 < kEps) continue;
+        if (r2 > cut2 || r2 < kEps) return;
         Slot<R> e;
         e.vx = v0; e.vy = v1; e.vz = v2;
         e.r2 = R(r2);
@@ -728,7 +740,7 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
         e.atom = j;
         e.rank = r;
         out[w++] = e;
-      }
+      });
     }
     total += wtot;
   }
@@ -1205,10 +1217,10 @@ __global__ __launch_bounds__(kBlock) void cn_chain_kernel(DevBatch b, const doub
     // ds*dEdcn_i once per self-image (ncoord.f90:214-216)
     const R de = R(dei + dedcn[j]);
     const R wsig = (j == i) ? R(dei) : R(0.5) * de;
-    for (int t = 0; t < g.img_cn_count; ++t) {
+    for_images(g, kImgCn, g.img_cn_count, xi, xj, [&](int t) {
       double u0, u1, u2;
       const double r2d = pair_vec(xi, xj, b.img[g.img_cn_begin + t], swap, u0, u1, u2);
-      if (r2d > cut2 || r2d < kCnFloor) continue;
+      if (r2d > cut2 || r2d < kCnFloor) return;
       const R r2 = R(r2d);
       const R r1 = d_sqrt(r2);
       const R expterm = d_exp(-R(kKcn) * (rc / r1 - R(1)));
@@ -1222,7 +1234,7 @@ __global__ __launch_bounds__(kBlock) void cn_chain_kernel(DevBatch b, const doub
       const R h = f * wsig;
       sxx += double(h * v0 * v0); syy += double(h * v1 * v1); szz += double(h * v2 * v2);
       sxy += double(h * v0 * v1); sxz += double(h * v0 * v2); syz += double(h * v1 * v2);
-    }
+    });
   }
   g0 = wave_sum(g0); g1 = wave_sum(g1); g2 = wave_sum(g2);
   sxx = wave_sum(sxx); syy = wave_sum(syy); szz = wave_sum(szz);
