@@ -805,9 +805,9 @@ struct PairEntry<double> {
   double C;
 };
 
-// Per-slot sums of a tile pair: M = sum (f_is + f_jk), K = sum f_jk v_partner
-// (three components), D = sum dE (qx_s + q_partner-side).
-constexpr int kAcc = 5;
+// Per-slot sums of a tile pair: G = sum dE/dv_s = sum (f_is + f_jk) v_s -
+// f_jk v_partner (three components), D = sum dE (qx_s + q_partner-side).
+constexpr int kAcc = 4;
 
 // Loads slot idx of the list of atom i into position l of a tile; returns
 // the slot's atom (-1 past the end of the list).
@@ -832,24 +832,25 @@ __device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restr
 
 // Adds the sums of slot s (vector v, atom k) to the gradient and dE/dCN of
 // atom k and to the i-side sums:
-//   grad_k += M v - K,  grad_i -= M v - K,  dEdcn_k -= D / 2,
-//   sigma  += M v v^T - sym(v K^T).
-__device__ __forceinline__ void flush_slot(double M, double Kx, double Ky, double Kz, double D, double vx, double vy,
+//   grad_k += G,  grad_i -= G,  dEdcn_k -= D / 2,  sigma += sym(v G^T).
+// The virial of a triple (i, a, b) with pair forces f_X v_X is
+// f_A v_a v_a^T + f_B v_b v_b^T + f_C v_ab v_ab^T = v_a G_a^T + v_b G_b^T
+// (v relative to atom i, G_s = dE/dv_s), so the slot sums carry it.
+__device__ __forceinline__ void flush_slot(double Gx, double Gy, double Gz, double D, double vx, double vy,
                                            double vz, int k, double* __restrict__ grad, double* __restrict__ dedcn,
                                            double* gi, double* sg) {
-  if (M == 0.0 && D == 0.0 && Kx == 0.0 && Ky == 0.0 && Kz == 0.0) return;
-  const double gx = M * vx - Kx, gy = M * vy - Ky, gz = M * vz - Kz;
-  atomicAdd(grad + 3 * k, gx);
-  atomicAdd(grad + 3 * k + 1, gy);
-  atomicAdd(grad + 3 * k + 2, gz);
+  if (D == 0.0 && Gx == 0.0 && Gy == 0.0 && Gz == 0.0) return;
+  atomicAdd(grad + 3 * k, Gx);
+  atomicAdd(grad + 3 * k + 1, Gy);
+  atomicAdd(grad + 3 * k + 2, Gz);
   atomicAdd(dedcn + k, -0.5 * D);
-  gi[0] -= gx; gi[1] -= gy; gi[2] -= gz;
-  sg[0] += M * vx * vx - vx * Kx;
-  sg[1] += M * vy * vy - vy * Ky;
-  sg[2] += M * vz * vz - vz * Kz;
-  sg[3] += M * vx * vy - 0.5 * (vx * Ky + vy * Kx);
-  sg[4] += M * vx * vz - 0.5 * (vx * Kz + vz * Kx);
-  sg[5] += M * vy * vz - 0.5 * (vy * Kz + vz * Ky);
+  gi[0] -= Gx; gi[1] -= Gy; gi[2] -= Gz;
+  sg[0] += vx * Gx;
+  sg[1] += vy * Gy;
+  sg[2] += vz * Gz;
+  sg[3] += 0.5 * (vx * Gy + vy * Gx);
+  sg[4] += 0.5 * (vx * Gz + vz * Gx);
+  sg[5] += 0.5 * (vy * Gz + vz * Gy);
 }
 
 // Three-body hot path: x**y for x > 0, 1/x and 1/sqrt(x) for normal x > 0.
@@ -893,7 +894,7 @@ template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return
 // atm.f90:754-760 in factored form.  The radial factor of pair X is
 // f_X = -c9 fdmp S^(-5/2) (3/4 X dP/dX + K1) / X with the damping part
 // K1 = -15/8 P - 3S + 2 alp pp fdmp (3/8 P + S); dG_X = f_X v_X.
-// The per-slot sums M, K, D (see kAcc) give the gradient and virial when the
+// The per-slot sums G, D (see kAcc) give the gradient and virial when the
 // slot is flushed (flush_slot); the weight of a triple, triple_scale times 2
 // for two images of one atom (atm.f90:954-980), is 1/2 per slot that is an
 // image of atom i (folded into sc6h) and 4/3 more when both are.
@@ -1082,16 +1083,16 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
               dE *= sw;
             }
             di += double(dE * (a1.z + b1.z));
-            atomicAdd(&accA[0 * kTile + a], double(fA + fC));
-            atomicAdd(&accA[1 * kTile + a], double(fC * b0.x));
-            atomicAdd(&accA[2 * kTile + a], double(fC * b0.y));
-            atomicAdd(&accA[3 * kTile + a], double(fC * b0.z));
-            atomicAdd(&accA[4 * kTile + a], double(dE * (a1.w + q.y)));
-            atomicAdd(&accb[0 * kTile + bb], double(fB + fC));
-            atomicAdd(&accb[1 * kTile + bb], double(fC * a0.x));
-            atomicAdd(&accb[2 * kTile + bb], double(fC * a0.y));
-            atomicAdd(&accb[3 * kTile + bb], double(fC * a0.z));
-            atomicAdd(&accb[4 * kTile + bb], double(dE * (b1.w + q.z)));
+            // dE/dv_a = fA v_a + fC (v_a - v_b), dE/dv_b = fB v_b + fC (v_b - v_a)
+            const R cx = fC * (a0.x - b0.x), cy = fC * (a0.y - b0.y), cz = fC * (a0.z - b0.z);
+            atomicAdd(&accA[0 * kTile + a], double(fA * a0.x + cx));
+            atomicAdd(&accA[1 * kTile + a], double(fA * a0.y + cy));
+            atomicAdd(&accA[2 * kTile + a], double(fA * a0.z + cz));
+            atomicAdd(&accA[3 * kTile + a], double(dE * (a1.w + q.y)));
+            atomicAdd(&accb[0 * kTile + bb], double(fB * b0.x - cx));
+            atomicAdd(&accb[1 * kTile + bb], double(fB * b0.y - cy));
+            atomicAdd(&accb[2 * kTile + bb], double(fB * b0.z - cz));
+            atomicAdd(&accb[3 * kTile + bb], double(dE * (b1.w + q.z)));
           } else if (kSmooth) {
             dE *= sw;
           }
@@ -1110,10 +1111,10 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       if (done) break;
     }
     if (kGrad && !diag) {  // flush the b-slots of this tile pair
-      const double M = accB[0 * kTile + l], Kx = accB[1 * kTile + l], Ky = accB[2 * kTile + l];
-      const double Kz = accB[3 * kTile + l], D = accB[4 * kTile + l];
+      const double Gx = accB[0 * kTile + l], Gy = accB[1 * kTile + l], Gz = accB[2 * kTile + l];
+      const double D = accB[3 * kTile + l];
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
-      if (Q * kTile + l < m) flush_slot(M, Kx, Ky, Kz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, grad, dedcn, gi, sg);
+      if (Q * kTile + l < m) flush_slot(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, grad, dedcn, gi, sg);
     }
     if (kPairs && !diag) {  // pair {i, b-slot atom}
       const double t = accB[l];
@@ -1125,8 +1126,8 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   }
   __syncthreads();
   if (kGrad && w == 0 && aval)
-    flush_slot(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l],
-               accA[4 * kTile + l], ax, ay, az, jat, grad, dedcn, gi, sg);
+    flush_slot(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l], ax, ay, az,
+               jat, grad, dedcn, gi, sg);
   if (kPairs && w == 0 && aval && accA[l] != 0.0)  // pair {i, a-slot atom}
     atomicAdd(pair3s + g.c6_begin + irow + akey.y, accA[l]);
   block_add(e_atom + i, ei, red);
