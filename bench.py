@@ -38,7 +38,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
 # rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch) scaled by a 256 MiB device copy (FETCH x2.00, WRITE x1.00);
 # commands and figures in profiles/atm_kernel__config1_periodic_2000.md.
-ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4>", "fetch_kib": 12_304_000, "write_kib": 6_878_000,
+ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4>", "fetch_kib": 12_324_000, "write_kib": 6_878_000,
                   "fetch_scale": 2.0, "write_scale": 1.0}
 
 
