@@ -980,7 +980,11 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   for (int c = 0; c < kNAcc; ++c) accB[c * kTile + l] = 0.0;
   __syncthreads();
   const bool aval = P * kTile + l < m;
-  const bool aval_half = aval && l < kTile / 2;  // step 32 of a diagonal tile
+  // Slots past the end of the list hold the coordinates 1e30, so every test
+  // with one of them fails (r^2 ~ 1e60, or 0 < eps for two of them) and the
+  // test needs no validity flag.  Step 32 of a diagonal tile pairs a with
+  // a + 32 and a + 32 with a: only lanes l < 32 take it (cutoff -1 otherwise).
+  const double cut2_half = l < kTile / 2 ? cut2 : -1.0;
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
   const int2 akey = tileA.key[l];
   const unsigned arow = aval ? static_cast<unsigned>(akey.y) * static_cast<unsigned>(n) : 0u;
@@ -1009,8 +1013,8 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
         const int bl = (l + st) & (kTile - 1);
         double v0, v1, v2;
         const double Cd = diff_norm2(TB.vx[bl], TB.vy[bl], TB.vz[bl], ax, ay, az, v0, v1, v2);
-        const bool lane_ok = (diag && st == kTile / 2) ? aval_half : aval;
-        const bool ok = lane_ok & (Cd <= cut2) & (Cd >= kEps);
+        const double cut2_step = (diag && st == kTile / 2) ? cut2_half : cut2;
+        const bool ok = (Cd <= cut2_step) & (Cd >= kEps);
         const unsigned long long bal = __ballot(ok);
         if (ok) {
           const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
