@@ -879,12 +879,11 @@ template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return
 //    reference's FP64 r_jk^2 (atm.f90:723-724) and appends accepted pairs
 //    (ballot + mbcnt compaction);
 //  * terms: each lane takes one queued pair, so every lane evaluates a
-//    triple (the pair-record gather of the next 64 entries is issued before
-//    the current ones are evaluated); the per-slot sums go into LDS with
-//    atomics in type R.
+//    triple; its per-slot sums go into double-precision LDS sums with
+//    atomics (eight per triple).
 // A diagonal tile (Q == P) runs the steps 1..32 (32 only for l < 32) so that
-// every unordered pair is visited once.  The per-slot sums of a tile pair
-// (at most 64 terms each) are added to double-precision sums at its end.
+// every unordered pair is visited once.  The sums of the b-slots are flushed
+// at the end of each tile pair, those of the a-slots at the end of the item.
 //
 // With A = r_ij^2, B = r_ik^2, C = r_jk^2, S = ABC, t1 = A+B-C, t2 = A-B+C,
 // t3 = -A+B+C, P = t1 t2 t3 the angular factor is
@@ -1032,8 +1031,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       const bool done = st > s1;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       __builtin_amdgcn_wave_barrier();
-      // term phase: one queued pair per lane; the entries and pair records of
-      // the next 64 entries are fetched before the current ones are used
+      // term phase: one queued pair per lane, 64 entries at a time
       for (; tail - head >= kWave || (done && tail > head); head += kWave) {
         if (head + l < tail) {
           const PairEntry<R> e = q_ring[(head + l) & (kQueue - 1)];
