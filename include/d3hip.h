@@ -36,6 +36,10 @@ typedef struct _dftd3_param* dftd3_param;
  * API this library mirrors (1.6.0). */
 D3HIP_API int dftd3_get_version(void);
 
+/* s-dftd3.h:74-78 dftd3_has_feature (api.f90:150-162, feature.F90:37-52):
+ * "mpi" is the only optional feature and this library has none of them. */
+D3HIP_API bool dftd3_has_feature(char* feature);
+
 /* s-dftd3.h:85-100 error handle (api.f90:176-238). */
 D3HIP_API dftd3_error dftd3_new_error(void);
 D3HIP_API int dftd3_check_error(dftd3_error error);
@@ -76,11 +80,26 @@ D3HIP_API void dftd3_set_model_realspace_cutoff_smooth(dftd3_error error, dftd3_
  * expansion of the reference C6 (rank > 0 fixed, else the smallest rank
  * within tolerance, default 1e-4) for every C6 of the model, and under 3D
  * periodic boundary conditions the two-body term summed over the reciprocal
- * lattice within kcut (> 0, else from the damping radii).  mesh is accepted
- * for the reference's signature: the vectors are always summed directly
- * (the reference's mesh < 0 branch). */
+ * lattice within kcut (> 0, else from the damping radii): mesh < 0 sums the
+ * vectors directly (fourier/ewald.f90), mesh = 0 uses a particle mesh derived
+ * from the cutoff and mesh > 0 that many points per direction, rounded up to
+ * a power of two (fourier/spme.f90). */
 D3HIP_API void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int rank, double tolerance, double kcut,
                                      int mesh);
+
+/* s-dftd3.h:185-196 dftd3_set_model_work_partition (api.f90:459-480,
+ * partition.f90:57-148): evaluate only part `part` (zero based) of `nparts`
+ * of the pair, triple and reciprocal-space loops; the structure dependent
+ * quantities (CN, weights, C6) and the CN chain rule stay whole, so summing
+ * the results of all parts gives the complete calculation.  Applies to
+ * dftd3_get_dispersion and dftd3_get_dispersion_hessian.  An invalid
+ * partition ("Invalid dispersion work partition") leaves the model serial. */
+D3HIP_API void dftd3_set_model_work_partition(dftd3_error error, dftd3_model model, int part, int nparts);
+
+/* s-dftd3.h:198-205 dftd3_set_model_mpi_comm (api.f90:484-507): this library
+ * has no MPI backend; like a reference build without MPI it reports
+ * "s-dftd3 was built without MPI support" (and leaves the model serial). */
+D3HIP_API void dftd3_set_model_mpi_comm(dftd3_error error, dftd3_model model, int comm);
 
 /* No counterpart in the reference (its d3_lowrank_c6%rank and %error):
  * the rank and largest relative error of the model's low-rank expansion

@@ -63,6 +63,27 @@ static double g_aatoau;
 static int g_sample_stride = 1;
 void d3o_set_sample_stride(int stride) { g_sample_stride = stride > 0 ? stride : 1; }
 
+/* partition.f90:57-148: the externally managed work partition of the pair,
+ * triple and reciprocal-space loops (part of nparts; nparts = 1 keeps every
+ * term).  owns_pair takes 0-based atom indices jat <= iat: the pair's index
+ * (iat - 1) iat / 2 + jat - 1 of the 1-based reference, modulo nparts;
+ * owns_index a 0-based loop index (the reference's idx - 1). */
+static int g_part = 0, g_nparts = 1;
+int d3o_set_partition(int part, int nparts)
+{
+    if (nparts <= 0 || part < 0 || part >= nparts) { g_part = 0; g_nparts = 1; return 1; }
+    g_part = part;
+    g_nparts = nparts;
+    return 0;
+}
+static int owns_pair(int iat, int jat)
+{
+    if (g_nparts == 1) return 1;
+    long long idx = (long long)iat * (iat + 1) / 2 + jat;
+    return idx % g_nparts == g_part;
+}
+static int owns_index(long long idx) { return g_nparts == 1 || idx % g_nparts == g_part; }
+
 /* Copy the element tables (data/d3_tables.npz) into the oracle. */
 void d3o_set_tables(const int *nref, const double *refcn, const double *c6ref,
                     const double *r4_over_r2, const double *vdwrad_aa,
@@ -644,6 +665,7 @@ static void disp2(const d3o_param *p, int nat, const int *num, const double *xyz
         for (int iat = 0; iat < nat; ++iat) {
             if (iat % g_sample_stride) continue; /* bounded timing sample */
             for (int jat = 0; jat <= iat; ++jat) {
+                if (!owns_pair(iat, jat)) continue; /* rational.f90:453 */
                 pair_const pc = pair_setup(p, num[iat], num[jat], id[iat], id[jat], c6[FIDX(jat, iat)]);
                 double c6ij = pc.c6ij;
                 for (int jtr = 0; jtr < ntr; ++jtr) {
@@ -726,6 +748,7 @@ static void atm(int nat, const int *num, const double *xyz, int ntr, const doubl
             if (iat % g_sample_stride) continue; /* bounded timing sample */
             int zi = num[iat];
             for (int jat = 0; jat <= iat; ++jat) {
+                if (!owns_pair(iat, jat)) continue; /* atm.f90:709 */
                 int zj = num[jat];
                 double c6ij = c6[FIDX(jat, iat)];
                 double r0ij = rs9 * d3o_rvdw(zj, zi);
@@ -1357,6 +1380,7 @@ static void disp2_hessian(const d3o_param *p, int nat, const int *num, const dou
     species_ids(nat, num, id);
     for (int iat = 0; iat < nat; ++iat) {
         for (int jat = 0; jat <= iat; ++jat) {
+            if (!owns_pair(iat, jat)) continue; /* damping.f90:452 */
             double c6ij = c6[FIDX(jat, iat)];
             double fac, cnd[2], cnd2[2][2];
             int nc, cnat[2];
@@ -1437,6 +1461,7 @@ static void atm_hessian(int nat, const int *num, const double *xyz, int ntr, con
     for (int iat = 0; iat < nat; ++iat) {
         int zi = num[iat];
         for (int jat = 0; jat <= iat; ++jat) {
+            if (!owns_pair(iat, jat)) continue; /* atm.f90:136 */
             int zj = num[jat];
             for (int jtr = 0; jtr < ntr; ++jtr) {
                 double vec[3][3], u[3];  /* vec[pair][xyz] */
@@ -1989,7 +2014,7 @@ static int ewald2(const d3o_param *p, int nat, const int *num, const int *id, co
     d3o_lattice_points(pbc3, rec, kcut, kp, nk);
     double *esum = (double *)calloc(nat, sizeof(double)), *dsum = (double *)calloc(nat, sizeof(double));
     double *gsum = (double *)calloc(3 * (size_t)nat, sizeof(double)), ssum[9] = {0};
-    for (int i = 0; i < nat; ++i) {
+    for (int i = 0; i < nat && owns_index(0); ++i) { /* ewald.f90:127: the self term on the first part */
         double pval = 0.0;
         int a = id[i] - 1;
         for (int t = 0; t < nterm[a * nid + a]; ++t) pval += potential_zero(terms + 2 * (a * nid + a) + t);
@@ -2012,6 +2037,7 @@ static int ewald2(const d3o_param *p, int nat, const int *num, const int *id, co
 #pragma omp for schedule(dynamic, 16)
     for (int ik = 0; ik < nk; ++ik) {
         if (ik % g_sample_stride) continue; /* bounded timing sample */
+        if (!owns_index(ik)) continue;      /* ewald.f90:161 */
         const double *kv = kp + 3 * ik;
         double knorm = sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
         if (knorm > kcut) continue;
@@ -2288,6 +2314,7 @@ static int spme2(const d3o_param *p, int nat, const int *num, const int *id, con
         double pval = 0.0;
         for (int t = 0; t < nterm[a * nid + a]; ++t) pval += potential_zero(terms + 2 * (a * nid + a) + t);
         for (int l = 0; l < R; ++l) {
+            if (!owns_index(l)) continue; /* spme.f90:166 */
             esum[i] += 0.5 * lr->lambda[l] * c6l[(size_t)i * R + l] * c6l[(size_t)i * R + l] * pval;
             if (grad) dsum[i] += lr->lambda[l] * c6l[(size_t)i * R + l] * dc6l[(size_t)i * R + l] * pval;
         }
@@ -2296,6 +2323,7 @@ static int spme2(const d3o_param *p, int nat, const int *num, const int *id, con
     double *tr = (double *)malloc(sizeof(double) * ng), *ti = (double *)malloc(sizeof(double) * ng);
     double erecip = 0.0, escale = -0.5 / vol;
     for (int l = 0; l < R; ++l) {
+        if (!owns_index(l)) continue; /* spme.f90:198 */
         memset(qr, 0, sizeof(double) * nid * ng);
         memset(qi, 0, sizeof(double) * nid * ng);
         for (int i = 0; i < nat; ++i) {

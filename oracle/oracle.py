@@ -374,6 +374,30 @@ def set_sample_stride(stride: int):
     lib().d3o_set_sample_stride(int(stride))
 
 
+def set_partition(part: int, nparts: int) -> bool:
+    """Work partition of the pair, triple and reciprocal-space loops from now
+    on (partition.f90:57-148; nparts = 1: every term).  An invalid partition
+    resets to the serial one and returns False."""
+    return lib().d3o_set_partition(int(part), int(nparts)) == 0
+
+
+class partition:
+    """``with partition(p, n):`` evaluates part p of n and restores the
+    serial loops afterwards."""
+
+    def __init__(self, part: int, nparts: int):
+        self.part, self.nparts = part, nparts
+
+    def __enter__(self):
+        if not set_partition(self.part, self.nparts):
+            raise ValueError("Invalid dispersion work partition")
+        return self
+
+    def __exit__(self, *exc):
+        set_partition(0, 1)
+        return False
+
+
 def time_sampled_dispersion(stride, *args, **kwargs):
     """Wall time of one sampled E+grad+sigma evaluation (seconds).  The
     results of a sampled run are partial sums and are not returned."""

@@ -42,6 +42,7 @@ struct _dftd3_model {
   double cutoff[5] = {40.0, 60.0, 40.0, 0.0, 0.0};  // cn, disp2, disp3, width2, width3 (cutoff.f90:26-32)
   std::vector<int> ghost;
   std::unique_ptr<d3::LowRank> lowrank;  // set_model_ewald (model.f90:185-196)
+  int part = 0, nparts = 1;               // set_model_work_partition (partition.f90:57-84)
   int precision = d3::kFP64;
   std::unique_ptr<d3::Engine> engine;
   hipStream_t stream = nullptr;
@@ -207,6 +208,13 @@ d3::SysParam make_sys_param(const _dftd3_param* p, const double* cutoff) {
 // ------------------------------------------------------------ error handle
 int dftd3_get_version(void) { return 10600; }
 
+// feature.F90:37-52: "mpi" is the only optional feature; this library has no
+// MPI backend, so every feature is unavailable.
+bool dftd3_has_feature(char* feature) {
+  (void)feature;
+  return false;
+}
+
 dftd3_error dftd3_new_error(void) { return new _dftd3_error(); }
 
 int dftd3_check_error(dftd3_error error) { return error && error->set ? 1 : 0; }
@@ -337,6 +345,30 @@ void dftd3_set_model_ewald(dftd3_error error, dftd3_model model, int rank, doubl
     lr->mesh = mesh;
     model->lowrank = std::move(lr);
   });
+}
+
+void dftd3_set_model_work_partition(dftd3_error error, dftd3_model model, int part, int nparts) {
+  // api.f90:459-480, partition.f90:57-84: parts are zero based; an invalid
+  // partition is an error and leaves the model serial (the partition is
+  // reinitialised before the check)
+  if (!error) return;
+  if (!model) return fatal(error, "D3 dispersion model is missing");
+  model->part = 0;
+  model->nparts = 1;
+  if (nparts <= 0 || part < 0 || part >= nparts) return fatal(error, "Invalid dispersion work partition");
+  model->part = part;
+  model->nparts = nparts;
+}
+
+void dftd3_set_model_mpi_comm(dftd3_error error, dftd3_model model, int comm) {
+  // api.f90:484-507 in a build without MPI (mpi_utils.F90:39-92): the
+  // partition is reset and the missing backend reported
+  (void)comm;
+  if (!error) return;
+  if (!model) return fatal(error, "D3 dispersion model is missing");
+  model->part = 0;
+  model->nparts = 1;
+  fatal(error, "s-dftd3 was built without MPI support");
 }
 
 void d3hip_get_model_lowrank(dftd3_error error, dftd3_model model, int* rank, double* max_error, double* kcut,
@@ -536,6 +568,8 @@ void dftd3_get_dispersion(dftd3_error error, dftd3_structure mol, dftd3_model di
     std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
     sys[0].param = make_sys_param(param, disp->cutoff);
     sys[0].lowrank = disp->lowrank.get();
+    sys[0].part = disp->part;  // api.f90:1099
+    sys[0].nparts = disp->nparts;
     const bool grad = gradient || sigma;
     d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
                                  disp->stream),
@@ -618,6 +652,8 @@ void dftd3_get_dispersion_hessian(dftd3_error error, dftd3_structure mol, dftd3_
     std::memcpy(sys[0].periodic, mol->periodic, sizeof mol->periodic);
     sys[0].param = make_sys_param(param, disp->cutoff);
     sys[0].lowrank = disp->lowrank.get();
+    sys[0].part = disp->part;  // api.f90:1170
+    sys[0].nparts = disp->nparts;
     d3::hip_check(hipMemcpyAsync(disp->d_xyz, mol->xyz.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice,
                                  disp->stream),
                   "upload positions");

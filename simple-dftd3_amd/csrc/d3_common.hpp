@@ -31,6 +31,10 @@ struct SysParam {
   int ewald;  // two-body term in reciprocal space (the real-space pass adds nothing)
   double s6, s8, s9, a1, a2, alp, bet, a3, a4;
   double cut_cn, cut2, cut3, width2, width3;
+  // externally managed work partition (partition.f90:57-148): this system's
+  // pair, triple and reciprocal-space loops keep only the terms of part
+  // `part` of `nparts` (nparts = 1: every term)
+  int part, nparts;
 };
 
 // One term pref r^m / (r^alpha + radius^alpha) of a damped pair potential

@@ -430,6 +430,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     g.natoms = sd.natoms;
     a += sd.natoms;
     par[k] = sd.param;
+    if (sd.nparts < 1 || sd.part < 0 || sd.part >= sd.nparts) throw Error("Invalid dispersion work partition");
+    par[k].part = sd.part;
+    par[k].nparts = sd.nparts;
     if (lr && sd.periodic[0] && sd.periodic[1] && sd.periodic[2]) {  // disp.f90:104-111
       if (!supports_ewald(sd.param)) throw Error("Damping function does not support Ewald summation");
       par[k].ewald = 1;
@@ -606,6 +609,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     e.chunk_seg = reinterpret_cast<const int*>(dmeta + o_cseg);
     e.groups = reinterpret_cast<const int4*>(dmeta + o_groups);
     e.spos = reinterpret_cast<const int*>(dmeta + o_spos);
+    e.part = systems[0].part;
+    e.nparts = systems[0].nparts;
+    e.mesh = pb.spme ? 1 : 0;
     if (pb.spme) {
       SpmeBatch& q = pb.sb;
       q.m0 = sm.m[0];
@@ -710,6 +716,7 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
       spme_key_grad = grad;
     }
     for (int l = 0; l < e.rank; ++l) {
+      if (e.nparts > 1 && l % e.nparts != e.part) continue;  // spme.f90:198
       timed("spme_spread", stream, [&] {
         hip_check(hipMemsetAsync(sw.Q, 0, sizeof(double) * size_t(e.nsp) * q.ng, stream), "hipMemsetAsync");
         launch_spme_spread(db, e, q, ew, sw, l, stream);

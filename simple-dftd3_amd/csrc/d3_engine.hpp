@@ -85,6 +85,11 @@ struct SystemDesc {
   // reciprocal lattice (disp.f90:104-111).  A batch with a low-rank system
   // holds only that system.
   const LowRank* lowrank = nullptr;
+  // Externally managed work partition (partition.f90:57-148): only the
+  // pair, triple and reciprocal-space terms of part `part` of `nparts`
+  // (disp.f90:141-175); summing the results of all parts gives the whole.
+  int part = 0;
+  int nparts = 1;
 };
 
 // Outputs of a batch run (device pointers; any may be null except energy).

@@ -139,6 +139,14 @@ __device__ __forceinline__ double diff_norm2(double bx, double by, double bz, do
   return v0 * v0 + v1 * v1 + v2 * v2;
 }
 
+// partition.f90:127-148 owns_pair with 0-based atom indices il >= jl: the
+// pair's index in the lower triangle, il (il + 1) / 2 + jl, modulo nparts.
+__device__ __forceinline__ bool owns_pair(const SysParam& p, int il, int jl) {
+  if (p.nparts <= 1) return true;
+  const long long idx = static_cast<long long>(il) * (il + 1) / 2 + jl;
+  return idx % p.nparts == p.part;
+}
+
 // cutoff.f90:73-99 smooth_cutoff
 template <typename R>
 __device__ __forceinline__ void smooth_cutoff(R r, R cutoff, R width, R& sw, R& dswdr) {
@@ -576,6 +584,10 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
     for (int jr = lane; jr < n; jr += kWave) {
       const int j = b.order[g.atom_begin + jr];
       if (kPairs && j > i) continue;
+      if (p.nparts > 1) {  // rational.f90:453 and the other families
+        const int il = i - g.atom_begin, jl = j - g.atom_begin;
+        if (!owns_pair(p, max(il, jl), min(il, jl))) continue;
+      }
       double ej = 0.0;
       const double4 xj = b.pos[j];
       const int spj = b.sp[j];
@@ -791,6 +803,7 @@ struct TileSlots {
   R4<R> g1[kTile];                          // pdmp, sc6h, qi, qx
   int2 key[kTile];                          // {queue tag of slot l as b, rank}: tag = l << 8, plus 1 << 16
                                             // when the slot is an image of atom i
+  int atom[kTile];                          // atom of the slot (for the work partition)
 };
 // Accepted slot pair (a, b) of a tile pair.
 template <typename R>
@@ -820,6 +833,7 @@ __device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restr
     T.g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
     T.g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
     T.key[l] = make_int2((l << 8) | (s.atom == i ? 1 << 16 : 0), s.rank);
+    T.atom[l] = s.atom;
     return s.atom;
   }
   // fails every distance test
@@ -827,6 +841,7 @@ __device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restr
   T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
   T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
   T.key[l] = make_int2(l << 8, 0);
+  T.atom[l] = i;
   return -1;
 }
 
@@ -984,6 +999,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   // test needs no validity flag.  Step 32 of a diagonal tile pairs a with
   // a + 32 and a + 32 with a: only lanes l < 32 take it (cutoff -1 otherwise).
   const double cut2_half = l < kTile / 2 ? cut2 : -1.0;
+  const int aatom = tileA.atom[l];
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
   const int2 akey = tileA.key[l];
   const unsigned arow = aval ? static_cast<unsigned>(akey.y) * static_cast<unsigned>(n) : 0u;
@@ -1013,7 +1029,9 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
         double v0, v1, v2;
         const double Cd = diff_norm2(TB.vx[bl], TB.vy[bl], TB.vz[bl], ax, ay, az, v0, v1, v2);
         const double cut2_step = (diag && st == kTile / 2) ? cut2_half : cut2;
-        const bool ok = (Cd <= cut2_step) & (Cd >= kEps);
+        bool ok = (Cd <= cut2_step) & (Cd >= kEps);
+        if (p.nparts > 1)  // atm.f90:709: the triple belongs to the pair (i, larger partner)
+          ok = ok && owns_pair(p, i - g.atom_begin, max(aatom, TB.atom[bl]) - g.atom_begin);
         const unsigned long long bal = __ballot(ok);
         if (ok) {
           const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
@@ -1650,6 +1668,7 @@ __global__ __launch_bounds__(kBlock) void hess2_kernel(DevBatch b, const double*
   double hii[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double dii[3] = {0.0, 0.0, 0.0}, e_cn = 0.0, e_cc = 0.0;
   for (int jl = lane; jl < n; jl += kWave) {
+    if (!owns_pair(p, max(il, jl), min(il, jl))) continue;  // damping.f90:452
     const int j = g.atom_begin + jl;
     const int spj = b.sp[j];
     const double rrij = 3.0 * b.r4r2[spi] * b.r4r2[spj];
@@ -1786,7 +1805,11 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
                            c6[c0 + static_cast<long long>(at2) * n + at1]};
     if (fabs(c6t[0]) < kEps || fabs(c6t[1]) < kEps || fabs(c6t[2]) < kEps) continue;
     const bool selfa = sa.atom == i, selfb = sb.atom == i;
-    const double w = (selfa ? 0.5 : 1.0) * (selfb ? 0.5 : 1.0) * (selfa && selfb ? 4.0 / 3.0 : 1.0);
+    // a triple of another part of a work partition (atm.f90:136) gets weight
+    // 0: a zero contribution rather than a branch out of this loop
+    const double w = owns_pair(p, at0, max(at1, at2))
+                         ? (selfa ? 0.5 : 1.0) * (selfb ? 0.5 : 1.0) * (selfa && selfb ? 4.0 / 3.0 : 1.0)
+                         : 0.0;
     double swp[3], dswp[3], d2swp[3];
     for (int q = 0; q < 3; ++q) switch_r2(u[q], p.cut3, p.width3, swp[q], dswp[q], d2swp[q]);
     const double sval = swp[0] * swp[1] * swp[2];
@@ -2136,7 +2159,16 @@ __device__ __forceinline__ TileVec tile_vector(const EwaldBatch& e, const int4 t
   v.kx = e.rec[0] * tile.x + e.rec[3] * tile.y + e.rec[6] * l;
   v.ky = e.rec[1] * tile.x + e.rec[4] * tile.y + e.rec[7] * l;
   v.kz = e.rec[2] * tile.x + e.rec[5] * tile.y + e.rec[8] * l;
-  v.w = !valid ? 0.0 : (tile.x == 0 && tile.y == 0 && l == 0) ? 1.0 : 2.0;
+  const bool origin = tile.x == 0 && tile.y == 0 && l == 0;
+  if (e.nparts <= 1) {
+    v.w = !valid ? 0.0 : origin ? 1.0 : 2.0;
+  } else if (!valid) {
+    v.w = 0.0;
+  } else {  // ewald.f90:161: the vector and its negative by their box indices
+    const long long w1 = 2 * e.rep1 + 1, w2 = 2 * e.rep2 + 1, nbox = (2 * e.rep0 + 1) * w1 * w2;
+    const long long t = ((tile.x + e.rep0) * w1 + (tile.y + e.rep1)) * w2 + (l + e.rep2);
+    v.w = (t % e.nparts == e.part ? 1.0 : 0.0) + (!origin && (nbox - 1 - t) % e.nparts == e.part ? 1.0 : 0.0);
+  }
   v.e1 = e.toff0 + e.rep0 + tile.x;
   v.e2 = e.toff1 + e.rep1 + tile.y;
   v.e3 = e.toff2 + e.rep2 + l;
@@ -2202,6 +2234,9 @@ __global__ __launch_bounds__(kBlock) void lowrank_kernel(DevBatch b, EwaldBatch 
     if (l >= e.rank) continue;
     lc6l[l * ld + row] = e.lambda[l] * a;
     if (dgw) ldc6l[l * ld + row] = e.lambda[l] * d;
+    // partitioned: the mesh keeps the terms l of its part (spme.f90:166), the
+    // direct sum the whole self-interaction on part 0 (ewald.f90:127)
+    if (e.nparts > 1 && (e.mesh ? l % e.nparts != e.part : e.part != 0)) continue;
     es += 0.5 * e.lambda[l] * a * a * e.pval[sp];
     ds += e.lambda[l] * a * d * e.pval[sp];
   }

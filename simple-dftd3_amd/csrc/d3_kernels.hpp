@@ -141,6 +141,9 @@ struct EwaldBatch {
   const int4* chunks;        // [nchunks] {species, begin, end, 0} into sorder
   const int* chunk_seg;      // [nsp + 1] chunks of each species
   const int4* groups;        // [ngroups] {species, begin, end, 0} into sorder, end - begin <= kEwaldGroup
+  // work partition: vectors by their index in get_lattice_points' box
+  // (direct sum, self-interaction on part 0) or expansion terms l (mesh)
+  int part, nparts, mesh;
 };
 struct EwaldWork {
   // rows and columns by position in species order (sorder)

@@ -21,11 +21,10 @@ this library implements:
   the ``energy`` and the analytic (3n, 3n) ``hessian`` (interface.py:637-658).
 
 All evaluation happens on the GPU through libd3hip.so.  Additions beyond the
-reference: ``DispersionModel.set_precision("fp64" | "mixed")``.  Ewald
-summation, work partitions and counterpoise correction of the reference are
-not part of this library; their classes
-exist so that imports keep working, and raise ``NotImplementedError`` when
-used.
+reference: ``DispersionModel.set_precision("fp64" | "mixed")`` and
+``get_lowrank()``.  The counterpoise correction (GCP) of the reference is not
+part of this library; its classes exist so that imports keep working, and
+raise ``NotImplementedError`` when used.
 """
 from typing import Optional
 
@@ -293,8 +292,9 @@ class DispersionModel(Structure):
         boundary conditions the two-body term is summed over the reciprocal
         lattice within ``kcut`` (default from the damping radii) and the
         two-body real-space cutoff is ignored.  Rational and zero damping
-        only.  The vectors are always summed directly, the reference's
-        ``mesh < 0`` branch; ``mesh`` is accepted and has no effect."""
+        only.  ``mesh < 0`` sums the vectors directly, ``mesh = 0`` uses a
+        particle mesh derived from the cutoff and ``mesh > 0`` that many mesh
+        points per direction (rounded up to a power of two)."""
         library.set_model_ewald(self._disp, rank, tolerance, kcut, mesh)
 
     def get_lowrank(self) -> tuple:
@@ -303,7 +303,17 @@ class DispersionModel(Structure):
         return library.get_model_lowrank(self._disp)
 
     def set_work_partition(self, part: int, nparts: int) -> None:
-        raise NotImplementedError("Work partitions are not available in this library")
+        """
+        Assign an externally managed part of the interaction loops to this
+        model (interface.py:575-589).
+
+        Parts are zero based, summing the results of all parts reproduces the
+        complete calculation.  Structure dependent quantities are evaluated
+        for the full system on every part, only the pairwise, three-body and
+        reciprocal space loops are partitioned.  Applies to get_dispersion and
+        get_hessian.
+        """
+        library.set_model_work_partition(self._disp, part, nparts)
 
     def get_pairwise_dispersion(self, param: DampingParam) -> dict:
         """Evaluate pairwise representation of the dispersion energy

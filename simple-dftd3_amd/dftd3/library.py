@@ -45,6 +45,9 @@ _SIGNATURES = {
     "dftd3_set_model_realspace_cutoff_smooth": (None, [_vp, _vp, _d, _d, _d, _d, _d]),
     "dftd3_set_model_ghost_index": (None, [_vp, _vp, _ip, ctypes.c_int]),
     "dftd3_set_model_ewald": (None, [_vp, _vp, ctypes.c_int, _d, _d, ctypes.c_int]),
+    "dftd3_set_model_work_partition": (None, [_vp, _vp, ctypes.c_int, ctypes.c_int]),
+    "dftd3_set_model_mpi_comm": (None, [_vp, _vp, ctypes.c_int]),
+    "dftd3_has_feature": (ctypes.c_bool, [ctypes.c_char_p]),
     "d3hip_get_model_lowrank": (None, [_vp, _vp, _ip, _dp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "dftd3_delete_model": (None, [ctypes.POINTER(_vp)]),
     "dftd3_new_zero_damping": (_vp, [_vp, _d, _d, _d, _d, _d, _d]),
@@ -219,6 +222,16 @@ def set_model_ghost_index(disp: ModelHandle, ghost: np.ndarray) -> None:
 def set_model_ewald(disp: ModelHandle, rank: int, tolerance: float, kcut: float, mesh: int) -> None:
     """Use a low-rank C6 expansion and the reciprocal-space two-body sum"""
     return error_check(lib.dftd3_set_model_ewald)(disp.handle, int(rank), float(tolerance), float(kcut), int(mesh))
+
+
+def has_feature(feature: str) -> bool:
+    """Whether an optional feature ("mpi") is available in this library"""
+    return bool(lib.dftd3_has_feature(feature.encode()))
+
+
+def set_model_work_partition(disp: ModelHandle, part: int, nparts: int) -> None:
+    """Assign an externally managed part of the interaction loops to the model"""
+    return error_check(lib.dftd3_set_model_work_partition)(disp.handle, int(part), int(nparts))
 
 
 def get_model_lowrank(disp: ModelHandle, last_evaluation: bool = False) -> tuple:

@@ -273,3 +273,25 @@ def test_ewald_without_gpu_fails_loudly(numbers, positions):
     model.set_ewald_summation()
     with pytest.raises(RuntimeError, match="HIP"):
         model.get_dispersion(RationalDampingParam(method="pbe0"), grad=True)
+
+
+def test_work_partition_and_mpi_handles(numbers, positions):
+    """api.f90:459-507 on the host: the invalid partitions of
+    api-test.c:992-1035, a missing model, the MPI entry point of a build
+    without MPI (mpi_utils.F90:39-92) and the feature query (feature.F90)."""
+    from dftd3 import library as L
+    from dftd3.interface import DispersionModel
+    model = DispersionModel(numbers, positions)
+    for part, nparts in ((-1, 3), (3, 3), (0, 0), (1, 1)):
+        with pytest.raises(RuntimeError, match="Invalid dispersion work partition"):
+            model.set_work_partition(part, nparts)
+    model.set_work_partition(2, 3)
+    with pytest.raises(RuntimeError, match="D3 dispersion model is missing"):
+        L.set_model_work_partition(L.ModelHandle.null(), 0, 1)
+    err = L._Error()
+    L.lib.dftd3_set_model_mpi_comm(err.handle, model._disp.handle, 0)
+    assert L.lib.dftd3_check_error(err.handle)
+    buf = ctypes.create_string_buffer(128)
+    L.lib.dftd3_get_error(err.handle, buf, None)
+    assert buf.value.decode() == "s-dftd3 was built without MPI support"
+    assert not L.has_feature("mpi")
