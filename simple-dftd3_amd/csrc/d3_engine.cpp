@@ -154,6 +154,7 @@ struct Prepared {
   bool any_atm = false;        // some system has s9 != 0
   bool smooth = false;         // some three-body system has a nonzero width3
   bool periodic_atm = false;   // some three-body system has lattice images
+  bool partitioned = false;    // some system evaluates one part of a work partition
   long long slot_bound = 0;    // sum over three-body systems of (il + 1) * images
   long long item_bound = 0;    // work items of those bounds
   bool ewald = false;          // the (single) system sums its two-body term in reciprocal space
@@ -433,6 +434,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     if (sd.nparts < 1 || sd.part < 0 || sd.part >= sd.nparts) throw Error("Invalid dispersion work partition");
     par[k].part = sd.part;
     par[k].nparts = sd.nparts;
+    pb.partitioned = pb.partitioned || sd.nparts > 1;
     if (lr && sd.periodic[0] && sd.periodic[1] && sd.periodic[2]) {  // disp.f90:104-111
       if (!supports_ewald(sd.param)) throw Error("Damping function does not support Ewald summation");
       par[k].ewald = 1;
@@ -763,7 +765,7 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
     timed("atm_list", stream, [&] { launch_atm_fill(db, w.pair, mixed, o, s, capacity, stream); });
     unsigned long long* wc = shape.blocks_per_cu > 0 ? counter.as<unsigned long long>(1) : nullptr;
     timed("atm", stream,
-          [&] { launch_atm(db, mixed, grad, smooth, shape, io, max_items, o, s, w, wc, stream); });
+          [&] { launch_atm(db, mixed, grad, smooth, pb.partitioned, shape, io, max_items, o, s, w, wc, stream); });
   }
   if (grad) timed("cn_chain", stream, [&] { launch_cn_chain(db, mixed, w, stream); });
   timed("reduce", stream, [&] { launch_reduce(db, w, out.energy, grad ? out.sigma : nullptr, stream); });

@@ -803,7 +803,6 @@ struct TileSlots {
   R4<R> g1[kTile];                          // pdmp, sc6h, qi, qx
   int2 key[kTile];                          // {queue tag of slot l as b, rank}: tag = l << 8, plus 1 << 16
                                             // when the slot is an image of atom i
-  int atom[kTile];                          // atom of the slot (for the work partition)
 };
 // Accepted slot pair (a, b) of a tile pair.
 template <typename R>
@@ -833,7 +832,6 @@ __device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restr
     T.g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
     T.g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
     T.key[l] = make_int2((l << 8) | (s.atom == i ? 1 << 16 : 0), s.rank);
-    T.atom[l] = s.atom;
     return s.atom;
   }
   // fails every distance test
@@ -841,7 +839,6 @@ __device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restr
   T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
   T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
   T.key[l] = make_int2(l << 8, 0);
-  T.atom[l] = i;
   return -1;
 }
 
@@ -930,7 +927,7 @@ template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return
 // (pair3_finalize_kernel).
 constexpr int kAccPairs = 1;  // per-slot sums of the pairwise mode: sum t
 
-template <typename R, bool kGrad, bool kSmooth, int kW, bool kPairs>
+template <typename R, bool kGrad, bool kSmooth, int kW, bool kPairs, bool kPart = false>
 __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict__ item_off,
                                          const long long* __restrict__ offset, const Slot<R>* __restrict__ slots,
                                          const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
@@ -999,7 +996,6 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   // test needs no validity flag.  Step 32 of a diagonal tile pairs a with
   // a + 32 and a + 32 with a: only lanes l < 32 take it (cutoff -1 otherwise).
   const double cut2_half = l < kTile / 2 ? cut2 : -1.0;
-  const int aatom = tileA.atom[l];
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
   const int2 akey = tileA.key[l];
   const unsigned arow = aval ? static_cast<unsigned>(akey.y) * static_cast<unsigned>(n) : 0u;
@@ -1030,8 +1026,11 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
         const double Cd = diff_norm2(TB.vx[bl], TB.vy[bl], TB.vz[bl], ax, ay, az, v0, v1, v2);
         const double cut2_step = (diag && st == kTile / 2) ? cut2_half : cut2;
         bool ok = (Cd <= cut2_step) & (Cd >= kEps);
-        if (p.nparts > 1)  // atm.f90:709: the triple belongs to the pair (i, larger partner)
-          ok = ok && owns_pair(p, i - g.atom_begin, max(aatom, TB.atom[bl]) - g.atom_begin);
+        if (kPart) {  // atm.f90:709: the triple belongs to the pair (i, larger partner);
+                      // a slot's atom is the system's atom of its rank in the spatial order
+          const int ja = b.order[g.atom_begin + tileA.key[l].y], jb = b.order[g.atom_begin + TB.key[bl].y];
+          ok = ok && owns_pair(p, i - g.atom_begin, max(ja, jb) - g.atom_begin);
+        }
         const unsigned long long bal = __ballot(ok);
         if (ok) {
           const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
@@ -1170,7 +1169,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
 // At least three wavefronts per SIMD: the FP64 E+grad instantiation otherwise
 // takes 184 VGPRs (two wavefronts) and runs 14 % slower; with the hint it
 // takes 168 and spills 20 bytes.
-template <typename R, bool kGrad, bool kSmooth, int kW>
+template <typename R, bool kGrad, bool kSmooth, int kW, bool kPart>
 __global__ __launch_bounds__(kW * kWave) __attribute__((amdgpu_waves_per_eu(3))) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
                                                          const long long* __restrict__ offset,
                                                          const Slot<R>* __restrict__ slots,
@@ -1178,7 +1177,7 @@ __global__ __launch_bounds__(kW * kWave) __attribute__((amdgpu_waves_per_eu(3)))
                                                          double* __restrict__ dedcn, double* __restrict__ grad,
                                                          double* __restrict__ sig_atom,
                                                          unsigned long long* __restrict__ work_counter) {
-  atm_body<R, kGrad, kSmooth, kW, false>(b, item_off, offset, slots, pair, e_atom, dedcn, grad, sig_atom,
+  atm_body<R, kGrad, kSmooth, kW, false, kPart>(b, item_off, offset, slots, pair, e_atom, dedcn, grad, sig_atom,
                                          work_counter, nullptr);
 }
 
@@ -2885,7 +2884,7 @@ void launch_atm_items(const long long* offset, int n, long long* item_count, hip
 namespace {
 __global__ void zero_counter(unsigned long long* c) { *c = 0ull; }
 
-template <typename R, bool kGrad, bool kSmooth, int kW>
+template <typename R, bool kGrad, bool kSmooth, int kW, bool kPart>
 void launch_atm_t(const DevBatch& b, const long long* item_off, long long max_items, const long long* offset,
                   const void* slots, const DevWork& w, const AtmShape& shape, unsigned long long* counter,
                   hipStream_t st) {
@@ -2896,37 +2895,39 @@ void launch_atm_t(const DevBatch& b, const long long* item_off, long long max_it
     wc = counter;
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, st, wc);
   }
-  hipLaunchKernelGGL((atm_kernel<R, kGrad, kSmooth, kW>), dim3(static_cast<unsigned>(grid)), dim3(kW * kWave), 0,
+  hipLaunchKernelGGL((atm_kernel<R, kGrad, kSmooth, kW, kPart>), dim3(static_cast<unsigned>(grid)), dim3(kW * kWave), 0,
                      st, b, item_off, offset, static_cast<const Slot<R>*>(slots),
                      static_cast<const R4<R>*>(w.pair), w.e_atom, w.dedcn, w.grad, w.sig_atom, wc);
 }
 
-template <int kW>
+template <int kW, bool kPart>
 void launch_atm_w(const DevBatch& b, bool mixed, bool grad, bool smooth, const long long* item_off,
                   long long max_items, const long long* offset, const void* slots, const DevWork& w,
                   const AtmShape& shape, unsigned long long* counter, hipStream_t st) {
   using L = void (*)(const DevBatch&, const long long*, long long, const long long*, const void*, const DevWork&,
                      const AtmShape&, unsigned long long*, hipStream_t);
   static const L table[8] = {
-      launch_atm_t<double, false, false, kW>, launch_atm_t<double, false, true, kW>,
-      launch_atm_t<double, true, false, kW>,  launch_atm_t<double, true, true, kW>,
-      launch_atm_t<float, false, false, kW>,  launch_atm_t<float, false, true, kW>,
-      launch_atm_t<float, true, false, kW>,   launch_atm_t<float, true, true, kW>};
+      launch_atm_t<double, false, false, kW, kPart>, launch_atm_t<double, false, true, kW, kPart>,
+      launch_atm_t<double, true, false, kW, kPart>,  launch_atm_t<double, true, true, kW, kPart>,
+      launch_atm_t<float, false, false, kW, kPart>,  launch_atm_t<float, false, true, kW, kPart>,
+      launch_atm_t<float, true, false, kW, kPart>,   launch_atm_t<float, true, true, kW, kPart>};
   table[(mixed ? 4 : 0) + (grad ? 2 : 0) + (smooth ? 1 : 0)](b, item_off, max_items, offset, slots, w, shape,
                                                              counter, st);
 }
 }  // namespace
 
-void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const AtmShape& shape,
+void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, bool partitioned, const AtmShape& shape,
                 const long long* item_off, long long max_items, const long long* offset, const void* slots,
                 const DevWork& w, unsigned long long* counter, hipStream_t st) {
   if (max_items <= 0) return;
   if (shape.blocks_per_cu > 0 && (!counter || shape.num_cus <= 0))
     throw std::invalid_argument("persistent three-body launch needs a work counter and the CU count");
-  if (shape.waves == 2)
-    launch_atm_w<2>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
+  if (partitioned)  // its own instantiation (four wavefronts), so the others keep their registers
+    launch_atm_w<4, true>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
+  else if (shape.waves == 2)
+    launch_atm_w<2, false>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
   else if (shape.waves == 4)
-    launch_atm_w<4>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
+    launch_atm_w<4, false>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
   else
     throw std::invalid_argument("three-body launch: wavefronts per block must be 2 or 4");
 }

@@ -93,7 +93,8 @@ struct AtmShape {
 // smooth: some system has a nonzero three-body switching width.  max_items
 // bounds the number of work items (item_off[natoms]); counter is device
 // memory for the work counter of a persistent shape.
-void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, const AtmShape& shape,
+// partitioned: some system has a work partition of more than one part.
+void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, bool partitioned, const AtmShape& shape,
                 const long long* item_off, long long max_items, const long long* offset, const void* slots,
                 const DevWork& w, unsigned long long* counter, hipStream_t stream);
 // Pairwise three-body energies: adds the pair sums of every triple term to
