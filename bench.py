@@ -367,6 +367,54 @@ def bench_hessian(args, systems, config, spec, positions, counts):
     return 0
 
 
+def bench_partition(args, systems, config, spec, positions, counts):
+    """Work partition (set_work_partition, partition.f90): E+grad+sigma of the
+    workload's system as P parts, each part evaluated in turn on the one GPU
+    through the model API (host positions in, results out), mixed precision;
+    reports the time per part and checks that the parts sum to the whole."""
+    from dftd3.interface import DispersionModel, RationalDampingParam
+    s = systems[0]
+    p = s["param"]
+    param = RationalDampingParam(s6=p["s6"], s8=p["s8"], s9=p["s9"], a1=p["a1"], a2=p["a2"], alp=p["alp"])
+
+    def model(part, nparts):
+        m = DispersionModel(s["numbers"], s["positions"], s.get("lattice"),
+                            None if s.get("periodic") is None else np.asarray(s["periodic"]))
+        m.set_precision("mixed")
+        if nparts > 1:
+            m.set_work_partition(part, nparts)
+        return m
+
+    def time_model(m):
+        for _ in range(args.warmup):
+            m.get_dispersion(param, grad=True)
+        t = timed_steps(lambda: m.get_dispersion(param, grad=True), args.steps, lambda: None)
+        return t, m.get_dispersion(param, grad=True)
+
+    t_whole, whole = time_model(model(0, 1))
+    nparts = args.partition
+    per_part, e_sum, g_sum = [], 0.0, 0.0
+    for part in range(nparts):
+        t, res = time_model(model(part, nparts))
+        per_part.append(t * 1e3)
+        e_sum += res["energy"]
+        g_sum = g_sum + res["gradient"]
+    de = abs(e_sum - whole["energy"])
+    dg = float(np.abs(g_sum - whole["gradient"]).max())
+    log(f"mixed: whole {t_whole * 1e3:.2f} ms, {nparts} parts: " + " ".join(f"{t:.2f}" for t in per_part)
+        + f" ms; sum of parts vs whole: dE {de:.1e}, max dg {dg:.1e}")
+    line = dict(
+        metric=METRIC, value=counts[1] / (max(per_part) * 1e-3), unit="pair-terms/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=max(per_part), higher_is_better=True, scaling="strong", vs_baseline=None,
+        dtype="f32", data="synthetic",
+        config=dict(config, pair_terms=int(counts[1]), triple_terms=int(counts[2]), parallelism=f"parts{nparts}",
+                    step="one part of a work partition through the model API (host buffers), the slowest part"),
+        whole_ms=t_whole * 1e3, part_ms=per_part, sum_of_parts_error=dict(energy=de, max_gradient=dg),
+        roofline=None, cpu_baseline=None,
+    )
+    print(json.dumps(line), flush=True)
+
+
 def bench_ewald(args, systems, config, spec, positions, counts):
     """Reciprocal-space two-body term (set_ewald_summation: low-rank C6 with
     the default tolerance, cutoff from the damping radii, direct sum over the
@@ -480,6 +528,8 @@ def main():
                     help="time the energy and analytic Hessian of the workload instead (FP64)")
     ap.add_argument("--ewald", action="store_true",
                     help="time the workload with the two-body term summed in reciprocal space instead (FP64)")
+    ap.add_argument("--partition", type=int, default=1,
+                    help="time the P parts of the reference's work partition, one after another (P > 1)")
     ap.add_argument("--ewald-mesh", type=int, default=-1,
                     help="with --ewald: < 0 sums the reciprocal vectors directly, >= 0 uses a particle mesh "
                          "(0: derived from the cutoff)")
@@ -516,6 +566,8 @@ def main():
         return bench_hessian(args, systems, config, spec, positions, counts)
     if args.ewald:
         return bench_ewald(args, systems, config, spec, positions, counts)
+    if args.partition > 1:
+        return bench_partition(args, systems, config, spec, positions, counts)
     ws = torch_api.workspace(dev)
     outs = {}
     res = {}

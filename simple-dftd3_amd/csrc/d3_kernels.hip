@@ -946,6 +946,8 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   __shared__ PairEntry<R> ring[kW][kQueue];
   __shared__ double red[kW];
   __shared__ long long next_item;
+  // the atoms of the A tile and of each wavefront's B tile (work partition only)
+  __shared__ int part_atom[kPart ? (kW + 1) * kTile : 1];
   const long long total = item_off[b.natoms];
   long long k_item = blockIdx.x;
   if (work_counter) {
@@ -986,6 +988,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   int jat = -1;  // atom of a-slot l
   if (w == 0) {
     jat = load_tile(tileA, slots, L0, P * kTile + l, m, l, i);
+    if (kPart) part_atom[l] = jat < 0 ? i : jat;
     for (int c = 0; c < kNAcc; ++c) accA[c * kTile + l] = 0.0;
   }
   for (int c = 0; c < kNAcc; ++c) accB[c * kTile + l] = 0.0;
@@ -1014,6 +1017,8 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
     double* __restrict__ accb = diag ? accA : accB;  // b-slots of a diagonal tile are a-slots
     int katl = -1;  // atom of b-slot l of this tile
     if (!diag) katl = load_tile(TB, slots, L0, Q * kTile + l, m, l, i);
+    if (kPart && !diag) part_atom[(w + 1) * kTile + l] = katl < 0 ? i : katl;
+    const int* __restrict__ batom = part_atom + (kPart && !diag ? (w + 1) * kTile : 0);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const int s0 = diag ? 1 : 0, s1 = diag ? kTile / 2 : kTile - 1;
@@ -1026,11 +1031,8 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
         const double Cd = diff_norm2(TB.vx[bl], TB.vy[bl], TB.vz[bl], ax, ay, az, v0, v1, v2);
         const double cut2_step = (diag && st == kTile / 2) ? cut2_half : cut2;
         bool ok = (Cd <= cut2_step) & (Cd >= kEps);
-        if (kPart) {  // atm.f90:709: the triple belongs to the pair (i, larger partner);
-                      // a slot's atom is the system's atom of its rank in the spatial order
-          const int ja = b.order[g.atom_begin + tileA.key[l].y], jb = b.order[g.atom_begin + TB.key[bl].y];
-          ok = ok && owns_pair(p, i - g.atom_begin, max(ja, jb) - g.atom_begin);
-        }
+        if (kPart)  // atm.f90:709: the triple belongs to the pair (i, larger partner)
+          ok = ok && owns_pair(p, i - g.atom_begin, max(part_atom[l], batom[bl]) - g.atom_begin);
         const unsigned long long bal = __ballot(ok);
         if (ok) {
           const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
