@@ -38,7 +38,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
 # rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
 # dispatch) scaled by a 256 MiB device copy (FETCH x2.00, WRITE x1.00);
 # commands and figures in profiles/atm_kernel__config1_periodic_2000.md.
-ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4>", "fetch_kib": 12_324_000, "write_kib": 6_878_000,
+ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4, false>", "fetch_kib": 12_324_000, "write_kib": 6_878_000,
                   "fetch_scale": 2.0, "write_scale": 1.0}
 
 
@@ -614,7 +614,7 @@ def main():
         mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"],
         mixed_vs_fp64_error=dict(energy_per_atom=de / natoms, max_gradient=dg, max_sigma=ds),
         kernel_ms=dict(fp64=fp64["kernel_ms"], mixed=mixed["kernel_ms"]),
-        roofline=dict(kernel="atm_kernel<float, true, false, 4>", bound="valu", achieved=mixed["atm_tflops"],
+        roofline=dict(kernel="atm_kernel<float, true, false, 4, false>", bound="valu", achieved=mixed["atm_tflops"],
                       peak=PEAK_TFLOPS["mixed"], unit="TFLOP/s", frac=mixed["atm_tflops"] / PEAK_TFLOPS["mixed"],
                       traffic=traffic, ops_per_triple=OPS_PER_TRIPLE, triples_per_launch=int(counts[2]),
                       launch_ms=mixed["atm_kernel_ms"], traffic_unit="bytes per launch (HBM read + write)"),
