@@ -270,7 +270,7 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
-      item_off, slots, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
+      item_off, slots, gw_rank, sp_rank, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
       ew_phi, ew_f, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
   // batched real <-> half-complex 3D transforms of the particle mesh (one per
   // species), kept while the mesh and the species count stay the same
@@ -660,6 +660,8 @@ DevWork Workspace::work(const Prepared& pb, bool mixed, bool grad, const BatchOu
   w.dedcn = debug && debug->dedcn ? debug->dedcn : dedcn.as<double>(natoms);
   w.grad = out.gradient ? out.gradient : this->grad.as<double>(size_t(natoms) * 3);
   w.sig_atom = sig.as<double>(size_t(natoms) * 6);
+  w.gwr = gw_rank.as<double>(size_t(natoms) * 2 * db.mref);
+  w.spr = sp_rank.as<int>(natoms);
   return w;
 }
 
@@ -686,7 +688,7 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
   const DevWork w = work(pb, mixed, grad, out, debug);
   timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
   timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, w.dgw, stream); });
-  timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.pair, stream); });
+  timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, w.dgw, w.gwr, w.spr, w.pair, stream); });
   timed("disp2", stream, [&] { launch_disp2(db, mixed, grad, w, nullptr, stream); });
   if (pb.ewald && pb.spme) {
     // fourier/spme.f90:78-328, term by term of the expansion
@@ -801,7 +803,7 @@ void Engine::run_pairwise(const std::vector<SystemDesc>& systems, const int* num
   const DevWork w = m.work(pb, mixed, false, BatchOutput{}, nullptr);
   m.timed("cn", stream, [&] { launch_cn(db, mixed, w.cn, stream); });
   m.timed("weights", stream, [&] { launch_weights(db, w.cn, w.gw, nullptr, stream); });
-  m.timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, nullptr, w.pair, stream); });
+  m.timed("c6", stream, [&] { launch_pair(db, mixed, w.gw, nullptr, w.gwr, w.spr, w.pair, stream); });
   m.timed("disp2", stream, [&] { launch_disp2(db, mixed, false, w, pair2, stream); });
   const size_t bytes = sizeof(double) * static_cast<size_t>(pb.pair_total);
   if (!pb.any_atm) {
@@ -958,7 +960,7 @@ Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, cons
         const DevWork w = s.ws.work(s.pb, s.mixed, grad, BatchOutput{}, nullptr);
         launch_cn(s.pb.db, s.mixed, w.cn, stream);
         launch_weights(s.pb.db, w.cn, w.gw, w.dgw, stream);
-        launch_pair(s.pb.db, s.mixed, w.gw, w.dgw, w.pair, stream);
+        launch_pair(s.pb.db, s.mixed, w.gw, w.dgw, w.gwr, w.spr, w.pair, stream);
         s.ws.list_offsets(s.pb, w.pair, s.mixed, stream);
         long long total = 0;
         hip_check(hipMemcpyAsync(&total, s.ws.off.as<long long>(natoms + 1) + natoms, sizeof total,

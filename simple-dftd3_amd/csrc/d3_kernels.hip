@@ -317,9 +317,31 @@ __global__ void weights_kernel(DevBatch b, const double* __restrict__ cn, double
 // (dC6_ij/dCN_j)/C6_ij, (4/3 rvdw_ij)^(alp/3)} (the last two feed the ATM
 // term, atm.f90:667-668,811-816).  One block per row i of the system block;
 // rows and columns are laid out in the spatial order of the atoms.
+// The weights, dw/dCN, species and ghost flags of the atoms at position x of
+// the batch's spatial order, reference-major: gwr[r][x], gwr[mref + r][x]
+// (with dw/dCN) and spr[x] = species | kGhostBit for a ghost atom, so that
+// the lanes of pair_kernel, which walk a row in that order, read consecutive
+// addresses.
+constexpr int kGhostBit = 1 << 20;
+__global__ __launch_bounds__(kBlock) void rank_weights_kernel(DevBatch b, const double* __restrict__ gw,
+                                                              const double* __restrict__ dgw,
+                                                              double* __restrict__ gwr, int* __restrict__ spr) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= b.natoms) return;
+  const int j = b.order[x];
+  const int mref = b.mref, nat = b.natoms;
+  for (int r = 0; r < mref; ++r) {
+    gwr[static_cast<long long>(r) * nat + x] = gw[j * mref + r];
+    if (dgw) gwr[static_cast<long long>(mref + r) * nat + x] = dgw[j * mref + r];
+  }
+  spr[x] = b.sp[j] | (b.ghost && b.ghost[j] ? kGhostBit : 0);
+}
+
 template <typename R>
 __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* __restrict__ gw,
-                                                      const double* __restrict__ dgw, R4<R>* __restrict__ pair) {
+                                                      const double* __restrict__ dgw,
+                                                      const double* __restrict__ gwr, const int* __restrict__ spr,
+                                                      R4<R>* __restrict__ pair) {
   const int i = blockIdx.x;
   const int s = b.sys[i];
   const SysGeom g = b.geom[s];
@@ -352,18 +374,20 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
   __syncthreads();
   const bool gi_ghost = b.ghost && b.ghost[i];
   R4<R>* row = pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n;
+  const long long nat = b.natoms;
   for (int jr = threadIdx.x; jr < n; jr += blockDim.x) {
-    const int j = b.order[g.atom_begin + jr];
-    const int spj = b.sp[j];
+    const int x = g.atom_begin + jr;  // position of partner j in the spatial order
+    const int code = spr[x];
+    const int spj = code & (kGhostBit - 1);
     const int nrj = b.nref[spj];
     double cc = 0.0, di = 0.0, dj = 0.0;
-    if (!(gi_ghost || (b.ghost && b.ghost[j]))) {  // model.f90:436-440
+    if (!(gi_ghost || (code & kGhostBit))) {  // model.f90:436-440
       for (int rj = 0; rj < nrj; ++rj) {
-        const double w = gw[j * mref + rj];
+        const double w = gwr[rj * nat + x];
         cc += w * h[spj][rj];
         if (dgw) {
           di += w * dh[spj][rj];
-          dj += dgw[j * mref + rj] * h[spj][rj];
+          dj += gwr[(mref + rj) * nat + x] * h[spj][rj];
         }
       }
     }
@@ -2810,12 +2834,15 @@ void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw
   hipLaunchKernelGGL(weights_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b, cn, gw, dgw);
 }
 
-void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, void* pair, hipStream_t st) {
+void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, double* gwr, int* spr,
+                 void* pair, hipStream_t st) {
+  hipLaunchKernelGGL(rank_weights_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b, gw, dgw, gwr,
+                     spr);
   if (mixed)
-    hipLaunchKernelGGL(pair_kernel<float>, dim3(b.natoms), dim3(kBlock), 0, st, b, gw, dgw,
+    hipLaunchKernelGGL(pair_kernel<float>, dim3(b.natoms), dim3(kBlock), 0, st, b, gw, dgw, gwr, spr,
                        static_cast<float4*>(pair));
   else
-    hipLaunchKernelGGL(pair_kernel<double>, dim3(b.natoms), dim3(kBlock), 0, st, b, gw, dgw,
+    hipLaunchKernelGGL(pair_kernel<double>, dim3(b.natoms), dim3(kBlock), 0, st, b, gw, dgw, gwr, spr,
                        static_cast<double4*>(pair));
 }
 

@@ -48,6 +48,8 @@ struct DevWork {
   double* dedcn;    // [N]
   double* grad;     // [N][3]
   double* sig_atom; // [N][6]
+  double* gwr;      // [2 mref][N] weights and dw/dCN in spatial order (pair records)
+  int* spr;         // [N] species | ghost flag in spatial order
 };
 
 // Size in bytes of one pair record and one three-body slot.
@@ -64,7 +66,10 @@ void launch_spatial_order(const DevBatch& b, void* temp, size_t temp_bytes, int*
                           hipStream_t stream);
 void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t stream);
 void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw, hipStream_t stream);
-void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, void* pair, hipStream_t stream);
+// gwr [(dgw ? 2 : 1) * mref][natoms] and spr [natoms]: scratch for the
+// weights in spatial order.
+void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, double* gwr, int* spr,
+                 void* pair, hipStream_t stream);
 // pair2 (energy only, may be null): per system an n x n block (offset
 // c6_begin, atom order) of pairwise two-body energies.
 void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, double* pair2, hipStream_t stream);
