@@ -270,7 +270,7 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
-      item_off, slots, gw_rank, sp_rank, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
+      item_off, slots, gw_rank, sp_rank, pos_rank, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
       ew_phi, ew_f, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
   // batched real <-> half-complex 3D transforms of the particle mesh (one per
   // species), kept while the mesh and the species count stay the same
@@ -327,6 +327,7 @@ struct Workspace {
         launch_spatial_order(pb.db, t, bytes, const_cast<int*>(pb.db.rank), const_cast<int*>(pb.db.order), stream);
       });
     }
+    timed("pack", stream, [&] { launch_rank_pos(pb.db, stream); });
   }
   DevWork work(const Prepared& pb, bool mixed, bool grad, const BatchOutput& out, const BatchDebug* debug);
   // count -> scan -> work items -> scan (three-body lists of every atom)
@@ -575,6 +576,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   db.ghost = ghost_host ? reinterpret_cast<const int*>(dmeta + o_ghost) : nullptr;
   db.rank = rank.as<int>(natoms);
   db.order = order.as<int>(natoms);
+  db.posr = pos_rank.as<double4>(natoms);
   pb.pair_total = pair_total;
   pb.any_atm = any_atm;
   pb.smooth = smooth;

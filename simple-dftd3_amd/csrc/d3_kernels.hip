@@ -613,7 +613,7 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
         if (!owns_pair(p, max(il, jl), min(il, jl))) continue;
       }
       double ej = 0.0;
-      const double4 xj = b.pos[j];
+      const double4 xj = b.posr[g.atom_begin + jr];  // = pos[j], read in spatial order
       const int spj = b.sp[j];
       const R rrij = R(3) * r4r2i * R(b.r4r2[spj]);
       const R4<R> q = row[jr];
@@ -744,7 +744,7 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
       q = row[r];
       valid = !(double(q.x) * double(q.x) < kEps);
     }
-    const double4 xj = valid ? b.pos[j] : xi;
+    const double4 xj = valid ? b.posr[g.atom_begin + r] : xi;  // = pos[j], read in spatial order
     if (valid) {
       for_images(g, kImg3, g.img3_count, xi, xj, [&](int t) {
         double v0, v1, v2;
@@ -2832,6 +2832,18 @@ void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t st) {
 
 void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw, hipStream_t st) {
   hipLaunchKernelGGL(weights_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b, cn, gw, dgw);
+}
+
+namespace {
+__global__ __launch_bounds__(kBlock) void rank_pos_kernel(DevBatch b, double4* __restrict__ posr) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < b.natoms) posr[x] = b.pos[b.order[x]];
+}
+}  // namespace
+
+void launch_rank_pos(const DevBatch& b, hipStream_t st) {
+  hipLaunchKernelGGL(rank_pos_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b,
+                     const_cast<double4*>(b.posr));
 }
 
 void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, double* gwr, int* spr,

@@ -35,6 +35,7 @@ struct DevBatch {
   // the pair records and of the three-body lists (no result depends on it):
   const int* rank;      // [N] position of the atom in its system's order
   const int* order;     // [N] atom at position atom_begin + r of that order
+  const double4* posr;  // [N] positions in that order, posr[x] = pos[order[x]] (refreshed with pos)
 };
 
 // Per-evaluation device arrays.
@@ -66,6 +67,8 @@ void launch_spatial_order(const DevBatch& b, void* temp, size_t temp_bytes, int*
                           hipStream_t stream);
 void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t stream);
 void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw, hipStream_t stream);
+// posr[x] = pos[order[x]] for every position x of the batch's spatial order.
+void launch_rank_pos(const DevBatch& b, hipStream_t stream);
 // gwr [(dgw ? 2 : 1) * mref][natoms] and spr [natoms]: scratch for the
 // weights in spatial order.
 void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, double* gwr, int* spr,
