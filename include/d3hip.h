@@ -30,6 +30,17 @@ typedef struct _dftd3_structure* dftd3_structure;
 typedef struct _dftd3_model* dftd3_model;
 typedef struct _dftd3_param* dftd3_param;
 
+/* s-dftd3.h:56-64: type-generic release of a handle (C11 _Generic), for the
+ * handle types of this library (the reference's gcp handle is not part of it). */
+#ifndef __cplusplus
+#define dftd3_delete(ptr) _Generic((ptr), \
+                       dftd3_error: dftd3_delete_error, \
+                   dftd3_structure: dftd3_delete_structure, \
+                       dftd3_model: dftd3_delete_model, \
+                       dftd3_param: dftd3_delete_param \
+                                  )(&ptr)
+#endif
+
 /* ------------------------------------------------------------------ Part 1 */
 
 /* s-dftd3.h:71-72 dftd3_get_version: major*10000 + minor*100 + patch of the

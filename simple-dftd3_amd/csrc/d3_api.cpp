@@ -120,9 +120,10 @@ void wrap_to_central_cell(int nat, double* xyz, const double* lat, const int* pe
   }
 }
 
-// api.f90:1517-1530 verify_structure, with a cell grid so that large
-// structures stay O(N): only pairs in neighbouring cells of edge >= 1e-9 can be
-// closer than 1e-9 bohr.
+// api.f90:1517-1530 verify_structure.  Small structures test every pair;
+// larger ones sort the atoms by x and sweep: only atoms whose x coordinates
+// differ by less than 1e-9 bohr can be closer than that, so the test is
+// O(N log N) for any structure without many atoms in one thin slab.
 bool too_close(int nat, const double* xyz) {
   const double thr = 1.0e-9;
   if (nat < 2) return false;

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -357,6 +358,12 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   int natoms = 0;
   for (const auto& s : systems) {
     if (s.natoms <= 0) throw Error("Invalid molecular structure data provided");
+    // pair-record indices of the three-body kernel hold rank_j n + rank_k in
+    // 47 bits (n up to 11.8 million atoms, far beyond device memory)
+    if (static_cast<long long>(s.natoms) * s.natoms >= (1LL << 47))
+      throw Error("Molecular structure too large: " + std::to_string(s.natoms) + " atoms");
+    if (natoms > std::numeric_limits<int>::max() - s.natoms)
+      throw Error("Too many atoms in one batch");
     natoms += s.natoms;
   }
   const auto& tab = element_tables();
@@ -764,7 +771,7 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
       s = slots.get(std::max<long long>(total, 1) * slot_record_bytes(mixed));
     } else {
       s = slots.get(std::max<long long>(capacity, 1) * slot_record_bytes(mixed));
-      launch_atm_guard(o, natoms, capacity, io, flag.as<int>(1), stream);
+      launch_atm_guard(db, o, capacity, io, flag.as<int>(1), w, grad, stream);
     }
     timed("atm_list", stream, [&] { launch_atm_fill(db, w.pair, mixed, o, s, capacity, stream); });
     unsigned long long* wc = shape.blocks_per_cu > 0 ? counter.as<unsigned long long>(1) : nullptr;

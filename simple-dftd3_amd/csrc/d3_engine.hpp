@@ -174,7 +174,9 @@ class Engine {
 // per molecule; a periodic system gets 1.5 times the list size of the
 // coordinates the plan is made with (and at least 64 more slots per atom),
 // capped by the exact bound.  An evaluation whose lists do not fit adds no
-// three-body terms and raises the overflow flag that overflowed() reports.
+// three-body terms, returns NaN energies, gradients and virials for every
+// system with a three-body term, and raises the overflow flag that
+// overflowed() reports.
 class Plan {
  public:
   Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, const double* positions_dev,

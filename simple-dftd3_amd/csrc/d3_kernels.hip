@@ -828,18 +828,25 @@ struct TileSlots {
   int2 key[kTile];                          // {queue tag of slot l as b, rank}: tag = l << 8, plus 1 << 16
                                             // when the slot is an image of atom i
 };
-// Accepted slot pair (a, b) of a tile pair.
+// Accepted slot pair (a, b) of a tile pair.  The index of the (j, k) pair
+// record within the system (rank_j n + rank_k, below 2^47: the host refuses
+// larger systems) is split over the two words:
+//   lo = a | b << 8 | (both slots are images of atom i) << 16 | rec[0:15) << 17
+//   hi = rec >> 15
 template <typename R>
 struct PairEntry {
-  unsigned ab;   // a | b << 8 | (both slots are images of atom i) << 16
-  unsigned rec;  // index of the (j, k) pair record within the system
-  R C;           // r_jk^2
+  unsigned lo, hi;
+  R C;  // r_jk^2
 };
 template <>
 struct PairEntry<double> {
-  unsigned ab, rec;
+  unsigned lo, hi;
   double C;
 };
+constexpr int kRecLoBits = 15;
+__device__ __forceinline__ unsigned long long entry_rec(unsigned lo, unsigned hi) {
+  return static_cast<unsigned long long>(lo >> 17) | (static_cast<unsigned long long>(hi) << kRecLoBits);
+}
 
 // Per-slot sums of a tile pair: G = sum dE/dv_s = sum (f_is + f_jk) v_s -
 // f_jk v_partner (three components), D = sum dE (qx_s + q_partner-side).
@@ -1025,7 +1032,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   const double cut2_half = l < kTile / 2 ? cut2 : -1.0;
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
   const int2 akey = tileA.key[l];
-  const unsigned arow = aval ? static_cast<unsigned>(akey.y) * static_cast<unsigned>(n) : 0u;
+  const unsigned long long arow = aval ? static_cast<unsigned long long>(akey.y) * static_cast<unsigned>(n) : 0ull;
   // entry tag of the b-slot, keeping the self flag only when a is also an image of i
   const unsigned amask = (akey.x & (1 << 16)) ? 0x1FF00u : 0xFF00u;
   const long long irow = kPairs ? static_cast<long long>(b.rank[i]) * n : 0;  // pairs {i, s} in pair3s
@@ -1064,8 +1071,9 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
           const int2 kb = TB.key[bl];
           PairEntry<R> e;
-          e.ab = (static_cast<unsigned>(kb.x) & amask) | static_cast<unsigned>(l);
-          e.rec = arow + static_cast<unsigned>(kb.y);
+          const unsigned long long rec = arow + static_cast<unsigned>(kb.y);
+          e.lo = (static_cast<unsigned>(kb.x) & amask) | static_cast<unsigned>(l) | (static_cast<unsigned>(rec) << 17);
+          e.hi = static_cast<unsigned>(rec >> kRecLoBits);
           e.C = R(Cd);
           q_ring[pos & (kQueue - 1)] = e;
         }
@@ -1078,8 +1086,9 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       for (; tail - head >= kWave || (done && tail > head); head += kWave) {
         if (head + l < tail) {
           const PairEntry<R> e = q_ring[(head + l) & (kQueue - 1)];
-          const R4<R> q = prow[e.rec];
-          const int a = e.ab & 0xff, bb = (e.ab >> 8) & 0xff;
+          const unsigned long long rec = entry_rec(e.lo, e.hi);
+          const R4<R> q = prow[rec];
+          const int a = e.lo & 0xff, bb = (e.lo >> 8) & 0xff;
           const R C = e.C;
           const R4<R> a0 = tileA.g0[a], a1 = tileA.g1[a];
           const R4<R> b0 = TB.g0[bb], b1 = TB.g1[bb];
@@ -1098,7 +1107,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
           const R pp = a1.x * b1.x * (q.w * pow_pos(C, malp6));
           const R fd = rcp_fast(R(1) + R(6) * pp);
           R c9 = ns9 * a1.y * b1.y * q.x;
-          if (e.ab >> 16) c9 *= R(4.0 / 3.0);  // j == k == i
+          if ((e.lo >> 16) & 1u) c9 *= R(4.0 / 3.0);  // j == k == i
           const R cwf = c9 * fd;
           R dE = ang * cwf;
           R sw = R(1), swa = R(1), dswa = R(0), swb = R(1), dswb = R(0), swc = R(1), dswc = R(0);
@@ -1146,7 +1155,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
             const double t = double(dE) * (-1.0 / 6.0);
             atomicAdd(&accA[a], t);
             atomicAdd(&accb[bb], t);
-            atomicAdd(pair3s + g.c6_begin + e.rec, t);
+            atomicAdd(pair3s + g.c6_begin + rec, t);
           }
         }
       }
@@ -2904,18 +2913,38 @@ void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long
 }
 
 namespace {
-__global__ void atm_guard_kernel(const long long* __restrict__ offset, int n, long long capacity,
-                                 long long* __restrict__ item_off, int* __restrict__ overflow) {
-  if (offset[n] > capacity) {
+// One thread per atom.  When the lists hold more than `capacity` slots no
+// list is filled: the flag is raised, the work-item range is emptied, and the
+// outputs of every atom of a system with a three-body term become NaN (its
+// energy, gradient, dE/dCN -- which carries the NaN through the chain rule --
+// and virial), so a replayed evaluation cannot return two-body-only results
+// as if they were complete.  Systems without a three-body term keep theirs.
+__global__ void atm_guard_kernel(DevBatch b, const long long* __restrict__ offset, long long capacity,
+                                 long long* __restrict__ item_off, int* __restrict__ overflow,
+                                 double* __restrict__ e_atom, double* __restrict__ dedcn, double* __restrict__ grad,
+                                 double* __restrict__ sig_atom) {
+  if (offset[b.natoms] <= capacity) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
     *overflow = 1;
-    item_off[n] = 0;
+    item_off[b.natoms] = 0;
   }
+  if (i >= b.natoms || fabs(b.par[b.sys[i]].s9) < kEps) return;
+  const double nan = __builtin_nan("");
+  e_atom[i] = nan;
+  if (dedcn) dedcn[i] = nan;
+  if (grad) grad[3 * i] = grad[3 * i + 1] = grad[3 * i + 2] = nan;
+  if (sig_atom)
+    for (int c = 0; c < 6; ++c) sig_atom[6 * i + c] = nan;
 }
+
 }  // namespace
 
-void launch_atm_guard(const long long* offset, int n, long long capacity, long long* item_off, int* overflow,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(atm_guard_kernel, dim3(1), dim3(1), 0, st, offset, n, capacity, item_off, overflow);
+void launch_atm_guard(const DevBatch& b, const long long* offset, long long capacity, long long* item_off,
+                      int* overflow, const DevWork& w, bool grad, hipStream_t st) {
+  hipLaunchKernelGGL(atm_guard_kernel, dim3(blocks_for(std::max(b.natoms, 1), kBlock)), dim3(kBlock), 0, st, b,
+                     offset, capacity, item_off, overflow, w.e_atom, grad ? w.dedcn : nullptr,
+                     grad ? w.grad : nullptr, grad ? w.sig_atom : nullptr);
 }
 
 void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t st) {

@@ -82,10 +82,13 @@ void launch_scan(const long long* count, int n, long long* offset, hipStream_t s
 // Writes nothing when the lists hold more than `capacity` slots.
 void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long long* offset, void* slots,
                      long long capacity, hipStream_t stream);
-// When the lists hold more than `capacity` slots: sets *overflow = 1 and
-// empties the work-item range (item_off[n] = 0), so no triple is evaluated.
-void launch_atm_guard(const long long* offset, int n, long long capacity, long long* item_off, int* overflow,
-                      hipStream_t stream);
+// When the lists hold more than `capacity` slots: sets *overflow = 1,
+// empties the work-item range (item_off[natoms] = 0), so no triple is
+// evaluated, and writes NaN into the energy, dE/dCN, gradient and virial
+// accumulators of w (the last three with grad) of every atom of a system with
+// a three-body term.  Runs after the two-body term.
+void launch_atm_guard(const DevBatch& b, const long long* offset, long long capacity, long long* item_off,
+                      int* overflow, const DevWork& w, bool grad, hipStream_t stream);
 // Work items (atom i, slot block P) of the tiled three-body kernel.
 void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t stream);
 

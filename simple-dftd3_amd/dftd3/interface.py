@@ -23,8 +23,8 @@ this library implements:
 All evaluation happens on the GPU through libd3hip.so.  Additions beyond the
 reference: ``DispersionModel.set_precision("fp64" | "mixed")`` and
 ``get_lowrank()``.  The counterpoise correction (GCP) of the reference is not
-part of this library; its classes exist so that imports keep working, and
-raise ``NotImplementedError`` when used.
+part of this library; its class ``GeometricCounterpoise`` exists so that
+imports keep working, and raises ``NotImplementedError`` when constructed.
 """
 from typing import Optional
 
@@ -343,3 +343,27 @@ def _rename_kwargs(kwargs, old_name, new_name):
     if old_name in kwargs and new_name not in kwargs:
         kwargs[new_name] = kwargs[old_name]
         del kwargs[old_name]
+
+
+class GeometricCounterpoise(Structure):
+    """Counterpoise correction (interface.py:661-724 of the reference).
+
+    The geometric counterpoise correction (gCP) is not part of this library:
+    the class exists with the reference's constructor signature and methods
+    so that code importing it keeps working, and constructing it raises
+    ``NotImplementedError``.
+    """
+
+    def __init__(self, numbers: np.ndarray, positions: np.ndarray, lattice: Optional[np.ndarray] = None,
+                 periodic: Optional[np.ndarray] = None, method: Optional[str] = None,
+                 basis: Optional[str] = None):
+        raise NotImplementedError("The geometric counterpoise correction is not implemented by this library")
+
+    def set_realspace_cutoff(self, bas: float, srb: float):
+        raise NotImplementedError("The geometric counterpoise correction is not implemented by this library")
+
+    def set_work_partition(self, part: int, nparts: int) -> None:
+        raise NotImplementedError("The geometric counterpoise correction is not implemented by this library")
+
+    def get_counterpoise(self, grad: bool) -> dict:
+        raise NotImplementedError("The geometric counterpoise correction is not implemented by this library")

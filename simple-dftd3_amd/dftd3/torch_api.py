@@ -301,6 +301,8 @@ class BatchPlan:
 
     def overflowed(self) -> bool:
         """True when an evaluation had more three-body slots than the plan
-        holds (its three-body terms were skipped).  Synchronizes."""
+        holds.  Such an evaluation skips the three-body terms and returns NaN
+        energies, gradients and virials for every system with a three-body
+        term (systems without one are unaffected).  Synchronizes."""
         with torch.cuda.device(self.device):
             return library.plan_overflowed(self.handle, self._stream())

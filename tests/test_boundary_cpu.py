@@ -295,3 +295,58 @@ def test_work_partition_and_mpi_handles(numbers, positions):
     L.lib.dftd3_get_error(err.handle, buf, None)
     assert buf.value.decode() == "s-dftd3 was built without MPI support"
     assert not L.has_feature("mpi")
+
+
+def test_system_size_guard_before_any_gpu_work():
+    """The pair records of the three-body kernel are indexed by rank_j n +
+    rank_k in 47 bits: a system of 2^23.5 atoms or more (far beyond device
+    memory) is refused with a clear message before anything is read or
+    allocated, instead of wrapping the index."""
+    from dftd3 import library as L
+    from dftd3 import torch_api as T
+    ws = L.new_workspace()
+    row = np.ascontiguousarray([T.param_row("rational", dict(s8=1.2177, a1=0.4145, a2=4.8593))])
+    dummy = ctypes.c_double(0.0)
+    for natoms, msg in (([11_900_000], "Molecular structure too large: 11900000 atoms"),
+                        ([11_000_000] * 200, "Too many atoms in one batch")):
+        nat = np.ascontiguousarray(natoms, dtype=np.int32)
+        params = np.ascontiguousarray(np.repeat(row, len(natoms), axis=0))
+        numbers = np.ones(4, dtype=np.int32)  # never read: the sizes are checked first
+        with pytest.raises(RuntimeError, match=msg):
+            L.get_dispersion_batch(ws, nat, numbers, None, None, params, None, L.PRECISION_MIXED,
+                                   ctypes.addressof(dummy), ctypes.addressof(dummy), None, None, None)
+
+
+def test_geometric_counterpoise_is_importable_and_refused(numbers, positions):
+    """interface.py:661 of the reference: the class exists; gCP itself is not
+    part of this library, so using it raises NotImplementedError."""
+    from dftd3.interface import GeometricCounterpoise
+    with pytest.raises(NotImplementedError):
+        GeometricCounterpoise(numbers, positions, method="b973c")
+
+
+def test_header_generic_delete_macro_compiles(tmp_path):
+    """s-dftd3.h:56-64: C code that releases handles with the type-generic
+    dftd3_delete() compiles against include/d3hip.h and links."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "use_delete.c"
+    src.write_text(
+        '#include "d3hip.h"\n'
+        "int main(void) {\n"
+        "  dftd3_error err = dftd3_new_error();\n"
+        "  dftd3_param par = dftd3_new_rational_damping(err, 1.0, 1.2177, 1.0, 0.4145, 4.8593, 14.0);\n"
+        "  dftd3_delete(par);\n"
+        "  int ok = par == 0;\n"
+        "  dftd3_delete(err);\n"
+        "  return ok && err == 0 ? 0 : 1;\n"
+        "}\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "simple-dftd3_amd", "dftd3")
+    exe = tmp_path / "use_delete"
+    subprocess.run([cc, "-std=c11", "-Wall", "-Werror", "-I", os.path.join(root, "include"), str(src), "-o",
+                    str(exe), "-L", libdir, "-l:libd3hip.so", "-Wl,-rpath," + libdir], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0

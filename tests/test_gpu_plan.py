@@ -144,7 +144,8 @@ def test_graph_replays_follow_new_coordinates(grad):
 def test_slot_capacity_guard():
     """A 2D slab planned with its atoms far apart along the non-periodic axis
     gets lists sized for that geometry; compressing the slab overflows them:
-    the plan reports it and adds no three-body terms."""
+    the plan reports it, and the outputs of the system with a three-body
+    term are NaN while a system without one in the same batch is exact."""
     from dftd3 import torch_api
     _step("slot capacity guard")
     num = np.array([6, 7, 8, 16, 1, 6])
@@ -154,17 +155,31 @@ def test_slot_capacity_guard():
     packed = spread.copy()
     packed[:, 2] = 2.5 * np.arange(6)
     sysd = dict(numbers=num, positions=spread, lattice=lat, periodic=[True, True, False], param=BJ)
-    spec, pos = _spec([sysd])
+    # second system of the batch: the same slab without the three-body term
+    sys2 = dict(sysd, param=dict(BJ, s9=0.0))
+    spec, pos = _spec([sysd, sys2])
     plan = torch_api.BatchPlan(spec, pos, precision="fp64")
-    plan.execute()
+    out = plan.execute()
+    torch.cuda.synchronize()
     assert not plan.overflowed()
-    packed_t = torch.as_tensor(packed, dtype=torch.float64, device="cuda").contiguous()
+    assert torch.isfinite(out["energy"]).all() and torch.isfinite(out["gradient"]).all()
+    packed_pos = np.concatenate([packed, packed])
+    packed_t = torch.as_tensor(packed_pos, dtype=torch.float64, device="cuda").contiguous()
     out = plan.execute(packed_t)
     torch.cuda.synchronize()
     assert plan.overflowed()
-    full = torch_api.run_batch(spec, packed_t, precision="fp64")
-    spec0, _ = _spec([dict(sysd, positions=packed, param=dict(BJ, s9=0.0))])
-    two_body = torch_api.run_batch(spec0, packed_t, precision="fp64")
-    e2 = two_body["energy"][0].item()
-    assert abs(out["energy"][0].item() - e2) <= 1e-12 * abs(e2)
-    assert abs(full["energy"][0].item() - e2) > 1e-9
+    # the three-body system: every output NaN
+    assert torch.isnan(out["energy"][0]).item()
+    assert torch.isnan(out["gradient"][:6]).all().item()
+    assert torch.isnan(out["sigma"][0]).all().item()
+    # the two-body-only system: exact
+    ref = torch_api.run_batch(spec, packed_t, precision="fp64")
+    assert torch.isfinite(ref["energy"]).all()
+    e2 = ref["energy"][1].item()
+    assert abs(out["energy"][1].item() - e2) <= 1e-12 * abs(e2)
+    assert torch.allclose(out["gradient"][6:], ref["gradient"][6:], rtol=0, atol=1e-13)
+    # a later evaluation that fits is complete again; the flag stays raised
+    out = plan.execute(torch.as_tensor(np.concatenate([spread, spread]), dtype=torch.float64,
+                                       device="cuda").contiguous())
+    torch.cuda.synchronize()
+    assert torch.isfinite(out["energy"]).all() and plan.overflowed()
