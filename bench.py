@@ -34,12 +34,16 @@ OPS_PER_PAIR_BJ = 63    # two-body BJ E+grad+sigma (rational.f90:462-497)
 OPS_PER_TRIPLE_E = 60   # ATM energy only (atm.f90:500-621), the arithmetic of a pairwise triple term
 KERNEL_GROUPS = ("pack", "order", "cn", "weights", "c6", "disp2", "atm_list", "atm", "cn_chain", "reduce")
 PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
-# HBM bytes per launch of the mixed ATM kernel on config 1, measured by two
-# rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
-# dispatch) scaled by a 256 MiB device copy (FETCH x2.00, WRITE x1.00);
-# commands and figures in profiles/atm_kernel__config1_periodic_2000.md.
+# HBM bytes per launch of the mixed ATM kernel on the 2,000-atom cell,
+# measured by two rocprofv3 --pmc runs of this script (FETCH_SIZE, then
+# WRITE_SIZE, KiB per dispatch) scaled by a 256 MiB device copy (FETCH x2.00,
+# WRITE x1.00); commands and figures in
+# profiles/atm_kernel__config1_periodic_2000.md.  "kernel_sha" is the SHA-256
+# (first 16 hex digits) of csrc/d3_kernels.hip the counters were read on: a
+# report on other kernel sources gives traffic = null.
 ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4, false>", "fetch_kib": 12_324_000, "write_kib": 6_878_000,
-                  "fetch_scale": 2.0, "write_scale": 1.0}
+                  "fetch_scale": 2.0, "write_scale": 1.0, "kernel_sha": None}
+KERNEL_SOURCE = os.path.join(ROOT, "simple-dftd3_amd", "csrc", "d3_kernels.hip")
 
 
 def log(msg):
@@ -90,26 +94,62 @@ def timed_steps(run, steps, sync):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(systems, counts, stride, threads):
-    """Oracle (FP64 C, OpenMP) on every stride-th outer atom of each loop."""
+def host_cpu():
+    """Model name, logical CPUs and physical cores of the host, and the CPUs
+    this process may run on."""
+    model, logical, cores = "unknown", 0, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name":
+                    model = val
+                    logical += 1
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id":
+                    core = val
+                elif not key and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+    except OSError:
+        pass
+    return dict(model=model, logical_cpus=logical or os.cpu_count(), physical_cores=len(cores) or None,
+                usable_cpus=len(os.sched_getaffinity(0)))
+
+
+def cpu_baseline(systems, counts, stride, threads, reps=3):
+    """Oracle (FP64 C, OpenMP) on every stride-th outer atom of each loop:
+    the median of `reps` timings on `threads` threads, and one timing on one
+    thread of a ten times sparser sample."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     s = systems[0]
     args_ = (s["numbers"], s["positions"], s.get("lattice"), s.get("periodic"), oracle.RATIONAL, s["param"])
-    t = oracle.time_sampled_dispersion(stride, *args_)
+    oracle.set_num_threads(threads)
+    times = sorted(oracle.time_sampled_dispersion(stride, *args_) for _ in range(reps))
+    t = times[len(times) // 2]
     t_full = t * stride
     cores = oracle.num_threads()
-    # one thread on a ten times sparser sample
     oracle.set_num_threads(1)
     t1 = oracle.time_sampled_dispersion(stride * 10, *args_) * stride * 10
     oracle.set_num_threads(cores)
     return dict(value=float(counts[1] / t_full), unit="pair-terms/s", cores=cores, kind="port",
-                sample=f"oracle E+grad+sigma with the outer atom loops restricted to every {stride}-th atom "
-                       f"({t:.2f} s), scaled by {stride} to one full evaluation ({t_full:.1f} s)",
-                seconds_per_eval_estimate=t_full,
+                sample=f"oracle E+grad+sigma with the outer atom loops restricted to every {stride}-th atom, "
+                       f"median of {reps} runs ({', '.join(f'{x:.2f}' for x in times)} s), scaled by {stride} "
+                       f"to one full evaluation ({t_full:.1f} s)",
+                seconds_per_eval_estimate=t_full, host=host_cpu(),
                 one_thread=dict(value=float(counts[1] / t1), seconds_per_eval_estimate=t1,
                                 sample=f"every {stride * 10}-th outer atom, scaled"))
+
+
+def source_sha(path):
+    import hashlib
+    with open(path, "rb") as fh:
+        return hashlib.sha256(fh.read()).hexdigest()[:16]
 
 
 def bench_batched(args, systems, config, spec, positions, counts):
@@ -186,8 +226,7 @@ def bench_batched(args, systems, config, spec, positions, counts):
         fp64_batched_vs_per_molecule_energy_per_atom=float(d_single),
         roofline=None,
     )
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 def bench_autograd(args, systems, config, spec, positions, counts):
@@ -518,10 +557,14 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=10)
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one device copy of this size first (reference kernel for FETCH/WRITE_SIZE counters)")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=len(os.sched_getaffinity(0)),
+                    help="threads of the CPU baseline (default: every CPU this process may use)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--oracle-check", action="store_true",
-                    help="also evaluate the workload once with the FP64 oracle and report both paths' errors")
+    ap.add_argument("--no-oracle-check", action="store_true",
+                    help="skip the errors of both paths against one full FP64 oracle evaluation (default "
+                         "workload only)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="report only the 2,000-atom cell (no 20,000-atom ellipsoid and 4,096-molecule batch keys)")
     ap.add_argument("--pairwise", action="store_true",
                     help="time the pairwise-resolved energies (pair energy matrices) of the workload instead")
     ap.add_argument("--hessian", action="store_true",
@@ -556,8 +599,6 @@ def main():
     positions = torch.as_tensor(pos, dtype=torch.float64, device=dev).contiguous()
     counts = torch_api.count_terms(spec, positions).sum(axis=0)
     log(f"{config['workload']}: CN terms {counts[0]}, pair terms {counts[1]}, triple terms {counts[2]}")
-    if args.config == "c3":
-        return bench_batched(args, systems, config, spec, positions, counts)
     if args.config == "c4":
         return bench_autograd(args, systems, config, spec, positions, counts)
     if args.pairwise:
@@ -568,7 +609,54 @@ def main():
         return bench_ewald(args, systems, config, spec, positions, counts)
     if args.partition > 1:
         return bench_partition(args, systems, config, spec, positions, counts)
-    ws = torch_api.workspace(dev)
+    if args.config == "c3":
+        print(json.dumps(bench_batched(args, systems, config, spec, positions, counts)), flush=True)
+        return 0
+    line = bench_single(args, systems, config, spec, positions, counts,
+                        cpu=not args.no_cpu_baseline, check=args.config == "c1" and not args.no_oracle_check)
+    if args.config == "c1" and not args.no_extra:
+        line["config2"] = extra_config(args, "c2")
+        line["config3"] = extra_config(args, "c3")
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def extra_config(args, name):
+    """The 20,000-atom ellipsoid (single system, both paths, ATM roofline) or
+    the 4,096-molecule batch (planned, graph-replayed) as a summary for the
+    default report line."""
+    import torch
+    from dftd3 import torch_api
+    systems, config = workload(name)
+    spec, pos = torch_api.batch_spec(systems)
+    positions = torch.as_tensor(pos, dtype=torch.float64, device="cuda:0").contiguous()
+    counts = torch_api.count_terms(spec, positions).sum(axis=0)
+    log(f"{config['workload']}: CN terms {counts[0]}, pair terms {counts[1]}, triple terms {counts[2]}")
+    if name == "c3":
+        b = bench_batched(args, systems, config, spec, positions, counts)
+        keep = ("ms_per_step", "molecules_per_s", "triple_terms_per_s", "fp64_batched", "fp64_per_molecule",
+                "mixed_batched_over_fp64_per_molecule", "mixed_over_fp64", "mixed_shape", "mixed_vs_fp64_error",
+                "fp64_batched_vs_per_molecule_energy_per_atom")
+        out = dict(workload=config["workload"], unit="molecules/s", value=b["molecules_per_s"],
+                   pair_terms_per_s=b["value"], **{k: b[k] for k in keep})
+        out["config"] = b["config"]
+        return out
+    b = bench_single(args, systems, config, spec, positions, counts, cpu=False, check=False)
+    keep = ("ms_per_step", "molecules_per_s", "triple_terms_per_s", "fp64", "mixed_over_fp64", "mixed_vs_fp64_error",
+            "roofline")
+    out = dict(workload=config["workload"], unit="pair-terms/s", value=b["value"], **{k: b[k] for k in keep})
+    out["config"] = b["config"]
+    return out
+
+
+def bench_single(args, systems, config, spec, positions, counts, cpu, check):
+    """One system (the 2,000-atom cell or the 20,000-atom ellipsoid): both
+    paths timed per evaluation and per kernel (HIP events on the launch
+    stream), the ATM roofline, and optionally the CPU baseline and the
+    errors against one full oracle evaluation."""
+    import torch
+    from dftd3 import library, torch_api
+    ws = torch_api.workspace(positions.device)
     outs = {}
     res = {}
     for prec in ("fp64", "mixed"):
@@ -596,10 +684,14 @@ def main():
     dg = (outs["mixed"]["gradient"] - outs["fp64"]["gradient"]).abs().max().item()
     ds = (outs["mixed"]["sigma"] - outs["fp64"]["sigma"]).abs().max().item()
     mixed, fp64 = res["mixed"], res["fp64"]
-    traffic = None
-    if args.config == "c1":
+    traffic, traffic_note = None, None
+    if config["workload"].startswith("config1"):
         t = ATM_TRAFFIC_C1
-        traffic = (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
+        if t["kernel_sha"] == source_sha(KERNEL_SOURCE):
+            traffic = (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"]) * 1024.0
+            traffic_note = f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on d3_kernels.hip sha256 {t['kernel_sha']}"
+        else:
+            traffic_note = "not measured on the current kernel source"
     line = dict(
         metric=METRIC, value=mixed["pair_terms_per_s"], unit="pair-terms/s", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=mixed["ms_per_step"], higher_is_better=True, scaling="weak",
@@ -617,15 +709,17 @@ def main():
         roofline=dict(kernel="atm_kernel<float, true, false, 4, false>", bound="valu", achieved=mixed["atm_tflops"],
                       peak=PEAK_TFLOPS["mixed"], unit="TFLOP/s", frac=mixed["atm_tflops"] / PEAK_TFLOPS["mixed"],
                       traffic=traffic, ops_per_triple=OPS_PER_TRIPLE, triples_per_launch=int(counts[2]),
-                      launch_ms=mixed["atm_kernel_ms"], traffic_unit="bytes per launch (HBM read + write)"),
+                      launch_ms=mixed["atm_kernel_ms"], traffic_unit="bytes per launch (HBM read + write)",
+                      traffic_source=traffic_note, kernel_sha=source_sha(KERNEL_SOURCE)),
     )
-    if not args.no_cpu_baseline:
-        log(f"cpu baseline: oracle on every {args.cpu_stride}-th atom, {args.cpu_threads} threads")
+    if cpu:
+        log(f"cpu baseline: oracle on every {args.cpu_stride}-th atom, {args.cpu_threads} threads, median of 3")
         line["cpu_baseline"] = cpu_baseline(systems, counts, args.cpu_stride, args.cpu_threads)
-    if args.oracle_check:
+    if check:
         line["error_vs_oracle"] = oracle_errors(systems, outs, args.cpu_threads)
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
+
+
 
 
 def oracle_errors(systems, outs, threads):

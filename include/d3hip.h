@@ -273,8 +273,10 @@ D3HIP_API void d3hip_get_kernel_time(dftd3_error error, dftd3_model model, const
  * graph (d3hip_plan_capture) and replayed (d3hip_plan_replay).
  * Three-body lists: molecules get the exact bound; periodic systems get 1.5x
  * the list size at the planning coordinates (at least 64 more slots per
- * atom).  An evaluation whose lists do not fit adds no three-body terms and
- * sets the flag that d3hip_plan_overflowed reports. */
+ * atom of a system with a three-body term).  An evaluation whose lists do
+ * not fit adds no three-body terms, writes NaN into the energy, gradient and
+ * virial of every system with a three-body term, and sets the flag that
+ * d3hip_plan_overflowed reports. */
 typedef struct _d3hip_plan* d3hip_plan;
 
 /* Same batch description as d3hip_get_dispersion_batch; grad != 0 plans

@@ -10,7 +10,7 @@ namespace d3 {
 constexpr int kMaxElem = 103;   // elements with reference data (reference.f90:27)
 constexpr int kMaxRef = 7;      // reference systems per element (reference.f90:28)
 constexpr int kNumPairs = kMaxElem * (kMaxElem + 1) / 2;
-constexpr int kMaxSpecies = 24; // distinct elements in one batch (LDS tables)
+constexpr int kMaxSpecies = kMaxElem; // distinct elements in one batch: any set of elements with reference data
 
 // Damping families of the two-body term (the three-body term is the same
 // for all): rational (Becke-Johnson; the modified rational parameters use it
@@ -43,7 +43,7 @@ struct FourierTerm {
   double pref, radius;
   int m, alpha;
 };
-constexpr int kMaxEwaldSpecies = 8;  // distinct elements of a reciprocal-space system
+constexpr int kMaxEwaldSpecies = 8;  // species whose structure factors the reciprocal-space kernels keep in registers (more: a generic instantiation)
 
 // Atom range and lattice-image ranges of one system of a batch.  Images are
 // stored per cutoff (CN, two-body, three-body) exactly as get_lattice_points

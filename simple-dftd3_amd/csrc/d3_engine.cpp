@@ -157,6 +157,7 @@ struct Prepared {
   bool periodic_atm = false;   // some three-body system has lattice images
   bool partitioned = false;    // some system evaluates one part of a work partition
   long long slot_bound = 0;    // sum over three-body systems of (il + 1) * images
+  long long atm_atoms = 0;     // atoms of the systems with a three-body term
   long long item_bound = 0;    // work items of those bounds
   bool ewald = false;          // the (single) system sums its two-body term in reciprocal space
   EwaldBatch eb{};             // its vectors, expansion and species chunks (nk = 0: no two-body term)
@@ -272,7 +273,7 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
       item_off, slots, gw_rank, sp_rank, pos_rank, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
-      ew_phi, ew_f, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
+      ew_phi, ew_f, ew_s, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
   // batched real <-> half-complex 3D transforms of the particle mesh (one per
   // species), kept while the mesh and the species count stay the same
   hipfftHandle fft_fwd = 0, fft_inv = 0;
@@ -430,7 +431,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   std::vector<SysGeom> geom(nsys);
   std::vector<SysParam> par(nsys);
   std::vector<double4> img;
-  long long pair_total = 0, slot_bound = 0, item_bound = 0;
+  long long pair_total = 0, slot_bound = 0, item_bound = 0, atm_atoms = 0;
   bool any_atm = false, smooth = false, periodic_atm = false;
   for (int k = 0, a = 0; k < nsys; ++k) {
     const SystemDesc& sd = systems[k];
@@ -466,6 +467,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
       any_atm = true;
       smooth |= sd.param.width3 > 0.0;
       if (g.img3_count > 1) periodic_atm = true;
+      atm_atoms += sd.natoms;
       // slot j <= i of atom i: at most (il + 1) images each
       for (long long il = 0; il < sd.natoms; ++il) {
         const long long m = (il + 1) * g.img3_count;
@@ -483,8 +485,6 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   std::vector<int4> chunks, groups;
   if (pb.ewald) {
     const SystemDesc& sd = systems[0];
-    if (nsp > kMaxEwaldSpecies)
-      throw Error("Ewald summation supports at most " + std::to_string(kMaxEwaldSpecies) + " distinct elements");
     terms.assign(size_t(nsp) * nsp * 2, FourierTerm{0.0, 1.0, 0, 2});
     nterm.assign(size_t(nsp) * nsp, 0);
     double kcut = 0.0;
@@ -589,6 +589,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   pb.smooth = smooth;
   pb.periodic_atm = periodic_atm;
   pb.slot_bound = slot_bound;
+  pb.atm_atoms = atm_atoms;
   pb.item_bound = item_bound;
   if (pb.ewald) {
     EwaldBatch& e = pb.eb;
@@ -751,6 +752,7 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
     ew.part = ew_part.as<double2>(size_t(e.nchunks) * R * nk);
     ew.phi = ew_phi.as<double2>(size_t(e.nsp) * (e.nsp + 1) / 2 * nk);
     ew.F = ew_f.as<double2>(size_t(e.nsp) * R * nk);
+    ew.S = e.nsp > kMaxEwaldSpecies ? ew_s.as<double2>(size_t(e.nsp) * R * nk) : nullptr;
     timed("lowrank", stream, [&] { launch_lowrank(db, e, grad, w, ew, stream); });
     timed("ewald_sf", stream, [&] { launch_ewald_sf(e, ew, stream); });
     timed("ewald_f", stream, [&] { launch_ewald_f(e, grad, w, ew, stream); });
@@ -976,7 +978,7 @@ Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, cons
                                  hipMemcpyDeviceToHost, stream),
                   "list size");
         hip_check(hipStreamSynchronize(stream), "list size sync");
-        cap = std::min(cap, std::max(total + total / 2, total + 64LL * natoms));
+        cap = std::min(cap, std::max(total + total / 2, total + 64LL * s.pb.atm_atoms));
       }
       s.capacity = cap;
       s.max_items = std::min(s.pb.item_bound, cap / kTile + natoms);

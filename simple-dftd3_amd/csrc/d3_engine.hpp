@@ -172,7 +172,8 @@ class Engine {
 //
 // Three-body lists: a batch of molecules gets the exact bound n(n+1)/2 slots
 // per molecule; a periodic system gets 1.5 times the list size of the
-// coordinates the plan is made with (and at least 64 more slots per atom),
+// coordinates the plan is made with (and at least 64 more slots per atom of a
+// system with a three-body term),
 // capped by the exact bound.  An evaluation whose lists do not fit adds no
 // three-body terms, returns NaN energies, gradients and virials for every
 // system with a three-body term, and raises the overflow flag that

@@ -357,8 +357,12 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
   }
   // per species s of the batch: h[s][r] = sum_ri w_i[ri] C6ref[ri][r] and the
   // same with dw_i/dCN_i, so that a record costs three nref_j-term dots; and
-  // (4/3 rvdw_is)^(alp/3)
-  __shared__ double h[kMaxSpecies][kMaxRef], dh[kMaxSpecies][kMaxRef], r0p_s[kMaxSpecies];
+  // (4/3 rvdw_is)^(alp/3), in dynamic LDS ((2 kMaxRef + 1) nsp doubles, sized
+  // at the launch).
+  extern __shared__ double pair_lds[];
+  double* __restrict__ h = pair_lds;                   // [nsp][kMaxRef]
+  double* __restrict__ dh = pair_lds + b.nsp * kMaxRef;  // [nsp][kMaxRef]
+  double* __restrict__ r0p_s = dh + b.nsp * kMaxRef;     // [nsp]
   for (int t = threadIdx.x; t < b.nsp * kMaxRef; t += blockDim.x) {
     const int sj = t / kMaxRef, rj = t % kMaxRef;
     const double* ref = b.c6ref + (spi * b.nsp + sj) * (kMaxRef * kMaxRef);
@@ -367,8 +371,8 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
       a += gi[ri] * ref[ri * kMaxRef + rj];
       d += dgi[ri] * ref[ri * kMaxRef + rj];
     }
-    h[sj][rj] = a;
-    dh[sj][rj] = d;
+    h[t] = a;
+    dh[t] = d;
   }
   for (int t = threadIdx.x; t < b.nsp; t += blockDim.x) r0p_s[t] = pow(kRs9 * b.rvdw[t * b.nsp + spi], alp3);
   __syncthreads();
@@ -384,10 +388,10 @@ __global__ __launch_bounds__(kBlock) void pair_kernel(DevBatch b, const double* 
     if (!(gi_ghost || (code & kGhostBit))) {  // model.f90:436-440
       for (int rj = 0; rj < nrj; ++rj) {
         const double w = gwr[rj * nat + x];
-        cc += w * h[spj][rj];
+        cc += w * h[spj * kMaxRef + rj];
         if (dgw) {
-          di += w * dh[spj][rj];
-          dj += gwr[(mref + rj) * nat + x] * h[spj][rj];
+          di += w * dh[spj * kMaxRef + rj];
+          dj += gwr[(mref + rj) * nat + x] * h[spj * kMaxRef + rj];
         }
       }
     }
@@ -970,8 +974,9 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   constexpr int kAccLen = kNAcc > 0 ? kNAcc * kTile : 1;
   __shared__ TileSlots<R> tileA;
   __shared__ TileSlots<R> tileB[kW];
-  // per-slot sums in double (FP32 LDS atomics measured several times slower):
-  // the A tile's for the whole item, each wavefront's B tile for one tile pair
+  // per-slot sums in double (FP32 LDS atomics, ds_add_f32, measured five
+  // times slower than FP64 ones in this kernel): the A tile's for the whole
+  // item, each wavefront's B tile for one tile pair
   __shared__ double accA[kAccLen];
   __shared__ double accBw[kW][kAccLen];
   __shared__ PairEntry<R> ring[kW][kQueue];
@@ -1054,12 +1059,20 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
     __builtin_amdgcn_wave_barrier();
     const int s0 = diag ? 1 : 0, s1 = diag ? kTile / 2 : kTile - 1;
     int st = s0, head = 0, tail = 0;
+    // the b-slot vector of the next test step, read one step ahead
+    double nbx = TB.vx[(l + s0) & (kTile - 1)], nby = TB.vy[(l + s0) & (kTile - 1)],
+           nbz = TB.vz[(l + s0) & (kTile - 1)];
     while (true) {
       // test phase: append the accepted pairs of the next steps
       for (; st <= s1 && tail - head <= kQueue - kWave; ++st) {
         const int bl = (l + st) & (kTile - 1);
+        const double bx = nbx, by = nby, bz = nbz;
+        const int bn = (bl + 1) & (kTile - 1);  // in range for every step: the read past s1 is unused
+        nbx = TB.vx[bn];
+        nby = TB.vy[bn];
+        nbz = TB.vz[bn];
         double v0, v1, v2;
-        const double Cd = diff_norm2(TB.vx[bl], TB.vy[bl], TB.vz[bl], ax, ay, az, v0, v1, v2);
+        const double Cd = diff_norm2(bx, by, bz, ax, ay, az, v0, v1, v2);
         const double cut2_step = (diag && st == kTile / 2) ? cut2_half : cut2;
         bool ok = (Cd <= cut2_step) & (Cd >= kEps);
         if (kPart)  // atm.f90:709: the triple belongs to the pair (i, larger partner)
@@ -2329,9 +2342,14 @@ __global__ __launch_bounds__(kBlock) void ewald_sf_kernel(EwaldBatch e, const do
 // Re(S_al conj S_bl), minus its energy E_k on the diagonal (ewald.f90:
 // 248-250).  The virial is summed over the wavefront and added to the
 // per-atom virial of the system's first atom.
+// kMany (more than kMaxEwaldSpecies species): the structure factors S_al of
+// the lane's vector go to Sbuf [nsp][rank][nk] (each lane reads back only its
+// own entries) instead of registers.
+template <bool kMany>
 __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const double2* __restrict__ part,
                                                          double2* __restrict__ phi, double2* __restrict__ F,
-                                                         double* __restrict__ sig_atom, int grad) {
+                                                         double2* __restrict__ Sbuf, double* __restrict__ sig_atom,
+                                                         int grad) {
   const long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long long nk = static_cast<long long>(e.ntiles) * kWave;
   const int t = static_cast<int>(k / kWave);
@@ -2352,7 +2370,36 @@ __global__ __launch_bounds__(kBlock) void ewald_f_kernel(EwaldBatch e, const dou
         phi[static_cast<long long>(a * (a + 1) / 2 + c) * nk + k] = make_double2(p, dp);
       }
     double ek = 0.0, qq = 0.0;
-    for (int l = 0; l < e.rank; ++l) {
+    for (int l = 0; l < e.rank && kMany; ++l) {
+      // S_al of this lane's vector, then F_al = sum_c phihat_ac S_cl
+      for (int a = 0; a < nsp; ++a) {
+        double sr = 0.0, si = 0.0;
+        for (int ch = e.chunk_seg[a]; ch < e.chunk_seg[a + 1]; ++ch) {
+          const double2 pv = part[(static_cast<long long>(ch) * e.rank + l) * nk + k];
+          sr += pv.x;
+          si += pv.y;
+        }
+        Sbuf[(static_cast<long long>(a) * e.rank + l) * nk + k] = make_double2(sr, si);
+      }
+      double el = 0.0, ql = 0.0;
+      for (int a = 0; a < nsp; ++a) {
+        const double2 sa = Sbuf[(static_cast<long long>(a) * e.rank + l) * nk + k];
+        double f_re = 0.0, f_im = 0.0;
+        for (int c = 0; c < nsp; ++c) {
+          const double2 sc = Sbuf[(static_cast<long long>(c) * e.rank + l) * nk + k];
+          const int pr = a >= c ? a * (a + 1) / 2 + c : c * (c + 1) / 2 + a;
+          const double2 pv = phi[static_cast<long long>(pr) * nk + k];
+          f_re += pv.x * sc.x;
+          f_im += pv.x * sc.y;
+          ql += pv.y * (sa.x * sc.x + sa.y * sc.y);
+        }
+        F[(static_cast<long long>(a) * e.rank + l) * nk + k] = make_double2(f_re, f_im);
+        el += sa.x * f_re + sa.y * f_im;
+      }
+      ek += e.lambda[l] * el;
+      qq += e.lambda[l] * ql;
+    }
+    for (int l = 0; l < e.rank && !kMany; ++l) {
       double s_re[kMaxEwaldSpecies], s_im[kMaxEwaldSpecies];
 #pragma unroll
       for (int a = 0; a < kMaxEwaldSpecies; ++a) {
@@ -2646,6 +2693,9 @@ __global__ __launch_bounds__(kBlock) void spme_spread_kernel(DevBatch b, EwaldBa
 // stored) times k k^T / (2 |k| V).  Grid-stride over the points; the virial
 // is summed over the block and added to the per-atom virial of the system's
 // first atom once per block.
+// kMany (more than kMaxEwaldSpecies species): the transforms Q_c are read
+// from Qh in the inner loop instead of registers.
+template <bool kMany>
 __global__ __launch_bounds__(kBlock) void spme_conv_kernel(EwaldBatch e, SpmeBatch s, const double2* __restrict__ Qh,
                                                            const double* __restrict__ K, const double* __restrict__ Dm,
                                                            const double* __restrict__ Dmir, int l,
@@ -2654,18 +2704,37 @@ __global__ __launch_bounds__(kBlock) void spme_conv_kernel(EwaldBatch e, SpmeBat
   double sg[6] = {0, 0, 0, 0, 0, 0};
   for (long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; p < s.nch;
        p += static_cast<long long>(gridDim.x) * blockDim.x) {
+    double qm = 0.0, qx = 0.0;
+    if (kMany) {
+      for (int a = 0; a < nsp; ++a) {
+        const double2 qa = Qh[static_cast<long long>(a) * s.nch + p];
+        double tr = 0.0, ti = 0.0;
+        for (int c = 0; c < nsp; ++c) {
+          const double2 qc = Qh[static_cast<long long>(c) * s.nch + p];
+          const long long o = static_cast<long long>(a >= c ? a * (a + 1) / 2 + c : c * (c + 1) / 2 + a) * s.nch + p;
+          const double kk = K[o];
+          tr += kk * qc.x;
+          ti += kk * qc.y;
+          if (Dm && c <= a) {  // each pair once
+            const double re = (c == a ? 1.0 : 2.0) * (qa.x * qc.x + qa.y * qc.y);
+            qm += Dm[o] * re;
+            qx += Dmir[o] * re;
+          }
+        }
+        Th[static_cast<long long>(a) * s.nch + p] = make_double2(tr, ti);
+      }
+    }
     double qr[kMaxEwaldSpecies], qi[kMaxEwaldSpecies];
 #pragma unroll
     for (int a = 0; a < kMaxEwaldSpecies; ++a) {
-      const double2 v = a < nsp ? Qh[static_cast<long long>(a) * s.nch + p] : make_double2(0.0, 0.0);
+      const double2 v = (!kMany && a < nsp) ? Qh[static_cast<long long>(a) * s.nch + p] : make_double2(0.0, 0.0);
       qr[a] = v.x;
       qi[a] = v.y;
     }
-    double qm = 0.0, qx = 0.0;
     // a runtime loop (a fully unrolled pair loop hoists every table load
     // into registers and runs at one wavefront per SIMD)
 #pragma unroll 1
-    for (int a = 0; a < nsp; ++a) {
+    for (int a = 0; a < nsp && !kMany; ++a) {
       double ar = 0.0, ai = 0.0;  // Q_a, selected without indexing the register array
 #pragma unroll
       for (int c = 0; c < kMaxEwaldSpecies; ++c)
@@ -2859,11 +2928,12 @@ void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* 
                  void* pair, hipStream_t st) {
   hipLaunchKernelGGL(rank_weights_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b, gw, dgw, gwr,
                      spr);
+  const size_t lds = sizeof(double) * static_cast<size_t>(b.nsp) * (2 * kMaxRef + 1);  // h, dh, r0p
   if (mixed)
-    hipLaunchKernelGGL(pair_kernel<float>, dim3(b.natoms), dim3(kBlock), 0, st, b, gw, dgw, gwr, spr,
+    hipLaunchKernelGGL(pair_kernel<float>, dim3(b.natoms), dim3(kBlock), lds, st, b, gw, dgw, gwr, spr,
                        static_cast<float4*>(pair));
   else
-    hipLaunchKernelGGL(pair_kernel<double>, dim3(b.natoms), dim3(kBlock), 0, st, b, gw, dgw, gwr, spr,
+    hipLaunchKernelGGL(pair_kernel<double>, dim3(b.natoms), dim3(kBlock), lds, st, b, gw, dgw, gwr, spr,
                        static_cast<double4*>(pair));
 }
 
@@ -3056,8 +3126,12 @@ void launch_spme_spread(const DevBatch& b, const EwaldBatch& e, const SpmeBatch&
 void launch_spme_conv(const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w, const SpmeWork& sw, int l,
                       hipStream_t st) {
   const int blocks = static_cast<int>(std::min<long long>(blocks_for(s.nch, kBlock), 4096));
-  hipLaunchKernelGGL(spme_conv_kernel, dim3(blocks), dim3(kBlock), 0, st, e, s, sw.Qh, sw.K,
-                     grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr, l, sw.Th, w.sig_atom);
+  if (e.nsp > kMaxEwaldSpecies)
+    hipLaunchKernelGGL(spme_conv_kernel<true>, dim3(blocks), dim3(kBlock), 0, st, e, s, sw.Qh, sw.K,
+                       grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr, l, sw.Th, w.sig_atom);
+  else
+    hipLaunchKernelGGL(spme_conv_kernel<false>, dim3(blocks), dim3(kBlock), 0, st, e, s, sw.Qh, sw.K,
+                       grad ? sw.Dm : nullptr, grad ? sw.Dmir : nullptr, l, sw.Th, w.sig_atom);
 }
 
 void launch_spme_gather(const DevBatch& b, const EwaldBatch& e, const SpmeBatch& s, bool grad, const DevWork& w,
@@ -3087,8 +3161,13 @@ void launch_ewald_sf(const EwaldBatch& e, const EwaldWork& ew, hipStream_t st) {
 }
 
 void launch_ewald_f(const EwaldBatch& e, bool grad, const DevWork& w, const EwaldWork& ew, hipStream_t st) {
-  hipLaunchKernelGGL(ewald_f_kernel, dim3(blocks_for(static_cast<long long>(e.ntiles) * kWave, kBlock)), dim3(kBlock),
-                     0, st, e, ew.part, ew.phi, ew.F, w.sig_atom, grad ? 1 : 0);
+  const dim3 grid(blocks_for(static_cast<long long>(e.ntiles) * kWave, kBlock));
+  if (e.nsp > kMaxEwaldSpecies)
+    hipLaunchKernelGGL(ewald_f_kernel<true>, grid, dim3(kBlock), 0, st, e, ew.part, ew.phi, ew.F, ew.S, w.sig_atom,
+                       grad ? 1 : 0);
+  else
+    hipLaunchKernelGGL(ewald_f_kernel<false>, grid, dim3(kBlock), 0, st, e, ew.part, ew.phi, ew.F, ew.S, w.sig_atom,
+                       grad ? 1 : 0);
 }
 
 int ewald_atom_split(const EwaldBatch& e) {

@@ -165,6 +165,7 @@ struct EwaldWork {
   double2 *part;             // [nchunks][rank][nk] partial structure factors (nk = ntiles * kWave)
   double2 *phi;              // [nsp (nsp + 1) / 2][nk] transform and its k-derivative
   double2 *F;                // [nsp][rank][nk] sum_b phihat_ab S_bl
+  double2 *S;                // [nsp][rank][nk] S_al, only for more than kMaxEwaldSpecies species
 };
 // The reciprocal-space energy (and dE/dCN, gradient, virial), added to the
 // accumulators of w after launch_disp2, in this order: the phase tables (if

@@ -4,12 +4,19 @@
 system of a batch as a tensor that autograd differentiates with respect to
 the positions (N, 3) and the lattices (nsys, 3, 3), both in bohr:
 
-* the forward pass runs the energy-only kernels of the library (no gradient
-  work, nothing is kept but the inputs);
-* the backward pass runs the energy / gradient / virial kernels on the saved
-  inputs (the coordination-number derivatives are recomputed pair by pair,
-  no 3 x N x N dCN/dR array exists) and turns gradient G and virial sigma
-  into the adjoints of the inputs.
+* when an input requires a gradient, the forward pass runs the energy /
+  gradient / virial kernels once (the coordination-number derivatives are
+  recomputed pair by pair, no 3 x N x N dCN/dR array exists) and keeps the
+  gradient G and the virial sigma; otherwise it runs the energy-only
+  kernels;
+* the backward pass turns G and sigma into the adjoints of the inputs
+  (no second evaluation).
+
+The lattices are read on the host (the lattice images are enumerated
+there); a lattice tensor passed again unchanged (the same tensor object with
+the same version counter) reuses the previous host copy, so a step with a
+fixed cell does not synchronise on it.  (Writes through ``.data`` do not
+advance the version counter: pass a new tensor after such a write.)
 
 Lattice adjoint: with the Cartesian positions X (rows x_i) and the lattice L
 (rows = lattice vectors) as independent inputs, the strain derivative of the
@@ -34,9 +41,16 @@ from . import torch_api
 def _spec_with_lattice(spec: torch_api.BatchSpec, lattice: Optional[torch.Tensor]) -> torch_api.BatchSpec:
     if lattice is None:
         return spec
+    # the same tensor object (held here, so its memory is not reused) with
+    # the same version counter holds the same values
+    cached = getattr(spec, "_lattice_spec", None)
+    if cached is not None and cached[0] is lattice and cached[1] == lattice._version:
+        return cached[2]
     lat = lattice.detach().to(torch.float64).cpu().numpy().reshape(len(spec), 3, 3)
     periodic = spec.periodic if spec.periodic is not None else np.ones((len(spec), 3), dtype=bool)
-    return torch_api.BatchSpec(spec.natoms, spec.numbers, lat, periodic, spec.params, spec.cutoffs)
+    run_spec = torch_api.BatchSpec(spec.natoms, spec.numbers, lat, periodic, spec.params, spec.cutoffs)
+    spec._lattice_spec = (lattice, lattice._version, run_spec)
+    return run_spec
 
 
 def lattice_adjoint(gradient: torch.Tensor, sigma: torch.Tensor, positions: torch.Tensor, lattice: torch.Tensor,
@@ -52,32 +66,41 @@ def lattice_adjoint(gradient: torch.Tensor, sigma: torch.Tensor, positions: torc
     return torch.linalg.solve(lattice.transpose(1, 2), (sig - gx).transpose(1, 2))
 
 
+def _owner(spec: torch_api.BatchSpec, device) -> torch.Tensor:
+    """System index of every atom (cached on the spec per device)."""
+    cache = spec.__dict__.setdefault("_owner", {})
+    if device not in cache:
+        natoms = torch.as_tensor(spec.natoms, dtype=torch.long, device=device)
+        cache[device] = torch.repeat_interleave(torch.arange(len(spec), device=device), natoms)
+    return cache[device]
+
+
 class _Dispersion(torch.autograd.Function):
     @staticmethod
     def forward(ctx, positions, lattice, spec, precision):
         pos64 = positions.detach().to(torch.float64).contiguous()
         run_spec = _spec_with_lattice(spec, lattice)
-        out = torch_api.run_batch(run_spec, pos64, precision=precision, grad=False)
-        ctx.spec = run_spec
-        ctx.precision = precision
+        need = ctx.needs_input_grad[0] or (lattice is not None and ctx.needs_input_grad[1])
+        out = torch_api.run_batch(run_spec, pos64, precision=precision, grad=need)
         ctx.has_lattice = lattice is not None
-        ctx.save_for_backward(positions, lattice if lattice is not None else positions.new_empty(0))
+        ctx.spec = run_spec
+        if need:
+            lat64 = lattice.detach().to(torch.float64) if lattice is not None else pos64.new_empty(0)
+            ctx.save_for_backward(out["gradient"], out["sigma"], pos64, lat64)
+        ctx.dtypes = (positions.dtype, lattice.dtype if lattice is not None else None)
         return out["energy"].to(positions.dtype)
 
     @staticmethod
+    @torch.autograd.function.once_differentiable
     def backward(ctx, grad_energy):
-        positions, lattice = ctx.saved_tensors
-        pos64 = positions.detach().to(torch.float64).contiguous()
-        out = torch_api.run_batch(ctx.spec, pos64, precision=ctx.precision, grad=True)
-        natoms = torch.as_tensor(ctx.spec.natoms, dtype=torch.long, device=positions.device)
-        owner = torch.repeat_interleave(torch.arange(len(ctx.spec), device=positions.device), natoms)
+        g, sigma, pos64, lat64 = ctx.saved_tensors
+        owner = _owner(ctx.spec, g.device)
         ge = grad_energy.to(torch.float64)
-        g = out["gradient"]
-        grad_pos = (g * ge[owner].unsqueeze(1)).to(positions.dtype) if ctx.needs_input_grad[0] else None
+        grad_pos = (g * ge[owner].unsqueeze(1)).to(ctx.dtypes[0]) if ctx.needs_input_grad[0] else None
         grad_lat = None
         if ctx.has_lattice and ctx.needs_input_grad[1]:
-            d = lattice_adjoint(g, out["sigma"], pos64, lattice.detach().to(torch.float64), owner)
-            grad_lat = (d * ge.view(-1, 1, 1)).to(lattice.dtype)
+            d = lattice_adjoint(g, sigma, pos64, lat64, owner)
+            grad_lat = (d * ge.view(-1, 1, 1)).to(ctx.dtypes[1])
         return grad_pos, grad_lat, None, None
 
 
@@ -108,8 +131,7 @@ def forces_and_stress(spec: torch_api.BatchSpec, positions: torch.Tensor, lattic
     lat = lattice.detach().requires_grad_(True)
     energy = dispersion_energy(spec, pos, lat, precision)
     gpos, glat = torch.autograd.grad(energy.sum(), (pos, lat))
-    natoms = torch.as_tensor(spec.natoms, dtype=torch.long, device=positions.device)
-    owner = torch.repeat_interleave(torch.arange(len(spec), device=positions.device), natoms)
+    owner = _owner(spec, positions.device)
     p64, l64 = pos.detach().double(), lat.detach().double()
     sig = torch.zeros(len(spec), 3, 3, dtype=torch.float64, device=positions.device)
     sig.index_add_(0, owner, gpos.double().unsqueeze(2) * p64.unsqueeze(1))

@@ -329,13 +329,21 @@ def python_fixtures(ref):
     vals = [float(v) for v in re.findall(r"[+-]?\d+\.\d+", m.group(2))]
     pos = _triples(vals)
     refs = {}
-    for test, kind, method in (("test_pbe0_d3_bj", "rational", "pbe0"), ("test_b3lyp_d3_zero", "zero", "b3lyp")):
-        mm = re.search(r"def " + test + r".*?ref = ([-\d.e]+) if atm else ([-\d.e]+)", text, re.S)
+    # (test, damping kind, method name; "" for explicit parameters)
+    cases = (("test_pbe0_d3_bj", "rational", "pbe0"), ("test_b3lyp_d3_zero", "zero", "b3lyp"),
+             ("test_pbe_d3_bjm", "mrational", "pbe"), ("test_bp_d3_zerom", "mzero", "bp"),
+             ("test_b97d_d3_op", "optimizedpower", "b97d"), ("test_pbe_d3_z", "z", ""))
+    explicit = {}
+    for test, kind, method in cases:
+        mm = re.search(r"def " + test + r"\(.*?ref = ([-\d.e]+) if atm else ([-\d.e]+)(.*?)\n\n", text, re.S)
         refs[(kind, method)] = (float(mm.group(1)), float(mm.group(2)))
-    return dict(name="python_interface_model", source="python/dftd3/test_interface.py:78-100,236-240,319-323",
+        if not method:  # ZDampingParam(a1=..., s9=1.0 if atm else 0.0), other keywords at their defaults
+            explicit[kind] = {k: float(v) for k, v in re.findall(r"(\w+)=([-\d.]+)(?:,|\))", mm.group(3))}
+    return dict(name="python_interface_model",
+                source="python/dftd3/test_interface.py:78-100,236-240,319-356",
                 numbers=nums, positions=pos, lattice=None, periodic=None,
                 energies={f"{k}:{m}": {"atm": v[0], "noatm": v[1]} for (k, m), v in refs.items()},
-                rel=1.0e-6)
+                explicit=explicit, rel=1.0e-6)
 
 
 def main():

@@ -171,3 +171,45 @@ def test_molecules_without_lattice():
     ref = O.dispersion(num, xyz, None, None, O.RATIONAL, BJ, O.DEFAULT_CUTOFF)
     assert abs(e.item() - ref["energy"]) <= 1e-10 * abs(ref["energy"])
     np.testing.assert_allclose(g.cpu().numpy(), ref["gradient"], rtol=0, atol=1e-9)
+
+
+def test_one_evaluation_per_step_and_lattice_cache(monkeypatch):
+    """With inputs that require gradients the forward pass evaluates energy,
+    gradient and virial once and the backward pass evaluates nothing; without
+    them the forward pass runs the energy-only kernels.  A lattice tensor
+    changed in place is read again; the same tensor unchanged is not."""
+    from dftd3 import autograd, torch_api
+    _step("one evaluation per step, lattice cache")
+    calls = []
+    orig = torch_api.run_batch
+
+    def counting(*a, **k):
+        calls.append(k.get("grad", True))
+        return orig(*a, **k)
+
+    monkeypatch.setattr(torch_api, "run_batch", counting)
+    systems = _cells(seed=61)
+    spec, pos, lat = _inputs(systems, torch.float64)
+    e = autograd.dispersion_energy(spec, pos, lat)
+    gpos, glat = torch.autograd.grad(e.sum(), (pos, lat))
+    assert calls == [True]
+    with torch.no_grad():
+        e_plain = autograd.dispersion_energy(spec, pos.detach(), lat.detach())
+    assert calls == [True, False]
+    torch.testing.assert_close(e_plain, e.detach(), rtol=1e-14, atol=0)
+    # the cached host copy follows in-place changes of the lattice
+    lat2 = lat.detach().clone()
+    e1 = autograd.dispersion_energy(spec, pos.detach(), lat2)
+    with torch.no_grad():
+        lat2.mul_(1.02)
+    e2 = autograd.dispersion_energy(spec, pos.detach(), lat2)
+    e_ref = autograd.dispersion_energy(spec, pos.detach(), lat2.clone())
+    assert (e1 - e2).abs().max().item() > 1e-6
+    torch.testing.assert_close(e2, e_ref, rtol=1e-14, atol=0)
+    # gradients of the single evaluation against the oracle
+    off = 0
+    for k, s in enumerate(systems):
+        n = len(s["numbers"])
+        ref = _oracle(s)
+        np.testing.assert_allclose(gpos[off:off + n].cpu().numpy(), ref["gradient"], rtol=0, atol=1e-9)
+        off += n

@@ -75,6 +75,13 @@ def _check(res, ref, n, precision="fp64", grad=True):
             assert np.abs(res["gradient"] - ref["gradient"]).max() <= 1e-7
 
 
+def many_species(seed, n, edge, zs=(1, 3, 5, 6, 7, 8, 9, 11, 14, 15, 16)):
+    """A strained cell holding every element of zs (more species than the
+    reciprocal-space kernels keep in registers)."""
+    _, xyz, lat = cell(seed, n, edge)
+    return np.array([zs[k % len(zs)] for k in range(n)]), xyz, lat
+
+
 CASES = [
     ("cubic-bj-full", lambda: cubic(8.0, 1), O.RATIONAL, PBE_BJ, dict(rank=10000, kcut=10.0, mesh=-1)),
     ("cubic-bj-default", lambda: cubic(8.0, 1), O.RATIONAL, PBE_BJ, dict(mesh=-1)),
@@ -90,6 +97,10 @@ CASES = [
     ("mesh-triclinic-bj-atm-24", lambda: cell(7, 10, 11.0), O.RATIONAL, dict(PBE_BJ, s9=1.0),
      dict(rank=4, kcut=6.0, mesh=24)),
     ("mesh-triclinic-zero-atm", lambda: cell(11, 24, 13.0), O.ZERO, dict(PBE_ZERO, s9=1.0), dict(kcut=8.0, mesh=0)),
+    # eleven species
+    ("species11-bj-atm", lambda: many_species(13, 22, 12.5), O.RATIONAL, dict(PBE_BJ, s9=1.0),
+     dict(rank=10000, kcut=8.0, mesh=-1)),
+    ("mesh-species11-zero", lambda: many_species(14, 22, 12.5), O.ZERO, PBE_ZERO, dict(kcut=7.0, mesh=24)),
 ]
 
 

@@ -102,16 +102,29 @@ def test_validation_goldens_through_interface(goldens, name):
 
 
 @pytest.mark.parametrize("atm", [True, False])
-@pytest.mark.parametrize("kind,method", [("rational", "pbe0"), ("zero", "b3lyp")])
+@pytest.mark.parametrize("kind,method", [("rational", "pbe0"), ("zero", "b3lyp"), ("mrational", "pbe"),
+                                         ("mzero", "bp"), ("optimizedpower", "b97d"), ("z", "")])
 def test_python_interface_model(python_model, kind, method, atm):
-    from dftd3.interface import DispersionModel, RationalDampingParam, ZeroDampingParam
+    """python/dftd3/test_interface.py:236-356 through dftd3.interface, and
+    against the oracle on the same parameters."""
+    from dftd3 import interface as I
+    from test_oracle_golden import KINDS, python_model_param
     _step(f"python model {kind} {method} atm={atm}")
-    m = DispersionModel(np.array(python_model["numbers"]), np.array(python_model["positions"]))
-    cls = RationalDampingParam if kind == "rational" else ZeroDampingParam
-    res = m.get_dispersion(cls(method=method, atm=atm), grad=False)
+    m = I.DispersionModel(np.array(python_model["numbers"]), np.array(python_model["positions"]))
+    cls = dict(rational=I.RationalDampingParam, zero=I.ZeroDampingParam, mrational=I.ModifiedRationalDampingParam,
+               mzero=I.ModifiedZeroDampingParam, optimizedpower=I.OptimizedPowerDampingParam,
+               z=I.ZDampingParam)[kind]
+    if method:
+        param = cls(method=method, atm=atm)
+    else:  # ZDampingParam(a1=..., s9=1.0 if atm else 0.0)
+        param = cls(**python_model["explicit"][kind], s9=1.0 if atm else 0.0)
+    res = m.get_dispersion(param, grad=False)
     ref = python_model["energies"][f"{kind}:{method}"]["atm" if atm else "noatm"]
     assert float(res["energy"]) == pytest.approx(ref, rel=python_model["rel"])
     assert set(res) == {"energy"} and res["energy"].shape == ()
+    okind, p = python_model_param(python_model, f"{kind}:{method}", atm)
+    orc = O.dispersion(python_model["numbers"], python_model["positions"], None, None, KINDS[okind], p, grad=False)
+    assert abs(float(res["energy"]) - orc["energy"]) <= E_REL * abs(orc["energy"])
 
 
 def test_config0_molecule_fp64():
@@ -202,6 +215,22 @@ def test_periodic_2d_slab():
     _check(r, _oracle(num, pos, lat, [True, True, False], "rational", BJ), 10)
 
 
+@pytest.mark.parametrize("kind", ["rational", "zero"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_periodic_1d_chain(kind, precision):
+    """One periodic direction (cutoff.f90:157-163): a chain of 12 atoms with
+    a 15-bohr repeat, BJ and zero damping with ATM."""
+    _step(f"1D chain {kind} {precision}")
+    rng = np.random.default_rng(8)
+    num = rng.choice([1, 6, 7, 8, 16], size=12)
+    pos = np.column_stack([np.linspace(0.0, 13.6, 12), rng.uniform(0, 5, 12), rng.uniform(0, 5, 12)])
+    lat = np.array([[15.0, 0.3, -0.2], [0.0, 40.0, 0.0], [0.0, 0.0, 40.0]])
+    p = BJ if kind == "rational" else ZERO
+    r = _batch([dict(numbers=num, positions=pos, lattice=lat, periodic=[True, False, False], kind=kind, param=p)],
+               precision=precision)[0]
+    _check(r, _oracle(num, pos, lat, [True, False, False], kind, p), 12, mixed=precision == "mixed")
+
+
 def test_term_counts_match_oracle_enumeration():
     """GPU work-unit counts (used for pair-/triple-terms/s) equal the
     oracle's enumeration of the reference's loops."""
@@ -236,3 +265,25 @@ def test_batch_mixed_systems():
     for prec in ("fp64", "mixed"):
         for r, ref, s in zip(_batch(systems, precision=prec), refs, systems):
             _check(r, ref, len(s["numbers"]), mixed=prec == "mixed")
+
+
+def test_many_elements_in_one_batch():
+    """No limit on the distinct elements of a batch (model.f90:87-181
+    allocates per species): three molecules drawing on 40 elements (H to Zr),
+    one of them holding 30 distinct elements, against the oracle."""
+    _step("40 distinct elements in one batch")
+    rng = np.random.default_rng(12)
+    systems = []
+    for k, (zs, n) in enumerate([(np.arange(1, 31), 36), (np.arange(20, 41), 24), (np.array([1, 6, 7, 8]), 12)]):
+        num = np.concatenate([zs, rng.choice(zs, size=n - len(zs))]) if n > len(zs) else zs[:n]
+        edge = (n / 0.01) ** (1.0 / 3.0)  # 0.01 atoms per bohr^3, no two closer than 2.5 bohr
+        pts = []
+        while len(pts) < n:
+            x = rng.uniform(0, edge, 3)
+            if all(np.linalg.norm(x - q) > 2.5 for q in pts):
+                pts.append(x)
+        systems.append(dict(numbers=num, positions=np.array(pts), param=BJ))
+    res = _batch(systems)
+    assert len(set(np.concatenate([s["numbers"] for s in systems]).tolist())) == 40
+    for s, r in zip(systems, res):
+        _check(r, _oracle(s["numbers"], s["positions"]), len(s["numbers"]))

@@ -55,16 +55,35 @@ def test_validation_goldens(goldens, name):
         np.testing.assert_allclose(r["virial"], np.array(e["virial"]), rtol=0, atol=f["atol"])
 
 
-@pytest.mark.parametrize("key", ["rational:pbe0", "zero:b3lyp"])
-@pytest.mark.parametrize("atm", [True, False])
-def test_python_interface_energies(python_model, key, atm):
+def python_model_param(python_model, key, atm):
+    """Damping kind and parameters of an energy of the reference's 16-atom
+    Python test model: loaded by method name from the parameter database
+    (`load_param(method, atm)`: s9 = 1 with the three-body term, else 0), or
+    given explicitly (Z damping: a1 with the other keywords at their
+    defaults, s9 = 1 or 0)."""
     import json
     import os
-    params = json.load(open(os.path.join(os.path.dirname(O.TABLES), "d3_params.json")))
     kind, method = key.split(":")
-    p = dict(params["defaults"])
-    p.update(params[kind][params["aliases"][method]])
+    if method:
+        params = json.load(open(os.path.join(os.path.dirname(O.TABLES), "d3_params.json")))
+        p = dict(params["defaults"])
+        p.update(params[kind][params["aliases"][method]])
+    else:
+        p = dict(s6=1.0, s8=1.0, alp=14.0)
+        p.update(python_model["explicit"][kind])
     p["s9"] = 1.0 if atm else 0.0
+    return ("rational" if kind == "mrational" else kind), p
+
+
+PY_KEYS = ["rational:pbe0", "zero:b3lyp", "mrational:pbe", "mzero:bp", "optimizedpower:b97d", "z:"]
+
+
+@pytest.mark.parametrize("key", PY_KEYS)
+@pytest.mark.parametrize("atm", [True, False])
+def test_python_interface_energies(python_model, key, atm):
+    """python/dftd3/test_interface.py:236-356: BJ, zero, modified rational,
+    modified zero, optimized power and Z damping on the 16-atom model."""
+    kind, p = python_model_param(python_model, key, atm)
     ref = python_model["energies"][key]["atm" if atm else "noatm"]
     r = O.dispersion(python_model["numbers"], python_model["positions"], None, None, _kind(kind), p, grad=False)
     assert r["energy"] == pytest.approx(ref, rel=python_model["rel"])
@@ -297,6 +316,72 @@ def test_lattice_points_molecule_and_2d():
     inside = allt[np.linalg.norm(allt, axis=1) <= 30.0]
     have = {tuple(np.round(t, 9)) for t in tr}
     assert all(tuple(np.round(t, 9)) in have for t in inside)
+
+
+def test_lattice_points_1d():
+    """One periodic direction (cutoff.f90:157-163): rep = ceiling(r / |a_k|)
+    along the periodic lattice vector, 0 along the others; the images are
+    the multiples n a_k, |n| <= rep, in increasing n, and cover every such
+    translation within the cutoff."""
+    lat = np.array([[4.3, 0.7, -0.2], [0.5, 11.0, 0.3], [-0.4, 0.2, 13.0]])
+    for axis in range(3):
+        pbc = [k == axis for k in range(3)]
+        for rthr in (9.0, 40.0, 60.0):
+            tr = O.lattice_points(pbc, lat, rthr)
+            rep = int(np.ceil(rthr / np.linalg.norm(lat[axis])))
+            expect = np.outer(np.arange(-rep, rep + 1), lat[axis])
+            np.testing.assert_allclose(tr, expect, rtol=0, atol=1e-12)
+            n = np.arange(-200, 201)
+            inside = [k for k in n if np.linalg.norm(k * lat[axis]) <= rthr]
+            assert min(inside) >= -rep and max(inside) <= rep
+
+
+def test_1d_chain_equals_3d_with_vacuum():
+    """A system periodic along one lattice vector equals the 3D-periodic
+    system whose other two lattice vectors are longer than every cutoff plus
+    the system's extent (the 3D enumeration is pinned by the Ta90 golden);
+    BJ and zero damping with ATM, energy, gradient and virial."""
+    from helpers import BJ, ZERO
+    rng = np.random.default_rng(5)
+    a = np.array([7.5, 0.4, -0.3])
+    num = rng.choice([1, 6, 7, 8], size=8)
+    pos = np.column_stack([rng.uniform(0, 7.5, 8), rng.uniform(0, 5, 8), rng.uniform(0, 5, 8)])
+    lat1 = np.array([a, [0.0, 150.0, 0.0], [0.0, 0.0, 150.0]])
+    for kind, par in ((O.RATIONAL, BJ), (O.ZERO, ZERO)):
+        r1 = O.dispersion(num, pos, lat1, [True, False, False], kind, par)
+        r3 = O.dispersion(num, pos, lat1, [True, True, True], kind, par)
+        assert abs(r1["energy"] - r3["energy"]) <= 1e-13 * abs(r3["energy"])
+        np.testing.assert_allclose(r1["gradient"], r3["gradient"], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(r1["virial"], r3["virial"], rtol=0, atol=1e-13)
+        # and differs from the molecule (the images along a interact)
+        r0 = O.dispersion(num, pos, None, None, kind, par, grad=False)
+        assert abs(r1["energy"] - r0["energy"]) > 1e-4
+
+
+def test_cso_sigmoid_against_published_form():
+    """CSO damping (cso.f90:300-320; Schroeder, Creon, Schwabe, JCTC 2015):
+    E = -C6 (s6 + a1 sf(r)) / (r^6 + (a3 r0 + a4)^6) with the sigmoid
+    sf(r) = 1 / (1 + exp(r - a2 r0)) and r0 = sqrt(3 r4r2_i r4r2_j).  The
+    rational damping with s8 = 0, a1 = a3, a2 = a4 is -C6 s6 / (r^6 + (a3 r0
+    + a4)^6) with the same C6 (pinned by the goldens and its dimer closed
+    form), so a dimer's sf(r) = (E_cso / E_rat - s6) / a1 follows from two
+    oracle energies; it must be the published sigmoid with r0 from the r4r2
+    table: s6 + a1 at short range, s6 alone at long range, a1 / 2 added at
+    r = a2 r0."""
+    s6, a1, a2, a3, a4 = 1.0, 0.86, 2.5, 0.0, 6.25
+    r0 = np.sqrt(3.0 * _r4r2(6) * _r4r2(8))
+    for r in (0.8, a2 * r0 - 1.5, a2 * r0, a2 * r0 + 2.0, 30.0, 55.0):
+        pos = [[0.0, 0.0, 0.0], [0.0, 0.3 * r, np.sqrt(0.91) * r]]
+        e_cso = O.dispersion([6, 8], pos, kind=O.CSO, param=dict(s6=s6, a1=a1, a2=a2, a3=a3, a4=a4, s9=0.0),
+                             grad=False)["energy"]
+        e_rat = O.dispersion([6, 8], pos, kind=O.RATIONAL, param=dict(s6=1.0, s8=0.0, a1=a3, a2=a4, s9=0.0),
+                             grad=False)["energy"]
+        sf = (e_cso / e_rat - s6) / a1
+        assert abs(sf - 1.0 / (1.0 + np.exp(r - a2 * r0))) <= 1e-12, (r, sf)
+        if r < 1.0:  # short range: s6 + a1
+            assert sf > 1.0 - 1e-4
+        if r > 50.0:  # long range: s6 alone
+            assert abs(sf) <= 1e-15
 
 
 def test_wrap_to_central_cell():
