@@ -217,6 +217,26 @@ D3HIP_API void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspa
                                           const double* positions, double* energy, double* gradient,
                                           double* sigma, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): d3hip_get_dispersion_batch without a workspace and
+ * without any allocation.  Every work buffer comes from a fixed work area of
+ * D3HIP_NOALLOC_BYTES per GPU, a static device array of the library (held
+ * until the process ends); the metadata is copied from pageable host memory.
+ * A batch that needs more than the area (about 72 n^2 bytes for an FP64
+ * molecule of n atoms with the three-body term, 44 n^2 mixed: n up to some
+ * 950 / 1200) is rejected with an error before any output is written.  No
+ * low-rank model.  Calls that overlap on one GPU race: they share its area,
+ * which keeps the last call's contents.  Same arguments and results as
+ * d3hip_get_dispersion_batch otherwise (including the read-back of the
+ * three-body list size). */
+#define D3HIP_NOALLOC_BYTES (64u << 20)
+D3HIP_API void d3hip_get_dispersion_batch_noalloc(dftd3_error error, int nsys, const int* natoms,
+                                                  const int* numbers, const double* lattices, const bool* periodic,
+                                                  const double* params, const double* cutoffs, int precision,
+                                                  const double* positions, double* energy, double* gradient,
+                                                  double* sigma, void* stream);
+
 /* Pairwise-resolved energies of a batch (device arrays; same batch
  * description as d3hip_get_dispersion_batch): pair_energy2 and pair_energy3
  * hold per system an n x n block in atom order, the blocks one after another

@@ -723,6 +723,26 @@ void d3hip_get_dispersion_batch(dftd3_error error, dftd3_model workspace, int ns
   });
 }
 
+static_assert(D3HIP_NOALLOC_BYTES == d3::kNoAllocBytes, "work area size of the header and the library");
+
+void d3hip_get_dispersion_batch_noalloc(dftd3_error error, int nsys, const int* natoms, const int* numbers,
+                                        const double* lattices, const bool* periodic, const double* params,
+                                        const double* cutoffs, int precision, const double* positions,
+                                        double* energy, double* gradient, double* sigma, void* stream) {
+  if (!error) return;
+  if (nsys <= 0 || !natoms || !numbers || !params || !positions || !energy)
+    return fatal(error, "Invalid batch description");
+  if (precision != d3::kFP64 && precision != d3::kMixed) return fatal(error, "Unknown precision mode");
+  guarded(error, [&] {
+    std::vector<d3::SystemDesc> sys = batch_systems(nsys, natoms, lattices, periodic, params, cutoffs);
+    d3::BatchOutput out;
+    out.energy = energy;
+    out.gradient = gradient;
+    out.sigma = sigma;
+    d3::run_noalloc(sys, numbers, positions, precision, out, static_cast<hipStream_t>(stream));
+  });
+}
+
 void d3hip_get_pairwise_dispersion_batch(dftd3_error error, dftd3_model workspace, int nsys, const int* natoms,
                                          const int* numbers, const double* lattices, const bool* periodic,
                                          const double* params, const double* cutoffs, int precision,

@@ -11,6 +11,8 @@ constexpr int kMaxElem = 103;   // elements with reference data (reference.f90:2
 constexpr int kMaxRef = 7;      // reference systems per element (reference.f90:28)
 constexpr int kNumPairs = kMaxElem * (kMaxElem + 1) / 2;
 constexpr int kMaxSpecies = kMaxElem; // distinct elements in one batch: any set of elements with reference data
+// Fixed device work area of the allocation-free entry point, per GPU.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
 
 // Damping families of the two-body term (the three-body term is the same
 // for all): rational (Becke-Johnson; the modified rational parameters use it

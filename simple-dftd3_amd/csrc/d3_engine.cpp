@@ -16,6 +16,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,14 +33,58 @@ namespace {
 
 constexpr double kEps = 2.220446049250313e-16;
 
-// Grow-only device allocation.
+// A fixed region of device memory handed out front to back (256-byte
+// aligned); reset() makes all of it free again.  Used by the allocation-free
+// entry point, whose region is the library's static work area.
+class Arena {
+ public:
+  void bind(void* base, size_t bytes) {
+    base_ = static_cast<char*>(base);
+    size_ = bytes;
+    used_ = 0;
+  }
+  void reset() { used_ = 0; }
+  void* take(size_t bytes) {
+    const size_t at = (used_ + 255) / 256 * 256;
+    if (at > size_ || bytes > size_ - at)
+      throw Error("The batch needs more device memory than the fixed work area of the allocation-free entry point (" +
+                  std::to_string(size_ >> 20) + " MiB)");
+    used_ = at + bytes;
+    return base_ + at;
+  }
+
+ private:
+  char* base_ = nullptr;
+  size_t size_ = 0, used_ = 0;
+};
+
+// Grow-only device allocation; with an arena, taken from it instead (no
+// allocation call).
 class DeviceBuffer {
  public:
   ~DeviceBuffer() {
-    if (ptr_) (void)hipFree(ptr_);
+    if (ptr_ && !arena_) (void)hipFree(ptr_);
+  }
+  void use_arena(Arena* a) {
+    if (ptr_ && !arena_) hip_check(hipFree(ptr_), "hipFree");
+    arena_ = a;
+    ptr_ = nullptr;
+    cap_ = 0;
+  }
+  // arena mode: forget the region (the arena is reset by its owner)
+  void release() {
+    if (arena_) {
+      ptr_ = nullptr;
+      cap_ = 0;
+    }
   }
   void* get(size_t bytes) {
     if (bytes > cap_) {
+      if (arena_) {
+        ptr_ = arena_->take(bytes);
+        cap_ = bytes;
+        return ptr_;
+      }
       if (ptr_) hip_check(hipFree(ptr_), "hipFree");
       ptr_ = nullptr;
       cap_ = 0;
@@ -57,6 +102,7 @@ class DeviceBuffer {
  private:
   void* ptr_ = nullptr;
   size_t cap_ = 0;
+  Arena* arena_ = nullptr;
 };
 
 // Pinned host buffer for the metadata upload; an event recorded after the copy
@@ -304,6 +350,24 @@ struct Workspace {
   std::vector<char> staging;
   PinnedStaging pinned;
   KernelTimer timer;
+  // allocation-free mode: every device buffer comes from the arena, the
+  // metadata is copied from pageable host memory (no pinned staging)
+  Arena* arena = nullptr;
+  std::vector<DeviceBuffer*> buffers() {
+    return {&meta, &pos, &rank, &order, &sort_tmp, &cn, &gw, &dgw, &pair, &e_atom, &dedcn, &grad, &sig, &cnt, &off,
+            &items, &item_off, &slots, &gw_rank, &sp_rank, &pos_rank, &counts, &counter, &flag, &pair3s, &hbuf,
+            &e_out, &ew_tab, &ew_c6l, &ew_lc6l, &ew_ldc6l, &ew_part, &ew_phi, &ew_f, &ew_s, &sp_base, &sp_theta,
+            &sp_dtheta, &sp_q, &sp_qh, &sp_th, &sp_k, &sp_dm, &sp_dmir};
+  }
+  void use_arena(Arena* a) {
+    arena = a;
+    for (DeviceBuffer* b : buffers()) b->use_arena(a);
+  }
+  // start of an allocation-free call: the whole arena is free again
+  void reset_arena() {
+    arena->reset();
+    for (DeviceBuffer* b : buffers()) b->release();
+  }
 
   template <typename F>
   void timed(const char* name, hipStream_t st, F&& launch) {
@@ -557,10 +621,15 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
                o_sorder = stage(sb, sorder), o_chunks = stage(sb, chunks), o_cseg = stage(sb, chunk_seg),
                o_groups = stage(sb, groups), o_spos = stage(sb, spos), o_bfac = stage(sb, sm.bfac);
   char* dmeta = static_cast<char*>(meta.get(sb.size()));
-  char* host = pinned.get(sb.size());
-  std::memcpy(host, sb.data(), sb.size());
-  hip_check(hipMemcpyAsync(dmeta, host, sb.size(), hipMemcpyHostToDevice, stream), "upload");
-  pinned.record(stream);
+  if (arena) {
+    // from pageable memory: the copy has read the host data when it returns
+    hip_check(hipMemcpyAsync(dmeta, sb.data(), sb.size(), hipMemcpyHostToDevice, stream), "upload");
+  } else {
+    char* host = pinned.get(sb.size());
+    std::memcpy(host, sb.data(), sb.size());
+    hip_check(hipMemcpyAsync(dmeta, host, sb.size(), hipMemcpyHostToDevice, stream), "upload");
+    pinned.record(stream);
+  }
 
   DevBatch& db = pb.db;
   db.natoms = natoms;
@@ -795,6 +864,45 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
   m.evaluate(pb, positions_dev, true, precision == kMixed, grad, pb.smooth, out, debug, stats, -1, 0, AtmShape{},
              stream);
   if (m.timer.enabled) m.timer.collect();
+}
+
+void run_noalloc(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,
+                 int precision, const BatchOutput& out, hipStream_t stream) {
+  if (systems.empty()) return;
+  for (const auto& sd : systems)
+    if (sd.lowrank) throw Error("The allocation-free entry point does not take a low-rank model");
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hipError_t err = hipSuccess;
+  void* area = noalloc_area(dev, &err);
+  hip_check(err, "hipGetSymbolAddress");
+  struct State {
+    Arena arena;
+    Workspace ws;
+  };
+  static std::mutex mu;
+  static std::map<int, State*> states;  // one per device, kept to the end of the process
+  std::lock_guard<std::mutex> lock(mu);
+  State*& st = states[dev];
+  if (!st) {
+    st = new State;
+    st->arena.bind(area, kNoAllocBytes);
+    st->ws.use_arena(&st->arena);
+  }
+  Workspace& m = st->ws;
+  m.reset_arena();
+  const Prepared pb = m.upload(systems, numbers, stream, nullptr);
+  const bool grad = out.gradient || out.sigma;
+  // the gradient is summed in the work area and copied out at the end, so
+  // that a batch rejected for its size leaves every output untouched
+  BatchOutput inner = out;
+  inner.gradient = nullptr;
+  m.evaluate(pb, positions_dev, true, precision == kMixed, grad, pb.smooth, inner, nullptr, nullptr, -1, 0,
+             AtmShape{}, stream);
+  if (out.gradient)
+    hip_check(hipMemcpyAsync(out.gradient, m.grad.as<double>(size_t(pb.db.natoms) * 3),
+                             sizeof(double) * 3 * size_t(pb.db.natoms), hipMemcpyDeviceToDevice, stream),
+              "gradient copy");
 }
 
 void Engine::run_pairwise(const std::vector<SystemDesc>& systems, const int* numbers, const double* positions_dev,

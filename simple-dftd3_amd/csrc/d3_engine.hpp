@@ -114,6 +114,16 @@ std::vector<double> lattice_points(const int periodic[3], const double lattice[9
 
 struct Workspace;
 
+// Allocation-free evaluation of a batch: as Engine::run (real-space path, no
+// low-rank model, no ghost atoms), with every work buffer taken from the
+// library's fixed work area of the current GPU (kNoAllocBytes), metadata
+// copied from pageable host memory, and no allocation, free, event or plan
+// creation.  Throws Error, before any output is written, when the batch needs
+// more than the area.  Calls that overlap on one GPU race: they share its
+// area, which keeps the contents of the last call.
+void run_noalloc(const std::vector<SystemDesc>& systems, const int* numbers_host, const double* positions_dev,
+                 int precision, const BatchOutput& out, hipStream_t stream);
+
 class Engine {
  public:
   Engine();

@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <map>
+#include <mutex>
 #include <string>
 
 #include "d3_kernels.hpp"
@@ -2918,6 +2920,25 @@ __global__ __launch_bounds__(kBlock) void rank_pos_kernel(DevBatch b, double4* _
   if (x < b.natoms) posr[x] = b.pos[b.order[x]];
 }
 }  // namespace
+
+// Work area of the allocation-free entry point.  In host code this name is a
+// stand-in, not a device address: only the address from hipGetSymbolAddress
+// (noalloc_area) may be passed to kernels or copies.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+void* noalloc_area(int device, hipError_t* hip_err) {
+  static std::mutex mu;
+  static std::map<int, void*> addr;  // per device: each GPU holds its own copy of the array
+  std::lock_guard<std::mutex> lock(mu);
+  *hip_err = hipSuccess;
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  void* p = nullptr;
+  *hip_err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (*hip_err != hipSuccess) return nullptr;
+  addr.emplace(device, p);
+  return p;
+}
 
 void launch_rank_pos(const DevBatch& b, hipStream_t st) {
   hipLaunchKernelGGL(rank_pos_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b,

@@ -216,4 +216,11 @@ void launch_count_pairs(const DevBatch& b, unsigned long long* counts, hipStream
 void launch_count_triples(const DevBatch& b, bool mixed, const long long* offset, const void* slots,
                           unsigned long long* counts, hipStream_t stream);
 
+// The allocation-free entry point's work area: a static device array of
+// kNoAllocBytes per GPU (held from the loading of the library to the end of
+// the process).  Returns its device address on the current device (looked
+// up once per device with hipGetSymbolAddress); hip_err receives the HIP
+// status of the lookup.
+void* noalloc_area(int device, hipError_t* hip_err);
+
 }  // namespace d3

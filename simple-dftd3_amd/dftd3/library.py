@@ -74,6 +74,8 @@ _SIGNATURES = {
     "d3hip_new_workspace": (_vp, [_vp]),
     "d3hip_get_dispersion_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, ctypes.c_int,
                                           _vp, _vp, _vp, _vp, _vp]),
+    "d3hip_get_dispersion_batch_noalloc": (None, [_vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, ctypes.c_int,
+                                                  _vp, _vp, _vp, _vp, _vp]),
     "d3hip_get_pairwise_dispersion_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp,
                                                    ctypes.c_int, _vp, _vp, _vp, _vp]),
     "d3hip_count_terms_batch": (None, [_vp, _vp, ctypes.c_int, _ip, _ip, _dp, _bp, _dp, _dp, _vp,
@@ -399,6 +401,16 @@ def get_dispersion_batch(workspace: ModelHandle, natoms, numbers, lattices, peri
         workspace.handle, len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices),
         _cast(_bp, periodic), _cast(_dp, params), _cast(_dp, cutoffs), precision, positions_ptr, energy_ptr,
         gradient_ptr, sigma_ptr, stream)
+
+
+def get_dispersion_batch_noalloc(natoms, numbers, lattices, periodic, params, cutoffs, precision: int,
+                                 positions_ptr: int, energy_ptr: int, gradient_ptr: Optional[int],
+                                 sigma_ptr: Optional[int], stream: Optional[int]) -> None:
+    """Batched evaluation on device pointers without any device allocation
+    (include/d3hip.h, d3hip_get_dispersion_batch_noalloc)"""
+    return error_check(lib.d3hip_get_dispersion_batch_noalloc)(
+        len(natoms), _cast(_ip, natoms), _cast(_ip, numbers), _cast(_dp, lattices), _cast(_bp, periodic),
+        _cast(_dp, params), _cast(_dp, cutoffs), precision, positions_ptr, energy_ptr, gradient_ptr, sigma_ptr, stream)
 
 
 def new_plan(natoms, numbers, lattices, periodic, params, cutoffs, precision: int, grad: bool,
