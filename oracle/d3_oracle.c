@@ -1749,6 +1749,46 @@ int d3o_get_dispersion_hessian(int nat, const int *num, const double *xyz, const
     return 0;
 }
 
+/*
+ * The three-body part of get_dispersion_hessian before the CN chain rule
+ * (disp.f90:288-298): the C6 matrices of c6_second and the four accumulators
+ * of atm_hessian (atm.f90:38-429), all column-major as FIDX / HIDX, for
+ * checks of per-triple Hessian code.  cut as d3o_get_dispersion_hessian.
+ */
+int d3o_atm_hessian_parts(int nat, const int *num, const double *xyz, const double *lattice,
+                          const int *periodic, const double *cut, double s9, double alp, double *c6, double *dc6,
+                          double *d2c6, double *d2c6ij, double *hessian, double *dEdcn, double *dEdcndr,
+                          double *dEdcndcn)
+{
+    int ndim = 3 * nat, mref = 0;
+    for (int i = 0; i < nat; ++i)
+        if (g_nref[num[i] - 1] > mref) mref = g_nref[num[i] - 1];
+    double zero_lat[9] = {0};
+    int zero_pbc[3] = {0, 0, 0};
+    if (!lattice) lattice = zero_lat;
+    if (!periodic) periodic = zero_pbc;
+    double *cn = (double *)calloc(nat, sizeof(double));
+    double *trans;
+    int ntr = lattice_alloc(periodic, lattice, cut[0], &trans);
+    d3o_cn(nat, num, xyz, ntr, trans, cut[0], cn);
+    free(trans);
+    double *gw = (double *)calloc((size_t)nat * mref, sizeof(double));
+    double *gwd = (double *)calloc((size_t)nat * mref, sizeof(double));
+    double *gwd2 = (double *)calloc((size_t)nat * mref, sizeof(double));
+    weights2(nat, num, cn, mref, gw, gwd, gwd2);
+    c6_second(nat, num, mref, gw, gwd, gwd2, NULL, c6, dc6, d2c6, d2c6ij);
+    memset(hessian, 0, sizeof(double) * (size_t)ndim * ndim);
+    memset(dEdcn, 0, sizeof(double) * nat);
+    memset(dEdcndr, 0, sizeof(double) * 3 * (size_t)nat * nat);
+    memset(dEdcndcn, 0, sizeof(double) * (size_t)nat * nat);
+    ntr = lattice_alloc(periodic, lattice, cut[2], &trans);
+    atm_hessian(nat, num, xyz, ntr, trans, cut[2], cut[4], s9, 4.0 / 3.0, alp + 2.0, c6, dc6, d2c6, d2c6ij,
+                hessian, dEdcn, dEdcndr, dEdcndcn);
+    free(trans);
+    free(cn); free(gw); free(gwd); free(gwd2);
+    return 0;
+}
+
 /* ------------------------------------------------------ reciprocal space */
 /* fourier/jacobi.f90:40-122 symmetric_eigendecomposition: cyclic Jacobi
  * rotations (at most 100 sweeps), eigenvalues sorted by decreasing magnitude.

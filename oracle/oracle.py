@@ -244,6 +244,28 @@ def hessian(numbers, positions, lattice=None, periodic=None, kind=RATIONAL, para
     return dict(energy=float(energy[0]), hessian=h.T.copy())
 
 
+def atm_hessian_parts(numbers, positions, lattice=None, periodic=None, s9=1.0, alp=14.0, cutoff=DEFAULT_CUTOFF):
+    """The three-body part of the Hessian before the CN chain rule
+    (atm.f90:38-429, as called from disp.f90:288-298), under the partition
+    of set_partition.  Row-major: c6 etc. [i, j] with the derivatives by
+    CN_i, hessian [3 i + a, 3 j + b], dEdcndr [3 i + a, k], dEdcndcn [k, l]."""
+    L = lib()
+    num, xyz, lat, pbc = _geom(numbers, positions, lattice, periodic)
+    cut = np.array(cutoff, dtype=np.float64)
+    n = len(num)
+    mats = [np.zeros((n, n)) for _ in range(4)]
+    h = np.zeros((3 * n, 3 * n))
+    dedcn = np.zeros(n)
+    dedcndr = np.zeros((3 * n, n))
+    dedcndcn = np.zeros((n, n))
+    L.d3o_atm_hessian_parts(n, _ptr(num, _ip), _ptr(xyz), _ptr(lat), _ptr(pbc, _ip), _ptr(cut),
+                            ctypes.c_double(s9), ctypes.c_double(alp), *[_ptr(m) for m in mats], _ptr(h),
+                            _ptr(dedcn), _ptr(dedcndr), _ptr(dedcndcn))
+    c6, dc6, d2c6, d2c6ij = (m.T.copy() for m in mats)
+    return dict(c6=c6, dc6=dc6, d2c6=d2c6, d2c6ij=d2c6ij, hessian=h.T.copy(), dedcn=dedcn, dedcndr=dedcndr,
+                dedcndcn=dedcndcn)
+
+
 class EwaldUnsupported(ValueError):
     """The damping function has no reciprocal-space representation."""
 

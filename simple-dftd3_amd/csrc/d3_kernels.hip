@@ -19,6 +19,7 @@
 #include <mutex>
 #include <string>
 
+#include "d3_hess3.hpp"
 #include "d3_kernels.hpp"
 
 namespace d3 {
@@ -129,24 +130,6 @@ __device__ __forceinline__ double atm_vec(const double4& xj, const double4& t, c
   v1 = xj.y + t.y - xi.y;
   v2 = xj.z + t.z - xi.z;
   return v0 * v0 + v1 * v1 + v2 * v2;
-}
-
-// vjk = vik - vij (atm.f90:723) and its squared length, as the reference.
-__device__ __forceinline__ double diff_norm2(double bx, double by, double bz, double ax, double ay, double az,
-                                             double& v0, double& v1, double& v2) {
-#pragma clang fp contract(off)
-  v0 = bx - ax;
-  v1 = by - ay;
-  v2 = bz - az;
-  return v0 * v0 + v1 * v1 + v2 * v2;
-}
-
-// partition.f90:127-148 owns_pair with 0-based atom indices il >= jl: the
-// pair's index in the lower triangle, il (il + 1) / 2 + jl, modulo nparts.
-__device__ __forceinline__ bool owns_pair(const SysParam& p, int il, int jl) {
-  if (p.nparts <= 1) return true;
-  const long long idx = static_cast<long long>(il) * (il + 1) / 2 + jl;
-  return idx % p.nparts == p.part;
 }
 
 // cutoff.f90:73-99 smooth_cutoff
@@ -1555,26 +1538,6 @@ __global__ __launch_bounds__(kBlock) void hess_dcndr_kernel(DevBatch b, double* 
   }
 }
 
-// cutoff.f90:104-137 smooth_cutoff_r2.
-__device__ __forceinline__ void switch_r2(double r2, double cutoff, double width, double& sw, double& dsw,
-                                          double& d2sw) {
-  const double r1 = sqrt(r2);
-  double dswdr = 0.0, d2swdr2 = 0.0;
-  sw = 1.0;
-  if (width > 0.0 && width < cutoff && r1 > cutoff - width) {
-    if (r1 >= cutoff) {
-      sw = 0.0;
-    } else {
-      const double x = (cutoff - r1) / width;
-      sw = x * x * x * (10.0 + x * (-15.0 + 6.0 * x));
-      dswdr = -30.0 * x * x * (1.0 - x) * (1.0 - x) / width;
-      d2swdr2 = 60.0 * x * (1.0 - x) * (1.0 - 2.0 * x) / (width * width);
-    }
-  }
-  dsw = 0.5 * dswdr / r1;
-  d2sw = 0.25 * (d2swdr2 - dswdr / r1) / r2;
-}
-
 // get_damping_kernel of every family (rational.f90, zero.f90, mzero.f90,
 // optimizedpower.f90, cso.f90, z.f90): e(u, C6) for u = r^2 and e_u, e_c,
 // e_uu, e_uc, e_cc.
@@ -1791,6 +1754,32 @@ __global__ __launch_bounds__(kBlock) void hess2_kernel(DevBatch b, const double*
 // Hessian blocks, dEdcndr, dEdcndcn and dEdcn entries every term of the
 // wavefront shares) are summed in registers and added once per wavefront;
 // the entries of the atom of b are added with atomics.
+// Accumulation of hess3_triple for one wavefront: the entries among triple
+// positions 0 and 1 (atom i and the atom of slot a, the same for every lane)
+// in registers, added once per wavefront; the others with FP64 atomics.
+struct Hess3DeviceSink {
+  double hf_[2][2][9], df_[2][3][2], cf_[2][2], ef_[2];
+  double* hess;      // the system's 3n x 3n block
+  double* dedcn;     // the system's atoms
+  double* dEdcndr;   // the system's 3n x n block
+  double* dEdcndcn;  // the system's n x n block
+  int n, ndim;
+  __device__ __forceinline__ void hf(int x, int y, int k, double v) { hf_[x][y][k] += v; }
+  __device__ __forceinline__ void h(int row, int col, double v) {
+    atomicAdd(hess + static_cast<long long>(row) * ndim + col, v);
+  }
+  __device__ __forceinline__ void cf(int x, int y, double v) { cf_[x][y] += v; }
+  __device__ __forceinline__ void c(int r, int col, double v) {
+    atomicAdd(dEdcndcn + static_cast<long long>(r) * n + col, v);
+  }
+  __device__ __forceinline__ void df(int x, int c, int y, double v) { df_[x][c][y] += v; }
+  __device__ __forceinline__ void d(int row, int atom, double v) {
+    atomicAdd(dEdcndr + static_cast<long long>(row) * n + atom, v);
+  }
+  __device__ __forceinline__ void ef(int x, double v) { ef_[x] += v; }
+  __device__ __forceinline__ void e(int atom, double v) { atomicAdd(dedcn + atom, v); }
+};
+
 __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long long* __restrict__ offset,
                                                        const Slot<double>* __restrict__ slots,
                                                        const double* __restrict__ c6,
@@ -1810,239 +1799,62 @@ __global__ __launch_bounds__(kBlock) void hess3_kernel(DevBatch b, const long lo
     else hi = mid;
   }
   const int i = lo;
-  const int s = b.sys[i];
-  const SysGeom g = b.geom[s];
-  const SysParam p = b.par[s];
-  const int n = g.natoms, ndim = 3 * n;
+  const SysGeom g = b.geom[b.sys[i]];
+  const int n = g.natoms;
   const long long c0 = g.c6_begin;
-  const double cut2 = p.cut3 * p.cut3, alp3 = (p.alp + 2.0) / 3.0, aexp = 0.5 * alp3;
-  const int sigp[3][3] = {{-1, -1, 0}, {1, 0, -1}, {0, 1, 1}};   // du_pair/dR_atom = 2 sigp[atom][pair] v_pair
-  const double cl[3][3] = {{1, -1, 1}, {1, 1, -1}, {-1, 1, 1}};  // coefficient of u_pair in l_l
-  const Slot<double> sa = slots[t];
-  const long long end = offset[i + 1];
-  const int at0 = i - g.atom_begin, at1 = sa.atom - g.atom_begin;
-  // Entries between the triple positions 0 (atom i) and 1 (atom of slot a),
-  // the same for every lane, are summed in registers (static indices) and
-  // added once per wavefront; entries with position 2 (atom of slot b) are
-  // global atomics.
-  double hf[2][2][9], df[2][3][2], cf[2][2], ef[2];
+  const Hess3System sys{b.par[b.sys[i]], n, c6 + c0, dc6 + c0, d2c6 + c0, d2c6ij + c0, b.rvdw, b.nsp};
+  Hess3DeviceSink sink;
 #pragma unroll
   for (int x = 0; x < 2; ++x) {
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
-      for (int c = 0; c < 9; ++c) hf[x][y][c] = 0.0;
+      for (int c = 0; c < 9; ++c) sink.hf_[x][y][c] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) df[x][c][0] = df[x][c][1] = 0.0;
-    cf[x][0] = cf[x][1] = 0.0;
-    ef[x] = 0.0;
+    for (int c = 0; c < 3; ++c) sink.df_[x][c][0] = sink.df_[x][c][1] = 0.0;
+    sink.cf_[x][0] = sink.cf_[x][1] = 0.0;
+    sink.ef_[x] = 0.0;
   }
+  sink.hess = hess + 9 * c0;
+  sink.dedcn = dedcn + g.atom_begin;
+  sink.dEdcndr = dEdcndr + 3 * c0;
+  sink.dEdcndcn = dEdcndcn + c0;
+  sink.n = n;
+  sink.ndim = 3 * n;
+  const Slot<double> sa = slots[t];
+  const long long end = offset[i + 1];
+  const int at0 = i - g.atom_begin, at1 = sa.atom - g.atom_begin;
+  const int sp0 = b.sp[i], sp1 = b.sp[sa.atom];
+  const double va[3] = {sa.vx, sa.vy, sa.vz};
   for (long long bi = t + 1 + lane; bi < end; bi += kWave) {
     const Slot<double> sb = slots[bi];
-    double vec[3][3];
-    vec[0][0] = sa.vx; vec[0][1] = sa.vy; vec[0][2] = sa.vz;
-    vec[1][0] = sb.vx; vec[1][1] = sb.vy; vec[1][2] = sb.vz;
-    double u[3];
-    u[2] = diff_norm2(sb.vx, sb.vy, sb.vz, sa.vx, sa.vy, sa.vz, vec[2][0], vec[2][1], vec[2][2]);
-    if (u[2] > cut2 || u[2] < kEps) continue;
-    u[0] = sa.r2;
-    u[1] = sb.r2;
-    const int at2 = sb.atom - g.atom_begin;
-    const int at[3] = {at0, at1, at2};
-    const double c6t[3] = {c6[c0 + static_cast<long long>(at1) * n + at0],
-                           c6[c0 + static_cast<long long>(at2) * n + at0],
-                           c6[c0 + static_cast<long long>(at2) * n + at1]};
-    if (fabs(c6t[0]) < kEps || fabs(c6t[1]) < kEps || fabs(c6t[2]) < kEps) continue;
-    const bool selfa = sa.atom == i, selfb = sb.atom == i;
-    // a triple of another part of a work partition (atm.f90:136) gets weight
-    // 0: a zero contribution rather than a branch out of this loop
-    const double w = owns_pair(p, at0, max(at1, at2))
-                         ? (selfa ? 0.5 : 1.0) * (selfb ? 0.5 : 1.0) * (selfa && selfb ? 4.0 / 3.0 : 1.0)
-                         : 0.0;
-    double swp[3], dswp[3], d2swp[3];
-    for (int q = 0; q < 3; ++q) switch_r2(u[q], p.cut3, p.width3, swp[q], dswp[q], d2swp[q]);
-    const double sval = swp[0] * swp[1] * swp[2];
-    const double sp[3] = {dswp[0] * swp[1] * swp[2], swp[0] * dswp[1] * swp[2], swp[0] * swp[1] * dswp[2]};
-    double spq[3][3];
-    spq[0][0] = d2swp[0] * swp[1] * swp[2];
-    spq[1][1] = swp[0] * d2swp[1] * swp[2];
-    spq[2][2] = swp[0] * swp[1] * d2swp[2];
-    spq[0][1] = spq[1][0] = dswp[0] * dswp[1] * swp[2];
-    spq[0][2] = spq[2][0] = dswp[0] * swp[1] * dswp[2];
-    spq[1][2] = spq[2][1] = swp[0] * dswp[1] * dswp[2];
-    const double ww = u[0] * u[1] * u[2];
-    const double wp[3] = {u[1] * u[2], u[0] * u[2], u[0] * u[1]};
-    const double wpq[3][3] = {{0.0, u[2], u[1]}, {u[2], 0.0, u[0]}, {u[1], u[0], 0.0}};
-    const double lv[3] = {u[0] + u[2] - u[1], u[0] - u[2] + u[1], -u[0] + u[2] + u[1]};
-    const double nval = lv[0] * lv[1] * lv[2];
-    double np_[3], npq[3][3];
-    for (int q = 0; q < 3; ++q) np_[q] = cl[0][q] * lv[1] * lv[2] + cl[1][q] * lv[0] * lv[2] + cl[2][q] * lv[0] * lv[1];
-    for (int q = 0; q < 3; ++q)
-      for (int r = 0; r < 3; ++r) {
-        double tmp = 0.0;
-        for (int l = 0; l < 3; ++l)
-          for (int m = 0; m < 3; ++m)
-            if (l != m) tmp += cl[l][q] * cl[m][r] * lv[3 - l - m];
-        npq[q][r] = tmp;
-      }
-    const double gv = pow(ww, -2.5), gd = -2.5 * pow(ww, -3.5), gdd = 8.75 * pow(ww, -4.5);
-    const double hv = pow(ww, -1.5), hd = -1.5 * pow(ww, -2.5), hdd = 3.75 * pow(ww, -3.5);
-    const double ang = 0.375 * nval * gv + hv;
-    const int spi = b.sp[i], spa = b.sp[sa.atom], spb = b.sp[sb.atom];
-    const double r0 = kRs9 * b.rvdw[spa * b.nsp + spi] * kRs9 * b.rvdw[spb * b.nsp + spi] * kRs9 *
-                      b.rvdw[spb * b.nsp + spa];
-    const double tval = 6.0 * pow(r0, alp3) * pow(ww, -aexp);
-    const double fd = 1.0 / (1.0 + tval);
-    double angp[3], tp[3], fdp[3], ap[3], kp[3];
-    for (int q = 0; q < 3; ++q) {
-      angp[q] = 0.375 * (np_[q] * gv + nval * gd * wp[q]) + hd * wp[q];
-      tp[q] = -aexp * tval * wp[q] / ww;
-      fdp[q] = -tp[q] * fd * fd;
-    }
-    const double av = ang * fd;
-    for (int q = 0; q < 3; ++q) ap[q] = angp[q] * fd + ang * fdp[q];
-    const double kv = sval * av;
-    for (int q = 0; q < 3; ++q) kp[q] = sp[q] * av + sval * ap[q];
-    double kpq[3][3];
-    for (int q = 0; q < 3; ++q)
-      for (int r = 0; r < 3; ++r) {
-        const double angpq = 0.375 * (npq[q][r] * gv + np_[q] * gd * wp[r] + np_[r] * gd * wp[q] +
-                                      nval * (gdd * wp[q] * wp[r] + gd * wpq[q][r])) +
-                             hdd * wp[q] * wp[r] + hd * wpq[q][r];
-        const double tpq = aexp * (aexp + 1.0) * tval * wp[q] * wp[r] / (ww * ww) - aexp * tval * wpq[q][r] / ww;
-        const double fdpq = -tpq * fd * fd + 2.0 * tp[q] * tp[r] * fd * fd * fd;
-        const double apq = angpq * fd + angp[q] * fdp[r] + angp[r] * fdp[q] + ang * fdpq;
-        kpq[q][r] = spq[q][r] * av + sp[q] * ap[r] + sp[r] * ap[q] + sval * apq;
-      }
-    const double qv = sqrt(fabs(c6t[0] * c6t[1] * c6t[2]));
-    const double pref = p.s9 * w, pq = pref * qv;
-    // Cartesian second derivatives at fixed CN
-    double gk[3][3][3];
-    for (int a = 0; a < 3; ++a)
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-          double tmp = 0.0;
-          for (int q = 0; q < 3; ++q) tmp += kpq[q][r] * 2.0 * sigp[a][q] * vec[q][c];
-          gk[a][r][c] = pq * tmp;
-        }
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int bb = 0; bb < 3; ++bb) {
-        double dg = 0.0;
-        for (int q = 0; q < 3; ++q) dg += kp[q] * sigp[a][q] * sigp[bb][q];
-        dg *= 2.0 * pq;
-        double* __restrict__ blk = hess + 9 * c0 + static_cast<long long>(3 * at[a]) * ndim + 3 * at[bb];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            double tmp = c == d ? dg : 0.0;
-            for (int r = 0; r < 3; ++r) tmp += gk[a][r][c] * 2.0 * sigp[bb][r] * vec[r][d];
-            if (a < 2 && bb < 2) hf[a < 2 ? a : 0][bb < 2 ? bb : 0][3 * c + d] += tmp;
-            else atomicAdd(blk + static_cast<long long>(c) * ndim + d, tmp);
-          }
-      }
-    // derivatives through C6: pairs (i, a), (i, b), (a, b)
-    double ec[3], euc[3][3], ecc[3][3];
-    for (int q = 0; q < 3; ++q) {
-      const double qm = 0.5 * qv / c6t[q];
-      ec[q] = pref * qm * kv;
-      for (int r = 0; r < 3; ++r) {
-        const double qmn = q == r ? -0.25 * qv / (c6t[q] * c6t[q]) : 0.25 * qv / (c6t[q] * c6t[r]);
-        ecc[q][r] = pref * qmn * kv;
-        euc[r][q] = pref * qm * kp[r];
-      }
-    }
-    // the (position, pair) entries of dC6/dCN: both atoms of each pair, or
-    // one entry of twice the derivative when the pair's two atoms coincide
-    // (the second then has coefficient 0)
-    constexpr int kEntPos[6] = {0, 1, 0, 2, 1, 2}, kEntPair[6] = {0, 0, 1, 1, 2, 2};
-    double ent_coef[6];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int a0 = at[kEntPos[2 * q]], a1 = at[kEntPos[2 * q + 1]];
-      if (a0 != a1) {
-        ent_coef[2 * q] = dc6[c0 + static_cast<long long>(a0) * n + a1];
-        ent_coef[2 * q + 1] = dc6[c0 + static_cast<long long>(a1) * n + a0];
-      } else {
-        ent_coef[2 * q] = 2.0 * dc6[c0 + static_cast<long long>(a0) * n + a0];
-        ent_coef[2 * q + 1] = 0.0;
-      }
-    }
-#pragma unroll
-    for (int ie = 0; ie < 6; ++ie) {
-      const int kpos = kEntPos[ie], kq = kEntPair[ie], ka = at[kpos];
-      const double ce = ent_coef[ie];
-      if (ce == 0.0) continue;
-      if (kpos < 2) ef[kpos < 2 ? kpos : 0] += ec[kq] * ce;
-      else atomicAdd(dedcn + g.atom_begin + ka, ec[kq] * ce);
-#pragma unroll
-      for (int jf = 0; jf < 6; ++jf) {
-        const int jpos = kEntPos[jf];
-        const double v = ecc[kq][kEntPair[jf]] * ce * ent_coef[jf];
-        if (kpos < 2 && jpos < 2) cf[kpos < 2 ? kpos : 0][jpos < 2 ? jpos : 0] += v;
-        else if (ent_coef[jf] != 0.0) atomicAdd(dEdcndcn + c0 + static_cast<long long>(ka) * n + at[jpos], v);
-      }
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          double egr = 0.0;
-          for (int q = 0; q < 3; ++q) egr += euc[q][kq] * 2.0 * sigp[a][q] * vec[q][c];
-          if (a < 2 && kpos < 2) df[a < 2 ? a : 0][c][kpos < 2 ? kpos : 0] += egr * ce;
-          else atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * at[a] + c) * n + ka, egr * ce);
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int p0 = kEntPos[2 * q], p1 = kEntPos[2 * q + 1];  // positions of the pair's atoms
-      const int a0 = at[p0], a1 = at[p1];
-      const long long o00 = c0 + static_cast<long long>(a0) * n + a0, o01 = c0 + static_cast<long long>(a0) * n + a1;
-      if (a0 != a1) {
-        const long long o11 = c0 + static_cast<long long>(a1) * n + a1, o10 = c0 + static_cast<long long>(a1) * n + a0;
-        const double v00 = ec[q] * d2c6[o01], v11 = ec[q] * d2c6[o10], vx = ec[q] * d2c6ij[o01];
-        if (p1 < 2) {  // pair (i, a): positions 0 and 1
-          cf[0][0] += v00;
-          cf[1][1] += v11;
-          cf[0][1] += vx;
-          cf[1][0] += vx;
-        } else {
-          atomicAdd(dEdcndcn + o00, v00);
-          atomicAdd(dEdcndcn + o11, v11);
-          atomicAdd(dEdcndcn + o01, vx);
-          atomicAdd(dEdcndcn + o10, vx);
-        }
-      } else {
-        const double v = ec[q] * (2.0 * d2c6[o00] + 2.0 * d2c6ij[o00]);
-        if (p1 < 2) cf[0][0] += v;
-        else atomicAdd(dEdcndcn + o00, v);
-      }
-    }
+    const double vb[3] = {sb.vx, sb.vy, sb.vz};
+    hess3_triple(sys, at0, at1, sb.atom - g.atom_begin, sp0, sp1, b.sp[sb.atom], sa.atom == i, sb.atom == i, va,
+                 sa.r2, vb, sb.r2, sink);
   }
   // the entries between positions 0 and 1, once per wavefront (both refer
   // to atom i when slot a is an image of it)
   const int fat[2] = {at0, at1};
+  const int ndim = 3 * n;
 #pragma unroll
   for (int x = 0; x < 2; ++x) {
 #pragma unroll
     for (int y = 0; y < 2; ++y) {
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
-        const double v = wave_sum(hf[x][y][c]);
+        const double v = wave_sum(sink.hf_[x][y][c]);
         if (lane == 0 && v != 0.0)
           atomicAdd(hess + 9 * c0 + static_cast<long long>(3 * fat[x] + c / 3) * ndim + 3 * fat[y] + c % 3, v);
       }
-      const double v = wave_sum(cf[x][y]);
+      const double v = wave_sum(sink.cf_[x][y]);
       if (lane == 0 && v != 0.0) atomicAdd(dEdcndcn + c0 + static_cast<long long>(fat[x]) * n + fat[y], v);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double w = wave_sum(df[x][c][y]);
+        const double w = wave_sum(sink.df_[x][c][y]);
         if (lane == 0 && w != 0.0) atomicAdd(dEdcndr + 3 * c0 + static_cast<long long>(3 * fat[x] + c) * n + fat[y], w);
       }
     }
-    const double v = wave_sum(ef[x]);
+    const double v = wave_sum(sink.ef_[x]);
     if (lane == 0 && v != 0.0) atomicAdd(dedcn + g.atom_begin + fat[x], v);
   }
 }
