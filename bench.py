@@ -272,7 +272,7 @@ def bench_autograd(args, systems, config, spec, positions, counts):
         vs_baseline=None, dtype="f32", data="synthetic",
         config=dict(config, pair_terms=int(counts[1]), triple_terms=int(counts[2]), cn_terms=int(counts[0]),
                     parallelism="single-gpu", accumulation="f64",
-                    step="autograd forward (energy kernels) + backward (gradient kernels, adjoints)"),
+                    step="autograd forward (one E+grad+sigma evaluation) + backward (adjoints from the saved gradient and virial)"),
         molecules_per_s=mixed["cells_per_s"], triple_terms_per_s=mixed["triple_terms_per_s"], fp64=fp64,
         mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"],
         mixed_vs_fp64_error=dict(energy_per_atom=de, max_position_adjoint=dg, max_lattice_adjoint=dl),

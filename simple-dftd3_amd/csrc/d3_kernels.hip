@@ -862,27 +862,57 @@ __device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restr
   return -1;
 }
 
-// Adds the sums of slot s (vector v, atom k) to the gradient and dE/dCN of
-// atom k and to the i-side sums:
-//   grad_k += G,  grad_i -= G,  dEdcn_k -= D / 2,  sigma += sym(v G^T).
-// The virial of a triple (i, a, b) with pair forces f_X v_X is
-// f_A v_a v_a^T + f_B v_b v_b^T + f_C v_ab v_ab^T = v_a G_a^T + v_b G_b^T
-// (v relative to atom i, G_s = dE/dv_s), so the slot sums carry it.
-__device__ __forceinline__ void flush_slot(double Gx, double Gy, double Gz, double D, double vx, double vy,
+// Segmented sums over the lanes of a wavefront: lane l holds the values of
+// slot l of a tile and its atom k (k < 0: no slot).  A list holds the images
+// of an atom one after another, so the slots of one atom are a run of lanes;
+// on return the first lane of every run (head) holds the run's sums.  Every
+// lane of the wavefront takes part.
+template <int N>
+__device__ __forceinline__ bool run_sums(double (&v)[N], int k) {
+  const int l = wave_lane();
+  const int knext = __shfl_down(k, 1, kWave);
+  const unsigned long long last = __ballot(l == kWave - 1 || knext != k);
+  if (~last != 0ull) {  // some run is longer than one slot
+    const int e = l + __builtin_ctzll(last >> l);  // last lane of this lane's run
+    for (int off = 1; off < kWave; off <<= 1) {
+      double t[N];
+#pragma unroll
+      for (int c = 0; c < N; ++c) t[c] = __shfl_down(v[c], off, kWave);
+      if (l + off <= e) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) v[c] += t[c];
+      }
+    }
+  }
+  const int kprev = __shfl_up(k, 1, kWave);
+  return l == 0 || kprev != k;
+}
+
+// Adds the slot sums of one tile (lane l: slot l with vector v, sums G and
+// D, atom k; k < 0 for no slot) to the gradient and dE/dCN of the slots'
+// atoms and to the i-side sums:
+//   grad_k += G,  grad_i -= G,  dEdcn_k -= D / 2,  sigma += sym(v G^T),
+// one global atomic per atom and value (the runs of one atom's images are
+// summed first).  Every lane of the wavefront calls it.
+__device__ __forceinline__ void flush_tile(double Gx, double Gy, double Gz, double D, double vx, double vy,
                                            double vz, int k, double* __restrict__ grad, double* __restrict__ dedcn,
                                            double* gi, double* sg) {
-  if (D == 0.0 && Gx == 0.0 && Gy == 0.0 && Gz == 0.0) return;
-  atomicAdd(grad + 3 * k, Gx);
-  atomicAdd(grad + 3 * k + 1, Gy);
-  atomicAdd(grad + 3 * k + 2, Gz);
-  atomicAdd(dedcn + k, -0.5 * D);
-  gi[0] -= Gx; gi[1] -= Gy; gi[2] -= Gz;
-  sg[0] += vx * Gx;
-  sg[1] += vy * Gy;
-  sg[2] += vz * Gz;
-  sg[3] += 0.5 * (vx * Gy + vy * Gx);
-  sg[4] += 0.5 * (vx * Gz + vz * Gx);
-  sg[5] += 0.5 * (vy * Gz + vz * Gy);
+  if (k >= 0) {
+    gi[0] -= Gx; gi[1] -= Gy; gi[2] -= Gz;
+    sg[0] += vx * Gx;
+    sg[1] += vy * Gy;
+    sg[2] += vz * Gz;
+    sg[3] += 0.5 * (vx * Gy + vy * Gx);
+    sg[4] += 0.5 * (vx * Gz + vz * Gx);
+    sg[5] += 0.5 * (vy * Gz + vz * Gy);
+  }
+  double v[4] = {Gx, Gy, Gz, D};
+  const bool head = run_sums(v, k);
+  if (!head || k < 0 || (v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0 && v[3] == 0.0)) return;
+  atomicAdd(grad + 3 * k, v[0]);
+  atomicAdd(grad + 3 * k + 1, v[1]);
+  atomicAdd(grad + 3 * k + 2, v[2]);
+  atomicAdd(dedcn + k, -0.5 * v[3]);
 }
 
 // Three-body hot path: x**y for x > 0, 1/x and 1/sqrt(x) for normal x > 0.
@@ -926,7 +956,7 @@ template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return
 // f_X = -c9 fdmp S^(-5/2) (3/4 X dP/dX + K1) / X with the damping part
 // K1 = -15/8 P - 3S + 2 alp pp fdmp (3/8 P + S); dG_X = f_X v_X.
 // The per-slot sums G, D (see kAcc) give the gradient and virial when the
-// slot is flushed (flush_slot); the weight of a triple, triple_scale times 2
+// slot is flushed (flush_tile); the weight of a triple, triple_scale times 2
 // for two images of one atom (atm.f90:954-980), is 1/2 per slot that is an
 // image of atom i (folded into sc6h) and 4/3 more when both are.
 // kSmooth adds the switching function of a nonzero width3 (cutoff.f90:73-99,
@@ -946,6 +976,13 @@ template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return
 // flushed.  The pair energy of (x, y) is then pair3s(rx, ry) + pair3s(ry, rx)
 // (pair3_finalize_kernel).
 constexpr int kAccPairs = 1;  // per-slot sums of the pairwise mode: sum t
+
+// Lane l receives the value of lane (l + 1) mod 64 (DPP wave_rol:1; every
+// lane has a source, so the move's old value is never used).
+__device__ __forceinline__ int wave_rotate1(int x) { return __builtin_amdgcn_update_dpp(x, x, 0x134, 0xF, 0xF, false); }
+__device__ __forceinline__ double wave_rotate1(double v) {
+  return __hiloint2double(wave_rotate1(__double2hiint(v)), wave_rotate1(__double2loint(v)));
+}
 
 template <typename R, bool kGrad, bool kSmooth, int kW, bool kPairs, bool kPart = false>
 __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict__ item_off,
@@ -1044,18 +1081,15 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
     __builtin_amdgcn_wave_barrier();
     const int s0 = diag ? 1 : 0, s1 = diag ? kTile / 2 : kTile - 1;
     int st = s0, head = 0, tail = 0;
-    // the b-slot vector of the next test step, read one step ahead
-    double nbx = TB.vx[(l + s0) & (kTile - 1)], nby = TB.vy[(l + s0) & (kTile - 1)],
-           nbz = TB.vz[(l + s0) & (kTile - 1)];
+    // the b-slot of lane l at step st is (l + st) mod 64: its vector and key
+    // rotate by one lane after every step
+    double bx = TB.vx[(l + s0) & (kTile - 1)], by = TB.vy[(l + s0) & (kTile - 1)],
+           bz = TB.vz[(l + s0) & (kTile - 1)];
+    int2 kbr = TB.key[(l + s0) & (kTile - 1)];
     while (true) {
       // test phase: append the accepted pairs of the next steps
       for (; st <= s1 && tail - head <= kQueue - kWave; ++st) {
         const int bl = (l + st) & (kTile - 1);
-        const double bx = nbx, by = nby, bz = nbz;
-        const int bn = (bl + 1) & (kTile - 1);  // in range for every step: the read past s1 is unused
-        nbx = TB.vx[bn];
-        nby = TB.vy[bn];
-        nbz = TB.vz[bn];
         double v0, v1, v2;
         const double Cd = diff_norm2(bx, by, bz, ax, ay, az, v0, v1, v2);
         const double cut2_step = (diag && st == kTile / 2) ? cut2_half : cut2;
@@ -1067,7 +1101,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
           const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
                                      static_cast<unsigned>(bal >> 32),
                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
-          const int2 kb = TB.key[bl];
+          const int2 kb = kbr;
           PairEntry<R> e;
           const unsigned long long rec = arow + static_cast<unsigned>(kb.y);
           e.lo = (static_cast<unsigned>(kb.x) & amask) | static_cast<unsigned>(l) | (static_cast<unsigned>(rec) << 17);
@@ -1076,6 +1110,11 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
           q_ring[pos & (kQueue - 1)] = e;
         }
         tail += __popcll(bal);
+        bx = wave_rotate1(bx);
+        by = wave_rotate1(by);
+        bz = wave_rotate1(bz);
+        kbr.x = wave_rotate1(kbr.x);
+        kbr.y = wave_rotate1(kbr.y);
       }
       const bool done = st > s1;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -1166,22 +1205,24 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       const double Gx = accB[0 * kTile + l], Gy = accB[1 * kTile + l], Gz = accB[2 * kTile + l];
       const double D = accB[3 * kTile + l];
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
-      if (Q * kTile + l < m) flush_slot(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, grad, dedcn, gi, sg);
+      flush_tile(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, grad, dedcn, gi, sg);
     }
     if (kPairs && !diag) {  // pair {i, b-slot atom}
-      const double t = accB[l];
+      double t[1] = {accB[l]};
       accB[l] = 0.0;
-      if (Q * kTile + l < m && t != 0.0) atomicAdd(pair3s + g.c6_begin + irow + TB.key[l].y, t);
+      if (run_sums(t, katl) && katl >= 0 && t[0] != 0.0) atomicAdd(pair3s + g.c6_begin + irow + TB.key[l].y, t[0]);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  if (kGrad && w == 0 && aval)
-    flush_slot(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l], ax, ay, az,
+  if (kGrad && w == 0)
+    flush_tile(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l], ax, ay, az,
                jat, grad, dedcn, gi, sg);
-  if (kPairs && w == 0 && aval && accA[l] != 0.0)  // pair {i, a-slot atom}
-    atomicAdd(pair3s + g.c6_begin + irow + akey.y, accA[l]);
+  if (kPairs && w == 0) {  // pair {i, a-slot atom}
+    double t[1] = {accA[l]};
+    if (run_sums(t, jat) && jat >= 0 && t[0] != 0.0) atomicAdd(pair3s + g.c6_begin + irow + akey.y, t[0]);
+  }
   block_add(e_atom + i, ei, red);
   if (kGrad) {
     block_add(grad + 3 * i, gi[0], red);
