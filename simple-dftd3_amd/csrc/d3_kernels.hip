@@ -923,6 +923,25 @@ template <> __device__ __forceinline__ double pow_pos<double>(double x, double y
 template <> __device__ __forceinline__ float pow_pos<float>(float x, float y) {
   return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
 }
+// x**(-8/3) for normal x > 0, the damping power of the default alp = 14
+// (alp + 2 = 16, (r0/r)^((alp + 2)/3) with x = r^2): the FP32 hardware
+// estimate scaled into range in FP64, then two Newton steps on
+// t^6 x^16 = 1, t <- t + t (1 - t^6 x^16) / 6 (quadratic convergence from
+// ~2e-7: about one ulp).  The general power costs the device library's
+// log2 and exp2 in FP64.
+__device__ __forceinline__ double pow_m8_3(double x) {
+  const float e = (-8.0f / 3.0f) * __builtin_amdgcn_logf(static_cast<float>(x));
+  const float k = __builtin_rintf(e);
+  double t = ldexp(static_cast<double>(__builtin_amdgcn_exp2f(e - k)), static_cast<int>(k));
+  const double x2 = x * x, x4 = x2 * x2, x8 = x4 * x4, x16 = x8 * x8;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double t2 = t * t, t6 = t2 * t2 * t2;
+    const double r = fma(-t6, x16, 1.0);
+    t = fma(t * r, 1.0 / 6.0, t);
+  }
+  return t;
+}
 template <typename R> __device__ __forceinline__ R rcp_fast(R x);
 template <> __device__ __forceinline__ double rcp_fast<double>(double x) { return 1.0 / x; }
 template <> __device__ __forceinline__ float rcp_fast<float>(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -1034,6 +1053,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   const R alp = R(p.alp) + R(2);
   const R alp2 = R(2) * alp;
   const R malp6 = -alp / R(6);
+  const bool alp14 = sizeof(R) == 8 && p.alp == 14.0;  // FP64 with the default alp: pow_m8_3
   const R ns9 = -R(p.s9);
   const R cutr = R(p.cut3), width = R(p.width3);
   const R4<R>* __restrict__ prow = pair + g.c6_begin;  // n x n pair records of the system
@@ -1141,7 +1161,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
           const R Pt = t1 * u;
           const R core = R(0.375) * Pt + S;
           const R ang = ir5 * core;
-          const R pp = a1.x * b1.x * (q.w * pow_pos(C, malp6));
+          const R pp = a1.x * b1.x * (q.w * (alp14 ? R(pow_m8_3(double(C))) : pow_pos(C, malp6)));
           const R fd = rcp_fast(R(1) + R(6) * pp);
           R c9 = ns9 * a1.y * b1.y * q.x;
           if ((e.lo >> 16) & 1u) c9 *= R(4.0 / 3.0);  // j == k == i
