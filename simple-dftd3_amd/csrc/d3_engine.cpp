@@ -317,8 +317,8 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
 }
 
 struct Workspace {
-  DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, cnt, off, items,
-      item_off, slots, gw_rank, sp_rank, pos_rank, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
+  DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, atm_acc, cnt, off,
+      items, item_off, slots, gw_rank, sp_rank, pos_rank, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
       ew_phi, ew_f, ew_s, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
   // batched real <-> half-complex 3D transforms of the particle mesh (one per
   // species), kept while the mesh and the species count stay the same
@@ -354,7 +354,7 @@ struct Workspace {
   // metadata is copied from pageable host memory (no pinned staging)
   Arena* arena = nullptr;
   std::vector<DeviceBuffer*> buffers() {
-    return {&meta, &pos, &rank, &order, &sort_tmp, &cn, &gw, &dgw, &pair, &e_atom, &dedcn, &grad, &sig, &cnt, &off,
+    return {&meta, &pos, &rank, &order, &sort_tmp, &cn, &gw, &dgw, &pair, &e_atom, &dedcn, &grad, &sig, &atm_acc, &cnt, &off,
             &items, &item_off, &slots, &gw_rank, &sp_rank, &pos_rank, &counts, &counter, &flag, &pair3s, &hbuf,
             &e_out, &ew_tab, &ew_c6l, &ew_lc6l, &ew_ldc6l, &ew_part, &ew_phi, &ew_f, &ew_s, &sp_base, &sp_theta,
             &sp_dtheta, &sp_q, &sp_qh, &sp_th, &sp_k, &sp_dm, &sp_dmir};
@@ -739,6 +739,7 @@ DevWork Workspace::work(const Prepared& pb, bool mixed, bool grad, const BatchOu
   w.dedcn = debug && debug->dedcn ? debug->dedcn : dedcn.as<double>(natoms);
   w.grad = out.gradient ? out.gradient : this->grad.as<double>(size_t(natoms) * 3);
   w.sig_atom = sig.as<double>(size_t(natoms) * 6);
+  w.atm_acc = grad && pb.any_atm ? atm_acc.as<double4>(natoms) : nullptr;
   w.gwr = gw_rank.as<double>(size_t(natoms) * 2 * db.mref);
   w.spr = sp_rank.as<int>(natoms);
   return w;

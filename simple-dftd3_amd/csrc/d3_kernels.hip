@@ -889,14 +889,16 @@ __device__ __forceinline__ bool run_sums(double (&v)[N], int k) {
 }
 
 // Adds the slot sums of one tile (lane l: slot l with vector v, sums G and
-// D, atom k; k < 0 for no slot) to the gradient and dE/dCN of the slots'
-// atoms and to the i-side sums:
-//   grad_k += G,  grad_i -= G,  dEdcn_k -= D / 2,  sigma += sym(v G^T),
-// one global atomic per atom and value (the runs of one atom's images are
-// summed first).  Every lane of the wavefront calls it.
+// D, atom k at position r of its system's spatial order; k < 0 for no slot)
+// to the three-body sums of the slots' atoms, acc[r] = (G, D) of the system
+// (turned into gradient and dE/dCN by atm_acc_kernel), and to the i-side
+// sums:  grad_i -= G,  sigma += sym(v G^T).  The runs of one atom's images
+// are summed first, so one atomic per atom and value; the sums are indexed
+// by the spatial order, so the atomics of a tile touch few cache lines.
+// Every lane of the wavefront calls it.
 __device__ __forceinline__ void flush_tile(double Gx, double Gy, double Gz, double D, double vx, double vy,
-                                           double vz, int k, double* __restrict__ grad, double* __restrict__ dedcn,
-                                           double* gi, double* sg) {
+                                           double vz, int k, int r, double4* __restrict__ acc, double* gi,
+                                           double* sg) {
   if (k >= 0) {
     gi[0] -= Gx; gi[1] -= Gy; gi[2] -= Gz;
     sg[0] += vx * Gx;
@@ -909,10 +911,29 @@ __device__ __forceinline__ void flush_tile(double Gx, double Gy, double Gz, doub
   double v[4] = {Gx, Gy, Gz, D};
   const bool head = run_sums(v, k);
   if (!head || k < 0 || (v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0 && v[3] == 0.0)) return;
-  atomicAdd(grad + 3 * k, v[0]);
-  atomicAdd(grad + 3 * k + 1, v[1]);
-  atomicAdd(grad + 3 * k + 2, v[2]);
-  atomicAdd(dedcn + k, -0.5 * v[3]);
+  atomicAdd(&acc[r].x, v[0]);
+  atomicAdd(&acc[r].y, v[1]);
+  atomicAdd(&acc[r].z, v[2]);
+  atomicAdd(&acc[r].w, v[3]);
+}
+
+// grad += G and dE/dCN -= D / 2 of every atom from the three-body sums acc
+// (spatial order, thread per position).
+__global__ void atm_acc_kernel(DevBatch b, const double4* __restrict__ acc, double* __restrict__ grad,
+                               double* __restrict__ dedcn) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= b.natoms) return;
+  const int a = b.order[x];
+  const double4 s = acc[x];
+  grad[3 * a] += s.x;
+  grad[3 * a + 1] += s.y;
+  grad[3 * a + 2] += s.z;
+  dedcn[a] -= 0.5 * s.w;
+}
+
+__global__ void zero_double4(double4* __restrict__ p, int n) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < n) p[x] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
 // Three-body hot path: x**y for x > 0, 1/x and 1/sqrt(x) for normal x > 0.
@@ -1008,8 +1029,8 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
                                          const long long* __restrict__ offset, const Slot<R>* __restrict__ slots,
                                          const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
                                          double* __restrict__ dedcn, double* __restrict__ grad,
-                                         double* __restrict__ sig_atom, unsigned long long* __restrict__ work_counter,
-                                         double* __restrict__ pair3s) {
+                                         double* __restrict__ sig_atom, double4* __restrict__ slot_acc,
+                                         unsigned long long* __restrict__ work_counter, double* __restrict__ pair3s) {
   static_assert(!(kGrad && kPairs), "the pairwise mode is energy only");
   constexpr int kNAcc = kGrad ? kAcc : (kPairs ? kAccPairs : 0);
   constexpr int kAccLen = kNAcc > 0 ? kNAcc * kTile : 1;
@@ -1225,7 +1246,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       const double Gx = accB[0 * kTile + l], Gy = accB[1 * kTile + l], Gz = accB[2 * kTile + l];
       const double D = accB[3 * kTile + l];
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
-      flush_tile(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, grad, dedcn, gi, sg);
+      flush_tile(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, TB.key[l].y, slot_acc + g.atom_begin, gi, sg);
     }
     if (kPairs && !diag) {  // pair {i, b-slot atom}
       double t[1] = {accB[l]};
@@ -1238,7 +1259,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   __syncthreads();
   if (kGrad && w == 0)
     flush_tile(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l], ax, ay, az,
-               jat, grad, dedcn, gi, sg);
+               jat, akey.y, slot_acc + g.atom_begin, gi, sg);
   if (kPairs && w == 0) {  // pair {i, a-slot atom}
     double t[1] = {accA[l]};
     if (run_sums(t, jat) && jat >= 0 && t[0] != 0.0) atomicAdd(pair3s + g.c6_begin + irow + akey.y, t[0]);
@@ -1270,9 +1291,10 @@ __global__ __launch_bounds__(kW * kWave) __attribute__((amdgpu_waves_per_eu(3)))
                                                          const R4<R>* __restrict__ pair, double* __restrict__ e_atom,
                                                          double* __restrict__ dedcn, double* __restrict__ grad,
                                                          double* __restrict__ sig_atom,
+                                                         double4* __restrict__ slot_acc,
                                                          unsigned long long* __restrict__ work_counter) {
   atm_body<R, kGrad, kSmooth, kW, false, kPart>(b, item_off, offset, slots, pair, e_atom, dedcn, grad, sig_atom,
-                                         work_counter, nullptr);
+                                                slot_acc, work_counter, nullptr);
 }
 
 // Pairwise three-body energies (one block of four wavefronts per work item).
@@ -1284,7 +1306,7 @@ __global__ __launch_bounds__(4 * kWave) void atm_pairs_kernel(DevBatch b, const 
                                                               double* __restrict__ e_atom,
                                                               double* __restrict__ pair3s) {
   atm_body<R, false, kSmooth, 4, true>(b, item_off, offset, slots, pair, e_atom, nullptr, nullptr, nullptr, nullptr,
-                                       pair3s);
+                                       nullptr, pair3s);
 }
 
 // Pair energies in atom order from the storage-order sums of atm_pairs_kernel:
@@ -2929,9 +2951,16 @@ void launch_atm_t(const DevBatch& b, const long long* item_off, long long max_it
     wc = counter;
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, st, wc);
   }
+  if (kGrad) {
+    if (!w.atm_acc) throw std::invalid_argument("three-body gradient launch without its per-atom sums");
+    hipLaunchKernelGGL(zero_double4, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, w.atm_acc, b.natoms);
+  }
   hipLaunchKernelGGL((atm_kernel<R, kGrad, kSmooth, kW, kPart>), dim3(static_cast<unsigned>(grid)), dim3(kW * kWave), 0,
                      st, b, item_off, offset, static_cast<const Slot<R>*>(slots),
-                     static_cast<const R4<R>*>(w.pair), w.e_atom, w.dedcn, w.grad, w.sig_atom, wc);
+                     static_cast<const R4<R>*>(w.pair), w.e_atom, w.dedcn, w.grad, w.sig_atom, w.atm_acc, wc);
+  if (kGrad)
+    hipLaunchKernelGGL(atm_acc_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b, w.atm_acc,
+                       w.grad, w.dedcn);
 }
 
 template <int kW, bool kPart>

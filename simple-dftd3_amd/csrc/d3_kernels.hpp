@@ -49,6 +49,7 @@ struct DevWork {
   double* dedcn;    // [N]
   double* grad;     // [N][3]
   double* sig_atom; // [N][6]
+  double4* atm_acc; // [N] three-body slot sums per atom in spatial order (G xyz, D), E+grad only
   double* gwr;      // [2 mref][N] weights and dw/dCN in spatial order (pair records)
   int* spr;         // [N] species | ghost flag in spatial order
 };

@@ -206,6 +206,22 @@ def test_periodic_mixed_long_lists():
     _check(r, _oracle(num, pos, lat, [True] * 3, "rational", BJ), 32, mixed=True)
 
 
+@pytest.mark.parametrize("alp", [14.0, 16.0, 13.5])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_three_body_damping_exponent(alp, precision):
+    """The three-body damping power (r0/r)^((alp + 2)/3) for the default
+    alp = 14 (the FP64 kernel's Newton-refined power) and for other values
+    (the general power), periodic cell and molecule, against the oracle."""
+    _step(f"alp {alp} {precision}")
+    p = dict(BJ, alp=alp)
+    num, pos, lat = _cell(61, 20, 11.0)
+    mol_num, mol_pos, _ = _cell(62, 24, 14.0)
+    rs = _batch([dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3, param=p),
+                 dict(numbers=mol_num, positions=mol_pos, param=p)], precision=precision)
+    _check(rs[0], _oracle(num, pos, lat, [True] * 3, "rational", p), 20, mixed=precision == "mixed")
+    _check(rs[1], _oracle(mol_num, mol_pos, None, None, "rational", p), 24, mixed=precision == "mixed")
+
+
 def test_periodic_2d_slab():
     _step("2D slab BJ + ATM")
     num, pos, lat = _cell(22, 10, 9.0)
