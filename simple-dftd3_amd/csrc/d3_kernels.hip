@@ -938,7 +938,9 @@ __global__ void zero_double4(double4* __restrict__ p, int n) {
 
 // Three-body hot path: x**y for x > 0, 1/x and 1/sqrt(x) for normal x > 0.
 // FP32 uses the hardware approximations (v_exp_f32, v_log_f32, v_rcp_f32,
-// v_rsq_f32, about 1 ulp); FP64 the device-library functions.
+// v_rsq_f32, about 1 ulp); FP64 the device-library functions, except 1/x:
+// the hardware reciprocal refined by two Newton steps (about 1 ulp, not the
+// correctly rounded division).
 template <typename R> __device__ __forceinline__ R pow_pos(R x, R y);
 template <> __device__ __forceinline__ double pow_pos<double>(double x, double y) { return exp2(y * log2(x)); }
 template <> __device__ __forceinline__ float pow_pos<float>(float x, float y) {
@@ -964,7 +966,11 @@ __device__ __forceinline__ double pow_m8_3(double x) {
   return t;
 }
 template <typename R> __device__ __forceinline__ R rcp_fast(R x);
-template <> __device__ __forceinline__ double rcp_fast<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ double rcp_fast<double>(double x) {  // v_rcp_f64, two Newton steps
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
 template <> __device__ __forceinline__ float rcp_fast<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <typename R> __device__ __forceinline__ R rsqrt_fast(R x);
 template <> __device__ __forceinline__ double rsqrt_fast<double>(double x) { return rsqrt(x); }
