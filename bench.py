@@ -129,21 +129,27 @@ def cpu_baseline(systems, counts, stride, threads, reps=3):
     import oracle
     s = systems[0]
     args_ = (s["numbers"], s["positions"], s.get("lattice"), s.get("periodic"), oracle.RATIONAL, s["param"])
+    if len(s["numbers"]) <= 500:  # small systems: whole evaluations (a sample would time a few atoms)
+        stride = 1
+    stride1 = stride * 10 if stride > 1 else 1
     oracle.set_num_threads(threads)
     times = sorted(oracle.time_sampled_dispersion(stride, *args_) for _ in range(reps))
     t = times[len(times) // 2]
     t_full = t * stride
     cores = oracle.num_threads()
     oracle.set_num_threads(1)
-    t1 = oracle.time_sampled_dispersion(stride * 10, *args_) * stride * 10
+    t1 = oracle.time_sampled_dispersion(stride1, *args_) * stride1
     oracle.set_num_threads(cores)
     return dict(value=float(counts[1] / t_full), unit="pair-terms/s", cores=cores, kind="port",
-                sample=f"oracle E+grad+sigma with the outer atom loops restricted to every {stride}-th atom, "
-                       f"median of {reps} runs ({', '.join(f'{x:.2f}' for x in times)} s), scaled by {stride} "
-                       f"to one full evaluation ({t_full:.1f} s)",
+                sample=(f"oracle E+grad+sigma with the outer atom loops restricted to every {stride}-th atom, "
+                        f"median of {reps} runs ({', '.join(f'{x:.2f}' for x in times)} s), scaled by {stride} "
+                        f"to one full evaluation ({t_full:.1f} s)") if stride > 1 else
+                       (f"oracle E+grad+sigma, median of {reps} full evaluations "
+                        f"({', '.join(f'{x:.3f}' for x in times)} s)"),
                 seconds_per_eval_estimate=t_full, host=host_cpu(),
                 one_thread=dict(value=float(counts[1] / t1), seconds_per_eval_estimate=t1,
-                                sample=f"every {stride * 10}-th outer atom, scaled"))
+                                sample=f"every {stride1}-th outer atom, scaled" if stride1 > 1
+                                else "one full evaluation"))
 
 
 def source_sha(path):
