@@ -314,7 +314,7 @@ D3HIP_API void d3hip_plan_execute(dftd3_error error, d3hip_plan plan, const doub
                                   double* gradient, double* sigma, void* stream);
 
 /* Candidate launch shapes of the three-body kernel: wavefronts per block
- * (2 or 4) and persistent blocks per compute unit (0 = one block per work
+ * (1, 2 or 4) and persistent blocks per compute unit (0 = one block per work
  * item). */
 D3HIP_API int d3hip_plan_num_shapes(void);
 D3HIP_API void d3hip_plan_candidate_shape(int index, int* waves, int* blocks_per_cu);

@@ -1111,7 +1111,7 @@ long long Plan::slot_capacity() const { return st_->capacity; }
 const AtmShape& Plan::shape() const { return st_->shape; }
 
 void Plan::set_shape(int waves, int blocks_per_cu) {
-  if (waves != 2 && waves != 4) throw Error("three-body launch: wavefronts per block must be 2 or 4");
+  if (waves != 1 && waves != 2 && waves != 4) throw Error("three-body launch: wavefronts per block must be 1, 2 or 4");
   if (blocks_per_cu < 0 || blocks_per_cu > 64) throw Error("three-body launch: blocks per CU out of range");
   st_->shape.waves = waves;
   st_->shape.blocks_per_cu = blocks_per_cu;
@@ -1136,6 +1136,14 @@ std::vector<AtmShape> Plan::candidate_shapes() {
       a.blocks_per_cu = bpc;
       c.push_back(a);
     }
+  // one wavefront per block: for short lists (small molecules), whose work
+  // items have one or two tiles to pair with
+  for (int bpc : {0, 16}) {
+    AtmShape a;
+    a.waves = 1;
+    a.blocks_per_cu = bpc;
+    c.push_back(a);
+  }
   return c;
 }
 

@@ -2997,8 +2997,10 @@ void launch_atm(const DevBatch& b, bool mixed, bool grad, bool smooth, bool part
     launch_atm_w<2, false>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
   else if (shape.waves == 4)
     launch_atm_w<4, false>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
+  else if (shape.waves == 1)
+    launch_atm_w<1, false>(b, mixed, grad, smooth, item_off, max_items, offset, slots, w, shape, counter, st);
   else
-    throw std::invalid_argument("three-body launch: wavefronts per block must be 2 or 4");
+    throw std::invalid_argument("three-body launch: wavefronts per block must be 1, 2 or 4");
 }
 
 namespace {

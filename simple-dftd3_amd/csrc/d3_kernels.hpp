@@ -93,7 +93,7 @@ void launch_atm_guard(const DevBatch& b, const long long* offset, long long capa
 // Work items (atom i, slot block P) of the tiled three-body kernel.
 void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t stream);
 
-// Launch shape of the three-body kernel: wavefronts per block (2 or 4, the
+// Launch shape of the three-body kernel: wavefronts per block (1, 2 or 4, the
 // block shares one 64-slot tile of a list) and either one block per work
 // item (blocks_per_cu = 0) or a persistent grid of blocks_per_cu blocks per
 // compute unit drawing items from a work counter.

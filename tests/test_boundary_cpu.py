@@ -229,7 +229,7 @@ def test_plan_api_errors_without_a_gpu():
     """The plan entry points validate their arguments before any GPU work."""
     from dftd3 import library as L
     shapes = L.plan_candidate_shapes()
-    assert (4, 0) in shapes and all(w in (2, 4) and b >= 0 for w, b in shapes)
+    assert (4, 0) in shapes and (1, 0) in shapes and all(w in (1, 2, 4) and b >= 0 for w, b in shapes)
     buf = ctypes.create_string_buffer(128)
     err = L._Error()
     assert L.lib.d3hip_new_plan(err.handle, 0, None, None, None, None, None, None, 0, 1, None, None) is None
