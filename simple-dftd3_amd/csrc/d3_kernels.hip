@@ -1102,7 +1102,8 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   // Slots past the end of the list hold the coordinates 1e30, so every test
   // with one of them fails (r^2 ~ 1e60, or 0 < eps for two of them) and the
   // test needs no validity flag.  Step 32 of a diagonal tile pairs a with
-  // a + 32 and a + 32 with a: only lanes l < 32 take it (cutoff -1 otherwise).
+  // a + 32 and a + 32 with a: only lanes l < 32 take it (cutoff -1 otherwise),
+  // in a step of its own after the loop over the others.
   const double cut2_half = l < kTile / 2 ? cut2 : -1.0;
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
   const int2 akey = tileA.key[l];
@@ -1133,36 +1134,46 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
     double bx = TB.vx[(l + s0) & (kTile - 1)], by = TB.vy[(l + s0) & (kTile - 1)],
            bz = TB.vz[(l + s0) & (kTile - 1)];
     int2 kbr = TB.key[(l + s0) & (kTile - 1)];
+    // one test step: lane l tests a = l against its b-slot with the cutoff
+    // thr, appends an accepted pair to the ring, and the b-slots rotate
+    auto test_step = [&](double thr) {
+      const int bl = (l + st) & (kTile - 1);
+      double v0, v1, v2;
+      const double Cd = diff_norm2(bx, by, bz, ax, ay, az, v0, v1, v2);
+      const bool in_c = Cd <= thr, in_e = Cd >= kEps;
+      bool ok = in_c & in_e;
+      // the ballot from the two compares' lane masks (no VALU copy of ok)
+      unsigned long long bal = __builtin_amdgcn_ballot_w64(in_c) & __builtin_amdgcn_ballot_w64(in_e);
+      if (kPart) {  // atm.f90:709: the triple belongs to the pair (i, larger partner)
+        ok = ok && owns_pair(p, i - g.atom_begin, max(part_atom[l], batom[bl]) - g.atom_begin);
+        bal = __ballot(ok);
+      }
+      if (ok) {
+        const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                   static_cast<unsigned>(bal >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
+        const int2 kb = kbr;
+        PairEntry<R> e;
+        const unsigned long long rec = arow + static_cast<unsigned>(kb.y);
+        e.lo = (static_cast<unsigned>(kb.x) & amask) | static_cast<unsigned>(l) | (static_cast<unsigned>(rec) << 17);
+        e.hi = static_cast<unsigned>(rec >> kRecLoBits);
+        e.C = R(Cd);
+        q_ring[pos & (kQueue - 1)] = e;
+      }
+      tail += __popcll(bal);
+      bx = wave_rotate1(bx);
+      by = wave_rotate1(by);
+      bz = wave_rotate1(bz);
+      kbr.x = wave_rotate1(kbr.x);
+      kbr.y = wave_rotate1(kbr.y);
+      ++st;
+    };
+    // the steps before the half step of a diagonal tile, or all 64
+    const int s_full = diag ? kTile / 2 - 1 : kTile - 1;
     while (true) {
       // test phase: append the accepted pairs of the next steps
-      for (; st <= s1 && tail - head <= kQueue - kWave; ++st) {
-        const int bl = (l + st) & (kTile - 1);
-        double v0, v1, v2;
-        const double Cd = diff_norm2(bx, by, bz, ax, ay, az, v0, v1, v2);
-        const double cut2_step = (diag && st == kTile / 2) ? cut2_half : cut2;
-        bool ok = (Cd <= cut2_step) & (Cd >= kEps);
-        if (kPart)  // atm.f90:709: the triple belongs to the pair (i, larger partner)
-          ok = ok && owns_pair(p, i - g.atom_begin, max(part_atom[l], batom[bl]) - g.atom_begin);
-        const unsigned long long bal = __ballot(ok);
-        if (ok) {
-          const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                                     static_cast<unsigned>(bal >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
-          const int2 kb = kbr;
-          PairEntry<R> e;
-          const unsigned long long rec = arow + static_cast<unsigned>(kb.y);
-          e.lo = (static_cast<unsigned>(kb.x) & amask) | static_cast<unsigned>(l) | (static_cast<unsigned>(rec) << 17);
-          e.hi = static_cast<unsigned>(rec >> kRecLoBits);
-          e.C = R(Cd);
-          q_ring[pos & (kQueue - 1)] = e;
-        }
-        tail += __popcll(bal);
-        bx = wave_rotate1(bx);
-        by = wave_rotate1(by);
-        bz = wave_rotate1(bz);
-        kbr.x = wave_rotate1(kbr.x);
-        kbr.y = wave_rotate1(kbr.y);
-      }
+      while (st <= s_full && tail - head <= kQueue - kWave) test_step(cut2);
+      if (diag && st == kTile / 2 && tail - head <= kQueue - kWave) test_step(cut2_half);
       const bool done = st > s1;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
       __builtin_amdgcn_wave_barrier();
