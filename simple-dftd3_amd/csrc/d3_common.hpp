@@ -59,6 +59,11 @@ struct SysGeom {
   int img2_begin, img2_count;
   int img3_begin, img3_count;
   long long c6_begin;  // offset of the n x n C6 block of this system
+  // tiles of 64 consecutive positions of the system's spatial order, the
+  // first at tile_begin of the batch's tile list; cull = 1 (molecules of
+  // more than 8 tiles): pair loops skip a tile whose bounding box is beyond
+  // the cutoff
+  int tile_begin, cull;
   // 3D-periodic systems: the images of set s (kImgCn, kImg2, kImg3) form the
   // index box |n_d| <= box[s][d] in list order (n_x outer, n_z inner); the
   // rows of frac (b_d / 2 pi) give fractional coordinates and span[s][d] =

@@ -318,7 +318,7 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, atm_acc, cnt, off,
-      items, item_off, slots, gw_rank, sp_rank, pos_rank, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
+      items, item_off, slots, gw_rank, sp_rank, pos_rank, tbox, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
       ew_phi, ew_f, ew_s, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
   // batched real <-> half-complex 3D transforms of the particle mesh (one per
   // species), kept while the mesh and the species count stay the same
@@ -355,7 +355,7 @@ struct Workspace {
   Arena* arena = nullptr;
   std::vector<DeviceBuffer*> buffers() {
     return {&meta, &pos, &rank, &order, &sort_tmp, &cn, &gw, &dgw, &pair, &e_atom, &dedcn, &grad, &sig, &atm_acc, &cnt, &off,
-            &items, &item_off, &slots, &gw_rank, &sp_rank, &pos_rank, &counts, &counter, &flag, &pair3s, &hbuf,
+            &items, &item_off, &slots, &gw_rank, &sp_rank, &pos_rank, &tbox, &counts, &counter, &flag, &pair3s, &hbuf,
             &e_out, &ew_tab, &ew_c6l, &ew_lc6l, &ew_ldc6l, &ew_part, &ew_phi, &ew_f, &ew_s, &sp_base, &sp_theta,
             &sp_dtheta, &sp_q, &sp_qh, &sp_th, &sp_k, &sp_dm, &sp_dmir};
   }
@@ -495,6 +495,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   std::vector<SysGeom> geom(nsys);
   std::vector<SysParam> par(nsys);
   std::vector<double4> img;
+  std::vector<int2> tiles;  // 64-position tiles of every system's spatial order
   long long pair_total = 0, slot_bound = 0, item_bound = 0, atm_atoms = 0;
   bool any_atm = false, smooth = false, periodic_atm = false;
   for (int k = 0, a = 0; k < nsys; ++k) {
@@ -527,6 +528,12 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
     image_box(sd, sd.param.cut3, &img[g.img3_begin], g.img3_count, kImg3, g);
     g.c6_begin = pair_total;
     pair_total += static_cast<long long>(sd.natoms) * sd.natoms;
+    g.tile_begin = static_cast<int>(tiles.size());
+    // tile culling for molecules of more than 8 tiles (in smaller ones every
+    // tile is usually within the cutoffs and the box tests only cost)
+    g.cull = !(sd.periodic[0] || sd.periodic[1] || sd.periodic[2]) && sd.natoms > 8 * kTile;
+    for (int t = 0; t * kTile < sd.natoms; ++t)
+      tiles.push_back(make_int2(g.atom_begin + t * kTile, std::min(kTile, sd.natoms - t * kTile)));
     if (std::fabs(sd.param.s9) >= kEps) {
       any_atm = true;
       smooth |= sd.param.width3 > 0.0;
@@ -615,7 +622,7 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
                o_geom = stage(sb, geom),
                o_par = stage(sb, par), o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2),
                o_nref = stage(sb, nref), o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref),
-               o_rvdw = stage(sb, rvdw);
+               o_rvdw = stage(sb, rvdw), o_ptiles = stage(sb, tiles);
   const size_t o_tiles = stage(sb, rt.tiles), o_lam = stage(sb, lam), o_vec = stage(sb, vecb),
                o_pval = stage(sb, pval), o_terms = stage(sb, terms), o_nterm = stage(sb, nterm),
                o_sorder = stage(sb, sorder), o_chunks = stage(sb, chunks), o_cseg = stage(sb, chunk_seg),
@@ -653,6 +660,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   db.rank = rank.as<int>(natoms);
   db.order = order.as<int>(natoms);
   db.posr = pos_rank.as<double4>(natoms);
+  db.ntiles = static_cast<int>(tiles.size());
+  db.tiles = reinterpret_cast<const int2*>(dmeta + o_ptiles);
+  db.tbox = tbox.as<double4>(2 * tiles.size());
   pb.pair_total = pair_total;
   pb.any_atm = any_atm;
   pb.smooth = smooth;

@@ -132,6 +132,32 @@ __device__ __forceinline__ double atm_vec(const double4& xj, const double4& t, c
   return v0 * v0 + v1 * v1 + v2 * v2;
 }
 
+// Whether every position of tile t (64 consecutive positions of a system's
+// spatial order) lies farther than sqrt(cut2) from x: the distance from x to
+// the tile's bounding box exceeds the cutoff (with a margin against the
+// rounding of the box distance), so every exact pair test with the tile
+// fails and a pair loop may skip it.  Systems without lattice images only.
+__device__ __forceinline__ bool tile_beyond(const DevBatch& b, int t, const double4& x, double cut2) {
+  const double4 lo = b.tbox[2 * t], hi = b.tbox[2 * t + 1];
+  const double dx = fmax(0.0, fmax(lo.x - x.x, x.x - hi.x));
+  const double dy = fmax(0.0, fmax(lo.y - x.y, x.y - hi.y));
+  const double dz = fmax(0.0, fmax(lo.z - x.z, x.z - hi.z));
+  return dx * dx + dy * dy + dz * dz > cut2 * (1.0 + 1e-12) + 1e-12;
+}
+
+// Tiles t0 .. t0 + 63 of a system (bit k: tile t0 + k) that a pair loop of
+// the position x with cutoff sqrt(cut2) has to visit: the box tests of 64
+// tiles at once, one per lane, so that a loop over the tiles tests a bit
+// instead of waiting for a box load per tile.  Every tile unless g.cull.
+__device__ __forceinline__ unsigned long long near_tiles(const DevBatch& b, const SysGeom& g, int t0,
+                                                         const double4& x, double cut2) {
+  if (!g.cull) return ~0ull;
+  const int t = t0 + wave_lane();
+  bool near = false;
+  if (t * kWave < g.natoms) near = !tile_beyond(b, g.tile_begin + t, x, cut2);
+  return __ballot(near);
+}
+
 // cutoff.f90:73-99 smooth_cutoff
 template <typename R>
 __device__ __forceinline__ void smooth_cutoff(R r, R cutoff, R width, R& sw, R& dswdr) {
@@ -243,8 +269,7 @@ __global__ __launch_bounds__(kBlock) void cn_kernel(DevBatch b, double* __restri
   const double4 xi = b.pos[i];
   const R rci = R(b.rcov[b.sp[i]]);
   double acc = 0.0;
-  for (int j = g.atom_begin + lane; j < g.atom_begin + g.natoms; j += kWave) {
-    const double4 xj = b.pos[j];
+  auto partner = [&](int j, const double4& xj) {
     const R rc = rci + R(b.rcov[b.sp[j]]);
     const bool swap = j > i;
     for_images(g, kImgCn, g.img_cn_count, xi, xj, [&](int t) {
@@ -254,6 +279,18 @@ __global__ __launch_bounds__(kBlock) void cn_kernel(DevBatch b, double* __restri
       const R r1 = d_sqrt(R(r2));
       acc += double(d_rcp(R(1) + d_exp(-R(kKcn) * (rc / r1 - R(1)))));
     });
+  };
+  if (g.cull) {  // partners in spatial order, tiles beyond the cutoff skipped
+    unsigned long long near = 0;
+    for (int r0 = 0; r0 < g.natoms; r0 += kWave) {
+      const int t = r0 / kWave;
+      if ((t & (kWave - 1)) == 0) near = near_tiles(b, g, t, xi, cut2);
+      if (!((near >> (t & (kWave - 1))) & 1ull) || r0 + lane >= g.natoms) continue;
+      const int x = g.atom_begin + r0 + lane;
+      partner(b.order[x], b.posr[x]);
+    }
+  } else {
+    for (int j = g.atom_begin + lane; j < g.atom_begin + g.natoms; j += kWave) partner(j, b.pos[j]);
   }
   acc = wave_sum(acc);
   if (lane == 0) cn[i] = acc;
@@ -594,7 +631,14 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
   const int idi = b.sid[i];
   const R4<R>* row = pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n;
   if (active || kPairs) {  // the pairwise energies are written (as zeros) for s6 = s8 = 0 too
-    for (int jr = lane; jr < n; jr += kWave) {
+    unsigned long long near = 0;
+    for (int r0 = 0; r0 < n; r0 += kWave) {
+      // tiles beyond the cutoff skipped (the pairwise mode writes every pair)
+      const int t = r0 / kWave;
+      if (!kPairs && (t & (kWave - 1)) == 0) near = near_tiles(b, g, t, xi, cut2);
+      if (!kPairs && !((near >> (t & (kWave - 1))) & 1ull)) continue;
+      const int jr = r0 + lane;
+      if (jr >= n) continue;
       const int j = b.order[g.atom_begin + jr];
       if (kPairs && j > i) continue;
       if (p.nparts > 1) {  // rational.f90:453 and the other families
@@ -668,7 +712,10 @@ __device__ __forceinline__ void disp2_atom(const DevBatch& b, const R4<R>* __res
 // The image loop is specialised per damping family (the family is a
 // per-system parameter, uniform within a wavefront).
 template <typename R, bool kGrad, bool kPairs>
-__global__ __launch_bounds__(kBlock) void disp2_kernel(DevBatch b, const R4<R>* __restrict__ pair,
+// Mixed: at most 168 VGPRs, so three wavefronts per SIMD (it otherwise
+// takes 175 and two).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1)))
+void disp2_kernel(DevBatch b, const R4<R>* __restrict__ pair,
                                                        double* __restrict__ e_atom, double* __restrict__ dedcn,
                                                        double* __restrict__ grad, double* __restrict__ sig_atom,
                                                        double* __restrict__ pair2) {
@@ -723,7 +770,11 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
   const R4<R>* row = pair ? pair + g.c6_begin + static_cast<long long>(b.rank[i]) * n : nullptr;
   Slot<R>* out = kFill ? slots + offset[i] : nullptr;
   long long total = 0;
+  unsigned long long near = 0;
   for (int r0 = 0; r0 < n; r0 += kWave) {  // partners in spatial order
+    const int t = r0 / kWave;
+    if ((t & (kWave - 1)) == 0) near = near_tiles(b, g, t, xi, cut2);
+    if (!((near >> (t & (kWave - 1))) & 1ull)) continue;  // no slot in this tile
     const int r = r0 + lane;
     const int j = r < n ? b.order[g.atom_begin + r] : i + 1;
     int cnt = 0;
@@ -1362,8 +1413,7 @@ __global__ __launch_bounds__(kBlock) void cn_chain_kernel(DevBatch b, const doub
   const double dei = dedcn[i];
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
   double sxx = 0.0, syy = 0.0, szz = 0.0, sxy = 0.0, sxz = 0.0, syz = 0.0;
-  for (int j = g.atom_begin + lane; j < g.atom_begin + g.natoms; j += kWave) {
-    const double4 xj = b.pos[j];
+  auto partner = [&](int j, const double4& xj) {
     const R rc = rci + R(b.rcov[b.sp[j]]);
     const bool swap = j > i;
     // the reference adds ds*(dEdcn_i + dEdcn_j) once per unordered pair and
@@ -1388,6 +1438,18 @@ __global__ __launch_bounds__(kBlock) void cn_chain_kernel(DevBatch b, const doub
       sxx += double(h * v0 * v0); syy += double(h * v1 * v1); szz += double(h * v2 * v2);
       sxy += double(h * v0 * v1); sxz += double(h * v0 * v2); syz += double(h * v1 * v2);
     });
+  };
+  if (g.cull) {  // partners in spatial order, tiles beyond the cutoff skipped (as cn_kernel)
+    unsigned long long near = 0;
+    for (int r0 = 0; r0 < g.natoms; r0 += kWave) {
+      const int t = r0 / kWave;
+      if ((t & (kWave - 1)) == 0) near = near_tiles(b, g, t, xi, cut2);
+      if (!((near >> (t & (kWave - 1))) & 1ull) || r0 + lane >= g.natoms) continue;
+      const int x = g.atom_begin + r0 + lane;
+      partner(b.order[x], b.posr[x]);
+    }
+  } else {
+    for (int j = g.atom_begin + lane; j < g.atom_begin + g.natoms; j += kWave) partner(j, b.pos[j]);
   }
   g0 = wave_sum(g0); g1 = wave_sum(g1); g2 = wave_sum(g2);
   sxx = wave_sum(sxx); syy = wave_sum(syy); szz = wave_sum(szz);
@@ -2831,6 +2893,28 @@ __global__ __launch_bounds__(kBlock) void rank_pos_kernel(DevBatch b, double4* _
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x < b.natoms) posr[x] = b.pos[b.order[x]];
 }
+
+// Bounding box of every 64-position tile of the spatial order (wavefront per
+// tile), from the positions in that order.
+__global__ __launch_bounds__(kBlock) void tile_box_kernel(DevBatch b, double4* __restrict__ tbox) {
+  const int lane = wave_lane();
+  const int t = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (t >= b.ntiles) return;
+  const int2 tile = b.tiles[t];
+  const double4 p = b.posr[tile.x + min(lane, tile.y - 1)];  // lanes past the tile repeat its last position
+  double lo[3] = {p.x, p.y, p.z}, hi[3] = {p.x, p.y, p.z};
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = fmin(lo[c], __shfl_xor(lo[c], off, kWave));
+      hi[c] = fmax(hi[c], __shfl_xor(hi[c], off, kWave));
+    }
+  if (lane == 0) {
+    tbox[2 * t] = make_double4(lo[0], lo[1], lo[2], 0.0);
+    tbox[2 * t + 1] = make_double4(hi[0], hi[1], hi[2], 0.0);
+  }
+}
 }  // namespace
 
 // Work area of the allocation-free entry point.  In host code this name is a
@@ -2855,6 +2939,9 @@ void* noalloc_area(int device, hipError_t* hip_err) {
 void launch_rank_pos(const DevBatch& b, hipStream_t st) {
   hipLaunchKernelGGL(rank_pos_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b,
                      const_cast<double4*>(b.posr));
+  if (b.ntiles > 0)
+    hipLaunchKernelGGL(tile_box_kernel, dim3(blocks_for(b.ntiles, kWavesPerBlock)), dim3(kBlock), 0, st, b,
+                       const_cast<double4*>(b.tbox));
 }
 
 void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, double* gwr, int* spr,

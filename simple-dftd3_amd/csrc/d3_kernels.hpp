@@ -36,6 +36,9 @@ struct DevBatch {
   const int* rank;      // [N] position of the atom in its system's order
   const int* order;     // [N] atom at position atom_begin + r of that order
   const double4* posr;  // [N] positions in that order, posr[x] = pos[order[x]] (refreshed with pos)
+  int ntiles;
+  const int2* tiles;    // [ntiles] first position and count of each 64-position tile of that order
+  const double4* tbox;  // [2 ntiles] lower and upper corner of each tile's positions (refreshed with pos)
 };
 
 // Per-evaluation device arrays.

@@ -222,6 +222,23 @@ def test_three_body_damping_exponent(alp, precision):
     _check(rs[1], _oracle(mol_num, mol_pos, None, None, "rational", p), 24, mixed=precision == "mixed")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_elongated_molecule_tile_culling(precision):
+    """A 640-atom rod 420 bohr long: most 64-atom tiles of the spatial order
+    lie beyond the CN, two-body and three-body cutoffs of an atom and are
+    skipped by the pair loops (CN, two-body, three-body lists, chain rule);
+    energy, gradient and virial must be the oracle's."""
+    _step(f"640-atom rod {precision}")
+    rng = np.random.default_rng(17)
+    n = 640
+    num = rng.choice([1, 6, 7, 8], size=n, p=[0.5, 0.3, 0.1, 0.1])
+    pos = np.column_stack([np.linspace(0.0, 420.0, n) + rng.uniform(-0.2, 0.2, n),
+                           rng.uniform(-4.0, 4.0, n), rng.uniform(-4.0, 4.0, n)])
+    sys_ = dict(numbers=num, positions=pos, param=BJ)
+    r = _batch([sys_], precision=precision)[0]
+    _check(r, _oracle(num, pos), n, mixed=precision == "mixed")
+
+
 def test_periodic_2d_slab():
     _step("2D slab BJ + ATM")
     num, pos, lat = _cell(22, 10, 9.0)
