@@ -41,8 +41,8 @@ PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
 # profiles/atm_kernel__config1_periodic_2000.md.  "kernel_sha" is the SHA-256
 # (first 16 hex digits) of csrc/d3_kernels.hip the counters were read on: a
 # report on other kernel sources gives traffic = null.
-ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4, false>", "fetch_kib": 12_207_000, "write_kib": 4_878_000,
-                  "fetch_scale": 2.0, "write_scale": 1.0, "kernel_sha": "64e1b5df58421abc"}
+ATM_TRAFFIC_C1 = {"kernel": "atm_kernel<float, true, false, 4, false>", "fetch_kib": 12_216_000, "write_kib": 4_878_000,
+                  "fetch_scale": 2.0, "write_scale": 1.0, "kernel_sha": "fc54b12f63ca07ad"}
 KERNEL_SOURCE = os.path.join(ROOT, "simple-dftd3_amd", "csrc", "d3_kernels.hip")
 
 
