@@ -316,6 +316,10 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
   std::memcpy(g.frac, fr, sizeof fr);
 }
 
+// Largest slot buffer (bytes) sized by the exact list bound of molecules
+// instead of a read-back of the list size.
+constexpr long long kUnsyncedSlotBytes = 64LL << 20;
+
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, atm_acc, cnt, off,
       items, item_off, slots, gw_rank, sp_rank, pos_rank, tbox, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
@@ -843,7 +847,15 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
     long long* o = off.as<long long>(natoms + 1);
     long long* io = item_off.as<long long>(natoms + 1);
     void* s = nullptr;
-    if (capacity < 0) {
+    const long long bound_bytes = pb.slot_bound * static_cast<long long>(slot_record_bytes(mixed));
+    if (capacity < 0 && !stats && !arena && !pb.periodic_atm && bound_bytes <= kUnsyncedSlotBytes) {
+      // molecules whose exact list bound (j <= i: il + 1 slots per atom)
+      // fits a modest buffer: no read-back of the list size, so the call
+      // does not wait for the device here
+      capacity = std::max<long long>(pb.slot_bound, 1);
+      max_items = pb.item_bound;
+      s = slots.get(capacity * slot_record_bytes(mixed));
+    } else if (capacity < 0) {
       long long total = 0;
       hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
       hip_check(hipStreamSynchronize(stream), "list size sync");
