@@ -202,6 +202,7 @@ struct Prepared {
   bool smooth = false;         // some three-body system has a nonzero width3
   bool periodic_atm = false;   // some three-body system has lattice images
   bool partitioned = false;    // some system evaluates one part of a work partition
+  bool fixed_order = false;    // one system of at most 8 tiles: storage order = atom order, no sort
   long long slot_bound = 0;    // sum over three-body systems of (il + 1) * images
   long long atm_atoms = 0;     // atoms of the systems with a three-body term
   long long item_bound = 0;    // work items of those bounds
@@ -322,7 +323,7 @@ constexpr long long kUnsyncedSlotBytes = 64LL << 20;
 
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, atm_acc, cnt, off,
-      items, item_off, slots, gw_rank, sp_rank, pos_rank, tbox, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
+      item_off, slots, gw_rank, sp_rank, pos_rank, tbox, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
       ew_phi, ew_f, ew_s, sp_base, sp_theta, sp_dtheta, sp_q, sp_qh, sp_th, sp_k, sp_dm, sp_dmir;
   // batched real <-> half-complex 3D transforms of the particle mesh (one per
   // species), kept while the mesh and the species count stay the same
@@ -359,7 +360,7 @@ struct Workspace {
   Arena* arena = nullptr;
   std::vector<DeviceBuffer*> buffers() {
     return {&meta, &pos, &rank, &order, &sort_tmp, &cn, &gw, &dgw, &pair, &e_atom, &dedcn, &grad, &sig, &atm_acc, &cnt, &off,
-            &items, &item_off, &slots, &gw_rank, &sp_rank, &pos_rank, &tbox, &counts, &counter, &flag, &pair3s, &hbuf,
+            &item_off, &slots, &gw_rank, &sp_rank, &pos_rank, &tbox, &counts, &counter, &flag, &pair3s, &hbuf,
             &e_out, &ew_tab, &ew_c6l, &ew_lc6l, &ew_ldc6l, &ew_part, &ew_phi, &ew_f, &ew_s, &sp_base, &sp_theta,
             &sp_dtheta, &sp_q, &sp_qh, &sp_th, &sp_k, &sp_dm, &sp_dmir};
   }
@@ -390,7 +391,7 @@ struct Workspace {
   void pack(const Prepared& pb, const double* positions_dev, hipStream_t stream, bool order_atoms) {
     timed("pack", stream,
           [&] { launch_pack(positions_dev, pb.db.natoms, const_cast<double4*>(pb.db.pos), stream); });
-    if (order_atoms) {
+    if (order_atoms && !pb.fixed_order) {
       const size_t bytes = spatial_order_temp_bytes(pb.db.natoms);
       void* t = sort_tmp.get(bytes);
       timed("order", stream, [&] {
@@ -400,8 +401,8 @@ struct Workspace {
     timed("pack", stream, [&] { launch_rank_pos(pb.db, stream); });
   }
   DevWork work(const Prepared& pb, bool mixed, bool grad, const BatchOutput& out, const BatchDebug* debug);
-  // count -> scan -> work items -> scan (three-body lists of every atom)
-  void list_offsets(const Prepared& pb, const void* pair, bool mixed, hipStream_t stream);
+  // count -> scans of the list sizes and of their work items (three-body lists of every atom)
+  void list_offsets(const Prepared& pb, const void* pair, bool mixed, double4* acc, hipStream_t stream);
   // pack .. disp2; then the three-body lists and kernel; then the chain
   // rule and the reduction.  capacity < 0: read the list size back and size
   // the slot buffer to it; otherwise use the fixed capacity and item bound.
@@ -627,6 +628,16 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
                o_par = stage(sb, par), o_img = stage(sb, img), o_rcov = stage(sb, rcov), o_r4 = stage(sb, r4r2),
                o_nref = stage(sb, nref), o_refcn = stage(sb, refcn), o_c6r = stage(sb, c6ref),
                o_rvdw = stage(sb, rvdw), o_ptiles = stage(sb, tiles);
+  // a single small system (the usual call of the drop-in interface) keeps
+  // the atom order as its storage order: in one or a few tiles the spatial
+  // order gains nothing and its sort would run with every evaluation
+  pb.fixed_order = nsys == 1 && natoms <= 8 * kTile;
+  std::vector<int> iota;
+  if (pb.fixed_order) {
+    iota.resize(natoms);
+    for (int k = 0; k < natoms; ++k) iota[k] = k;
+  }
+  const size_t o_iota = stage(sb, iota);
   const size_t o_tiles = stage(sb, rt.tiles), o_lam = stage(sb, lam), o_vec = stage(sb, vecb),
                o_pval = stage(sb, pval), o_terms = stage(sb, terms), o_nterm = stage(sb, nterm),
                o_sorder = stage(sb, sorder), o_chunks = stage(sb, chunks), o_cseg = stage(sb, chunk_seg),
@@ -661,8 +672,8 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   db.c6ref = reinterpret_cast<const double*>(dmeta + o_c6r);
   db.rvdw = reinterpret_cast<const double*>(dmeta + o_rvdw);
   db.ghost = ghost_host ? reinterpret_cast<const int*>(dmeta + o_ghost) : nullptr;
-  db.rank = rank.as<int>(natoms);
-  db.order = order.as<int>(natoms);
+  db.rank = pb.fixed_order ? reinterpret_cast<const int*>(dmeta + o_iota) : rank.as<int>(natoms);
+  db.order = pb.fixed_order ? reinterpret_cast<const int*>(dmeta + o_iota) : order.as<int>(natoms);
   db.posr = pos_rank.as<double4>(natoms);
   db.ntiles = static_cast<int>(tiles.size());
   db.tiles = reinterpret_cast<const int2*>(dmeta + o_ptiles);
@@ -759,17 +770,15 @@ DevWork Workspace::work(const Prepared& pb, bool mixed, bool grad, const BatchOu
   return w;
 }
 
-void Workspace::list_offsets(const Prepared& pb, const void* pair_records, bool mixed, hipStream_t stream) {
+void Workspace::list_offsets(const Prepared& pb, const void* pair_records, bool mixed, double4* acc,
+                             hipStream_t stream) {
   const int natoms = pb.db.natoms;
   long long* c = cnt.as<long long>(natoms);
   long long* o = off.as<long long>(natoms + 1);
-  long long* it = items.as<long long>(natoms);
   long long* io = item_off.as<long long>(natoms + 1);
   timed("atm_list", stream, [&] {
-    launch_atm_count(pb.db, pair_records, mixed, c, stream);
-    launch_scan(c, natoms, o, stream);
-    launch_atm_items(o, natoms, it, stream);
-    launch_scan(it, natoms, io, stream);
+    launch_atm_count(pb.db, pair_records, mixed, c, acc, stream);
+    launch_list_scan(c, natoms, o, io, stream);
   });
 }
 
@@ -843,7 +852,7 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
     timed("ewald_atom", stream, [&] { launch_ewald_atom(e, grad, w, ew, stream); });
   }
   if (pb.any_atm) {
-    list_offsets(pb, w.pair, mixed, stream);
+    list_offsets(pb, w.pair, mixed, w.atm_acc, stream);
     long long* o = off.as<long long>(natoms + 1);
     long long* io = item_off.as<long long>(natoms + 1);
     void* s = nullptr;
@@ -953,7 +962,7 @@ void Engine::run_pairwise(const std::vector<SystemDesc>& systems, const int* num
   } else {
     double* S = m.pair3s.as<double>(static_cast<size_t>(pb.pair_total));
     hip_check(hipMemsetAsync(S, 0, bytes, stream), "hipMemsetAsync");
-    m.list_offsets(pb, w.pair, mixed, stream);
+    m.list_offsets(pb, w.pair, mixed, nullptr, stream);
     long long* o = m.off.as<long long>(natoms + 1);
     long long* io = m.item_off.as<long long>(natoms + 1);
     long long total = 0;
@@ -1032,7 +1041,7 @@ void Engine::count_terms(const std::vector<SystemDesc>& systems, const int* numb
   hip_check(hipMemsetAsync(dc, 0, sizeof(unsigned long long) * 3 * pb.db.nsys, stream), "hipMemsetAsync");
   launch_count_pairs(pb.db, dc, stream);
   // lists without the C6 test: every pair within the cutoff is a slot
-  m.list_offsets(pb, nullptr, false, stream);
+  m.list_offsets(pb, nullptr, false, nullptr, stream);
   long long* o = m.off.as<long long>(natoms + 1);
   long long total = 0;
   hip_check(hipMemcpyAsync(&total, o + natoms, sizeof total, hipMemcpyDeviceToHost, stream), "list size");
@@ -1103,7 +1112,7 @@ Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, cons
         launch_cn(s.pb.db, s.mixed, w.cn, stream);
         launch_weights(s.pb.db, w.cn, w.gw, w.dgw, stream);
         launch_pair(s.pb.db, s.mixed, w.gw, w.dgw, w.gwr, w.spr, w.pair, stream);
-        s.ws.list_offsets(s.pb, w.pair, s.mixed, stream);
+        s.ws.list_offsets(s.pb, w.pair, s.mixed, w.atm_acc, stream);
         long long total = 0;
         hip_check(hipMemcpyAsync(&total, s.ws.off.as<long long>(natoms + 1) + natoms, sizeof total,
                                  hipMemcpyDeviceToHost, stream),

@@ -747,15 +747,20 @@ void disp2_kernel(DevBatch b, const R4<R>* __restrict__ pair,
 // of different atoms is the reference's triple with the larger index as j, a
 // pair of two images of the same atom stands for the two triples (j, Tj, j,
 // Tk) and (j, Tk, j, Tj) of the reference, which have equal contributions.
+// The count pass also clears the atom's three-body sums acc (spatial order;
+// null when the gradient is not wanted) for the three-body kernel that
+// follows it in every evaluation.
 template <bool kFill, typename R>
 __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R>* __restrict__ pair,
                                                           long long* __restrict__ count,
                                                           const long long* __restrict__ offset,
-                                                          Slot<R>* __restrict__ slots, long long capacity) {
+                                                          Slot<R>* __restrict__ slots, long long capacity,
+                                                          double4* __restrict__ acc) {
   const int lane = wave_lane();
   const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (i >= b.natoms) return;
   if (kFill && offset[b.natoms] > capacity) return;  // the lists do not fit: no three-body terms
+  if (!kFill && acc && lane == 0) acc[b.geom[b.sys[i]].atom_begin + b.rank[i]] = make_double4(0.0, 0.0, 0.0, 0.0);
   const int s = b.sys[i];
   const SysGeom g = b.geom[s];
   const SysParam p = b.par[s];
@@ -823,37 +828,69 @@ __global__ __launch_bounds__(kBlock) void atm_list_kernel(DevBatch b, const R4<R
   if (!kFill && lane == 0) count[i] = total;
 }
 
-// Exclusive scan of n counts into offset[0..n] (single block).
-__global__ void scan_kernel(const long long* __restrict__ count, int n, long long* __restrict__ offset) {
-  __shared__ long long part[1024];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int chunk = (n + nt - 1) / nt;
+// The list offsets and the work-item offsets of n atoms in one block:
+// offset[0..n] = exclusive scan of the list sizes count[k], item_off[0..n] =
+// exclusive scan of their tile counts ceil(count[k] / 64).  Each thread sums
+// a chunk of consecutive atoms; the chunk sums are scanned across the
+// wavefronts (wavefront scans, then a scan of the wavefront totals).
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void list_scan_kernel(const long long* __restrict__ count, int n,
+                                                                long long* __restrict__ offset,
+                                                                long long* __restrict__ item_off) {
+  __shared__ long long wtot[2][kScanBlock / kWave];
+  const int tid = threadIdx.x, lane = wave_lane(), w = tid >> 6;
+  const int chunk = (n + kScanBlock - 1) / kScanBlock;
   const int lo = min(n, tid * chunk), hi = min(n, lo + chunk);
-  long long sum = 0;
-  for (int k = lo; k < hi; ++k) sum += count[k];
-  part[tid] = sum;
-  __syncthreads();
-  if (tid == 0) {
-    long long run = 0;
-    for (int k = 0; k < nt; ++k) {
-      long long v = part[k];
-      part[k] = run;
-      run += v;
-    }
-    offset[n] = run;
-  }
-  __syncthreads();
-  long long run = part[tid];
+  long long s0 = 0, s1 = 0;
   for (int k = lo; k < hi; ++k) {
-    offset[k] = run;
-    run += count[k];
+    const long long c = count[k];
+    s0 += c;
+    s1 += (c + kTile - 1) / kTile;
   }
-}
-
-// Number of slot blocks (work items) per atom.
-__global__ void atm_items_kernel(const long long* __restrict__ offset, int n, long long* __restrict__ items) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) items[i] = (offset[i + 1] - offset[i] + kTile - 1) / kTile;
+  long long i0 = s0, i1 = s1;  // inclusive scans within the wavefront
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const long long y0 = __shfl_up(i0, off, kWave), y1 = __shfl_up(i1, off, kWave);
+    if (lane >= off) {
+      i0 += y0;
+      i1 += y1;
+    }
+  }
+  if (lane == kWave - 1) {
+    wtot[0][w] = i0;
+    wtot[1][w] = i1;
+  }
+  __syncthreads();
+  if (w == 0) {  // exclusive scan of the wavefront totals
+    constexpr int kNw = kScanBlock / kWave;
+    long long t0 = lane < kNw ? wtot[0][lane] : 0, t1 = lane < kNw ? wtot[1][lane] : 0;
+    const long long e0 = t0, e1 = t1;
+#pragma unroll
+    for (int off = 1; off < kNw; off <<= 1) {
+      const long long y0 = __shfl_up(t0, off, kWave), y1 = __shfl_up(t1, off, kWave);
+      if (lane >= off) {
+        t0 += y0;
+        t1 += y1;
+      }
+    }
+    if (lane < kNw) {
+      wtot[0][lane] = t0 - e0;
+      wtot[1][lane] = t1 - e1;
+    }
+    if (lane == kNw - 1) {
+      offset[n] = t0;
+      item_off[n] = t1;
+    }
+  }
+  __syncthreads();
+  long long r0 = wtot[0][w] + i0 - s0, r1 = wtot[1][w] + i1 - s1;
+  for (int k = lo; k < hi; ++k) {
+    const long long c = count[k];
+    offset[k] = r0;
+    item_off[k] = r1;
+    r0 += c;
+    r1 += (c + kTile - 1) / kTile;
+  }
 }
 
 // --------------------------------------------------------- three-body term
@@ -943,7 +980,10 @@ __device__ __forceinline__ bool run_sums(double (&v)[N], int k) {
 // D, atom k at position r of its system's spatial order; k < 0 for no slot)
 // to the three-body sums of the slots' atoms, acc[r] = (G, D) of the system
 // (turned into gradient and dE/dCN by atm_acc_kernel), and to the i-side
-// sums:  grad_i -= G,  sigma += sym(v G^T).  The runs of one atom's images
+// sums:  grad_i -= G,  sigma += sym(v G^T).  (The virial of a triple
+// (i, a, b) with pair forces f_X v_X is f_A v_a v_a^T + f_B v_b v_b^T +
+// f_C v_ab v_ab^T = v_a G_a^T + v_b G_b^T, v relative to atom i and
+// G_s = dE/dv_s, so the slot sums carry it.)  The runs of one atom's images
 // are summed first, so one atomic per atom and value; the sums are indexed
 // by the spatial order, so the atomics of a tile touch few cache lines.
 // Every lane of the wavefront calls it.
@@ -980,11 +1020,6 @@ __global__ void atm_acc_kernel(DevBatch b, const double4* __restrict__ acc, doub
   grad[3 * a + 1] += s.y;
   grad[3 * a + 2] += s.z;
   dedcn[a] -= 0.5 * s.w;
-}
-
-__global__ void zero_double4(double4* __restrict__ p, int n) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < n) p[x] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
 // Three-body hot path: x**y for x > 0, 1/x and 1/sqrt(x) for normal x > 0.
@@ -2889,19 +2924,19 @@ void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw
 }
 
 namespace {
-__global__ __launch_bounds__(kBlock) void rank_pos_kernel(DevBatch b, double4* __restrict__ posr) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < b.natoms) posr[x] = b.pos[b.order[x]];
-}
 
-// Bounding box of every 64-position tile of the spatial order (wavefront per
-// tile), from the positions in that order.
-__global__ __launch_bounds__(kBlock) void tile_box_kernel(DevBatch b, double4* __restrict__ tbox) {
+// The positions in spatial order, posr[x] = pos[order[x]], and the bounding
+// box of every 64-position tile of that order (wavefront per tile; the tiles
+// cover every position).
+__global__ __launch_bounds__(kBlock) void tile_box_kernel(DevBatch b, double4* __restrict__ posr,
+                                                          double4* __restrict__ tbox) {
   const int lane = wave_lane();
   const int t = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (t >= b.ntiles) return;
   const int2 tile = b.tiles[t];
-  const double4 p = b.posr[tile.x + min(lane, tile.y - 1)];  // lanes past the tile repeat its last position
+  const int x = tile.x + min(lane, tile.y - 1);  // lanes past the tile repeat its last position
+  const double4 p = b.pos[b.order[x]];
+  if (lane < tile.y) posr[x] = p;
   double lo[3] = {p.x, p.y, p.z}, hi[3] = {p.x, p.y, p.z};
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1)
@@ -2937,11 +2972,9 @@ void* noalloc_area(int device, hipError_t* hip_err) {
 }
 
 void launch_rank_pos(const DevBatch& b, hipStream_t st) {
-  hipLaunchKernelGGL(rank_pos_kernel, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, b,
-                     const_cast<double4*>(b.posr));
   if (b.ntiles > 0)
     hipLaunchKernelGGL(tile_box_kernel, dim3(blocks_for(b.ntiles, kWavesPerBlock)), dim3(kBlock), 0, st, b,
-                       const_cast<double4*>(b.tbox));
+                       const_cast<double4*>(b.posr), const_cast<double4*>(b.tbox));
 }
 
 void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* dgw, double* gwr, int* spr,
@@ -2977,18 +3010,19 @@ void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, do
   }
 }
 
-void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, hipStream_t st) {
+void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, double4* acc,
+                      hipStream_t st) {
   const dim3 grid(blocks_for(b.natoms, kWavesPerBlock)), blk(kBlock);
   if (mixed)
     hipLaunchKernelGGL((atm_list_kernel<false, float>), grid, blk, 0, st, b, static_cast<const float4*>(pair), count,
-                       (const long long*)nullptr, (Slot<float>*)nullptr, 0LL);
+                       (const long long*)nullptr, (Slot<float>*)nullptr, 0LL, acc);
   else
     hipLaunchKernelGGL((atm_list_kernel<false, double>), grid, blk, 0, st, b, static_cast<const double4*>(pair),
-                       count, (const long long*)nullptr, (Slot<double>*)nullptr, 0LL);
+                       count, (const long long*)nullptr, (Slot<double>*)nullptr, 0LL, acc);
 }
 
-void launch_scan(const long long* count, int n, long long* offset, hipStream_t st) {
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, n, offset);
+void launch_list_scan(const long long* count, int n, long long* offset, long long* item_off, hipStream_t st) {
+  hipLaunchKernelGGL(list_scan_kernel, dim3(1), dim3(kScanBlock), 0, st, count, n, offset, item_off);
 }
 
 void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long long* offset, void* slots,
@@ -2996,10 +3030,10 @@ void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long
   const dim3 grid(blocks_for(b.natoms, kWavesPerBlock)), blk(kBlock);
   if (mixed)
     hipLaunchKernelGGL((atm_list_kernel<true, float>), grid, blk, 0, st, b, static_cast<const float4*>(pair),
-                       (long long*)nullptr, offset, static_cast<Slot<float>*>(slots), capacity);
+                       (long long*)nullptr, offset, static_cast<Slot<float>*>(slots), capacity, (double4*)nullptr);
   else
     hipLaunchKernelGGL((atm_list_kernel<true, double>), grid, blk, 0, st, b, static_cast<const double4*>(pair),
-                       (long long*)nullptr, offset, static_cast<Slot<double>*>(slots), capacity);
+                       (long long*)nullptr, offset, static_cast<Slot<double>*>(slots), capacity, (double4*)nullptr);
 }
 
 namespace {
@@ -3037,9 +3071,6 @@ void launch_atm_guard(const DevBatch& b, const long long* offset, long long capa
                      grad ? w.grad : nullptr, grad ? w.sig_atom : nullptr);
 }
 
-void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t st) {
-  hipLaunchKernelGGL(atm_items_kernel, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, offset, n, item_count);
-}
 
 namespace {
 __global__ void zero_counter(unsigned long long* c) { *c = 0ull; }
@@ -3055,10 +3086,8 @@ void launch_atm_t(const DevBatch& b, const long long* item_off, long long max_it
     wc = counter;
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, st, wc);
   }
-  if (kGrad) {
-    if (!w.atm_acc) throw std::invalid_argument("three-body gradient launch without its per-atom sums");
-    hipLaunchKernelGGL(zero_double4, dim3(blocks_for(b.natoms, kBlock)), dim3(kBlock), 0, st, w.atm_acc, b.natoms);
-  }
+  // w.atm_acc was cleared by the list count pass of this evaluation
+  if (kGrad && !w.atm_acc) throw std::invalid_argument("three-body gradient launch without its per-atom sums");
   hipLaunchKernelGGL((atm_kernel<R, kGrad, kSmooth, kW, kPart>), dim3(static_cast<unsigned>(grid)), dim3(kW * kWave), 0,
                      st, b, item_off, offset, static_cast<const Slot<R>*>(slots),
                      static_cast<const R4<R>*>(w.pair), w.e_atom, w.dedcn, w.grad, w.sig_atom, w.atm_acc, wc);

@@ -71,7 +71,8 @@ void launch_spatial_order(const DevBatch& b, void* temp, size_t temp_bytes, int*
                           hipStream_t stream);
 void launch_cn(const DevBatch& b, bool mixed, double* cn, hipStream_t stream);
 void launch_weights(const DevBatch& b, const double* cn, double* gw, double* dgw, hipStream_t stream);
-// posr[x] = pos[order[x]] for every position x of the batch's spatial order.
+// posr[x] = pos[order[x]] for every position x of the batch's spatial order,
+// and the bounding box of every 64-position tile of that order (tbox).
 void launch_rank_pos(const DevBatch& b, hipStream_t stream);
 // gwr [(dgw ? 2 : 1) * mref][natoms] and spr [natoms]: scratch for the
 // weights in spatial order.
@@ -81,8 +82,12 @@ void launch_pair(const DevBatch& b, bool mixed, const double* gw, const double* 
 // c6_begin, atom order) of pairwise two-body energies.
 void launch_disp2(const DevBatch& b, bool mixed, bool grad, const DevWork& w, double* pair2, hipStream_t stream);
 // Three-body lists: count per atom (pair may be null: no C6 test), fill slots.
-void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, hipStream_t stream);
-void launch_scan(const long long* count, int n, long long* offset, hipStream_t stream);
+// The count pass also clears acc (DevWork::atm_acc, may be null).
+void launch_atm_count(const DevBatch& b, const void* pair, bool mixed, long long* count, double4* acc,
+                      hipStream_t stream);
+// offset[0..n] and item_off[0..n]: exclusive scans of the list sizes and of
+// their 64-slot tile counts (one block).
+void launch_list_scan(const long long* count, int n, long long* offset, long long* item_off, hipStream_t stream);
 // Writes nothing when the lists hold more than `capacity` slots.
 void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long long* offset, void* slots,
                      long long capacity, hipStream_t stream);
@@ -93,8 +98,6 @@ void launch_atm_fill(const DevBatch& b, const void* pair, bool mixed, const long
 // a three-body term.  Runs after the two-body term.
 void launch_atm_guard(const DevBatch& b, const long long* offset, long long capacity, long long* item_off,
                       int* overflow, const DevWork& w, bool grad, hipStream_t stream);
-// Work items (atom i, slot block P) of the tiled three-body kernel.
-void launch_atm_items(const long long* offset, int n, long long* item_count, hipStream_t stream);
 
 // Launch shape of the three-body kernel: wavefronts per block (1, 2 or 4, the
 // block shares one 64-slot tile of a list) and either one block per work
