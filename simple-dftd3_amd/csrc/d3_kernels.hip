@@ -902,28 +902,22 @@ struct TileSlots {
   double vx[kTile], vy[kTile], vz[kTile];  // FP64 vectors for the exact r_jk tests
   R4<R> g0[kTile];                          // vx, vy, vz, r2
   R4<R> g1[kTile];                          // pdmp, sc6h, qi, qx
-  int2 key[kTile];                          // {queue tag of slot l as b, rank}: tag = l << 8, plus 1 << 16
-                                            // when the slot is an image of atom i
+  int rank[kTile];                          // storage index of the slot's atom (0 past the list's end)
 };
-// Accepted slot pair (a, b) of a tile pair.  The index of the (j, k) pair
-// record within the system (rank_j n + rank_k, below 2^47: the host refuses
-// larger systems) is split over the two words:
-//   lo = a | b << 8 | (both slots are images of atom i) << 16 | rec[0:15) << 17
-//   hi = rec >> 15
+// Accepted slot pair (a, b) of a tile pair, written at test step st:
+// lo = a | (a + st) << 8 (b = (a + st) mod 64), C = r_jk^2.  The term phase
+// finds the rest through the two slots: the (j, k) pair record rank_j n +
+// rank_k and the flag of a triple whose two slots are images of atom i.
 template <typename R>
 struct PairEntry {
-  unsigned lo, hi;
-  R C;  // r_jk^2
+  unsigned lo;
+  R C;
 };
 template <>
 struct PairEntry<double> {
-  unsigned lo, hi;
+  unsigned lo, pad;
   double C;
 };
-constexpr int kRecLoBits = 15;
-__device__ __forceinline__ unsigned long long entry_rec(unsigned lo, unsigned hi) {
-  return static_cast<unsigned long long>(lo >> 17) | (static_cast<unsigned long long>(hi) << kRecLoBits);
-}
 
 // Per-slot sums of a tile pair: G = sum dE/dv_s = sum (f_is + f_jk) v_s -
 // f_jk v_partner (three components), D = sum dE (qx_s + q_partner-side).
@@ -933,20 +927,20 @@ constexpr int kAcc = 4;
 // the slot's atom (-1 past the end of the list).
 template <typename R>
 __device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restrict__ slots, long long base, int idx,
-                                         int m, int l, int i) {
+                                         int m, int l) {
   if (idx < m) {
     const Slot<R> s = slots[base + idx];
     T.vx[l] = s.vx; T.vy[l] = s.vy; T.vz[l] = s.vz;
     T.g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
     T.g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
-    T.key[l] = make_int2((l << 8) | (s.atom == i ? 1 << 16 : 0), s.rank);
+    T.rank[l] = s.rank;
     return s.atom;
   }
   // fails every distance test
   T.vx[l] = 1e30; T.vy[l] = 1e30; T.vz[l] = 1e30;
   T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
   T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
-  T.key[l] = make_int2(l << 8, 0);
+  T.rank[l] = 0;
   return -1;
 }
 
@@ -1071,10 +1065,11 @@ template <> __device__ __forceinline__ float rsqrt_fast<float>(float x) { return
 // entries per wavefront:
 //  * test: at step s lane l tests a = l against b = (l + s) mod 64 with the
 //    reference's FP64 r_jk^2 (atm.f90:723-724) and appends accepted pairs
-//    (ballot + mbcnt compaction);
+//    (ballot + mbcnt compaction) as (a, b, r_jk^2): the b-slot vectors
+//    rotate across the lanes, the rest of the b-slot stays in LDS;
 //  * terms: each lane takes one queued pair, so every lane evaluates a
-//    triple; its per-slot sums go into double-precision LDS sums with
-//    atomics (eight per triple).
+//    triple (the pair record found from the two slots' ranks); its per-slot
+//    sums go into double-precision LDS sums with atomics (eight per triple).
 // A diagonal tile (Q == P) runs the steps 1..32 (32 only for l < 32) so that
 // every unordered pair is visited once.  The sums of the b-slots are flushed
 // at the end of each tile pair, those of the a-slots at the end of the item.
@@ -1178,13 +1173,12 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   double* __restrict__ accB = accBw[w];
   int jat = -1;  // atom of a-slot l
   if (w == 0) {
-    jat = load_tile(tileA, slots, L0, P * kTile + l, m, l, i);
+    jat = load_tile(tileA, slots, L0, P * kTile + l, m, l);
     if (kPart) part_atom[l] = jat < 0 ? i : jat;
     for (int c = 0; c < kNAcc; ++c) accA[c * kTile + l] = 0.0;
   }
   for (int c = 0; c < kNAcc; ++c) accB[c * kTile + l] = 0.0;
   __syncthreads();
-  const bool aval = P * kTile + l < m;
   // Slots past the end of the list hold the coordinates 1e30, so every test
   // with one of them fails (r^2 ~ 1e60, or 0 < eps for two of them) and the
   // test needs no validity flag.  Step 32 of a diagonal tile pairs a with
@@ -1192,11 +1186,10 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   // in a step of its own after the loop over the others.
   const double cut2_half = l < kTile / 2 ? cut2 : -1.0;
   const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
-  const int2 akey = tileA.key[l];
-  const unsigned long long arow = aval ? static_cast<unsigned long long>(akey.y) * static_cast<unsigned>(n) : 0ull;
-  // entry tag of the b-slot, keeping the self flag only when a is also an image of i
-  const unsigned amask = (akey.x & (1 << 16)) ? 0x1FF00u : 0xFF00u;
-  const long long irow = kPairs ? static_cast<long long>(b.rank[i]) * n : 0;  // pairs {i, s} in pair3s
+  const int arank = tileA.rank[l];
+  const int ri = b.rank[i];  // storage index of atom i: slots with this rank are its images
+  const unsigned lane_tag = static_cast<unsigned>(l) * 257u;  // entry word a | a << 8
+  const long long irow = kPairs ? static_cast<long long>(ri) * n : 0;  // pairs {i, s} in pair3s
 
   double ei = 0.0, di = 0.0;
   double gi[3] = {0.0, 0.0, 0.0};
@@ -1208,7 +1201,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
     TileSlots<R>& TB = diag ? tileA : tileB[w];
     double* __restrict__ accb = diag ? accA : accB;  // b-slots of a diagonal tile are a-slots
     int katl = -1;  // atom of b-slot l of this tile
-    if (!diag) katl = load_tile(TB, slots, L0, Q * kTile + l, m, l, i);
+    if (!diag) katl = load_tile(TB, slots, L0, Q * kTile + l, m, l);
     if (kPart && !diag) part_atom[(w + 1) * kTile + l] = katl < 0 ? i : katl;
     const int* __restrict__ batom = part_atom + (kPart && !diag ? (w + 1) * kTile : 0);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -1219,7 +1212,6 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
     // rotate by one lane after every step
     double bx = TB.vx[(l + s0) & (kTile - 1)], by = TB.vy[(l + s0) & (kTile - 1)],
            bz = TB.vz[(l + s0) & (kTile - 1)];
-    int2 kbr = TB.key[(l + s0) & (kTile - 1)];
     // one test step: lane l tests a = l against its b-slot with the cutoff
     // thr, appends an accepted pair to the ring, and the b-slots rotate
     auto test_step = [&](double thr) {
@@ -1235,14 +1227,11 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
         bal = __ballot(ok);
       }
       if (ok) {
-        const int pos = tail + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                                   static_cast<unsigned>(bal >> 32),
-                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal), 0u)));
-        const int2 kb = kbr;
+        const unsigned pos = __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(bal >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(bal),
+                                                                                 static_cast<unsigned>(tail)));
         PairEntry<R> e;
-        const unsigned long long rec = arow + static_cast<unsigned>(kb.y);
-        e.lo = (static_cast<unsigned>(kb.x) & amask) | static_cast<unsigned>(l) | (static_cast<unsigned>(rec) << 17);
-        e.hi = static_cast<unsigned>(rec >> kRecLoBits);
+        e.lo = lane_tag + (static_cast<unsigned>(st) << 8);
         e.C = R(Cd);
         q_ring[pos & (kQueue - 1)] = e;
       }
@@ -1250,8 +1239,6 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       bx = wave_rotate1(bx);
       by = wave_rotate1(by);
       bz = wave_rotate1(bz);
-      kbr.x = wave_rotate1(kbr.x);
-      kbr.y = wave_rotate1(kbr.y);
       ++st;
     };
     // the steps before the half step of a diagonal tile, or all 64
@@ -1267,9 +1254,12 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       for (; tail - head >= kWave || (done && tail > head); head += kWave) {
         if (head + l < tail) {
           const PairEntry<R> e = q_ring[(head + l) & (kQueue - 1)];
-          const unsigned long long rec = entry_rec(e.lo, e.hi);
+          const int a = e.lo & 0xff, bb = (e.lo >> 8) & (kTile - 1);
+          const int ka = tileA.rank[a], kb = TB.rank[bb];
+          const unsigned long long rec =
+              static_cast<unsigned long long>(static_cast<unsigned>(ka)) * static_cast<unsigned>(n) +
+              static_cast<unsigned>(kb);
           const R4<R> q = prow[rec];
-          const int a = e.lo & 0xff, bb = (e.lo >> 8) & 0xff;
           const R C = e.C;
           const R4<R> a0 = tileA.g0[a], a1 = tileA.g1[a];
           const R4<R> b0 = TB.g0[bb], b1 = TB.g1[bb];
@@ -1288,7 +1278,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
           const R pp = a1.x * b1.x * (q.w * (alp14 ? R(pow_m8_3(double(C))) : pow_pos(C, malp6)));
           const R fd = rcp_fast(R(1) + R(6) * pp);
           R c9 = ns9 * a1.y * b1.y * q.x;
-          if ((e.lo >> 16) & 1u) c9 *= R(4.0 / 3.0);  // j == k == i
+          if (ka == ri && kb == ri) c9 *= R(4.0 / 3.0);  // j == k == i
           const R cwf = c9 * fd;
           R dE = ang * cwf;
           R sw = R(1), swa = R(1), dswa = R(0), swb = R(1), dswb = R(0), swc = R(1), dswc = R(0);
@@ -1349,12 +1339,12 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       const double Gx = accB[0 * kTile + l], Gy = accB[1 * kTile + l], Gz = accB[2 * kTile + l];
       const double D = accB[3 * kTile + l];
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
-      flush_tile(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, TB.key[l].y, slot_acc + g.atom_begin, gi, sg);
+      flush_tile(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, TB.rank[l], slot_acc + g.atom_begin, gi, sg);
     }
     if (kPairs && !diag) {  // pair {i, b-slot atom}
       double t[1] = {accB[l]};
       accB[l] = 0.0;
-      if (run_sums(t, katl) && katl >= 0 && t[0] != 0.0) atomicAdd(pair3s + g.c6_begin + irow + TB.key[l].y, t[0]);
+      if (run_sums(t, katl) && katl >= 0 && t[0] != 0.0) atomicAdd(pair3s + g.c6_begin + irow + TB.rank[l], t[0]);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -1362,10 +1352,10 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   __syncthreads();
   if (kGrad && w == 0)
     flush_tile(accA[0 * kTile + l], accA[1 * kTile + l], accA[2 * kTile + l], accA[3 * kTile + l], ax, ay, az,
-               jat, akey.y, slot_acc + g.atom_begin, gi, sg);
+               jat, arank, slot_acc + g.atom_begin, gi, sg);
   if (kPairs && w == 0) {  // pair {i, a-slot atom}
     double t[1] = {accA[l]};
-    if (run_sums(t, jat) && jat >= 0 && t[0] != 0.0) atomicAdd(pair3s + g.c6_begin + irow + akey.y, t[0]);
+    if (run_sums(t, jat) && jat >= 0 && t[0] != 0.0) atomicAdd(pair3s + g.c6_begin + irow + arank, t[0]);
   }
   block_add(e_atom + i, ei, red);
   if (kGrad) {
