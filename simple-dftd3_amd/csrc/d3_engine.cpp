@@ -321,6 +321,38 @@ void image_box(const SystemDesc& sd, double cut, const double4* imgs, int count,
 // instead of a read-back of the list size.
 constexpr long long kUnsyncedSlotBytes = 64LL << 20;
 
+int device_cu_count() {
+  int dev = 0, cus = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+  return cus;
+}
+
+// Three-body launch shape of a call without a plan (waves == 0 asks for it).
+// Below kLargeAtmItems work items (one 200-atom molecule has about 400) four
+// wavefronts per block and one block per item, so that the few items spread
+// over the chip.  Above, a persistent grid, whose smaller blocks leave fewer
+// wavefronts idle at the end of an item: two wavefronts per block and 8
+// blocks per CU, or, for lists with lattice images in mixed precision, one
+// and 16.  Measured (whole evaluations, ms, this shape / four wavefronts and
+// one block per item / best of the ten shapes): 2,000-atom cell mixed 13.08 /
+// 14.13 / 13.08, FP64 19.01 / 20.51 / 19.01; 20,000-atom ellipsoid mixed
+// 109.8 / 116.5 / 109.2, FP64 161.0 / 177.3 / 161.0; 1,000- and 2,000-atom
+// molecules mixed 0.93 / 0.99 / 0.90 and 3.36 / 3.60 / 3.26.
+constexpr long long kLargeAtmItems = 8192;
+const AtmShape kPerCallShape{0, 0, 0};
+
+AtmShape per_call_shape(bool mixed, bool periodic, long long items) {
+  AtmShape a;
+  if (items >= kLargeAtmItems) {
+    const bool one = mixed && periodic;
+    a.waves = one ? 1 : 2;
+    a.blocks_per_cu = one ? 16 : 8;
+    a.num_cus = device_cu_count();
+  }
+  return a;
+}
+
 struct Workspace {
   DeviceBuffer meta, pos, rank, order, sort_tmp, cn, gw, dgw, pair, e_atom, dedcn, grad, sig, atm_acc, cnt, off,
       item_off, slots, gw_rank, sp_rank, pos_rank, tbox, counts, counter, flag, pair3s, hbuf, e_out, ew_tab, ew_c6l, ew_lc6l, ew_ldc6l, ew_part,
@@ -877,9 +909,10 @@ void Workspace::evaluate(const Prepared& pb, const double* positions_dev, bool o
       launch_atm_guard(db, o, capacity, io, flag.as<int>(1), w, grad, stream);
     }
     timed("atm_list", stream, [&] { launch_atm_fill(db, w.pair, mixed, o, s, capacity, stream); });
-    unsigned long long* wc = shape.blocks_per_cu > 0 ? counter.as<unsigned long long>(1) : nullptr;
+    const AtmShape sh = shape.waves == 0 ? per_call_shape(mixed, pb.periodic_atm, max_items) : shape;
+    unsigned long long* wc = sh.blocks_per_cu > 0 ? counter.as<unsigned long long>(1) : nullptr;
     timed("atm", stream,
-          [&] { launch_atm(db, mixed, grad, smooth, pb.partitioned, shape, io, max_items, o, s, w, wc, stream); });
+          [&] { launch_atm(db, mixed, grad, smooth, pb.partitioned, sh, io, max_items, o, s, w, wc, stream); });
   }
   if (grad) timed("cn_chain", stream, [&] { launch_cn_chain(db, mixed, w, stream); });
   timed("reduce", stream, [&] { launch_reduce(db, w, out.energy, grad ? out.sigma : nullptr, stream); });
@@ -893,7 +926,7 @@ void Engine::run(const std::vector<SystemDesc>& systems, const int* numbers, con
   Workspace& m = *ws_;
   const Prepared pb = m.upload(systems, numbers, stream, ghost_host);
   const bool grad = out.gradient || out.sigma;
-  m.evaluate(pb, positions_dev, true, precision == kMixed, grad, pb.smooth, out, debug, stats, -1, 0, AtmShape{},
+  m.evaluate(pb, positions_dev, true, precision == kMixed, grad, pb.smooth, out, debug, stats, -1, 0, kPerCallShape,
              stream);
   if (m.timer.enabled) m.timer.collect();
 }
@@ -930,7 +963,7 @@ void run_noalloc(const std::vector<SystemDesc>& systems, const int* numbers, con
   BatchOutput inner = out;
   inner.gradient = nullptr;
   m.evaluate(pb, positions_dev, true, precision == kMixed, grad, pb.smooth, inner, nullptr, nullptr, -1, 0,
-             AtmShape{}, stream);
+             kPerCallShape, stream);
   if (out.gradient)
     hip_check(hipMemcpyAsync(out.gradient, m.grad.as<double>(size_t(pb.db.natoms) * 3),
                              sizeof(double) * 3 * size_t(pb.db.natoms), hipMemcpyDeviceToDevice, stream),
@@ -995,7 +1028,7 @@ void Engine::run_hessian(const std::vector<SystemDesc>& systems, const int* numb
   BatchOutput out;
   out.energy = e;
   RunStats stats;
-  m.evaluate(pb, positions_dev, true, false, false, pb.smooth, out, nullptr, &stats, -1, 0, AtmShape{}, stream);
+  m.evaluate(pb, positions_dev, true, false, false, pb.smooth, out, nullptr, &stats, -1, 0, kPerCallShape, stream);
   const size_t nn = static_cast<size_t>(pb.pair_total), nm = static_cast<size_t>(natoms) * db.mref;
   const size_t words = 3 * nm + 4 * nn + 3 * nn + natoms + 3 * nn + nn + 3 * nn;
   double* base = m.hbuf.as<double>(words);
@@ -1079,14 +1112,6 @@ struct Plan::State {
   }
 };
 
-namespace {
-int device_cu_count() {
-  int dev = 0, cus = 0;
-  hip_check(hipGetDevice(&dev), "hipGetDevice");
-  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-  return cus;
-}
-}  // namespace
 
 Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, const double* positions_dev,
            int precision, bool grad, const int* ghost_host, hipStream_t stream)
@@ -1122,6 +1147,9 @@ Plan::Plan(const std::vector<SystemDesc>& systems, const int* numbers_host, cons
       }
       s.capacity = cap;
       s.max_items = std::min(s.pb.item_bound, cap / kTile + natoms);
+      const AtmShape a = per_call_shape(s.mixed, s.pb.periodic_atm, s.max_items);  // until set_shape or autotune
+      s.shape.waves = a.waves;
+      s.shape.blocks_per_cu = a.blocks_per_cu;
       // size every buffer now, so that an evaluation allocates nothing
       (void)s.ws.slots.get(std::max<long long>(cap, 1) * slot_record_bytes(s.mixed));
       (void)s.ws.counter.as<unsigned long long>(1);

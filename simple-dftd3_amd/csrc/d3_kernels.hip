@@ -899,7 +899,6 @@ constexpr int kQueue = 128;  // entries of the accepted-pair ring of a wavefront
 // Slot data of one 64-slot tile in LDS.
 template <typename R>
 struct TileSlots {
-  double vx[kTile], vy[kTile], vz[kTile];  // FP64 vectors for the exact r_jk tests
   R4<R> g0[kTile];                          // vx, vy, vz, r2
   R4<R> g1[kTile];                          // pdmp, sc6h, qi, qx
   int rank[kTile];                          // storage index of the slot's atom (0 past the list's end)
@@ -923,24 +922,29 @@ struct PairEntry<double> {
 // f_jk v_partner (three components), D = sum dE (qx_s + q_partner-side).
 constexpr int kAcc = 4;
 
-// Loads slot idx of the list of atom i into position l of a tile; returns
-// the slot's atom (-1 past the end of the list).
+// Loads slot idx of the list of atom i into position l of a tile and its
+// FP64 vector into (vx, vy, vz); returns the slot's atom (-1 past the end of
+// the list).  Without a tile (T == nullptr) only the vector is loaded.
 template <typename R>
-__device__ __forceinline__ int load_tile(TileSlots<R>& T, const Slot<R>* __restrict__ slots, long long base, int idx,
-                                         int m, int l) {
+__device__ __forceinline__ int load_tile(TileSlots<R>* T, const Slot<R>* __restrict__ slots, long long base, int idx,
+                                         int m, int l, double& vx, double& vy, double& vz) {
   if (idx < m) {
     const Slot<R> s = slots[base + idx];
-    T.vx[l] = s.vx; T.vy[l] = s.vy; T.vz[l] = s.vz;
-    T.g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
-    T.g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
-    T.rank[l] = s.rank;
+    vx = s.vx; vy = s.vy; vz = s.vz;
+    if (T) {
+      T->g0[l] = make_r4<R>(R(s.vx), R(s.vy), R(s.vz), s.r2);
+      T->g1[l] = make_r4<R>(s.pdmp, s.sc6h, s.qi, s.qx);
+      T->rank[l] = s.rank;
+    }
     return s.atom;
   }
   // fails every distance test
-  T.vx[l] = 1e30; T.vy[l] = 1e30; T.vz[l] = 1e30;
-  T.g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
-  T.g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
-  T.rank[l] = 0;
+  vx = 1e30; vy = 1e30; vz = 1e30;
+  if (T) {
+    T->g0[l] = make_r4<R>(R(0), R(0), R(0), R(0));
+    T->g1[l] = make_r4<R>(R(0), R(0), R(0), R(0));
+    T->rank[l] = 0;
+  }
   return -1;
 }
 
@@ -1172,10 +1176,13 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   // A tile (shared)
   double* __restrict__ accB = accBw[w];
   int jat = -1;  // atom of a-slot l
+  double ax, ay, az;  // FP64 vector of a-slot l (every wavefront holds them)
   if (w == 0) {
-    jat = load_tile(tileA, slots, L0, P * kTile + l, m, l);
+    jat = load_tile(&tileA, slots, L0, P * kTile + l, m, l, ax, ay, az);
     if (kPart) part_atom[l] = jat < 0 ? i : jat;
     for (int c = 0; c < kNAcc; ++c) accA[c * kTile + l] = 0.0;
+  } else {
+    load_tile<R>(nullptr, slots, L0, P * kTile + l, m, l, ax, ay, az);
   }
   for (int c = 0; c < kNAcc; ++c) accB[c * kTile + l] = 0.0;
   __syncthreads();
@@ -1185,7 +1192,6 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   // a + 32 and a + 32 with a: only lanes l < 32 take it (cutoff -1 otherwise),
   // in a step of its own after the loop over the others.
   const double cut2_half = l < kTile / 2 ? cut2 : -1.0;
-  const double ax = tileA.vx[l], ay = tileA.vy[l], az = tileA.vz[l];
   const int arank = tileA.rank[l];
   const int ri = b.rank[i];  // storage index of atom i: slots with this rank are its images
   const unsigned lane_tag = static_cast<unsigned>(l) * 257u;  // entry word a | a << 8
@@ -1200,18 +1206,23 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
     const bool diag = Q == P;
     TileSlots<R>& TB = diag ? tileA : tileB[w];
     double* __restrict__ accb = diag ? accA : accB;  // b-slots of a diagonal tile are a-slots
-    int katl = -1;  // atom of b-slot l of this tile
-    if (!diag) katl = load_tile(TB, slots, L0, Q * kTile + l, m, l);
+    int katl = -1;                          // atom of b-slot l of this tile
+    double vbx = 0.0, vby = 0.0, vbz = 0.0;  // and its FP64 vector
+    if (!diag) katl = load_tile(&TB, slots, L0, Q * kTile + l, m, l, vbx, vby, vbz);
     if (kPart && !diag) part_atom[(w + 1) * kTile + l] = katl < 0 ? i : katl;
     const int* __restrict__ batom = part_atom + (kPart && !diag ? (w + 1) * kTile : 0);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const int s0 = diag ? 1 : 0, s1 = diag ? kTile / 2 : kTile - 1;
     int st = s0, head = 0, tail = 0;
-    // the b-slot of lane l at step st is (l + st) mod 64: its vector and key
-    // rotate by one lane after every step
-    double bx = TB.vx[(l + s0) & (kTile - 1)], by = TB.vy[(l + s0) & (kTile - 1)],
-           bz = TB.vz[(l + s0) & (kTile - 1)];
+    // the b-slot of lane l at step st is (l + st) mod 64: its vector rotates
+    // by one lane after every step
+    double bx = vbx, by = vby, bz = vbz;
+    if (diag) {
+      bx = wave_rotate1(ax);
+      by = wave_rotate1(ay);
+      bz = wave_rotate1(az);
+    }
     // one test step: lane l tests a = l against its b-slot with the cutoff
     // thr, appends an accepted pair to the ring, and the b-slots rotate
     auto test_step = [&](double thr) {
@@ -1339,7 +1350,7 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
       const double Gx = accB[0 * kTile + l], Gy = accB[1 * kTile + l], Gz = accB[2 * kTile + l];
       const double D = accB[3 * kTile + l];
       for (int c = 0; c < kAcc; ++c) accB[c * kTile + l] = 0.0;
-      flush_tile(Gx, Gy, Gz, D, TB.vx[l], TB.vy[l], TB.vz[l], katl, TB.rank[l], slot_acc + g.atom_begin, gi, sg);
+      flush_tile(Gx, Gy, Gz, D, vbx, vby, vbz, katl, TB.rank[l], slot_acc + g.atom_begin, gi, sg);
     }
     if (kPairs && !diag) {  // pair {i, b-slot atom}
       double t[1] = {accB[l]};
@@ -1374,11 +1385,11 @@ __device__ __forceinline__ void atm_body(DevBatch b, const long long* __restrict
   }
 }
 
-// At least three wavefronts per SIMD: the FP64 E+grad instantiation otherwise
-// takes 184 VGPRs (two wavefronts) and runs 14 % slower; with the hint it
-// takes 168 and spills 20 bytes.
+// At least three wavefronts per SIMD in FP64 (the E+grad instantiation
+// otherwise takes 184 VGPRs, two wavefronts, and runs 14 % slower; with the
+// hint it takes 168 and spills a few dozen bytes) and four in FP32.
 template <typename R, bool kGrad, bool kSmooth, int kW, bool kPart>
-__global__ __launch_bounds__(kW * kWave) __attribute__((amdgpu_waves_per_eu(3))) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
+__global__ __launch_bounds__(kW * kWave) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 3))) void atm_kernel(DevBatch b, const long long* __restrict__ item_off,
                                                          const long long* __restrict__ offset,
                                                          const Slot<R>* __restrict__ slots,
                                                          const R4<R>* __restrict__ pair, double* __restrict__ e_atom,

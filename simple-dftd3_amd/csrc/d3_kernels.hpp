@@ -102,7 +102,8 @@ void launch_atm_guard(const DevBatch& b, const long long* offset, long long capa
 // Launch shape of the three-body kernel: wavefronts per block (1, 2 or 4, the
 // block shares one 64-slot tile of a list) and either one block per work
 // item (blocks_per_cu = 0) or a persistent grid of blocks_per_cu blocks per
-// compute unit drawing items from a work counter.
+// compute unit drawing items from a work counter.  The engine takes waves = 0
+// as "choose from the number of work items".
 struct AtmShape {
   int waves = 4;
   int blocks_per_cu = 0;
