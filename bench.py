@@ -41,12 +41,15 @@ PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}  # MI355X vector FP64 / FP32
 # profiles/atm_kernel__config1_periodic_2000.md.  "kernel_sha" is the SHA-256
 # (first 16 hex digits) of csrc/d3_kernels.hip the counters were read on: a
 # report on other kernel sources gives traffic = null.
-# The three-body kernel of a mixed-precision call on the 2,000-atom cell: the
-# per-call launch shape of a large evaluation, one wavefront per block (the
-# fourth template argument) and 16 persistent blocks per CU.
-ATM_KERNEL_C1 = "atm_kernel<float, true, false, 1, false>"
-ATM_TRAFFIC_C1 = {"kernel": ATM_KERNEL_C1, "fetch_kib": 12_216_000, "write_kib": 4_878_000,
-                  "fetch_scale": 2.0, "write_scale": 1.0, "kernel_sha": "fc54b12f63ca07ad"}
+# The mixed-precision three-body kernel of a large call without a plan: one
+# wavefront per block (the fourth template argument; 16 persistent blocks per
+# CU) for lists with lattice images, two (8 per CU) otherwise.
+def atm_kernel_name(periodic):
+    return f"atm_kernel<float, true, false, {1 if periodic else 2}, false>"
+
+
+ATM_TRAFFIC_C1 = {"kernel": atm_kernel_name(True), "fetch_kib": 12_889_626, "write_kib": 4_879_442,
+                  "fetch_scale": 2.0, "write_scale": 1.0, "kernel_sha": "a5ee85cc7b04f681"}
 KERNEL_SOURCE = os.path.join(ROOT, "simple-dftd3_amd", "csrc", "d3_kernels.hip")
 
 
@@ -716,7 +719,7 @@ def bench_single(args, systems, config, spec, positions, counts, cpu, check):
         mixed_over_fp64=fp64["ms_per_step"] / mixed["ms_per_step"],
         mixed_vs_fp64_error=dict(energy_per_atom=de / natoms, max_gradient=dg, max_sigma=ds),
         kernel_ms=dict(fp64=fp64["kernel_ms"], mixed=mixed["kernel_ms"]),
-        roofline=dict(kernel=ATM_KERNEL_C1, bound="valu", achieved=mixed["atm_tflops"],
+        roofline=dict(kernel=atm_kernel_name(systems[0].get("lattice") is not None), bound="valu", achieved=mixed["atm_tflops"],
                       peak=PEAK_TFLOPS["mixed"], unit="TFLOP/s", frac=mixed["atm_tflops"] / PEAK_TFLOPS["mixed"],
                       traffic=traffic, ops_per_triple=OPS_PER_TRIPLE, triples_per_launch=int(counts[2]),
                       launch_ms=mixed["atm_kernel_ms"], traffic_unit="bytes per launch (HBM read + write)",
