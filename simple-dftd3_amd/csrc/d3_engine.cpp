@@ -460,8 +460,9 @@ Prepared Workspace::upload(const std::vector<SystemDesc>& systems, const int* nu
   int natoms = 0;
   for (const auto& s : systems) {
     if (s.natoms <= 0) throw Error("Invalid molecular structure data provided");
-    // pair-record indices of the three-body kernel hold rank_j n + rank_k in
-    // 47 bits (n up to 11.8 million atoms, far beyond device memory)
+    // the n x n pair records of a system (16 or 32 bytes each) outgrow any
+    // device memory long before n^2 reaches 2^47 (11.8 million atoms); the
+    // bound keeps every index product far from 64-bit overflow
     if (static_cast<long long>(s.natoms) * s.natoms >= (1LL << 47))
       throw Error("Molecular structure too large: " + std::to_string(s.natoms) + " atoms");
     if (natoms > std::numeric_limits<int>::max() - s.natoms)
