@@ -239,6 +239,25 @@ def test_elongated_molecule_tile_culling(precision):
     _check(r, _oracle(num, pos), n, mixed=precision == "mixed")
 
 
+@pytest.mark.parametrize("scale", [1.0, 1.0 + 3e-7, 1.0 - 3e-7])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_three_body_cutoff_band(precision, scale):
+    """Slot pairs at the three-body cutoff: a 3 x 3 x 4 grid of spacing 3
+    bohr (times 1, 1 + 3e-7, 1 - 3e-7) with a 6-bohr three-body cutoff, so
+    that many r_jk^2 fall exactly at 36 (taken: r^2 <= c^2, atm.f90:723-724)
+    or within 2e-6 of it on either side; a triple taken or dropped wrongly
+    moves the energy by ~1e-5 Eh."""
+    _step(f"three-body cutoff band {precision} scale {scale}")
+    rng = np.random.default_rng(5)
+    g = np.array([[x, y, z] for x in range(3) for y in range(3) for z in range(4)], dtype=float)
+    pos = 3.0 * scale * g
+    num = rng.choice([1, 6], size=len(pos))
+    sys_ = dict(numbers=num, positions=pos, param=BJ, cutoff=dict(disp3=6.0))
+    r = _batch([sys_], precision=precision)[0]
+    _check(r, _oracle(num, pos, cutoff=(O.DEFAULT_CUTOFF[0], O.DEFAULT_CUTOFF[1], 6.0, 0.0, 0.0)), len(pos),
+           mixed=precision == "mixed")
+
+
 def test_periodic_2d_slab():
     _step("2D slab BJ + ATM")
     num, pos, lat = _cell(22, 10, 9.0)
