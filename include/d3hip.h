@@ -315,7 +315,10 @@ D3HIP_API void d3hip_plan_execute(dftd3_error error, d3hip_plan plan, const doub
 
 /* Candidate launch shapes of the three-body kernel: wavefronts per block
  * (1, 2 or 4) and persistent blocks per compute unit (0 = one block per work
- * item). */
+ * item).  A new plan starts with the shape of a call without a plan: four
+ * wavefronts and one block per item below 8,192 work items; above, two
+ * wavefronts and 8 blocks per CU, or one and 16 for mixed-precision lists
+ * with lattice images. */
 D3HIP_API int d3hip_plan_num_shapes(void);
 D3HIP_API void d3hip_plan_candidate_shape(int index, int* waves, int* blocks_per_cu);
 D3HIP_API void d3hip_plan_get_shape(dftd3_error error, d3hip_plan plan, int* waves, int* blocks_per_cu);

@@ -87,6 +87,26 @@ def test_noalloc_matches_batch_entry_two_calls(precision):
         _agree(out, ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_noalloc_persistent_three_body_grid(precision):
+    """A 320-atom cell (about 10,000 three-body work items, above the
+    threshold of the persistent launch: its work counter lives in the work
+    area too) through both entry points."""
+    from dftd3 import torch_api
+    dev = torch.device("cuda")
+    _step(f"{precision}: 320-atom cell, persistent three-body grid")
+    num, pos, lat = cell(41, 320, 28.0)
+    spec, pos = torch_api.batch_spec([dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3, param=BJ)])
+    p = torch.as_tensor(pos, dtype=torch.float64, device=dev).contiguous()
+    ref, out = _outputs(spec, dev), _outputs(spec, dev)
+    _call(False, spec, p, ref, precision)
+    free0, _ = torch.cuda.mem_get_info()
+    _call(True, spec, p, out, precision)
+    free1, _ = torch.cuda.mem_get_info()
+    assert free1 == free0, (free0, free1)
+    _agree(out, ref)
+
+
 def test_noalloc_rejects_a_batch_beyond_the_work_area():
     from dftd3 import synthetic, torch_api
     dev = torch.device("cuda")
