@@ -97,6 +97,28 @@ def test_every_launch_shape_gives_the_same_result(precision):
         _close(plan.execute(out=out), ref)
 
 
+def test_initial_launch_shape_follows_the_work():
+    """A new plan starts with the shape of a call without a plan: four
+    wavefronts and one block per work item for small work (a 200-atom
+    molecule, about 400 items), a persistent grid above 8,192 items (two
+    wavefronts and 8 blocks per CU; one and 16 for mixed-precision lists
+    with lattice images)."""
+    from dftd3 import synthetic, torch_api
+    _step("initial launch shapes")
+    num, pos = synthetic.molecule_c0()
+    small = [dict(numbers=num, positions=pos, param=BJ)]
+    num, pos = synthetic.molecule_c0(seed=5, n=1300)
+    molecule = [dict(numbers=num, positions=pos, param=BJ)]
+    num, pos, lat = cell(41, 320, 28.0)
+    crystal = [dict(numbers=num, positions=pos, lattice=lat, periodic=[True] * 3, param=BJ)]
+    for systems, fp64, mixed in ((small, (4, 0), (4, 0)), (molecule, (2, 8), (2, 8)), (crystal, (2, 8), (1, 16))):
+        spec, p = _spec(systems)
+        for precision, want in (("fp64", fp64), ("mixed", mixed)):
+            plan = torch_api.BatchPlan(spec, p, precision=precision)
+            assert plan.shape == want, (len(systems[0]["numbers"]), precision, plan.shape)
+            del plan
+
+
 def test_autotune_keeps_a_candidate_and_results():
     from dftd3 import library, torch_api
     _step("autotune")
